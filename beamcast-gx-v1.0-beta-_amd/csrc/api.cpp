@@ -1,0 +1,305 @@
+// api.cpp -- the C ABI of beamcast_hip.h: context, scene residency, ray queries, render.
+// Mirrors the OpenCL host runtime it replaces (gpu_opencl_windows.go:94-723): a persistent
+// context with a stream, scene buffers cached by revision, errors returned as status codes.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/beamcast_hip.h"
+#include "api_internal.hpp"
+#include "bvh8_build.hpp"
+#include "kernels.hpp"
+
+namespace {
+thread_local std::string g_create_error;
+
+bool ok(bc_ctx* c, hipError_t e, const char* what) {
+    if (e == hipSuccess) return true;
+    if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+void free_dev(void*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+template <class T>
+bool upload_vec(bc_ctx* c, T*& dst, size_t& cap_bytes, const std::vector<T>& v, const char* what) {
+    size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+    if (bytes > cap_bytes) {
+        void* p = dst;
+        free_dev(p);
+        dst = nullptr;
+        cap_bytes = 0;
+        if (!ok(c, hipMalloc(&p, bytes), what)) return false;
+        dst = (T*)p;
+        cap_bytes = bytes;
+    }
+    if (!v.empty() && !ok(c, hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), what))
+        return false;
+    return true;
+}
+}  // namespace
+
+bool bc_ensure(bc_ctx* c, void*& p, size_t& cap, size_t bytes, const char* what) {
+    if (bytes <= cap && p) return true;
+    free_dev(p);
+    cap = 0;
+    if (!ok(c, hipMalloc(&p, bytes), what)) return false;
+    cap = bytes;
+    return true;
+}
+
+bool bc_check(bc_ctx* c, hipError_t e, const char* what) { return ok(c, e, what); }
+
+extern "C" {
+
+int bc_device_available(char* name, int cap) {
+    int n = 0;
+    std::string msg;
+    int found = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        msg = "no HIP device";
+    } else {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, 0) == hipSuccess && std::strstr(p.gcnArchName, "gfx950")) {
+            msg = std::string(p.name) + " (" + p.gcnArchName + ")";
+            found = 1;
+        } else {
+            msg = "device 0 is not gfx950";
+        }
+    }
+    if (name && cap > 0) {
+        std::snprintf(name, (size_t)cap, "%s", msg.c_str());
+    }
+    return found;
+}
+
+bc_ctx* bc_create(int device_ordinal) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device_ordinal < 0 || device_ordinal >= n) {
+        g_create_error = "no HIP device with ordinal " + std::to_string(device_ordinal);
+        return nullptr;
+    }
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device_ordinal) != hipSuccess || !std::strstr(p.gcnArchName, "gfx950")) {
+        g_create_error = "device is not gfx950 (MI355X); this library is built for gfx950 only";
+        return nullptr;
+    }
+    if (hipSetDevice(device_ordinal) != hipSuccess) {
+        g_create_error = "hipSetDevice failed";
+        return nullptr;
+    }
+    bc_ctx* c = new bc_ctx();
+    c->device = device_ordinal;
+    c->name = std::string(p.name) + " (" + p.gcnArchName + ")";
+    c->num_cus = p.multiProcessorCount;
+    if (!ok(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
+        !ok(c, hipEventCreate(&c->ev0), "hipEventCreate") || !ok(c, hipEventCreate(&c->ev1), "hipEventCreate") ||
+        !ok(c, hipMalloc((void**)&c->d_err, 64), "hipMalloc(err)") ||
+        !ok(c, hipMemset(c->d_err, 0, 64), "hipMemset(err)")) {
+        g_create_error = c->err;
+        bc_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+void bc_destroy(bc_ctx* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+                    c->tmp_in, c->tmp_out};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    bc_render_release(c);
+    if (c->ev0) hipEventDestroy(c->ev0);
+    if (c->ev1) hipEventDestroy(c->ev1);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* bc_last_error(const bc_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+const char* bc_device_name(const bc_ctx* c) { return c ? c->name.c_str() : ""; }
+
+int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t geom) {
+    if (!c || !v) return BC_ERR_ARG;
+    if (c->has_scene && topo == c->topo && geom == c->geom) return BC_OK;
+    if (v->nmat <= 0) { c->err = "HIP backend requires at least one material"; return BC_ERR_SCENE; }
+    hipSetDevice(c->device);
+    bcx::Bvh8 b;
+    std::string err;
+    if (!bcx::build_bvh8(*v, b, err)) { c->err = err; return BC_ERR_SCENE; }
+    const int stack = bcd::stack_for_depth(b.depth);
+    if (stack < 0) { c->err = "BVH8 depth " + std::to_string(b.depth) + " exceeds the traversal stack (33 levels)"; return BC_ERR_SCENE; }
+    for (int i = 0; i < v->nprim; i++) {
+        if (v->prim_mat[i] < 0 || v->prim_mat[i] >= v->nmat) { c->err = "primitive material out of range"; return BC_ERR_SCENE; }
+    }
+    // materials (renderer.go:93-99)
+    std::vector<bc_mat> mats((size_t)v->nmat);
+    for (int i = 0; i < v->nmat; i++) {
+        const double* d = v->mat_data + 8 * (size_t)i;
+        bc_mat& m = mats[(size_t)i];
+        std::memset(&m, 0, sizeof m);
+        m.color[0] = (float)d[0]; m.color[1] = (float)d[1]; m.color[2] = (float)d[2]; m.rough = (float)d[3];
+        m.ior = (float)d[4];
+        m.emission[0] = (float)d[5]; m.emission[1] = (float)d[6]; m.emission[2] = (float)d[7];
+        m.kind = v->mat_kind[i];
+    }
+    // light distribution (rebuildLightDistribution, renderer.go:363-407; lightWeight :1343-1357)
+    std::vector<bc_light> lights;
+    std::vector<double> w;
+    for (int i = 0; i < v->nprim; i++) {
+        const int m = v->prim_mat[i];
+        if (v->mat_kind[m] != 3) continue;
+        const double* g = v->prim_geom + 10 * (size_t)i;
+        bc_light L;
+        std::memset(&L, 0, sizeof L);
+        L.prim = i;
+        double area;
+        if (v->prim_kind[i] == 0) {
+            L.kind = 0;
+            area = 4 * M_PI * g[3] * g[3];
+            L.g0[0] = (float)g[0]; L.g0[1] = (float)g[1]; L.g0[2] = (float)g[2]; L.g0[3] = (float)g[3];
+        } else {
+            if (!v->nee_triangles) continue;
+            L.kind = 1;
+            const double cx = g[4] * g[8] - g[5] * g[7], cy = g[5] * g[6] - g[3] * g[8], cz = g[3] * g[7] - g[4] * g[6];
+            area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            for (int k = 0; k < 3; k++) {
+                L.g0[k] = (float)g[k]; L.g1[k] = (float)g[3 + k]; L.g2[k] = (float)g[6 + k];
+            }
+        }
+        L.area = (float)area;
+        const double* md = v->mat_data + 8 * (size_t)m;
+        double wt = (0.2126 * md[5] + 0.7152 * md[6] + 0.0722 * md[7]) * area;
+        w.push_back(wt < 1e-9 ? 1e-9 : wt);
+        lights.push_back(L);
+    }
+    std::vector<float> pdf((size_t)v->nprim, 0.0f);
+    std::vector<int32_t> light_of((size_t)v->nprim, -1);
+    double total = 0;
+    std::vector<double> cdf(lights.size());
+    for (size_t i = 0; i < lights.size(); i++) { total += w[i]; cdf[i] = total; }
+    double prev = 0;
+    for (size_t i = 0; i < lights.size(); i++) {
+        double cv = cdf[i] / total;
+        pdf[(size_t)lights[i].prim] = (float)(cv - prev);
+        prev = cv;
+        cdf[i] = cv;
+    }
+    if (!lights.empty()) cdf.back() = 1;
+    for (size_t i = 0; i < lights.size(); i++) {
+        lights[i].cdf = (float)cdf[i];
+        light_of[(size_t)lights[i].prim] = (int32_t)i;
+    }
+    if (!upload_vec(c, c->d_nodes, c->cap_nodes, b.nodes, "upload nodes") ||
+        !upload_vec(c, c->d_prims, c->cap_prims, b.prims, "upload prims") ||
+        !upload_vec(c, c->d_mats, c->cap_mats, mats, "upload materials") ||
+        !upload_vec(c, c->d_lights, c->cap_lights, lights, "upload lights") ||
+        !upload_vec(c, c->d_pdf, c->cap_pdf, pdf, "upload light pdf") ||
+        !upload_vec(c, c->d_light_of, c->cap_light_of, light_of, "upload light map"))
+        return BC_ERR_DEVICE;
+    if (!ok(c, hipStreamSynchronize(c->stream), "scene upload")) return BC_ERR_DEVICE;
+    c->nnode8 = (int64_t)b.nodes.size();
+    c->depth8 = b.depth;
+    c->nprim = v->nprim;
+    c->nmat = v->nmat;
+    c->nlight = (int)lights.size();
+    c->stack = stack;
+    c->topo = topo;
+    c->geom = geom;
+    c->scene_bytes = (int64_t)(b.nodes.size() * sizeof(bc_node8) + b.prims.size() * sizeof(bc_prim));
+    c->has_scene = true;
+    return BC_OK;
+}
+
+int bc_scene_stats(const bc_ctx* c, int64_t* o) {
+    if (!c || !o) return BC_ERR_ARG;
+    o[0] = c->nnode8; o[1] = c->depth8; o[2] = c->nprim; o[3] = c->scene_bytes;
+    return BC_OK;
+}
+
+int bc_trace_device(bc_ctx* c, const void* d_rays, int64_t n, int any_hit, void* d_out) {
+    if (!c || n < 0 || (n > 0 && (!d_rays || !d_out))) return BC_ERR_ARG;
+    if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
+    hipError_t e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, (const bc_ray*)d_rays, n, d_out,
+                                     c->d_err, c->stream);
+    return ok(c, e, "trace launch") ? BC_OK : BC_ERR_DEVICE;
+}
+
+int bc_trace(bc_ctx* c, const float* rays, int64_t n, int any_hit, void* out) {
+    if (!c || n < 0 || (n > 0 && (!rays || !out))) return BC_ERR_ARG;
+    if (n == 0) return BC_OK;
+    hipSetDevice(c->device);
+    const size_t in_b = (size_t)n * sizeof(bc_ray), out_b = (size_t)n * (any_hit ? 4 : sizeof(bc_hit));
+    if (!bc_ensure(c, c->tmp_in, c->cap_in, in_b, "hipMalloc(rays)") ||
+        !bc_ensure(c, c->tmp_out, c->cap_out, out_b, "hipMalloc(hits)"))
+        return BC_ERR_NOMEM;
+    if (!ok(c, hipMemcpyAsync(c->tmp_in, rays, in_b, hipMemcpyHostToDevice, c->stream), "copy rays")) return BC_ERR_DEVICE;
+    int rc = bc_trace_device(c, c->tmp_in, n, any_hit, c->tmp_out);
+    if (rc != BC_OK) return rc;
+    if (!ok(c, hipMemcpyAsync(out, c->tmp_out, out_b, hipMemcpyDeviceToHost, c->stream), "copy hits") ||
+        !ok(c, hipStreamSynchronize(c->stream), "trace"))
+        return BC_ERR_DEVICE;
+    return bc_check_kernel_error(c);
+}
+
+void* bc_alloc(bc_ctx* c, uint64_t bytes) {
+    if (!c) return nullptr;
+    hipSetDevice(c->device);
+    void* p = nullptr;
+    if (!ok(c, hipMalloc(&p, std::max<uint64_t>(bytes, 16)), "bc_alloc")) return nullptr;
+    return p;
+}
+int bc_free(bc_ctx* c, void* p) {
+    if (!c) return BC_ERR_ARG;
+    return ok(c, hipFree(p), "bc_free") ? BC_OK : BC_ERR_DEVICE;
+}
+int bc_upload(bc_ctx* c, void* d, const void* s, uint64_t bytes) {
+    if (!c) return BC_ERR_ARG;
+    if (!ok(c, hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, c->stream), "bc_upload") ||
+        !ok(c, hipStreamSynchronize(c->stream), "bc_upload"))
+        return BC_ERR_DEVICE;
+    return BC_OK;
+}
+int bc_download(bc_ctx* c, void* d, const void* s, uint64_t bytes) {
+    if (!c) return BC_ERR_ARG;
+    if (!ok(c, hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToHost, c->stream), "bc_download") ||
+        !ok(c, hipStreamSynchronize(c->stream), "bc_download"))
+        return BC_ERR_DEVICE;
+    return bc_check_kernel_error(c);
+}
+int bc_synchronize(bc_ctx* c) {
+    if (!c) return BC_ERR_ARG;
+    if (!ok(c, hipStreamSynchronize(c->stream), "synchronize")) return BC_ERR_DEVICE;
+    return bc_check_kernel_error(c);
+}
+int bc_timer_start(bc_ctx* c) {
+    if (!c) return BC_ERR_ARG;
+    return ok(c, hipEventRecord(c->ev0, c->stream), "timer start") ? BC_OK : BC_ERR_DEVICE;
+}
+int bc_timer_stop(bc_ctx* c, float* ms) {
+    if (!c || !ms) return BC_ERR_ARG;
+    if (!ok(c, hipEventRecord(c->ev1, c->stream), "timer stop") || !ok(c, hipEventSynchronize(c->ev1), "timer sync") ||
+        !ok(c, hipEventElapsedTime(ms, c->ev0, c->ev1), "timer elapsed"))
+        return BC_ERR_DEVICE;
+    return BC_OK;
+}
+
+}  // extern "C"
+
+int bc_check_kernel_error(bc_ctx* c) {
+    int h = 0;
+    if (!ok(c, hipMemcpy(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost), "read kernel error flag")) return BC_ERR_DEVICE;
+    if (h) {
+        c->err = "kernel reported an error (traversal stack overflow or queue overflow)";
+        hipMemset(c->d_err, 0, sizeof(int));
+        return BC_ERR_DEVICE;
+    }
+    return BC_OK;
+}

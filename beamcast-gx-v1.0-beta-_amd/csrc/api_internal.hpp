@@ -1,0 +1,42 @@
+// api_internal.hpp -- the context behind bc_ctx (persistent runtime, scene residency).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/beamcast_hip.h"
+#include "bc_layout.h"
+
+struct RenderState;  // integrator buffers (integrator.hip)
+
+struct bc_ctx {
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err, name;
+    // resident scene
+    bool has_scene = false;
+    uint64_t topo = 0, geom = 0;
+    bc_node8* d_nodes = nullptr;
+    bc_prim* d_prims = nullptr;
+    bc_mat* d_mats = nullptr;
+    bc_light* d_lights = nullptr;
+    float* d_pdf = nullptr;          // light selection pdf by primitive
+    int32_t* d_light_of = nullptr;   // light index by primitive, -1 if none
+    size_t cap_nodes = 0, cap_prims = 0, cap_mats = 0, cap_lights = 0, cap_pdf = 0, cap_light_of = 0;
+    int64_t nnode8 = 0, scene_bytes = 0;
+    int depth8 = 0, nprim = 0, nmat = 0, nlight = 0, stack = 0;
+    int* d_err = nullptr;
+    // staging for host-pointer ray queries
+    void* tmp_in = nullptr;
+    void* tmp_out = nullptr;
+    size_t cap_in = 0, cap_out = 0;
+    // integrator
+    RenderState* rs = nullptr;
+};
+
+bool bc_ensure(bc_ctx* c, void*& p, size_t& cap, size_t bytes, const char* what);
+bool bc_check(bc_ctx* c, hipError_t e, const char* what);
+int bc_check_kernel_error(bc_ctx* c);
+void bc_render_release(bc_ctx* c);

@@ -1,0 +1,107 @@
+// bc_layout.h -- records resident in HBM, shared by the host packer and the HIP kernels.
+//
+// The reference GPU path uploads a threaded binary BVH of 32-byte nodes and 40-byte
+// triangles (gpu_opencl_windows.go:54-77, packed at :337-377).  This backend replaces
+// that with an 8-wide BVH whose child boxes are quantised to 8 bits per plane relative
+// to a per-node power-of-two grid, and with primitive records stored in leaf order so a
+// leaf's primitives are one contiguous run.
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define BC_HD __host__ __device__
+#else
+#define BC_HD
+#endif
+
+// One 8-wide node: 80 bytes = five 16-byte loads.
+//   child box (slot i, axis a) = [o_a + qlo_a[i] * 2^(e_a-127), o_a + qhi_a[i] * 2^(e_a-127)]
+//   imask bit i    : slot i is an internal child, node index = child_base + popcount(imask & ((1<<i)-1))
+//   meta[i] (leaf) : bits 0-4 offset of the first primitive (relative to prim_base), bits 5-7 count-1
+//   meta[i] = 0xFF : empty slot
+// Slot i holds the child lying most towards (+-x,+-y,+-z) = (i&1, i&2, i&4): a ray whose
+// direction sign bits are oct visits slots in the order k ^ oct, k = 0..7 (near to far).
+struct alignas(16) bc_node8 {
+    float ox, oy, oz;
+    uint8_t ex, ey, ez, imask;
+    uint32_t child_base;
+    uint32_t prim_base;
+    uint8_t meta[8];
+    uint8_t qlox[8], qloy[8], qloz[8];
+    uint8_t qhix[8], qhiy[8], qhiz[8];
+};
+static_assert(sizeof(bc_node8) == 80, "bc_node8 must be 80 bytes");
+
+#define BC_META_EMPTY 0xFFu
+#define BC_SPHERE_BIT 0x80000000u
+
+// Primitive record (48 bytes, three float4), in BVH8 leaf order.
+//   triangle: q0 = (A.x, A.y, A.z, E1.x)  q1 = (E1.y, E1.z, E2.x, E2.y)  q2 = (E2.z, id, mat, 0)
+//   sphere  : q0 = (C.x, C.y, C.z, r)      q1 = (r*r, 1/r, 0, 0)          q2 = (0, id|SPHERE_BIT, mat, 0)
+// "id" is the Scene.Primitives index (renderer.go:233), stored as raw bits.
+struct alignas(16) bc_prim {
+    float q0[4], q1[4], q2[4];
+};
+static_assert(sizeof(bc_prim) == 48, "bc_prim must be 48 bytes");
+
+// Material record: kinds as renderer.go:86-91.
+struct alignas(16) bc_mat {
+    float color[3], rough;
+    float emission[3], ior;
+    int32_t kind, pad0, pad1, pad2;
+};
+static_assert(sizeof(bc_mat) == 48, "bc_mat must be 48 bytes");
+
+// NEE light record (emissive spheres, and emissive triangles when enabled).
+struct alignas(16) bc_light {
+    float g0[4];   // sphere: C.xyz, r        triangle: A.xyz, 0
+    float g1[4];   // triangle: E1.xyz, 0
+    float g2[4];   // triangle: E2.xyz, 0
+    float area;    // surface area rounded from fp64
+    float cdf;     // normalised CDF value
+    int32_t prim;  // Scene.Primitives index
+    int32_t kind;  // 0 sphere, 1 triangle
+};
+static_assert(sizeof(bc_light) == 64, "bc_light must be 64 bytes");
+
+// Ray record used by the trace entry points: 32 bytes.
+struct alignas(16) bc_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+};
+// Closest-hit record: 8 bytes (t, primitive id; id = -1 on miss).
+struct bc_hit {
+    float t;
+    int32_t prim;
+};
+
+// Random-number dimension map of the integrator (counter-based; identical in the oracle).
+#define BC_DIM_CAM_X 0u
+#define BC_DIM_CAM_Y 1u
+#define BC_DIM_BOUNCE0 16u
+#define BC_DIMS_PER_BOUNCE 64u
+#define BC_D_LIGHT 0u
+#define BC_D_GLASS 1u
+#define BC_D_RR 2u
+#define BC_D_TRI 3u
+#define BC_D_SPH 5u
+#define BC_D_COS 21u
+#define BC_D_METAL 37u
+#define BC_NTRY_SPH 8
+#define BC_NTRY_COS 8
+#define BC_NTRY_METAL 9
+
+BC_HD static inline uint32_t bc_mix32(uint32_t x) {
+    // "lowbias32" integer hash (C. Wellons, hash-prospector; public domain)
+    x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+    return x;
+}
+BC_HD static inline uint32_t bc_path_key(uint32_t seed, uint32_t pixel, uint32_t sample) {
+    return bc_mix32(seed ^ bc_mix32(pixel ^ bc_mix32(sample + 0x9e3779b9u)));
+}
+BC_HD static inline uint32_t bc_rng_u32(uint32_t key, uint32_t dim) {
+    return bc_mix32(key ^ bc_mix32(dim * 0x85ebca6bu + 0x632be5abu));
+}
+BC_HD static inline float bc_rng(uint32_t key, uint32_t dim) {
+    return (float)(bc_rng_u32(key, dim) >> 8) * (1.0f / 16777216.0f);
+}

@@ -1,0 +1,20 @@
+// bvh8_build.hpp -- host flattening of the Go binary SAH BVH into bc_node8 / bc_prim arrays.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "../../include/beamcast_hip.h"
+#include "bc_layout.h"
+
+namespace bcx {
+
+struct Bvh8 {
+    std::vector<bc_node8> nodes;
+    std::vector<bc_prim> prims;  // leaf order
+    int depth = 0;               // levels of bc_node8 (root = 1)
+};
+
+// Returns false with a message when the view is not encodable.
+bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err);
+
+}  // namespace bcx

@@ -1,0 +1,20 @@
+// kernels.hpp -- host-callable launchers of the HIP kernels (all asynchronous on `st`).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "bc_layout.h"
+
+namespace bcd {
+
+constexpr int kTraceBlock = 256;
+
+// Smallest compiled LDS stack depth that holds a BVH8 of the given depth, or -1.
+int stack_for_depth(int depth);
+
+// Ray queries: any_hit writes int32 occlusion flags, else bc_hit records.  *err is set
+// non-zero by the kernel if a traversal stack would overflow.
+hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const bc_ray* rays,
+                        int64_t n, void* out, int* err, hipStream_t st);
+
+}  // namespace bcd

@@ -1,0 +1,383 @@
+// workloads.cpp -- seeded synthetic scenes and ray sets, plus the C entry points of the
+// host scene mirror (beamcast_host.h).  Every random value is a pure function of
+// (seed, item index, draw index), so any sub-range of a workload can be regenerated.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "../../include/beamcast_host.h"
+#include "host_scene.hpp"
+
+struct bch_scene {
+    bcx::Scene s;
+};
+
+namespace {
+
+using bcx::V;
+using bcx::Vec3;
+
+// splitmix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom number generators", 2014)
+inline uint64_t sm64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+// uniform double in [0,1) for (seed, item, draw)
+inline double u01(uint64_t seed, uint64_t item, uint64_t draw) {
+    uint64_t h = sm64(seed ^ sm64(item * 0x9e3779b97f4a7c15ull + sm64(draw + 0x632be59bd9b4e019ull)));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0);
+}
+inline Vec3 sphere_dir(double u, double v) {
+    double z = 1 - 2 * u, phi = 2 * M_PI * v, r = std::sqrt(std::max(0.0, 1 - z * z));
+    return V(r * std::cos(phi), r * std::sin(phi), z);
+}
+
+void add_quad(bcx::Scene& s, Vec3 a, Vec3 b, Vec3 c, Vec3 d, int mat) {
+    s.AddTriangle(a, b, c, mat);
+    s.AddTriangle(a, c, d, mat);
+}
+
+// axis-aligned box of size (sx,sy,sz) with base centre `base`, rotated by deg about +y
+void add_box(bcx::Scene& s, Vec3 base, double sx, double sy, double sz, double deg, int mat) {
+    double r = deg * M_PI / 180, c = std::cos(r), sn = std::sin(r);
+    auto P = [&](double x, double y, double z) {
+        double rx = c * x + sn * z, rz = -sn * x + c * z;
+        return V(base.x + rx, base.y + y, base.z + rz);
+    };
+    double hx = sx / 2, hz = sz / 2;
+    Vec3 v[8] = {P(-hx, 0, -hz), P(hx, 0, -hz), P(hx, sy, -hz), P(-hx, sy, -hz),
+                 P(-hx, 0, hz),  P(hx, 0, hz),  P(hx, sy, hz),  P(-hx, sy, hz)};
+    const int f[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {3, 7, 6, 2}, {0, 4, 7, 3}, {1, 2, 6, 5}};
+    for (auto& q : f) add_quad(s, v[q[0]], v[q[1]], v[q[2]], v[q[3]], mat);
+}
+
+int mat_diffuse(bcx::Scene& s, Vec3 c) {
+    bcx::Material m;
+    m.Kind = bcx::MatDiffuse;
+    m.Color = c;
+    return s.AddMaterial(m);
+}
+
+void set_cam(double* out, Vec3 p, Vec3 t, Vec3 u, double fov) {
+    if (!out) return;
+    double v[10] = {p.x, p.y, p.z, t.x, t.y, t.z, u.x, u.y, u.z, fov};
+    std::memcpy(out, v, sizeof v);
+}
+
+// closed box: walls at x=+-1, y=0, y=2, z=-2 (open towards the camera at z=0)
+void closed_box(bcx::Scene& s, int subdiv, int nmat_extra, uint64_t seed) {
+    int grey = mat_diffuse(s, V(.75, .75, .75));
+    int red = mat_diffuse(s, V(.63, .065, .05));
+    int green = mat_diffuse(s, V(.14, .45, .09));
+    bcx::Material lm;
+    lm.Kind = bcx::MatEmissive;
+    lm.Emission = V(17, 12, 4);
+    int light = s.AddMaterial(lm);
+    std::vector<int> extra;
+    for (int i = 0; i < nmat_extra; i++)
+        extra.push_back(mat_diffuse(s, V(.2 + .6 * u01(seed, (uint64_t)i, 0), .2 + .6 * u01(seed, (uint64_t)i, 1),
+                                         .2 + .6 * u01(seed, (uint64_t)i, 2))));
+    int quad_id = 0;
+    auto wall = [&](Vec3 a, Vec3 b, Vec3 c, Vec3 d, int mat) {
+        // subdivide the quad a-b-c-d into subdiv x subdiv cells
+        for (int i = 0; i < subdiv; i++)
+            for (int j = 0; j < subdiv; j++) {
+                double u0 = (double)i / subdiv, u1 = (double)(i + 1) / subdiv;
+                double v0 = (double)j / subdiv, v1 = (double)(j + 1) / subdiv;
+                auto L = [&](double u, double v) {
+                    Vec3 ab = a + (b - a) * u, dc = d + (c - d) * u;
+                    return ab + (dc - ab) * v;
+                };
+                int m = mat;
+                if (!extra.empty() && mat != light) m = extra[(size_t)(sm64(seed ^ (uint64_t)quad_id) % extra.size())];
+                quad_id++;
+                add_quad(s, L(u0, v0), L(u1, v0), L(u1, v1), L(u0, v1), m);
+            }
+    };
+    wall(V(-1, 0, 0), V(1, 0, 0), V(1, 0, -2), V(-1, 0, -2), grey);      // floor
+    wall(V(-1, 2, -2), V(1, 2, -2), V(1, 2, 0), V(-1, 2, 0), grey);      // ceiling
+    wall(V(-1, 0, -2), V(1, 0, -2), V(1, 2, -2), V(-1, 2, -2), grey);    // back
+    wall(V(-1, 0, 0), V(-1, 0, -2), V(-1, 2, -2), V(-1, 2, 0), red);     // left
+    wall(V(1, 0, -2), V(1, 0, 0), V(1, 2, 0), V(1, 2, -2), green);       // right
+    add_box(s, V(-.35, 0, -1.3), .6, 1.2, .6, 17, grey);
+    add_box(s, V(.35, 0, -.7), .6, .6, .6, -17, grey);
+    add_quad(s, V(-.25, 1.98, -1.25), V(.25, 1.98, -1.25), V(.25, 1.98, -.75), V(-.25, 1.98, -.75), light);
+}
+
+// value noise on an integer lattice, smooth-step interpolated
+double vnoise(uint64_t seed, double x, double y) {
+    double fx = std::floor(x), fy = std::floor(y);
+    int64_t ix = (int64_t)fx, iy = (int64_t)fy;
+    double tx = x - fx, ty = y - fy;
+    auto L = [&](int64_t a, int64_t b) { return u01(seed, (uint64_t)(a * 73856093) ^ (uint64_t)(b * 19349663), 7); };
+    double sx = tx * tx * (3 - 2 * tx), sy = ty * ty * (3 - 2 * ty);
+    double a = L(ix, iy), b = L(ix + 1, iy), c = L(ix, iy + 1), d = L(ix + 1, iy + 1);
+    return (a + (b - a) * sx) + ((c + (d - c) * sx) - (a + (b - a) * sx)) * sy;
+}
+
+}  // namespace
+
+extern "C" {
+
+bch_scene* bch_scene_new(void) { return new bch_scene(); }
+void bch_scene_free(bch_scene* s) { delete s; }
+int bch_add_material(bch_scene* s, int kind, const double* d) {
+    bcx::Material m;
+    m.Kind = kind;
+    m.Color = V(d[0], d[1], d[2]);
+    m.Roughness = d[3];
+    m.IOR = d[4];
+    m.Emission = V(d[5], d[6], d[7]);
+    return s->s.AddMaterial(m);
+}
+void bch_add_sphere(bch_scene* s, const double* c, double r, int mat) { s->s.AddSphere(V(c[0], c[1], c[2]), r, mat); }
+void bch_add_triangle(bch_scene* s, const double* a, const double* b, const double* c, int mat) {
+    s->s.AddTriangle(V(a[0], a[1], a[2]), V(b[0], b[1], b[2]), V(c[0], c[1], c[2]), mat);
+}
+void bch_build(bch_scene* s, int bins, int leaf) {
+    s->s.BVHBins = bins;
+    s->s.BVHLeafSize = leaf;
+    s->s.Build();
+}
+void bch_refit(bch_scene* s) { s->s.Refit(); }
+void bch_counts(const bch_scene* s, int32_t* o) {
+    o[0] = (int32_t)s->s.Primitives.size();
+    o[1] = (int32_t)s->s.Nodes.size();
+    o[2] = (int32_t)s->s.Materials.size();
+}
+void bch_revisions(const bch_scene* s, uint64_t* o) {
+    o[0] = s->s.GPUTopologyRevision;
+    o[1] = s->s.GPUGeometryRevision;
+}
+const bc_scene_view* bch_view(bch_scene* s, int nee) { return &s->s.View(nee != 0); }
+void bch_export_prims(const bch_scene* s, int32_t* kind, int32_t* mat, double* g) {
+    for (size_t i = 0; i < s->s.Primitives.size(); i++) {
+        const bcx::Primitive& p = s->s.Primitives[i];
+        kind[i] = p.Kind;
+        mat[i] = p.Material;
+        double* o = g + 9 * i;
+        if (p.Kind == bcx::PrimSphere) {
+            double v[9] = {p.Center.x, p.Center.y, p.Center.z, p.Radius, 0, 0, 0, 0, 0};
+            std::memcpy(o, v, sizeof v);
+        } else {
+            double v[9] = {p.A.x, p.A.y, p.A.z, p.B.x, p.B.y, p.B.z, p.C.x, p.C.y, p.C.z};
+            std::memcpy(o, v, sizeof v);
+        }
+    }
+}
+void bch_export_materials(const bch_scene* s, int32_t* kind, double* d) {
+    for (size_t i = 0; i < s->s.Materials.size(); i++) {
+        const bcx::Material& m = s->s.Materials[i];
+        kind[i] = m.Kind;
+        double v[8] = {m.Color.x, m.Color.y, m.Color.z, m.Roughness, m.IOR, m.Emission.x, m.Emission.y, m.Emission.z};
+        std::memcpy(d + 8 * i, v, sizeof v);
+    }
+}
+void bch_translate(bch_scene* s, int start, int count, const double* d) {  // translatePrimitive, renderer.go:282-303
+    Vec3 dv = V(d[0], d[1], d[2]);
+    int end = std::min((int)s->s.Primitives.size(), start + count);
+    for (int i = std::max(0, start); i < end; i++) {
+        bcx::Primitive& p = s->s.Primitives[(size_t)i];
+        if (p.Kind == bcx::PrimSphere) {
+            p.Center = p.Center + dv;
+            p.Centroid = p.Center;
+            Vec3 rv = V(p.Radius, p.Radius, p.Radius);
+            p.Box = bcx::AABB{p.Center - rv, p.Center + rv};
+            continue;
+        }
+        p.A = p.A + dv; p.B = p.B + dv; p.C = p.C + dv;
+        p.E1 = p.B - p.A; p.E2 = p.C - p.A;
+        p.Normal = bcx::Unit(bcx::Cross(p.E1, p.E2));
+        p.Centroid = (p.A + p.B + p.C) * (1.0 / 3.0);
+        Vec3 mn = V(std::min(p.A.x, std::min(p.B.x, p.C.x)), std::min(p.A.y, std::min(p.B.y, p.C.y)),
+                    std::min(p.A.z, std::min(p.B.z, p.C.z)));
+        Vec3 mx = V(std::max(p.A.x, std::max(p.B.x, p.C.x)), std::max(p.A.y, std::max(p.B.y, p.C.y)),
+                    std::max(p.A.z, std::max(p.B.z, p.C.z)));
+        p.Box = bcx::AABB{mn - V(1e-6, 1e-6, 1e-6), mx + V(1e-6, 1e-6, 1e-6)};
+    }
+}
+
+void bch_camera_basis(const double* pos, const double* target, const double* up, double fov, double aspect, double* o) {
+    bcx::Camera c;
+    c.Position = V(pos[0], pos[1], pos[2]);
+    c.Target = V(target[0], target[1], target[2]);
+    c.Up = V(up[0], up[1], up[2]);
+    c.FOV = fov;
+    Vec3 a, b, cc, d;
+    c.basis(aspect, a, b, cc, d);
+    double v[12] = {a.x, a.y, a.z, b.x, b.y, b.z, cc.x, cc.y, cc.z, d.x, d.y, d.z};
+    std::memcpy(o, v, sizeof v);
+}
+
+bch_scene* bch_workload_scene(int kind, uint64_t seed, int64_t param, int bins, int leaf, double* cam) {
+    bch_scene* w = new bch_scene();
+    bcx::Scene& s = w->s;
+    switch (kind) {
+    case 1:
+        s.Name = "closed box";
+        closed_box(s, 1, 0, seed);
+        set_cam(cam, V(0, 1, 3.4), V(0, 1, -1), V(0, 1, 0), 40);
+        break;
+    case 2: {
+        s.Name = "triangle soup";
+        int m = mat_diffuse(s, V(.75, .75, .75));
+        for (int64_t i = 0; i < param; i++) {
+            Vec3 c = V(u01(seed, (uint64_t)i, 0), u01(seed, (uint64_t)i, 1), u01(seed, (uint64_t)i, 2));
+            Vec3 v[3];
+            for (int k = 0; k < 3; k++) {
+                double L = .002 + .018 * u01(seed, (uint64_t)i, 3 + 3 * (uint64_t)k);
+                v[k] = c + sphere_dir(u01(seed, (uint64_t)i, 4 + 3 * (uint64_t)k), u01(seed, (uint64_t)i, 5 + 3 * (uint64_t)k)) * L;
+            }
+            s.AddTriangle(v[0], v[1], v[2], m);
+        }
+        set_cam(cam, V(.5, .5, -1.5), V(.5, .5, .5), V(0, 1, 0), 45);
+        break;
+    }
+    case 3: {  // NewShowcaseScene, renderer.go:2944-2959
+        s.Name = "Material Showcase";
+        int ground = mat_diffuse(s, V(.55, .57, .6));
+        int red = mat_diffuse(s, V(.72, .12, .10));
+        bcx::Material g; g.Kind = bcx::MatMetal; g.Color = V(.93, .72, .28); g.Roughness = .08;
+        int gold = s.AddMaterial(g);
+        bcx::Material gl; gl.Kind = bcx::MatGlass; gl.Color = V(.98, .99, 1); gl.IOR = 1.5;
+        int glass = s.AddMaterial(gl);
+        bcx::Material lm; lm.Kind = bcx::MatEmissive; lm.Emission = V(18, 15, 11);
+        int light = s.AddMaterial(lm);
+        s.AddSphere(V(0, -1001, -4), 1000, ground);
+        s.AddSphere(V(-1.8, 0, -4.2), 1, red);
+        s.AddSphere(V(.25, 0, -3.2), 1, glass);
+        s.AddSphere(V(2.2, 0, -4.4), 1, gold);
+        s.AddSphere(V(0, 5, -4), .8, light);
+        set_cam(cam, V(6, 3.2, 5.5), V(0, 0, -4), V(0, 1, 0), 40);
+        break;
+    }
+    case 4: {  // NewCornellScene, renderer.go:2960-2979
+        s.Name = "Cornell-like Room";
+        int white = mat_diffuse(s, V(.72, .72, .72));
+        int red = mat_diffuse(s, V(.70, .08, .06));
+        int green = mat_diffuse(s, V(.08, .55, .12));
+        bcx::Material gl; gl.Kind = bcx::MatGlass; gl.Color = V(.98, .99, 1); gl.IOR = 1.5;
+        int glass = s.AddMaterial(gl);
+        bcx::Material lm; lm.Kind = bcx::MatEmissive; lm.Emission = V(22, 20, 16);
+        int light = s.AddMaterial(lm);
+        s.AddTriangle(V(-3, -2, -8), V(3, -2, -8), V(3, -2, -2), white);
+        s.AddTriangle(V(-3, -2, -8), V(3, -2, -2), V(-3, -2, -2), white);
+        s.AddTriangle(V(-3, -2, -8), V(-3, -2, -2), V(-3, 4, -2), red);
+        s.AddTriangle(V(-3, -2, -8), V(-3, 4, -2), V(-3, 4, -8), red);
+        s.AddTriangle(V(3, -2, -2), V(3, -2, -8), V(3, 4, -8), green);
+        s.AddTriangle(V(3, -2, -2), V(3, 4, -8), V(3, 4, -2), green);
+        s.AddSphere(V(-.9, -1, -5.2), 1, glass);
+        s.AddSphere(V(1.1, -1.2, -4.2), .8, white);
+        s.AddSphere(V(0, 3.2, -5), .65, light);
+        set_cam(cam, V(0, .5, 5), V(0, 0, -5), V(0, 1, 0), 38);
+        break;
+    }
+    case 5: {  // NewStressScene, renderer.go:2980-2995
+        s.Name = "Stress Grid";
+        int ground = mat_diffuse(s, V(.45, .48, .52));
+        bcx::Material b; b.Kind = bcx::MatMetal; b.Color = V(.25, .52, .92); b.Roughness = .12;
+        int blue = s.AddMaterial(b);
+        bcx::Material lm; lm.Kind = bcx::MatEmissive; lm.Emission = V(25, 22, 18);
+        int light = s.AddMaterial(lm);
+        s.AddSphere(V(0, -1001, -10), 1000, ground);
+        for (int z = 0; z < 20; z++)
+            for (int x = -20; x <= 20; x++) s.AddSphere(V((double)x * .58, -.65, -3 - (double)z * .58), .22, blue);
+        s.AddSphere(V(0, 8, -8), 1.2, light);
+        set_cam(cam, V(9, 5, 8), V(0, -.5, -8), V(0, 1, 0), 42);
+        break;
+    }
+    case 6: {  // heightfield terrain: param x param cells over [0,1]^2, 8-octave fBm
+        s.Name = "terrain";
+        int n = (int)std::max<int64_t>(1, param);
+        std::vector<int> mats;
+        for (int i = 0; i < 64; i++) {
+            double a = .2 + .6 * u01(seed, (uint64_t)i, 100);
+            mats.push_back(mat_diffuse(s, V(a, a, a)));
+        }
+        std::vector<double> hgt((size_t)(n + 1) * (n + 1));
+        for (int j = 0; j <= n; j++)
+            for (int i = 0; i <= n; i++) {
+                double x = (double)i / n, y = (double)j / n, amp = .5, fr = 4, hsum = 0;
+                for (int o = 0; o < 8; o++) {
+                    hsum += amp * vnoise(seed + (uint64_t)o, x * fr, y * fr);
+                    amp *= .5;
+                    fr *= 2;
+                }
+                hgt[(size_t)j * (n + 1) + i] = .25 * hsum;
+            }
+        s.Primitives.reserve((size_t)n * n * 2);
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                auto P = [&](int a, int b) { return V((double)a / n, hgt[(size_t)b * (n + 1) + a], (double)b / n); };
+                int m = mats[(size_t)(sm64(seed ^ ((uint64_t)j * (uint64_t)n + (uint64_t)i)) % 64)];
+                s.AddTriangle(P(i, j), P(i, j + 1), P(i + 1, j + 1), m);
+                s.AddTriangle(P(i, j), P(i + 1, j + 1), P(i + 1, j), m);
+            }
+        set_cam(cam, V(.5, .6, -.35), V(.5, .1, .5), V(0, 1, 0), 50);
+        break;
+    }
+    case 7: {  // closed box retessellated to about `param` triangles with 64 materials
+        s.Name = "closed box (retessellated)";
+        int sub = (int)std::max<double>(1.0, std::floor(std::sqrt((double)param / 10.0)));
+        closed_box(s, sub, 64, seed);
+        set_cam(cam, V(0, 1, 3.4), V(0, 1, -1), V(0, 1, 0), 40);
+        break;
+    }
+    default:
+        delete w;
+        return nullptr;
+    }
+    s.BVHBins = bins;
+    s.BVHLeafSize = leaf;
+    s.Build();
+    return w;
+}
+
+int bch_workload_rays(int set, uint64_t seed, int64_t width, int any_hit, int64_t first, int64_t count, float* out) {
+    if (width <= 0 || first < 0 || count < 0) return -1;
+    Vec3 origin, lower, horiz, vert;
+    if (set == 0) {
+        bcx::Camera c;
+        c.Position = V(.5, .5, -1.5);
+        c.Target = V(.5, .5, .5);
+        c.Up = V(0, 1, 0);
+        c.FOV = 45;
+        c.basis(1.0, origin, lower, horiz, vert);
+    }
+    const uint64_t rseed = seed ^ (set == 0 ? 0x0c0e0e0aull : 0x1c0e0e0bull);
+    for (int64_t k = 0; k < count; k++) {
+        const uint64_t i = (uint64_t)(first + k);
+        Vec3 o, d;
+        double tmax = 1e30;
+        if (set == 0) {
+            int64_t x = (int64_t)(i % (uint64_t)width), y = (int64_t)(i / (uint64_t)width);
+            double u = ((double)x + .5) / (double)width, v = ((double)(width - 1 - y) + .5) / (double)width;
+            o = origin;
+            d = bcx::Unit(lower + horiz * u + vert * v - origin);
+            if (any_hit) {  // tmax = entry distance into the unit cube + U(.05, 1)
+                double tn = 0, tf = 1e30;
+                const double oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+                for (int a = 0; a < 3; a++) {
+                    if (std::fabs(dd[a]) < 1e-15) continue;
+                    double t0 = (0 - oo[a]) / dd[a], t1 = (1 - oo[a]) / dd[a];
+                    if (t0 > t1) std::swap(t0, t1);
+                    tn = std::max(tn, t0);
+                    tf = std::min(tf, t1);
+                }
+                tmax = (tn <= tf ? tn : 0) + .05 + .95 * u01(rseed, i, 9);
+            }
+        } else {
+            o = V(u01(rseed, i, 0), u01(rseed, i, 1), u01(rseed, i, 2));
+            d = sphere_dir(u01(rseed, i, 3), u01(rseed, i, 4));
+            if (any_hit) tmax = .05 + .95 * u01(rseed, i, 9);
+        }
+        float* r = out + 8 * k;
+        r[0] = (float)o.x; r[1] = (float)o.y; r[2] = (float)o.z; r[3] = 1e-4f;
+        r[4] = (float)d.x; r[5] = (float)d.y; r[6] = (float)d.z; r[7] = (float)tmax;
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+/*
+ * beamcast_hip.h -- C ABI of the MI355X path-tracing backend for the Beamcast Go renderer.
+ *
+ * This is the drop-in boundary that replaces the OpenCL GPU Dataflow path
+ * (windows_portable/gpu_opencl_windows.go, stub gpu_opencl_stub.go).  A cgo shim calls
+ * these entry points where the Go code today calls renderOpenCL / openCLGPUAvailable.
+ * Plain pointers and sizes only; every call returns 0 on success or a negative status,
+ * with a message retrievable from bc_last_error().  A context is not thread-safe: the
+ * caller serialises calls on it, as persistentCL.mu does today (gpu_opencl_windows.go:570-572).
+ */
+#ifndef BEAMCAST_HIP_H
+#define BEAMCAST_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bc_ctx bc_ctx;
+
+enum {
+    BC_OK = 0,
+    BC_ERR_ARG = -1,        /* invalid argument / size */
+    BC_ERR_DEVICE = -2,     /* HIP runtime failure */
+    BC_ERR_SCENE = -3,      /* scene missing or not encodable */
+    BC_ERR_CANCELLED = -4,  /* cancel flag observed between chunks */
+    BC_ERR_NOMEM = -5
+};
+
+/* Replaces openCLGPUAvailable() (gpu_opencl_windows.go:172; stub gpu_opencl_stub.go:10-12).
+ * Returns 1 and writes the device name when a gfx950 device is usable, else 0 and a reason. */
+int bc_device_available(char* name, int cap);
+
+/* Replaces the persistentCL singleton set-up (gpu_opencl_windows.go:94-131, initLocked :224-316).
+ * Returns NULL on failure; bc_last_error(NULL) then gives the reason. */
+bc_ctx* bc_create(int device_ordinal);
+void bc_destroy(bc_ctx* ctx);
+const char* bc_last_error(const bc_ctx* ctx);
+/* Device name, as renderOpenCL's deviceName result (gpu_opencl_windows.go:565-566). */
+const char* bc_device_name(const bc_ctx* ctx);
+
+/* Caller-owned view of a built Go Scene (renderer.go:225-241): the SAH BVH of
+ * Scene.Build (renderer.go:409-449) and the primitives/materials it indexes. */
+typedef struct bc_scene_view {
+    int32_t nnode;              /* len(Scene.Nodes) */
+    const double* node_box;     /* nnode x 6: Box.Min.xyz, Box.Max.xyz */
+    const int32_t* node_link;   /* nnode x 4: Left, Right, Start, Count */
+    int32_t nprim;              /* len(Scene.Primitives) == len(Scene.Order) */
+    const int32_t* order;       /* Scene.Order */
+    const int32_t* prim_kind;   /* 0 PrimSphere, 1 PrimTriangle */
+    const int32_t* prim_mat;    /* Primitive.Material */
+    const double* prim_geom;    /* nprim x 10: triangle A.xyz E1.xyz E2.xyz 0; sphere Center.xyz Radius 0.. */
+    int32_t nmat;               /* len(Scene.Materials) */
+    const int32_t* mat_kind;    /* 0 diffuse, 1 metal, 2 glass, 3 emissive */
+    const double* mat_data;     /* nmat x 8: Color.xyz, Roughness, IOR, Emission.xyz */
+    int32_t nee_triangles;      /* 1: emissive triangles are NEE lights too (Go samples spheres only) */
+} bc_scene_view;
+
+/* Replaces ensureSceneLocked / uploadFullSceneLocked / updateGeometryLocked
+ * (gpu_opencl_windows.go:379-495) including validateGPUSceneEncoding (:470-482).
+ * A change of topo_rev rebuilds the BVH8 from the binary BVH; a change of geom_rev alone
+ * re-quantises the existing BVH8 topology from the new boxes. */
+int bc_scene_upload(bc_ctx* ctx, const bc_scene_view* scene, uint64_t topo_rev, uint64_t geom_rev);
+/* Statistics of the uploaded scene: nodes8, depth8, prims, bytes resident. */
+int bc_scene_stats(const bc_ctx* ctx, int64_t* out4);
+
+/* Render description: the RenderSettings fields the path reads (renderer.go:1027-1062), the
+ * camera basis renderOpenCL passes as kernel args 11-14 (gpu_opencl_windows.go:580-612) in
+ * fp64, and this backend's own knobs. */
+typedef struct bc_render_desc {
+    int32_t width, height;      /* internalSize (renderer.go:2066-2079) */
+    int32_t spp;                /* effectiveSPP (renderer.go:2395-2417) */
+    int32_t bounces;            /* path segments per sample */
+    int32_t rr_depth;           /* RRDepth; < 1 means 4 (renderer.go:1550-1553) */
+    int32_t sampler;            /* 0 Random, 1 Halton (+ Cranley-Patterson rotation), 2 R2 */
+    int32_t mis;                /* power-heuristic MIS */
+    int32_t power_lights;       /* power-weighted light selection */
+    int32_t sky_mode;           /* 0 background() gradient of renderer.go:1326, 1 constant sky_color */
+    float firefly_clamp;        /* FireflyClamp; <= 0 off */
+    uint32_t seed;              /* base seed of the counter-based generator */
+    float sky_color[3];
+    double origin[3], lower[3], horiz[3], vert[3];   /* Camera.basis (renderer.go:947-960) */
+    int32_t mode;               /* 0 plain wavefront, 1 persistent + graph */
+    int32_t samples_per_pass;   /* 0 = automatic */
+    int32_t reserved[6];
+} bc_render_desc;
+
+/* Replaces renderOpenCL's chunk loop (gpu_opencl_windows.go:565-723): renders rows
+ * [y0, y0+rows) into caller-owned host buffers in the reference output layout:
+ *   rgba   rows*W*4 floats: mean radiance rgb, w = path segments (gpu_opencl_windows.go:820)
+ *   guides rows*W*3 u32 (may be NULL): depth bits, oct-normal|material|valid, jitter
+ *          (gpu_opencl_windows.go:787-797, decoded at :688-717)
+ * rays_out receives the path-segment count (Meta.Rays).  cancel (may be NULL) is polled
+ * between sample passes; a set flag returns BC_ERR_CANCELLED. */
+int bc_render(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* rgba,
+              uint32_t* guides, uint64_t* rays_out, const volatile int32_t* cancel);
+/* Same, with rgba/guides in device memory (no host copy). shadow_rays_out may be NULL. */
+int bc_render_device(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* d_rgba,
+                     uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_rays_out);
+
+/* Ray queries against the uploaded scene (the traversal of gpu_opencl_windows.go:758-777
+ * and of Scene.Hit / Scene.Occluded, renderer.go:777-895).  rays: n x 8 floats
+ * (o.xyz, tmin, d.xyz, tmax).  Closest hit writes n x (float t, int32 prim) with prim = -1
+ * on a miss; any hit writes n int32 (1 occluded, 0 clear). Host pointers. */
+int bc_trace(bc_ctx* ctx, const float* rays, int64_t n, int any_hit, void* out);
+/* Same with device pointers, enqueued on the context stream (asynchronous). */
+int bc_trace_device(bc_ctx* ctx, const void* d_rays, int64_t n, int any_hit, void* d_out);
+
+/* Device memory and stream helpers for callers that keep inputs resident in HBM. */
+void* bc_alloc(bc_ctx* ctx, uint64_t bytes);
+int bc_free(bc_ctx* ctx, void* d_ptr);
+int bc_upload(bc_ctx* ctx, void* d_dst, const void* src, uint64_t bytes);
+int bc_download(bc_ctx* ctx, void* dst, const void* d_src, uint64_t bytes);
+int bc_synchronize(bc_ctx* ctx);
+/* Stream-ordered timing: record a start marker, then read the elapsed ms since it. */
+int bc_timer_start(bc_ctx* ctx);
+int bc_timer_stop(bc_ctx* ctx, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,69 @@
+/*
+ * beamcast_host.h -- C entry points of the host-side scene mirror and workload generators.
+ *
+ * The Go renderer owns Scene / Camera / RenderSettings in the shipped product and hands
+ * the C ABI of beamcast_hip.h a bc_scene_view.  This header exposes the C++ mirror of
+ * those Go types (same names and semantics) so that callers without Go -- the tests,
+ * the benchmark -- can build scenes exactly as Scene.Build would, and generate the
+ * seeded synthetic workloads the benchmark and tests run on.
+ */
+#ifndef BEAMCAST_HOST_H
+#define BEAMCAST_HOST_H
+#include <stdint.h>
+
+#include "beamcast_hip.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct bch_scene bch_scene;
+
+/* Scene (renderer.go:225-241) */
+bch_scene* bch_scene_new(void);
+void bch_scene_free(bch_scene* s);
+/* Scene.AddMaterial (renderer.go:243): data8 = Color.xyz, Roughness, IOR, Emission.xyz */
+int bch_add_material(bch_scene* s, int kind, const double* data8);
+/* Scene.AddSphere / AddTriangle (renderer.go:246-270): degenerate input is dropped */
+void bch_add_sphere(bch_scene* s, const double* center, double radius, int mat);
+void bch_add_triangle(bch_scene* s, const double* a, const double* b, const double* c, int mat);
+/* Scene.Build with BVHMode SAH (renderer.go:409-449); bins/leaf clamped as there */
+void bch_build(bch_scene* s, int bins, int leaf);
+/* Scene.Refit (renderer.go:451-473) */
+void bch_refit(bch_scene* s);
+/* counts: nprim, nnode, nmat; revisions: topology, geometry */
+void bch_counts(const bch_scene* s, int32_t* out3);
+void bch_revisions(const bch_scene* s, uint64_t* out2);
+/* view for bc_scene_upload; valid until the scene is changed or freed */
+const bc_scene_view* bch_view(bch_scene* s, int nee_triangles);
+/* primitives as the Go fields A,B,C (triangles) or Center,Radius (spheres): nprim x 9 */
+void bch_export_prims(const bch_scene* s, int32_t* kind, int32_t* mat, double* geom9);
+void bch_export_materials(const bch_scene* s, int32_t* kind, double* data8);
+/* translate primitives [start, start+count) by d (translatePrimitive, renderer.go:282-303) */
+void bch_translate(bch_scene* s, int start, int count, const double* d);
+
+/* Camera.basis (renderer.go:947-960): out12 = origin, lower, horiz, vert */
+void bch_camera_basis(const double* pos, const double* target, const double* up, double fov,
+                      double aspect, double* out12);
+
+/* Seeded workloads.  camera_out (may be NULL) receives pos.xyz, target.xyz, up.xyz, fov.
+ *   1  closed box: 5 walls, 2 rotated boxes, ceiling quad light (36 triangles)
+ *   2  triangle soup: `param` triangles, centroids U[0,1]^3, edges U(.002,.02)
+ *   3  Material Showcase (renderer.go:2944-2959)
+ *   4  Cornell-like Room (renderer.go:2960-2979)
+ *   5  Stress Grid (renderer.go:2980-2995)
+ *   6  heightfield terrain: param x param grid cells, fBm heights, 64 materials
+ *   7  closed box retessellated to ~param triangles with 64 materials
+ * The scene is built (SAH, bins, leaf) before it is returned. */
+bch_scene* bch_workload_scene(int kind, uint64_t seed, int64_t param, int bins, int leaf, double* camera_out);
+/* Rays for the traversal workload (n x 8 floats: o.xyz, tmin, d.xyz, tmax):
+ *   set 0 coherent pinhole primaries through pixel centres of a width x width image
+ *   set 1 incoherent: origins U[0,1]^3, directions uniform on the sphere
+ * any_hit selects the any-hit tmax distribution.  first/count select a sub-range of the
+ * full set, so a sample can be regenerated without the rest. */
+int bch_workload_rays(int set, uint64_t seed, int64_t width, int any_hit, int64_t first,
+                      int64_t count, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
