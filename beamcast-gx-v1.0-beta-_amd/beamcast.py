@@ -1,0 +1,332 @@
+"""ctypes bindings of libbeamcast_hip.so (include/beamcast_hip.h, include/beamcast_host.h).
+
+Host-side helper for callers without Go: it exposes the C ABI that the Go renderer's cgo
+shim binds (the renderOpenCL / openCLGPUAvailable replacement of
+windows_portable/gpu_opencl_windows.go:172,565) and the C++ mirror of the Go Scene.
+There is no fallback: if the library is missing or no gfx950 device is present, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libbeamcast_hip.so")
+
+BC_OK = 0
+
+# material kinds (renderer.go:86-91) and primitive kinds (renderer.go:101-104)
+DIFFUSE, METAL, GLASS, EMISSIVE = 0, 1, 2, 3
+SPHERE, TRIANGLE = 0, 1
+
+# workload kinds of bch_workload_scene
+CLOSED_BOX, SOUP, SHOWCASE, CORNELL, STRESS, TERRAIN, BOX_RETESS = 1, 2, 3, 4, 5, 6, 7
+
+
+class SceneView(ctypes.Structure):
+    _fields_ = [
+        ("nnode", c_int32), ("node_box", POINTER(c_double)), ("node_link", POINTER(c_int32)),
+        ("nprim", c_int32), ("order", POINTER(c_int32)), ("prim_kind", POINTER(c_int32)),
+        ("prim_mat", POINTER(c_int32)), ("prim_geom", POINTER(c_double)),
+        ("nmat", c_int32), ("mat_kind", POINTER(c_int32)), ("mat_data", POINTER(c_double)),
+        ("nee_triangles", c_int32),
+    ]
+
+
+class RenderDesc(ctypes.Structure):
+    _fields_ = [
+        ("width", c_int32), ("height", c_int32), ("spp", c_int32), ("bounces", c_int32),
+        ("rr_depth", c_int32), ("sampler", c_int32), ("mis", c_int32), ("power_lights", c_int32),
+        ("sky_mode", c_int32), ("firefly_clamp", c_float), ("seed", c_uint32),
+        ("sky_color", c_float * 3),
+        ("origin", c_double * 3), ("lower", c_double * 3), ("horiz", c_double * 3), ("vert", c_double * 3),
+        ("mode", c_int32), ("samples_per_pass", c_int32), ("reserved", c_int32 * 6),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the in-tree library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64 = c_void_p, c_int64
+        sig = {
+            "bc_device_available": (c_int, [c_char_p, c_int]),
+            "bc_create": (vp, [c_int]),
+            "bc_destroy": (None, [vp]),
+            "bc_last_error": (c_char_p, [vp]),
+            "bc_device_name": (c_char_p, [vp]),
+            "bc_scene_upload": (c_int, [vp, vp, c_uint64, c_uint64]),
+            "bc_scene_stats": (c_int, [vp, POINTER(c_int64)]),
+            "bc_render": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), vp]),
+            "bc_render_device": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
+            "bc_trace": (c_int, [vp, vp, i64, c_int, vp]),
+            "bc_trace_device": (c_int, [vp, vp, i64, c_int, vp]),
+            "bc_alloc": (vp, [vp, c_uint64]),
+            "bc_free": (c_int, [vp, vp]),
+            "bc_upload": (c_int, [vp, vp, vp, c_uint64]),
+            "bc_download": (c_int, [vp, vp, vp, c_uint64]),
+            "bc_synchronize": (c_int, [vp]),
+            "bc_timer_start": (c_int, [vp]),
+            "bc_timer_stop": (c_int, [vp, POINTER(c_float)]),
+            "bch_scene_new": (vp, []),
+            "bch_scene_free": (None, [vp]),
+            "bch_add_material": (c_int, [vp, c_int, POINTER(c_double)]),
+            "bch_add_sphere": (None, [vp, POINTER(c_double), c_double, c_int]),
+            "bch_add_triangle": (None, [vp, POINTER(c_double), POINTER(c_double), POINTER(c_double), c_int]),
+            "bch_build": (None, [vp, c_int, c_int]),
+            "bch_refit": (None, [vp]),
+            "bch_counts": (None, [vp, POINTER(c_int32)]),
+            "bch_revisions": (None, [vp, POINTER(c_uint64)]),
+            "bch_view": (POINTER(SceneView), [vp, c_int]),
+            "bch_export_prims": (None, [vp, vp, vp, vp]),
+            "bch_export_materials": (None, [vp, vp, vp]),
+            "bch_translate": (None, [vp, c_int, c_int, POINTER(c_double)]),
+            "bch_camera_basis": (None, [POINTER(c_double)] * 3 + [c_double, c_double, POINTER(c_double)]),
+            "bch_workload_scene": (vp, [c_int, c_uint64, c_int64, c_int, c_int, POINTER(c_double)]),
+            "bch_workload_rays": (c_int, [c_int, c_uint64, c_int64, c_int, c_int64, c_int64, vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return c_void_p(a.ctypes.data) if a is not None else None
+
+
+def device_available():
+    buf = ctypes.create_string_buffer(256)
+    ok = lib().bc_device_available(buf, 256)
+    return bool(ok), buf.value.decode()
+
+
+def camera_basis(pos, target, up, fov, aspect):
+    out = np.zeros(12)
+    arr = [np.asarray(v, dtype=np.float64) for v in (pos, target, up)]
+    lib().bch_camera_basis(*[a.ctypes.data_as(POINTER(c_double)) for a in arr], float(fov), float(aspect),
+                           out.ctypes.data_as(POINTER(c_double)))
+    return out.reshape(4, 3)
+
+
+class HostScene:
+    """The C++ mirror of the Go Scene (renderer.go:225-641)."""
+
+    def __init__(self, handle=None, camera=None):
+        self.h = handle if handle is not None else lib().bch_scene_new()
+        if not self.h:
+            raise ValueError("invalid scene")
+        self.camera = camera
+
+    @classmethod
+    def workload(cls, kind, seed=0x5EED, param=0, bins=32, leaf=4):
+        cam = np.zeros(10)
+        h = lib().bch_workload_scene(kind, seed, param, bins, leaf, cam.ctypes.data_as(POINTER(c_double)))
+        if not h:
+            raise ValueError(f"unknown workload {kind}")
+        return cls(h, cam)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().bch_scene_free(self.h)
+            self.h = None
+
+    def add_material(self, kind, color=(0, 0, 0), rough=0.0, ior=0.0, emission=(0, 0, 0)):
+        d = np.array([*color, rough, ior, *emission], dtype=np.float64)
+        return lib().bch_add_material(self.h, kind, d.ctypes.data_as(POINTER(c_double)))
+
+    def add_sphere(self, c, r, mat):
+        c = np.asarray(c, dtype=np.float64)
+        lib().bch_add_sphere(self.h, c.ctypes.data_as(POINTER(c_double)), float(r), mat)
+
+    def add_triangle(self, a, b, c, mat):
+        a, b, c = (np.asarray(v, dtype=np.float64) for v in (a, b, c))
+        lib().bch_add_triangle(self.h, *(v.ctypes.data_as(POINTER(c_double)) for v in (a, b, c)), mat)
+
+    def build(self, bins=32, leaf=4):
+        lib().bch_build(self.h, bins, leaf)
+
+    def refit(self):
+        lib().bch_refit(self.h)
+
+    def translate(self, start, count, d):
+        d = np.asarray(d, dtype=np.float64)
+        lib().bch_translate(self.h, start, count, d.ctypes.data_as(POINTER(c_double)))
+
+    def counts(self):
+        o = (c_int32 * 3)()
+        lib().bch_counts(self.h, o)
+        return int(o[0]), int(o[1]), int(o[2])
+
+    def revisions(self):
+        o = (c_uint64 * 2)()
+        lib().bch_revisions(self.h, o)
+        return int(o[0]), int(o[1])
+
+    def view(self, nee_triangles=False):
+        return lib().bch_view(self.h, 1 if nee_triangles else 0)
+
+    def nodes(self):
+        """(node_box [n,6], node_link [n,4], order [nprim]) copied out of the view."""
+        v = self.view().contents
+        nn, npr = v.nnode, v.nprim
+        box = np.ctypeslib.as_array(v.node_box, shape=(nn * 6,)).reshape(nn, 6).copy()
+        link = np.ctypeslib.as_array(v.node_link, shape=(nn * 4,)).reshape(nn, 4).copy()
+        order = np.ctypeslib.as_array(v.order, shape=(npr,)).copy()
+        return box, link, order
+
+    def export(self):
+        """(prim_kind, prim_mat, geom9 [A,B,C or C,r], mat_kind, mat_data8) as numpy arrays."""
+        npr, _, nm = self.counts()
+        kind = np.zeros(npr, np.int32)
+        mat = np.zeros(npr, np.int32)
+        geom = np.zeros((npr, 9))
+        lib().bch_export_prims(self.h, _ptr(kind), _ptr(mat), _ptr(geom))
+        mk = np.zeros(nm, np.int32)
+        md = np.zeros((nm, 8))
+        lib().bch_export_materials(self.h, _ptr(mk), _ptr(md))
+        return kind, mat, geom, mk, md
+
+
+def workload_rays(set_id, width, any_hit=False, first=0, count=None, seed=0x5EED):
+    """Ray set of the traversal workload: n x 8 float32 (o.xyz, tmin, d.xyz, tmax)."""
+    if count is None:
+        count = width * width - first
+    out = np.empty((count, 8), np.float32)
+    rc = lib().bch_workload_rays(set_id, seed, width, 1 if any_hit else 0, first, count, _ptr(out))
+    if rc != 0:
+        raise ValueError("bad ray-set arguments")
+    return out
+
+
+class BeamcastError(RuntimeError):
+    pass
+
+
+class Device:
+    """A bc_ctx: persistent runtime on one gfx950 device."""
+
+    def __init__(self, ordinal=0):
+        L = lib()
+        self.ctx = L.bc_create(ordinal)
+        if not self.ctx:
+            raise BeamcastError(L.bc_last_error(None).decode())
+
+    def close(self):
+        if self.ctx:
+            lib().bc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != BC_OK:
+            raise BeamcastError(f"status {rc}: {lib().bc_last_error(self.ctx).decode()}")
+
+    @property
+    def name(self):
+        return lib().bc_device_name(self.ctx).decode()
+
+    def upload(self, scene: HostScene, nee_triangles=False, revisions=None):
+        topo, geom = revisions if revisions is not None else scene.revisions()
+        self._check(lib().bc_scene_upload(self.ctx, scene.view(nee_triangles), topo, geom))
+
+    def upload_view(self, view, topo, geom):
+        self._check(lib().bc_scene_upload(self.ctx, view, topo, geom))
+
+    def stats(self):
+        o = (c_int64 * 4)()
+        self._check(lib().bc_scene_stats(self.ctx, o))
+        return {"nodes8": o[0], "depth8": o[1], "prims": o[2], "bytes": o[3]}
+
+    def trace(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = rays.shape[0]
+        if any_hit:
+            out = np.empty(n, np.int32)
+        else:
+            out = np.empty(n, dtype=[("t", np.float32), ("prim", np.int32)])
+        self._check(lib().bc_trace(self.ctx, _ptr(rays), n, 1 if any_hit else 0, _ptr(out)))
+        return out
+
+    # --- device memory / timing helpers
+    def alloc(self, nbytes):
+        p = lib().bc_alloc(self.ctx, nbytes)
+        if not p:
+            raise BeamcastError(lib().bc_last_error(self.ctx).decode())
+        return p
+
+    def free(self, p):
+        self._check(lib().bc_free(self.ctx, p))
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(arr.nbytes)
+        self._check(lib().bc_upload(self.ctx, p, _ptr(arr), arr.nbytes))
+        return p
+
+    def download(self, p, out):
+        self._check(lib().bc_download(self.ctx, _ptr(out), p, out.nbytes))
+        return out
+
+    def trace_device(self, d_rays, n, any_hit, d_out):
+        self._check(lib().bc_trace_device(self.ctx, d_rays, n, 1 if any_hit else 0, d_out))
+
+    def sync(self):
+        self._check(lib().bc_synchronize(self.ctx))
+
+    def timer_start(self):
+        self._check(lib().bc_timer_start(self.ctx))
+
+    def timer_stop(self):
+        ms = c_float()
+        self._check(lib().bc_timer_stop(self.ctx, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def render(self, desc: RenderDesc, y0=0, rows=None, guides=False, cancel=None):
+        rows = desc.height - y0 if rows is None else rows
+        rgba = np.zeros((rows, desc.width, 4), np.float32)
+        g = np.zeros((rows, desc.width, 3), np.uint32) if guides else None
+        rays = c_uint64()
+        cflag = None
+        if cancel is not None:
+            cflag = np.array([1 if cancel else 0], np.int32)
+        self._check(lib().bc_render(self.ctx, ctypes.byref(desc), y0, rows, _ptr(rgba), _ptr(g) if g is not None else None,
+                                    ctypes.byref(rays), _ptr(cflag) if cflag is not None else None))
+        return rgba, g, int(rays.value)
+
+    def render_device(self, desc: RenderDesc, y0, rows, d_rgba, d_guides=None):
+        rays, shadow = c_uint64(), c_uint64()
+        self._check(lib().bc_render_device(self.ctx, ctypes.byref(desc), y0, rows, d_rgba, d_guides,
+                                           ctypes.byref(rays), ctypes.byref(shadow)))
+        return int(rays.value), int(shadow.value)
+
+
+def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampler=1, mis=True, power=True,
+                firefly_clamp=0.0, seed=0x9E3779B9, sky_mode=0, sky_color=(0.3, 0.3, 0.3), mode=0,
+                samples_per_pass=0):
+    """RenderDesc for a workload scene (camera from the workload, Camera.basis in fp64)."""
+    cam = scene.camera
+    basis = camera_basis(cam[0:3], cam[3:6], cam[6:9], cam[9], width / height)
+    d = RenderDesc()
+    d.width, d.height, d.spp, d.bounces, d.rr_depth = width, height, spp, bounces, rr_depth
+    d.sampler, d.mis, d.power_lights, d.sky_mode = sampler, int(mis), int(power), sky_mode
+    d.firefly_clamp, d.seed = firefly_clamp, seed
+    for k in range(3):
+        d.sky_color[k] = sky_color[k]
+        d.origin[k], d.lower[k], d.horiz[k], d.vert[k] = basis[0][k], basis[1][k], basis[2][k], basis[3][k]
+    d.mode, d.samples_per_pass = mode, samples_per_pass
+    return d

@@ -1,0 +1,124 @@
+"""GPU parity of the BVH8 traversal against the oracle (through the C ABI).
+
+Bar: closest-hit ids and distances bit-exact against the oracle's fp32 restatement of
+triHit / sphereHitAccel (same operation order, equal-t resolved to the smaller primitive
+index); against the fp64 Go-semantics Scene.Hit at least 99.99% identical ids, mismatches
+only where |dt|/t < 1e-5.  Any hit: identical occlusion flags.
+"""
+import numpy as np
+import pytest
+
+import beamcast as bc
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = bc.Device(0)
+    yield d
+    d.close()
+
+
+def soup_rays(n_per_set, width=1024, any_hit=False):
+    # strided picks over the full ray sets so the sample spans the whole image / cube
+    coh = bc.workload_rays(0, width, any_hit=any_hit, count=width * width)[:: max(1, width * width // n_per_set)]
+    inc = bc.workload_rays(1, width, any_hit=any_hit, count=n_per_set)
+    return np.concatenate([coh[:n_per_set], inc])
+
+
+def check_closest(dev, host, rays, fp64_check=True):
+    dev.upload(host)
+    got = dev.trace(rays)
+    o = orc.Scene.from_host(host)
+    prim, t = o.hit32(rays)
+    bad = np.nonzero((got["prim"] != prim) | (got["t"] != t))[0]
+    assert bad.size == 0, f"{bad.size} rays differ from the fp32 oracle, first {bad[:5]}"
+    if fp64_check:
+        p64, t64 = o.hit64(rays.astype(np.float64))
+        mism = got["prim"] != p64
+        assert mism.mean() <= 1e-4
+        m = mism & (p64 >= 0) & (got["prim"] >= 0)
+        if m.any():
+            rel = np.abs(got["t"][m] - t64[m]) / t64[m]
+            assert rel.max() < 1e-5
+    return got
+
+
+def test_soup_closest_and_any(dev):
+    print("case: soup 200k, 2x32768 rays", flush=True)
+    s = bc.HostScene.workload(bc.SOUP, param=200_000)
+    rays = soup_rays(32768)
+    got = check_closest(dev, s, rays)
+    assert (got["prim"] >= 0).mean() > 0.3
+    o = orc.Scene.from_host(s)
+    ra = soup_rays(32768, any_hit=True)
+    occ = dev.trace(ra, any_hit=True)
+    assert np.array_equal(occ, o.any32(ra))
+    assert 0.02 < occ.mean() < 0.98
+
+
+@pytest.mark.parametrize("kind", [bc.STRESS, bc.SHOWCASE, bc.CORNELL, bc.CLOSED_BOX])
+def test_builtin_scenes(dev, kind):
+    print(f"case: scene kind {kind}", flush=True)
+    s = bc.HostScene.workload(kind, bins=16, leaf=4)
+    pos = s.camera[:3]
+    rng = np.random.default_rng(kind)
+    d = rng.normal(size=(20000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = np.tile(pos, (20000, 1)) + rng.normal(size=(20000, 3)) * 0.5
+    rays = np.column_stack([o, np.full(20000, 1e-4), d, np.full(20000, 1e30)]).astype(np.float32)
+    check_closest(dev, s, rays)
+    rays[:, 7] = rng.random(20000) * 10
+    occ = dev.trace(rays, any_hit=True)
+    assert np.array_equal(occ, orc.Scene.from_host(s, bins=16).any32(rays))
+
+
+def test_edge_rays(dev):
+    """Axis-parallel directions (zero components), origins on box faces, empty ranges."""
+    print("case: edge rays", flush=True)
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s)
+    rows = []
+    for o in ([0, 1, 1], [0.5, 0.5, -1.0], [-1, 1, -1], [0.35, 0.0, -0.7], [0, 1.98, -1]):
+        for d in ([1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [0.6, 0.8, 0]):
+            for tmax in (1e30, 0.5, 1e-4):
+                rows.append([*o, 1e-4, *d, tmax])
+    rows.append([0, 1, 1, 2.0, 0, 0, -1, 1.0])  # tmin > tmax: no hit
+    rays = np.array(rows, np.float32)
+    check_closest(dev, s, rays, fp64_check=False)
+    occ = dev.trace(rays, any_hit=True)
+    assert np.array_equal(occ, orc.Scene.from_host(s).any32(rays))
+    assert dev.trace(np.zeros((0, 8), np.float32)).shape == (0,)
+
+
+def test_full_size_any_matches_closest(dev):
+    """1M-triangle soup, 2 x 1M rays: any-hit(tmax) == (closest hit exists with t <= tmax)."""
+    print("case: 1M soup property", flush=True)
+    s = bc.HostScene.workload(bc.SOUP, param=1_000_000)
+    dev.upload(s)
+    st = dev.stats()
+    assert st["prims"] == 1_000_000 and st["depth8"] >= 2
+    for set_id in (0, 1):
+        rays = bc.workload_rays(set_id, 1024, any_hit=True, count=1024 * 1024)
+        closest = rays.copy()
+        closest[:, 7] = 1e30
+        h = dev.trace(closest)
+        occ = dev.trace(rays, any_hit=True)
+        want = (h["prim"] >= 0) & (h["t"] <= rays[:, 7])
+        assert np.array_equal(occ.astype(bool), want)
+
+
+def test_scene_encoding_errors(dev):
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    vp = s.view()
+    v = vp.contents
+    link = np.ctypeslib.as_array(v.node_link, shape=(v.nnode * 4,))
+    saved = link.copy()
+    leaf = int(np.nonzero(link[3::4] > 0)[0][0])
+    link[4 * leaf + 3] = 17  # validateGPUSceneEncoding: leaf count > 16 (gpu_opencl_windows.go:470-482)
+    with pytest.raises(bc.BeamcastError, match="unsupported count"):
+        dev.upload_view(vp, 99, 99)
+    link[:] = saved
+    dev.upload_view(vp, 100, 100)

@@ -1,0 +1,94 @@
+"""CPU checks of the product library: it loads, exports every declared symbol, and its host
+scene mirror builds the same SAH BVH as the oracle's restatement of Scene.Build."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import beamcast as bc
+import oracle as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(bch?_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header", ["beamcast_hip.h", "beamcast_host.h"])
+def test_library_exports_every_declared_symbol(header):
+    lib = bc.lib()
+    names = declared(header)
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in {header} but not exported"
+
+
+def test_no_device_reports_cleanly():
+    ok, msg = bc.device_available()
+    if not ok:
+        assert msg
+        with pytest.raises(bc.BeamcastError):
+            bc.Device(0)
+
+
+@pytest.mark.parametrize("kind,param,bins", [(bc.CLOSED_BOX, 0, 32), (bc.SHOWCASE, 0, 16), (bc.CORNELL, 0, 16),
+                                             (bc.STRESS, 0, 16), (bc.SOUP, 20000, 32), (bc.BOX_RETESS, 4000, 32),
+                                             (bc.TERRAIN, 48, 32)])
+def test_host_build_matches_oracle_build(kind, param, bins):
+    h = bc.HostScene.workload(kind, param=param, bins=bins, leaf=4)
+    o = orc.Scene.from_host(h, bins=bins, leaf=4)
+    hb, hl, ho = h.nodes()
+    ob, ol, oo = o.nodes()
+    assert np.array_equal(hb, ob) and np.array_equal(hl, ol) and np.array_equal(ho, oo)
+
+
+def test_revision_tracking():
+    """TestGPURevisionTracking (renderer_test.go:692-718): Build bumps both, Refit geometry only."""
+    s = bc.HostScene.workload(bc.SHOWCASE, bins=16)
+    t0, g0 = s.revisions()
+    assert t0 > 0 and g0 > 0
+    s.translate(1, 1, (0.5, 0, 0))
+    s.refit()
+    t1, g1 = s.revisions()
+    assert t1 == t0 and g1 == g0 + 1
+    s.build()
+    t2, g2 = s.revisions()
+    assert t2 == t1 + 1 and g2 == g1 + 1
+
+
+def test_refit_matches_rebuild_boxes():
+    """TestBVHRefitMatchesRebuild (renderer_test.go:464-493): refit after a move gives boxes that
+    contain every primitive, and hits agree with a full rebuild."""
+    s = bc.HostScene.workload(bc.STRESS, bins=16)
+    s.translate(10, 50, (0.1, 0.2, -0.1))
+    s.refit()
+    kind, mat, geom, mk, md = s.export()
+    box, link, order = s.nodes()
+    for i, (l, r, st, c) in enumerate(link):
+        if c:
+            for p in order[st:st + c]:
+                cc, rr = geom[p, :3], geom[p, 3]
+                assert np.all(box[i, :3] <= cc - rr + 1e-12) and np.all(box[i, 3:] >= cc + rr - 1e-12)
+    rebuilt = orc.Scene(kind, mat, geom, mk, md, bins=16, leaf=4)
+    rays = np.array([[9, 5, 8, 1e-4, *(np.array([x, -0.4, -1]) / np.linalg.norm([x, -0.4, -1])), 1e30]
+                     for x in np.linspace(-0.8, 0.2, 200)])
+    p1, t1 = rebuilt.hit64(rays)
+    p2, t2 = rebuilt.brute64(rays)
+    assert np.array_equal(p1, p2)
+
+
+def test_workload_rays_subrange_and_shape():
+    full = bc.workload_rays(1, 64, count=1000)
+    part = bc.workload_rays(1, 64, first=300, count=100)
+    assert np.array_equal(full[300:400], part)
+    d = full[:, 4:7]
+    assert np.allclose(np.linalg.norm(d, axis=1), 1, atol=1e-6)
+    assert np.all(full[:, 3] == np.float32(1e-4)) and np.all(full[:, 7] == np.float32(1e30))
+    coh = bc.workload_rays(0, 64, any_hit=True, count=64 * 64)
+    assert np.median(coh[:, 7]) > 1.5  # rays entering the unit cube get tmax beyond its entry plane
+    inc = bc.workload_rays(1, 64, any_hit=True, count=4096)
+    assert inc[:, 7].min() >= 0.05 and inc[:, 7].max() <= 1.0
