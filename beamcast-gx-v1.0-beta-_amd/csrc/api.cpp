@@ -176,6 +176,7 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
         }
         L.area = (float)area;
         const double* md = v->mat_data + 8 * (size_t)m;
+        L.le[0] = (float)md[5]; L.le[1] = (float)md[6]; L.le[2] = (float)md[7];
         double wt = (0.2126 * md[5] + 0.7152 * md[6] + 0.0722 * md[7]) * area;
         w.push_back(wt < 1e-9 ? 1e-9 : wt);
         lights.push_back(L);

@@ -57,12 +57,13 @@ struct alignas(16) bc_light {
     float g0[4];   // sphere: C.xyz, r        triangle: A.xyz, 0
     float g1[4];   // triangle: E1.xyz, 0
     float g2[4];   // triangle: E2.xyz, 0
+    float le[4];   // emission of the light's material (rgb), 0
     float area;    // surface area rounded from fp64
     float cdf;     // normalised CDF value
     int32_t prim;  // Scene.Primitives index
     int32_t kind;  // 0 sphere, 1 triangle
 };
-static_assert(sizeof(bc_light) == 64, "bc_light must be 64 bytes");
+static_assert(sizeof(bc_light) == 80, "bc_light must be 80 bytes");
 
 // Ray record used by the trace entry points: 32 bytes.
 struct alignas(16) bc_ray {

@@ -51,6 +51,47 @@ def gen_rays(bc, set_id, width, any_hit, threads):
     return out
 
 
+def bench_render(bc, dev, args, threads):
+    """Closed box 256x256, 16 spp, 4 segments: path-samples/s of the wavefront integrator
+    (rgb + guides left in HBM) next to the oracle's fp64 integrator on the host cores."""
+    box = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(box, nee_triangles=True)
+    w = h = 256
+    spp, bounces = 16, 4
+    d = bc.render_desc(box, w, h, spp, bounces, rr_depth=3, sampler=1, mis=True, power=True)
+    d_rgba = dev.alloc(w * h * 16)
+    d_g = dev.alloc(w * h * 12)
+    for _ in range(2):
+        dev.render_device(d, 0, h, d_rgba, d_g)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        segs, shadow = dev.render_device(d, 0, h, d_rgba, d_g)
+    sec = (time.perf_counter() - t0) / reps
+    dev.free(d_rgba)
+    dev.free(d_g)
+    samples = w * h * spp
+    out = {"workload": "closed box (36 tris, quad light), 256x256, 16 spp Halton, 4 segments, MIS + power NEE "
+                       "over emissive triangles, RR from bounce 3, rgb + guides",
+           "ms_per_frame": round(sec * 1e3, 3), "path_samples_per_s": round(samples / sec, 1),
+           "mrays_s_segments": round(segs / sec / 1e6, 2), "mrays_s_total": round((segs + shadow) / sec / 1e6, 2),
+           "segments_per_sample": round(segs / samples, 4)}
+    if not args.no_cpu:
+        import oracle as orc
+
+        o = orc.Scene.from_host(box, nee_triangles=True)
+        od = orc.desc_from(d)
+        rows = 16
+        t0 = time.perf_counter()
+        o.render(od, precision=64, y0=120, rows=rows, threads=threads)
+        csec = time.perf_counter() - t0
+        cpu_rate = w * rows * spp / csec
+        out["cpu_baseline"] = {"path_samples_per_s": round(cpu_rate, 1), "cores": threads, "kind": "port",
+                               "sample": f"rows 120-{120 + rows} of the same frame, fp64 trace restatement"}
+        out["gpu_over_cpu"] = round(samples / sec / cpu_rate, 1)
+    return out
+
+
 def host_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -67,6 +108,7 @@ def main():
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-render", action="store_true", help="skip the closed-box render measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this backend runs on one GPU; the multi-GPU split (per-tile or per-sample "
@@ -175,6 +217,8 @@ def main():
                "gpu_over_cpu": round(value / cpu_rate, 1)}
         log(f"[bench] CPU baseline: {cpu_rate:.2f} Mrays/s on {threads} threads ({secs:.1f}s)")
 
+    render = None if args.no_render else bench_render(bc, dev, args, threads)
+
     names = {("coherent", False): "coherent_closest", ("coherent", True): "coherent_any",
              ("incoherent", False): "incoherent_closest", ("incoherent", True): "incoherent_any"}
     result = {
@@ -210,6 +254,7 @@ def main():
             "alg_bytes_per_ray": round(alg[dominant], 1),
         },
         "cpu_baseline": cpu,
+        "render": render,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
                                "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4)}
                     for k in launches},

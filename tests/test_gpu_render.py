@@ -1,0 +1,123 @@
+"""GPU parity of the wavefront integrator against the oracle (through the C ABI).
+
+Bar (per pixel, identical sample indices): radiance within 1e-4 relative of the oracle's fp32
+restatement of trace (renderer.go:1484-1563), |gpu - ref| <= 1e-4 * |ref| + 1e-6; path-segment
+counts and compact guides identical.  Against the fp64 Go semantics: same-seed images agree to
+a small relative RMSE, and converged means agree.
+"""
+import numpy as np
+import pytest
+
+import beamcast as bc
+import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = bc.Device(0)
+    yield d
+    d.close()
+
+
+def parity(dev, scene, desc, nee=False, guides=True, bins=32):
+    dev.upload(scene, nee_triangles=nee)
+    img, g, rays = dev.render(desc, guides=guides)
+    o = orc.Scene.from_host(scene, bins=bins, nee_triangles=nee)
+    ref, rg, rrays, _ = o.render(orc.desc_from(desc), precision=32, guides=guides)
+    assert rays == rrays, f"path segments {rays} vs {rrays}"
+    assert np.array_equal(img[..., 3], ref[..., 3])
+    err = np.abs(img[..., :3] - ref[..., :3])
+    bad = err > 1e-4 * np.abs(ref[..., :3]) + 1e-6
+    assert not bad.any(), f"{bad.sum()} channel values outside 1e-4 rel; max err {err.max()}"
+    if guides:
+        assert np.array_equal(g, rg)
+    return img, ref, o
+
+
+def test_closed_box_parity(dev):
+    print("case: closed box 64x64 8spp", flush=True)
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    d = bc.render_desc(s, 64, 64, 8, 4, rr_depth=3, sampler=1, mis=True, power=True)
+    img, _, _ = parity(dev, s, d, nee=True)
+    assert img[..., :3].mean() > 0.05
+
+
+@pytest.mark.parametrize("kind,b", [(bc.SHOWCASE, 6), (bc.CORNELL, 5), (bc.STRESS, 6)])
+def test_builtin_scene_parity(dev, kind, b):
+    print(f"case: scene {kind}", flush=True)
+    s = bc.HostScene.workload(kind, bins=16, leaf=4)
+    d = bc.render_desc(s, 48, 32, 6, b, rr_depth=4, sampler=1, mis=True, power=True, firefly_clamp=24)
+    parity(dev, s, d, bins=16)
+
+
+@pytest.mark.parametrize("sampler,mis,power", [(0, False, False), (2, True, False), (1, False, True)])
+def test_sampler_and_light_modes(dev, sampler, mis, power):
+    print(f"case: sampler {sampler} mis {mis} power {power}", flush=True)
+    s = bc.HostScene.workload(bc.CORNELL, bins=16, leaf=4)
+    d = bc.render_desc(s, 40, 24, 4, 5, rr_depth=2, sampler=sampler, mis=mis, power=power, seed=1234)
+    parity(dev, s, d, bins=16, guides=False)
+
+
+def test_constant_sky_and_multipass(dev):
+    """samples_per_pass forces several passes; accumulation order stays the sample order."""
+    print("case: multipass", flush=True)
+    s = bc.HostScene.workload(bc.BOX_RETESS, param=3000)
+    d = bc.render_desc(s, 32, 32, 9, 4, rr_depth=2, sky_mode=1, sky_color=(0.3, 0.3, 0.3), samples_per_pass=2)
+    parity(dev, s, d, nee=True)
+
+
+def test_row_chunks_equal_full_frame(dev):
+    """renderOpenCL's row chunking (gpu_opencl_windows.go:631-721): chunks reproduce the frame."""
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s, nee_triangles=True)
+    d = bc.render_desc(s, 40, 30, 4, 4, rr_depth=3)
+    full, gf, rays = dev.render(d, guides=True)
+    parts, gp, total = [], [], 0
+    for y0 in range(0, 30, 8):
+        img, g, r = dev.render(d, y0=y0, rows=min(8, 30 - y0), guides=True)
+        parts.append(img)
+        gp.append(g)
+        total += r
+    assert np.array_equal(np.concatenate(parts), full) and np.array_equal(np.concatenate(gp), gf)
+    assert total == rays
+
+
+def test_cancel_and_bad_arguments(dev):
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s)
+    d = bc.render_desc(s, 16, 16, 1, 2)
+    with pytest.raises(bc.BeamcastError, match="cancelled"):
+        dev.render(d, cancel=True)
+    with pytest.raises(bc.BeamcastError):
+        dev.render(d, y0=10, rows=10)
+    d.spp = 0
+    with pytest.raises(bc.BeamcastError):
+        dev.render(d)
+
+
+def test_published_path_length_on_gpu(dev):
+    """GIGA GPU probe workload (benchmark.go:218-235): Showcase 640x360, 4 spp, 6 bounces, MIS and
+    power sampling off, RRDepth 4 -> 1,767,805 segments over 921,600 samples (GIGA_SAMPLE_THIS_ENV.txt:60)."""
+    s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
+    dev.upload(s)
+    d = bc.render_desc(s, 640, 360, 4, 6, rr_depth=4, sampler=0, mis=False, power=False, seed=77)
+    _, _, rays = dev.render(d)
+    assert abs(rays / 921600 / (1767805 / 921600) - 1) < 0.003
+
+
+def test_same_seed_fp64_and_converged_mean(dev):
+    """Against the fp64 Go semantics: same seeds give nearly the same image; the mean of a
+    converged render matches an independent-seed fp64 render."""
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s, nee_triangles=True)
+    o = orc.Scene.from_host(s, nee_triangles=True)
+    d = bc.render_desc(s, 48, 48, 64, 4, rr_depth=3)
+    img, _, _ = dev.render(d)
+    ref64, _, _, _ = o.render(orc.desc_from(d), precision=64)
+    rel_rmse = np.sqrt(np.mean((img[..., :3] - ref64[..., :3]) ** 2)) / ref64[..., :3].mean()
+    assert rel_rmse < 0.02
+    d2 = bc.render_desc(s, 48, 48, 512, 4, rr_depth=3, seed=99)
+    img2, _, _ = dev.render(d2)
+    assert abs(img2[..., :3].mean() / ref64[..., :3].mean() - 1) < 0.01
