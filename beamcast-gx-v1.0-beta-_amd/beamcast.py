@@ -69,6 +69,7 @@ def lib():
             "bc_scene_stats": (c_int, [vp, POINTER(c_int64)]),
             "bc_render": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), vp]),
             "bc_render_device": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
+            "bc_render_tuning": (c_int, [vp, POINTER(c_int32)]),
             "bc_trace": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_trace_device": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_alloc": (vp, [vp, c_uint64]),
@@ -308,6 +309,11 @@ class Device:
                                     ctypes.byref(rays), _ptr(cflag) if cflag is not None else None))
         return rgba, g, int(rays.value)
 
+    def tuning(self):
+        o = (c_int32 * 4)()
+        self._check(lib().bc_render_tuning(self.ctx, o))
+        return {"pool": o[0], "iters_per_graph": o[1], "stack": o[2], "windows": o[3]}
+
     def render_device(self, desc: RenderDesc, y0, rows, d_rgba, d_guides=None):
         rays, shadow = c_uint64(), c_uint64()
         self._check(lib().bc_render_device(self.ctx, ctypes.byref(desc), y0, rows, d_rgba, d_guides,
@@ -317,7 +323,7 @@ class Device:
 
 def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampler=1, mis=True, power=True,
                 firefly_clamp=0.0, seed=0x9E3779B9, sky_mode=0, sky_color=(0.3, 0.3, 0.3), mode=0,
-                samples_per_pass=0):
+                samples_per_pass=0, pool=0, iters=0):
     """RenderDesc for a workload scene (camera from the workload, Camera.basis in fp64)."""
     cam = scene.camera
     basis = camera_basis(cam[0:3], cam[3:6], cam[6:9], cam[9], width / height)
@@ -329,4 +335,5 @@ def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampl
         d.sky_color[k] = sky_color[k]
         d.origin[k], d.lower[k], d.horiz[k], d.vert[k] = basis[0][k], basis[1][k], basis[2][k], basis[3][k]
     d.mode, d.samples_per_pass = mode, samples_per_pass
+    d.reserved[0], d.reserved[1] = pool, iters
     return d

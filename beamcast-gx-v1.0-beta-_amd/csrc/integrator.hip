@@ -23,135 +23,12 @@
 
 #include "api_internal.hpp"
 #include "kernels.hpp"
+#include "integrator_device.hpp"
 #include "trace_device.hpp"
 
 namespace bcd {
 
-constexpr int kBlock = 256;
-
-struct RenderParams {
-    int32_t width, height, spp, bounces, rr_depth, sampler, mis, power, sky_mode, want_guides;
-    float firefly, sky[3];
-    uint32_t seed;
-    float origin[3], lower[3], horiz[3], vert[3];
-    int32_t y0, npix;          // chunk: first row, pixels in chunk
-    int32_t s0, ns;            // pass: first sample, samples in this pass
-    int32_t nmat, nlight;
-};
-
-struct PathBufs {
-    float4* o;      // ray origin
-    float4* d;      // ray direction
-    float4* T;      // throughput xyz, prevPdf w
-    float4* L;      // radiance
-    float4* pp;     // prevPoint xyz, w = prevDiffuse flag
-    uint4* meta;    // pixel, sample, key, packed jitter
-    uint32_t* nseg; // path segments
-    float2* hit;    // t, record index bits (0xffffffff: miss)
-    float4* so;     // shadow origin xyz, tmax w
-    float4* sd;     // shadow dir
-    float4* sc;     // shadow contribution
-    uint32_t* q[2];
-    uint32_t* sq;
-    uint32_t* cnt;  // [0],[1] queue lengths, [2] shadow queue length
-    unsigned long long* totals;  // [0] path segments, [1] shadow rays
-};
-
-struct SceneBufs {
-    const bc_node8* nodes;
-    const bc_prim* prims;
-    const bc_mat* mats;
-    const bc_light* lights;
-    const float* pdf;
-    const int32_t* light_of;
-    int* err;
-};
-
-__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
-__device__ __forceinline__ float3 add(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ float3 sub(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ float3 mul(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ float3 had(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float3 cross(float3 a, float3 b) {
-    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ float3 unitv(float3 a) {  // Vec3.Unit, renderer.go:32-38
-    const float l = sqrtf(dot(a, a));
-    if (l < (float)1e-12) return f3(0.0f, 0.0f, 0.0f);
-    return mul(a, 1.0f / l);
-}
-__device__ __forceinline__ float rmax(float a, float b) { return a > b ? a : b; }
-__device__ __forceinline__ float rmin(float a, float b) { return a < b ? a : b; }
-__device__ __forceinline__ float rclamp(float x, float a, float b) { return x < a ? a : (x > b ? b : x); }
-__device__ __forceinline__ float3 xyz(float4 v) { return f3(v.x, v.y, v.z); }
-
-#define PI_F ((float)3.141592653589793)
-#define INVPI_F ((float)0.3183098861837907)
-
-// Appends the calling lanes whose pred is true to a queue with one atomic per wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred, uint32_t& slot) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return 0;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    return (uint32_t)__popcll(m);
-}
-
-__device__ __forceinline__ double radical_inverse(uint64_t n, uint32_t base) {  // renderer.go:1844-1858
-    if (base == 2) return (double)__brevll(n) * 5.421010862427522e-20;
-    const double invBase = 1.0 / (double)base;
-    double inv = invBase, result = 0.0;
-    while (n > 0) {
-        const uint64_t digit = n % base;
-        result += (double)digit * inv;
-        n /= base;
-        inv *= invBase;
-    }
-    return result;
-}
-
-__device__ __forceinline__ void pixel_rotation(int x, int y, double& rx, double& ry) {  // renderer.go:1906-1915
-    uint32_t h1 = (uint32_t)x * 0x9e3779b1u ^ (uint32_t)y * 0x85ebca77u;
-    h1 ^= h1 >> 16;
-    h1 *= 0x7feb352du;
-    h1 ^= h1 >> 15;
-    uint32_t h2 = h1 ^ 0x68bc21ebu;
-    h2 ^= h2 >> 13;
-    const double inv = 1.0 / 4294967296.0;
-    rx = (double)h1 * inv;
-    ry = (double)h2 * inv;
-}
-
-__device__ __forceinline__ uint32_t pack_guide(float3 n, int mat) {  // gpu_opencl_windows.go:787-792
-    const float inv = 1.0f / (fabsf(n.x) + fabsf(n.y) + fabsf(n.z) + 1e-20f);
-    const float nx = n.x * inv, ny = n.y * inv, nz = n.z * inv;
-    float px = nx, py = ny;
-    if (nz < 0.0f) {
-        const float ax = 1.0f - fabsf(py), ay = 1.0f - fabsf(px);
-        px = copysignf(ax, px);
-        py = copysignf(ay, py);
-    }
-    px = px * 0.5f + 0.5f;
-    py = py * 0.5f + 0.5f;
-    px = px < 0.0f ? 0.0f : (px > 1.0f ? 1.0f : px);
-    py = py < 0.0f ? 0.0f : (py > 1.0f ? 1.0f : py);
-    const uint32_t qx = (uint32_t)rintf(px * 1023.0f), qy = (uint32_t)rintf(py * 1023.0f);
-    const uint32_t m = (uint32_t)(mat < 0 ? 0 : (mat > 2047 ? 2047 : mat));
-    return 0x80000000u | (m << 20) | (qy << 10) | qx;
-}
-__device__ __forceinline__ uint32_t pack_jitter(float ju, float jv) {  // gpu_opencl_windows.go:793
-    ju = ju < 0.0f ? 0.0f : (ju > 1.0f ? 1.0f : ju);
-    jv = jv < 0.0f ? 0.0f : (jv > 1.0f ? 1.0f : jv);
-    return ((uint32_t)rintf(jv * 65535.0f) << 16) | (uint32_t)rintf(ju * 65535.0f);
-}
-
 // ------------------------------------------------------------------------------ raygen
-// Camera ray of adaptivePixel (renderer.go:1940-1950) with adaptive sampling off.
 __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, float* __restrict__ acc,
                                                    uint32_t* __restrict__ guides) {
     const int n = P.ns * P.npix;
@@ -161,50 +38,9 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, f
         B.cnt[1] = 0;
         B.cnt[2] = 0;
     }
-    const double g = 1.324717957244746, a1 = 1.0 / g, a2 = 1.0 / (g * g);
-    const float3 origin = f3(P.origin[0], P.origin[1], P.origin[2]);
-    const float3 lower = f3(P.lower[0], P.lower[1], P.lower[2]);
-    const float3 horiz = f3(P.horiz[0], P.horiz[1], P.horiz[2]);
-    const float3 vert = f3(P.vert[0], P.vert[1], P.vert[2]);
     for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
         const int sl = j / P.npix, pl = j - sl * P.npix;
-        const int sample = P.s0 + sl;
-        const int x = pl % P.width, y = P.y0 + pl / P.width;
-        const uint32_t pixel = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
-        const uint32_t key = bc_path_key(P.seed, pixel, (uint32_t)sample);
-        double jx, jy;
-        if (P.sampler == 1) {
-            double rx, ry;
-            pixel_rotation(x, y, rx, ry);
-            jx = radical_inverse((uint64_t)sample + 1, 2) + rx;
-            jx = jx - floor(jx);
-            jy = radical_inverse((uint64_t)sample + 1, 3) + ry;
-            jy = jy - floor(jy);
-        } else if (P.sampler == 2) {
-            double rx, ry;
-            pixel_rotation(x, y, rx, ry);
-            double sx = 0.5 + (double)(sample + 1) * a1, sy = 0.5 + (double)(sample + 1) * a2;
-            sx = sx - floor(sx);
-            sy = sy - floor(sy);
-            jx = sx + rx;
-            jx = jx - floor(jx);
-            jy = sy + ry;
-            jy = jy - floor(jy);
-        } else {
-            jx = (double)(bc_rng_u32(key, BC_DIM_CAM_X) >> 8) * (1.0 / 16777216.0);
-            jy = (double)(bc_rng_u32(key, BC_DIM_CAM_Y) >> 8) * (1.0 / 16777216.0);
-        }
-        const double uu = ((double)x + jx) / (double)(P.width - 1 > 1 ? P.width - 1 : 1);
-        const double vv = ((double)(P.height - 1 - y) + jy) / (double)(P.height - 1 > 1 ? P.height - 1 : 1);
-        const float u = (float)uu, v = (float)vv;
-        const float3 dir = sub(add(add(lower, mul(horiz, u)), mul(vert, v)), origin);
-        B.o[j] = make_float4(origin.x, origin.y, origin.z, 0.0f);
-        B.d[j] = make_float4(dir.x, dir.y, dir.z, 0.0f);
-        B.T[j] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
-        B.L[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        B.meta[j] = make_uint4(pixel, (uint32_t)sample, key, pack_jitter((float)jx, (float)jy));
-        B.nseg[j] = 0;
+        init_camera_path(P, B, (uint32_t)j, pl, P.s0 + sl);
         B.q[0][j] = (uint32_t)j;
         if (P.s0 == 0 && sl == 0) {
             float* a = acc + 4 * (size_t)pl;
@@ -224,6 +60,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int 
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         B.cnt[cur ^ 1] = 0;  // next queue, filled by shade of this bounce
         B.cnt[2] = 0;        // shadow queue of this bounce
+        B.cnt[3] = 0;        // terminated paths (persistent schedule)
     }
     const uint32_t n = B.cnt[cur];
     const uint32_t stride = gridDim.x * kBlock;
@@ -242,263 +79,20 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int 
 }
 
 // ------------------------------------------------------------------------------ shade
-struct LightPick {
-    bool ok;
-    float3 q, lnrm;
-    bool sphere;
-    float sel, area;
-    int mat;
-};
-
-__device__ __forceinline__ float light_dir_pdf(const SceneBufs& S, const RenderParams& P, int prim, float3 from,
-                                               float3 lp, float3 ln) {  // renderer.go:1404-1425
-    const int li = S.light_of[prim];
-    if (li < 0) return 0.0f;
-    const float3 to = sub(lp, from);
-    const float dist2 = dot(to, to);
-    if (dist2 <= (float)1e-12) return 0.0f;
-    const float3 ld = unitv(to);
-    const float cosL = rmax(0.0f, dot(ln, mul(ld, -1.0f)));
-    if (cosL <= (float)1e-12) return 0.0f;
-    const float area = S.lights[li].area;
-    const float sel = P.power ? S.pdf[prim] : 1.0f / (float)P.nlight;
-    return sel * dist2 / (cosL * area);
-}
-
 __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, RenderParams P, int cur, int bounce,
                                                   uint32_t* __restrict__ guides) {
     const uint32_t n = B.cnt[cur];
     const uint32_t stride = gridDim.x * kBlock;
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
-    const uint32_t dim = BC_DIM_BOUNCE0 + BC_DIMS_PER_BOUNCE * (uint32_t)bounce;
-    const int rr = P.rr_depth < 1 ? 4 : P.rr_depth;
     const uint32_t nloop = (n + stride - 1) / stride * stride;  // uniform trip count for wave ops
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
         bool cont = false, shadow = false;
         uint32_t j = 0;
-        float4 shadow_o = make_float4(0, 0, 0, 0), shadow_d = make_float4(0, 0, 0, 0), shadow_c = make_float4(0, 0, 0, 0);
+        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
         if (i < n) {
             j = q[i];
-            const float2 hv = B.hit[j];
-            const uint32_t rec = __float_as_uint(hv.y);
-            const float t = hv.x;
-            float3 ro = xyz(B.o[j]), rd = xyz(B.d[j]);
-            float4 Tv = B.T[j];
-            float3 T = xyz(Tv);
-            float3 L = xyz(B.L[j]);
-            const uint4 meta = B.meta[j];
-            const uint32_t key = meta.z;
-            if (rec == 0xffffffffu) {  // background, renderer.go:1492-1495 / :1326-1331
-                float3 bg;
-                if (P.sky_mode == 1) {
-                    bg = f3(P.sky[0], P.sky[1], P.sky[2]);
-                } else {
-                    const float3 u = unitv(rd);
-                    const float tt = (float)0.5 * (u.y + 1.0f);
-                    bg = add(mul(f3((float)0.06, (float)0.08, (float)0.12), 1.0f - tt),
-                             mul(f3((float)0.32, (float)0.48, (float)0.72), tt));
-                }
-                L = add(L, had(T, bg));
-                B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
-            } else {
-                const float4* pr = reinterpret_cast<const float4*>(S.prims + rec);
-                const float4 q0 = pr[0], q1 = pr[1], q2 = pr[2];
-                const uint32_t idbits = __float_as_uint(q2.y);
-                const int id = (int)(idbits & ~BC_SPHERE_BIT);
-                const int mat = (int)__float_as_uint(q2.z);
-                float3 nrm;
-                if (idbits & BC_SPHERE_BIT) {
-                    const float3 pos = add(ro, mul(rd, t));
-                    nrm = mul(sub(pos, f3(q0.x, q0.y, q0.z)), q1.y);
-                } else {
-                    nrm = unitv(cross(f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x)));
-                }
-                const bool front = dot(rd, nrm) < 0.0f;
-                const float3 N = front ? nrm : mul(nrm, -1.0f);
-                const float3 p = add(ro, mul(rd, t));
-                if (bounce == 0 && P.want_guides && meta.y == 0) {  // gpu_opencl_windows.go:811
-                    uint32_t* gg = guides + 3 * (size_t)(meta.x - (uint32_t)P.y0 * (uint32_t)P.width);
-                    gg[0] = __float_as_uint(t * sqrtf(dot(rd, rd)));
-                    gg[1] = pack_guide(N, mat);
-                    gg[2] = meta.w;
-                }
-                const bc_mat M = S.mats[mat];
-                const float3 color = f3(M.color[0], M.color[1], M.color[2]);
-                bool alive = true;
-                if (M.kind == 3) {  // emissive: MIS weight on BSDF-sampled hits, renderer.go:1504-1511
-                    const float4 pp = B.pp[j];
-                    float w = 1.0f;
-                    if (P.mis && pp.w != 0.0f && id >= 0) {
-                        const float lpdf = light_dir_pdf(S, P, id, xyz(pp), p, N);
-                        const float a2 = Tv.w * Tv.w, b2 = lpdf * lpdf;
-                        w = (a2 + b2 <= (float)1e-20) ? 0.0f : a2 / (a2 + b2);
-                    }
-                    L = add(L, mul(had(T, f3(M.emission[0], M.emission[1], M.emission[2])), w));
-                    B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
-                    alive = false;
-                } else if (M.kind == 0) {  // diffuse: NEE then cosine sampling
-                    if (P.nlight > 0) {   // directLight, renderer.go:1427-1465
-                        const float u = bc_rng(key, dim + BC_D_LIGHT);
-                        int li;
-                        float sel;
-                        if (!P.power) {  // chooseLight, renderer.go:1371-1393 (Intn -> floor(u*n))
-                            li = (int)(u * (float)P.nlight);
-                            if (li >= P.nlight) li = P.nlight - 1;
-                            sel = 1.0f / (float)P.nlight;
-                        } else {
-                            int lo = 0, hi = P.nlight;
-                            while (lo < hi) {
-                                const int mid = (lo + hi) >> 1;
-                                if (S.lights[mid].cdf >= u) hi = mid; else lo = mid + 1;
-                            }
-                            li = lo >= P.nlight ? P.nlight - 1 : lo;
-                            sel = S.pdf[S.lights[li].prim];
-                            if (sel <= 0.0f) sel = 1.0f / (float)P.nlight;
-                        }
-                        const bc_light& Lr = S.lights[li];
-                        float3 qpt, lnrm;
-                        const bool sph = Lr.kind == 0;
-                        if (sph) {  // randUnitVector, renderer.go:1273-1285
-                            float3 ln = f3(0.0f, 0.0f, 1.0f);
-                            for (int k = 0; k < BC_NTRY_SPH; k++) {
-                                const float a = bc_rng(key, dim + BC_D_SPH + 2u * (uint32_t)k) * 2.0f - 1.0f;
-                                const float b = bc_rng(key, dim + BC_D_SPH + 2u * (uint32_t)k + 1u) * 2.0f - 1.0f;
-                                const float ss = a * a + b * b;
-                                if (ss >= 1.0f || ss <= (float)1e-18) continue;
-                                const float f = 2.0f * sqrtf(1.0f - ss);
-                                ln = f3(a * f, b * f, 1.0f - 2.0f * ss);
-                                break;
-                            }
-                            qpt = add(f3(Lr.g0[0], Lr.g0[1], Lr.g0[2]), mul(ln, Lr.g0[3]));
-                            lnrm = ln;
-                        } else {  // uniform point on an emissive triangle
-                            const float u1 = bc_rng(key, dim + BC_D_TRI), u2 = bc_rng(key, dim + BC_D_TRI + 1u);
-                            const float su = sqrtf(u1);
-                            const float b1 = su * (1.0f - u2), b2 = su * u2;
-                            const float3 e1 = f3(Lr.g1[0], Lr.g1[1], Lr.g1[2]), e2 = f3(Lr.g2[0], Lr.g2[1], Lr.g2[2]);
-                            qpt = add(add(f3(Lr.g0[0], Lr.g0[1], Lr.g0[2]), mul(e1, b1)), mul(e2, b2));
-                            lnrm = unitv(cross(e1, e2));
-                        }
-                        const float3 to = sub(qpt, p);
-                        const float dist2 = dot(to, to);
-                        if (!(dist2 < (float)1e-8)) {
-                            const float3 ld = unitv(to);
-                            const float nd = rmax(0.0f, dot(N, ld));
-                            const float cl = dot(mul(lnrm, -1.0f), ld);
-                            const float cosL = sph ? rmax(0.0f, cl) : fabsf(cl);
-                            if (!(nd <= 0.0f || cosL <= 0.0f)) {
-                                const float lightPDF = sel * dist2 / (cosL * Lr.area);
-                                if (lightPDF > 0.0f) {
-                                    const float bsdfPDF = nd / PI_F;
-                                    float w = 1.0f;
-                                    if (P.mis) {
-                                        const float a2 = lightPDF * lightPDF, b2 = bsdfPDF * bsdfPDF;
-                                        w = (a2 + b2 <= (float)1e-20) ? 0.0f : a2 / (a2 + b2);
-                                    }
-                                    const float3 f = mul(color, INVPI_F);
-                                    const float3 dl = mul(had(f, f3(Lr.le[0], Lr.le[1], Lr.le[2])), nd * w / lightPDF);
-                                    const float3 c = had(T, dl);  // throughput before the albedo update
-                                    // visible(), renderer.go:1333-1341
-                                    const float3 from = add(p, mul(N, (float)2e-4));
-                                    const float3 dd = sub(qpt, from);
-                                    const float dd2 = dot(dd, dd);
-                                    if (dd2 < (float)1e-12) {
-                                        L = add(L, c);
-                                        B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
-                                    } else {
-                                        const float dist = sqrtf(dd2);
-                                        const float3 dir = mul(dd, 1.0f / dist);
-                                        shadow_o = make_float4(from.x, from.y, from.z, dist - (float)1e-4);
-                                        shadow_d = make_float4(dir.x, dir.y, dir.z, 0.0f);
-                                        shadow_c = make_float4(c.x, c.y, c.z, 0.0f);
-                                        shadow = true;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    T = had(T, color);
-                    float x = 0.0f, y = 0.0f, ss = 0.0f;  // randCosHemisphere, renderer.go:1290-1311
-                    bool ok = false;
-                    for (int k = 0; k < BC_NTRY_COS && !ok; k++) {
-                        x = bc_rng(key, dim + BC_D_COS + 2u * (uint32_t)k) * 2.0f - 1.0f;
-                        y = bc_rng(key, dim + BC_D_COS + 2u * (uint32_t)k + 1u) * 2.0f - 1.0f;
-                        ss = x * x + y * y;
-                        ok = ss < 1.0f;
-                    }
-                    if (!ok) { x = 0.0f; y = 0.0f; ss = 0.0f; }
-                    const float z = sqrtf(rmax(0.0f, 1.0f - ss));
-                    const float sign = N.z < 0.0f ? -1.0f : 1.0f;
-                    const float a = -1.0f / (sign + N.z);
-                    const float b = N.x * N.y * a;
-                    const float3 tt = f3(1.0f + sign * N.x * N.x * a, sign * b, -sign * N.x);
-                    const float3 bt = f3(b, sign + N.y * N.y * a, -N.y);
-                    const float3 nd = add(add(mul(tt, x), mul(bt, y)), mul(N, z));
-                    Tv.w = rmax(0.0f, dot(N, nd)) / PI_F;
-                    B.pp[j] = make_float4(p.x, p.y, p.z, 1.0f);
-                    ro = add(p, mul(N, (float)2e-4));
-                    rd = nd;
-                } else if (M.kind == 1) {  // metal, renderer.go:1520-1528
-                    float3 ru = f3(0.0f, 0.0f, 0.0f);
-                    for (int k = 0; k < BC_NTRY_METAL; k++) {
-                        const float a = bc_rng(key, dim + BC_D_METAL + 3u * (uint32_t)k) * 2.0f - 1.0f;
-                        const float b = bc_rng(key, dim + BC_D_METAL + 3u * (uint32_t)k + 1u) * 2.0f - 1.0f;
-                        const float c = bc_rng(key, dim + BC_D_METAL + 3u * (uint32_t)k + 2u) * 2.0f - 1.0f;
-                        const float l2 = a * a + b * b + c * c;
-                        if (l2 < 1.0f && l2 > (float)1e-18) { ru = f3(a, b, c); break; }
-                    }
-                    const float3 u = unitv(rd);
-                    const float3 refl = sub(u, mul(N, 2.0f * dot(u, N)));
-                    const float3 nd = unitv(add(refl, mul(ru, rclamp(M.rough, 0.0f, 1.0f))));
-                    if (dot(nd, N) <= 0.0f) {
-                        alive = false;
-                    } else {
-                        B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                        T = had(T, color);
-                        ro = add(p, mul(N, (float)2e-4));
-                        rd = nd;
-                    }
-                } else if (M.kind == 2) {  // glass, renderer.go:1529-1549
-                    float eta = M.ior;
-                    if (eta <= 1.0f) eta = (float)1.5;
-                    const float ratio = front ? 1.0f / eta : eta;
-                    const float3 u = unitv(rd);
-                    const float cosT = rmin(dot(mul(u, -1.0f), N), 1.0f);
-                    const float sinT = sqrtf(rmax(0.0f, 1.0f - cosT * cosT));
-                    float r0 = (1.0f - ratio) / (1.0f + ratio);
-                    r0 *= r0;
-                    const float xx = 1.0f - cosT, x2 = xx * xx, x5 = x2 * x2 * xx;
-                    const float sch = r0 + (1.0f - r0) * x5;
-                    float3 nd;
-                    if (ratio * sinT > 1.0f || sch > bc_rng(key, dim + BC_D_GLASS)) {
-                        nd = sub(u, mul(N, 2.0f * dot(u, N)));
-                    } else {
-                        const float ct = rmin(dot(mul(u, -1.0f), N), 1.0f);
-                        const float3 perp = mul(add(u, mul(N, ct)), ratio);
-                        const float3 par = mul(N, -sqrtf(fabsf(1.0f - dot(perp, perp))));
-                        nd = add(perp, par);
-                    }
-                    B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    T = had(T, color);
-                    ro = add(p, mul(nd, (float)2e-4));
-                    rd = nd;
-                } else {
-                    alive = false;
-                }
-                if (alive && bounce >= rr) {  // Russian roulette, renderer.go:1550-1560
-                    const float mx = rmax(T.x, rmax(T.y, T.z));
-                    const float qq = rclamp(mx, (float)0.08, (float)0.95);
-                    if (bc_rng(key, dim + BC_D_RR) > qq) alive = false;
-                    else T = mul(T, 1.0f / qq);
-                }
-                if (alive) {
-                    B.o[j] = make_float4(ro.x, ro.y, ro.z, 0.0f);
-                    B.d[j] = make_float4(rd.x, rd.y, rd.z, 0.0f);
-                    B.T[j] = make_float4(T.x, T.y, T.z, Tv.w);
-                    cont = bounce + 1 < P.bounces;
-                }
-            }
+            cont = shade_path(S, B, P, j, bounce, guides, shadow, so, sd, sc) && bounce + 1 < P.bounces;
         }
         uint32_t slot = 0;
         wave_append(&B.cnt[cur ^ 1], cont, slot);
@@ -507,9 +101,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
         wave_append(&B.cnt[2], shadow, sslot);
         if (shadow) {
             B.sq[sslot] = j;
-            B.so[j] = shadow_o;
-            B.sd[j] = shadow_d;
-            B.sc[j] = shadow_c;
+            B.so[j] = so;
+            B.sd[j] = sd;
+            B.sc[j] = sc;
         }
     }
 }
@@ -586,30 +180,194 @@ __global__ __launch_bounds__(kBlock) void k_finalize(RenderParams P, const float
     }
 }
 
+
+// ------------------------------------------------------------------------------ persistent schedule
+// A fixed pool of path slots.  Each iteration: extend -> shade -> shadow -> regen.  Paths that
+// end leave their radiance in a per-sample slot and their path slot is refilled in place with
+// the next (pixel, sample) of the window, claimed from a device counter.  Samples are keyed by
+// (pixel, sample), never by slot, so the image does not depend on the schedule; the per-sample
+// results are reduced in sample order afterwards.  Parameters live in device memory so the
+// iteration sequence can be captured once as a hipGraph and replayed.
+
+__global__ void k_p_reset(PathBufs B) {
+    if (threadIdx.x < 5) B.cnt[threadIdx.x] = 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_p_fill(const RenderParams* __restrict__ Pd, PathBufs B,
+                                                   float4* __restrict__ acc, uint32_t* __restrict__ guides,
+                                                   uint32_t slots, int clear_pixels) {
+    const RenderParams P = *Pd;
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t total = (uint32_t)P.ns * (uint32_t)P.npix;
+    if (clear_pixels) {
+        for (uint32_t pl = blockIdx.x * kBlock + threadIdx.x; pl < (uint32_t)P.npix; pl += stride) {
+            acc[pl] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (P.want_guides) {
+                uint32_t* gg = guides + 3 * (size_t)pl;
+                gg[0] = 0u; gg[1] = 0u; gg[2] = 0u;
+            }
+        }
+    }
+    const uint32_t nloop = (slots + stride - 1) / stride * stride;
+    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < nloop; j += stride) {
+        uint32_t idx = 0xffffffffu;
+        wave_append(&B.cnt[4], j < slots, idx);
+        const bool start = j < slots && idx < total;
+        if (start) {
+            const uint32_t sl = idx / (uint32_t)P.npix, pl = idx - sl * (uint32_t)P.npix;
+            init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
+            B.bnc[j] = 0;
+        }
+        uint32_t slot = 0;
+        wave_append(&B.cnt[0], start, slot);
+        if (start) B.q[0][slot] = j;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
+                                                    int cur, uint32_t* __restrict__ guides) {
+    const RenderParams P = *Pd;
+    const uint32_t n = B.cnt[cur];
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t* q = B.q[cur];
+    uint32_t* qn = B.q[cur ^ 1];
+    const uint32_t nloop = (n + stride - 1) / stride * stride;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
+        bool cont = false, shadow = false, done = false;
+        uint32_t j = 0;
+        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+        if (i < n) {
+            j = q[i];
+            const int bounce = (int)B.bnc[j];
+            cont = shade_path(S, B, P, j, bounce, guides, shadow, so, sd, sc) && bounce + 1 < P.bounces;
+            if (cont) B.bnc[j] = (uint32_t)(bounce + 1);
+            done = !cont;
+        }
+        uint32_t slot = 0;
+        wave_append(&B.cnt[cur ^ 1], cont, slot);
+        if (cont) qn[slot] = j;
+        uint32_t dslot = 0;
+        wave_append(&B.cnt[3], done, dslot);
+        if (done) B.dq[dslot] = j;
+        uint32_t sslot = 0;
+        wave_append(&B.cnt[2], shadow, sslot);
+        if (shadow) {
+            B.sq[sslot] = j;
+            B.so[j] = so;
+            B.sd[j] = sd;
+            B.sc[j] = sc;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_p_regen(PathBufs B, const RenderParams* __restrict__ Pd, int cur) {
+    const RenderParams P = *Pd;
+    const uint32_t n = B.cnt[3];
+    const uint32_t total = (uint32_t)P.ns * (uint32_t)P.npix;
+    const uint32_t stride = gridDim.x * kBlock;
+    uint32_t* qn = B.q[cur ^ 1];
+    const uint32_t nloop = (n + stride - 1) / stride * stride;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
+        uint32_t j = 0;
+        if (i < n) {
+            j = B.dq[i];
+            const uint4 meta = B.meta[j];
+            const uint32_t pl = meta.x - (uint32_t)P.y0 * (uint32_t)P.width;
+            const uint32_t g = (meta.y - (uint32_t)P.s0) * (uint32_t)P.npix + pl;
+            const float4 l = B.L[j];
+            B.res[g] = make_float4(l.x, l.y, l.z, (float)B.nseg[j]);
+        }
+        uint32_t idx = 0xffffffffu;
+        wave_append(&B.cnt[4], i < n, idx);
+        const bool start = i < n && idx < total;
+        if (start) {
+            const uint32_t sl = idx / (uint32_t)P.npix, pl = idx - sl * (uint32_t)P.npix;
+            init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
+            B.bnc[j] = 0;
+        }
+        uint32_t slot = 0;
+        wave_append(&B.cnt[cur ^ 1], start, slot);
+        if (start) qn[slot] = j;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_p_reduce(const RenderParams* __restrict__ Pd, PathBufs B,
+                                                     float4* __restrict__ acc) {
+    const RenderParams P = *Pd;
+    const int stride = gridDim.x * kBlock;
+    const int nloop = (P.npix + stride - 1) / stride * stride;
+    for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < nloop; pl += stride) {
+        uint32_t segs = 0;
+        if (pl < P.npix) {
+            float4 a = acc[pl];
+            for (int sl = 0; sl < P.ns; sl++) {
+                const float4 r = B.res[(size_t)sl * (size_t)P.npix + (size_t)pl];
+                float cx = r.x, cy = r.y, cz = r.z;
+                if (P.firefly > 0.0f) {
+                    const float lim = P.firefly;
+                    const float m = rmax(cx, rmax(cy, cz));
+                    if (!(m <= lim || m <= 0.0f)) {
+                        const float k = lim / m;
+                        cx = cx * k; cy = cy * k; cz = cz * k;
+                    }
+                }
+                a.x = a.x + cx; a.y = a.y + cy; a.z = a.z + cz;
+                a.w = a.w + r.w;
+                segs += (uint32_t)r.w;
+            }
+            acc[pl] = a;
+        }
+        const uint32_t tot = wave_sum(segs);
+        if (__lane_id() == 0 && tot) atomicAdd(&B.totals[0], (unsigned long long)tot);
+    }
+}
+
 }  // namespace bcd
 
 // ------------------------------------------------------------------------------ host side
 struct RenderState {
-    size_t cap_paths = 0, cap_pix = 0;
-    void* buf[16] = {};
-    size_t bufcap[16] = {};
+    void* buf[24] = {};
+    size_t bufcap[24] = {};
     bcd::PathBufs B{};
     float4* acc = nullptr;
-    float4* rgba_tmp = nullptr;
     uint32_t* guide_tmp = nullptr;
-    unsigned long long* totals = nullptr;
+    bcd::RenderParams* Pd = nullptr;
+    bcd::RenderParams* h_params = nullptr;  // pinned staging for Pd
+    uint32_t* h_cnt = nullptr;              // pinned
+    // captured iteration graph of the persistent schedule
+    hipGraphExec_t exec = nullptr;
+    const void* g_key[11] = {};
+    int g_iters = 0, g_stack = 0, g_grid = 0;
+    // autotuner cache: (shape) -> (pool, iterations per graph)
+    struct Tune {
+        int w, rows, spp, bounces;
+        uint64_t topo, geom;
+        int pool, iters;
+    };
+    std::vector<Tune> tunes;
+    int last_pool = 0, last_iters = 0, last_windows = 0;
 };
 
 namespace {
 
-bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides) {
+constexpr size_t kResultCap = (size_t)1 << 25;  // per-sample results held per window (512 MB)
+
+bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t window_samples) {
     if (!c->rs) c->rs = new RenderState();
     RenderState& R = *c->rs;
+    if (!R.h_cnt) {
+        void* h = nullptr;
+        if (!bc_check(c, hipHostMalloc(&h, 4096, hipHostMallocDefault), "pinned staging")) return false;
+        R.h_cnt = (uint32_t*)h;
+        R.h_params = (bcd::RenderParams*)((char*)h + 256);
+    }
     struct Want { int slot; size_t bytes; };
     const Want w[] = {
-        {0, paths * 16}, {1, paths * 16}, {2, paths * 16}, {3, paths * 16}, {4, paths * 16}, {5, paths * 16},
-        {6, paths * 4},  {7, paths * 8},  {8, paths * 16}, {9, paths * 16}, {10, paths * 16}, {11, paths * 4},
-        {12, paths * 4}, {13, paths * 4}, {14, 64},        {15, npix * 16 * 2 + (guides ? npix * 12 : 0) + 64},
+        {0, paths * 16},  {1, paths * 16},  {2, paths * 16}, {3, paths * 16},  {4, paths * 16},
+        {5, paths * 16},  {6, paths * 4},   {7, paths * 8},  {8, paths * 16},  {9, paths * 16},
+        {10, paths * 16}, {11, paths * 4},  {12, paths * 4}, {13, paths * 4},  {14, 64},
+        {15, npix * 16 + (guides ? npix * 12 : 0) + 64},     {16, paths * 4},  {17, paths * 4},
+        {18, window_samples * 16},                           {19, sizeof(bcd::RenderParams)},
     };
     for (const Want& x : w)
         if (!bc_ensure(c, R.buf[x.slot], R.bufcap[x.slot], std::max<size_t>(x.bytes, 64), "render buffers")) return false;
@@ -629,12 +387,32 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides) {
     R.B.sq = (uint32_t*)R.buf[13];
     R.B.cnt = (uint32_t*)R.buf[14];
     R.B.totals = (unsigned long long*)((char*)R.buf[14] + 32);
+    R.B.bnc = (uint32_t*)R.buf[16];
+    R.B.dq = (uint32_t*)R.buf[17];
+    R.B.res = (float4*)R.buf[18];
     R.acc = (float4*)R.buf[15];
-    R.rgba_tmp = R.acc + npix;
-    R.guide_tmp = (uint32_t*)(R.rgba_tmp + npix);
-    R.cap_paths = paths;
-    R.cap_pix = npix;
+    R.guide_tmp = (uint32_t*)(R.acc + npix);
+    R.Pd = (bcd::RenderParams*)R.buf[19];
     return true;
+}
+
+bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, int rows, bool guides) {
+    bcd::RenderParams P{};
+    P.width = d->width; P.height = d->height; P.spp = d->spp; P.bounces = d->bounces; P.rr_depth = d->rr_depth;
+    P.sampler = d->sampler; P.mis = d->mis != 0; P.power = d->power_lights != 0; P.sky_mode = d->sky_mode;
+    P.want_guides = guides;
+    P.firefly = d->firefly_clamp;
+    for (int k = 0; k < 3; k++) {
+        P.sky[k] = d->sky_color[k];
+        P.origin[k] = (float)d->origin[k]; P.lower[k] = (float)d->lower[k];
+        P.horiz[k] = (float)d->horiz[k]; P.vert[k] = (float)d->vert[k];
+    }
+    P.seed = d->seed;
+    P.y0 = y0;
+    P.npix = rows * d->width;
+    P.nmat = c->nmat;
+    P.nlight = c->nlight;
+    return P;
 }
 
 template <int STACK>
@@ -645,49 +423,43 @@ void launch_bounce(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::R
     hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B);
 }
 
-}  // namespace
-
-void bc_render_release(bc_ctx* c) {
-    if (!c || !c->rs) return;
-    for (void* p : c->rs->buf)
-        if (p) (void)hipFree(p);
-    delete c->rs;
-    c->rs = nullptr;
+template <int STACK>
+void launch_iteration(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd, int cur,
+                      uint32_t* guides, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur);
+    hipLaunchKernelGGL(bcd::k_p_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides);
+    hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B);
+    hipLaunchKernelGGL(bcd::k_p_regen, dim3(grid), dim3(bcd::kBlock), 0, st, B, Pd, cur);
 }
 
-extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba,
-                                uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_out) {
-    if (!c || !d) return BC_ERR_ARG;
-    if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
-    if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->bounces < 0 || d->bounces > 1024 || y0 < 0 || rows <= 0 ||
-        y0 + rows > d->height || !d_rgba || d->sampler < 0 || d->sampler > 2) {
-        c->err = "invalid render description or row range";
-        return BC_ERR_ARG;
-    }
-    (void)hipSetDevice(c->device);
+int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, uint64_t* rays_out, uint64_t* shadow_out) {
+    RenderState& R = *c->rs;
+    hipStream_t st = c->stream;
+    hipLaunchKernelGGL(bcd::k_finalize, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, (const float4*)R.acc,
+                       (float4*)d_rgba);
+    if (!bc_check(c, hipGetLastError(), "finalize launch") ||
+        !bc_check(c, hipMemcpyAsync(R.h_cnt + 16, R.B.totals, 16, hipMemcpyDeviceToHost, st), "read totals") ||
+        !bc_check(c, hipStreamSynchronize(st), "render"))
+        return BC_ERR_DEVICE;
+    unsigned long long tot[2];
+    std::memcpy(tot, R.h_cnt + 16, 16);
+    if (rays_out) *rays_out = tot[0];
+    if (shadow_out) *shadow_out = tot[1];
+    return bc_check_kernel_error(c);
+}
+
+// Plain wavefront schedule: passes of pixels x samples, three kernels per bounce.
+int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides,
+                 uint64_t* rays_out, uint64_t* shadow_out) {
     const size_t npix = (size_t)rows * (size_t)d->width;
     const size_t max_paths = std::max<size_t>(npix, (size_t)1 << 22);
     int spp_pass = d->samples_per_pass > 0 ? d->samples_per_pass : (int)std::max<size_t>(1, max_paths / npix);
     spp_pass = std::min(spp_pass, d->spp);
     const size_t paths = npix * (size_t)spp_pass;
-    if (!ensure_state(c, paths, npix, d_guides != nullptr)) return BC_ERR_NOMEM;
+    if (!ensure_state(c, paths, npix, d_guides != nullptr, 1)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
-    bcd::RenderParams P{};
-    P.width = d->width; P.height = d->height; P.spp = d->spp; P.bounces = d->bounces; P.rr_depth = d->rr_depth;
-    P.sampler = d->sampler; P.mis = d->mis != 0; P.power = d->power_lights != 0; P.sky_mode = d->sky_mode;
-    P.want_guides = d_guides != nullptr;
-    P.firefly = d->firefly_clamp;
-    for (int k = 0; k < 3; k++) {
-        P.sky[k] = d->sky_color[k];
-        P.origin[k] = (float)d->origin[k]; P.lower[k] = (float)d->lower[k];
-        P.horiz[k] = (float)d->horiz[k]; P.vert[k] = (float)d->vert[k];
-    }
-    P.seed = d->seed;
-    P.y0 = y0;
-    P.npix = (int32_t)npix;
-    P.nmat = c->nmat;
-    P.nlight = c->nlight;
+    bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
     const int grid_paths = (int)std::min<size_t>((paths + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
     const int grid_pix = (int)std::min<size_t>((npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
     hipStream_t st = c->stream;
@@ -709,16 +481,162 @@ extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int 
         hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, R.B, R.acc);
         if (!bc_check(c, hipGetLastError(), "render launch")) return BC_ERR_DEVICE;
     }
-    hipLaunchKernelGGL(bcd::k_finalize, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, (const float4*)R.acc,
-                       (float4*)d_rgba);
-    unsigned long long tot[2] = {0, 0};
-    if (!bc_check(c, hipGetLastError(), "finalize launch") ||
-        !bc_check(c, hipMemcpyAsync(tot, R.B.totals, 16, hipMemcpyDeviceToHost, st), "read totals") ||
-        !bc_check(c, hipStreamSynchronize(st), "render"))
-        return BC_ERR_DEVICE;
-    if (rays_out) *rays_out = tot[0];
-    if (shadow_out) *shadow_out = tot[1];
-    return bc_check_kernel_error(c);
+    return finish(c, P, d_rgba, grid_pix, rays_out, shadow_out);
+}
+
+// Persistent schedule with in-place regeneration; `iters` iterations are one captured graph.
+int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides,
+                      uint64_t* rays_out, uint64_t* shadow_out, int pool, int iters) {
+    const size_t npix = (size_t)rows * (size_t)d->width;
+    const int window_ns = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, kResultCap / npix));
+    const size_t total_first = npix * (size_t)window_ns;
+    const size_t slots = std::min<size_t>((size_t)pool, total_first);
+    if (!ensure_state(c, slots, npix, d_guides != nullptr, total_first)) return BC_ERR_NOMEM;
+    RenderState& R = *c->rs;
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
+    const int grid = (int)std::min<size_t>((slots + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
+    const int grid_pix = (int)std::min<size_t>((npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
+    hipStream_t st = c->stream;
+    uint32_t* guides = d_guides ? d_guides : R.guide_tmp;
+    iters = std::max(2, iters & ~1);
+    const void* key[11] = {c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+                           R.B.o, R.B.res, R.Pd, guides};
+    if (!R.exec || R.g_iters != iters || R.g_stack != c->stack || R.g_grid != grid ||
+        std::memcmp(key, R.g_key, sizeof key) != 0) {
+        if (R.exec) { (void)hipGraphExecDestroy(R.exec); R.exec = nullptr; }
+        hipGraph_t graph = nullptr;
+        if (!bc_check(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture")) return BC_ERR_DEVICE;
+        for (int it = 0; it < iters; it++) {
+            const int cur = it & 1;
+            switch (c->stack) {
+            case 8: launch_iteration<8>(S, R.B, R.Pd, cur, guides, grid, st); break;
+            case 16: launch_iteration<16>(S, R.B, R.Pd, cur, guides, grid, st); break;
+            case 24: launch_iteration<24>(S, R.B, R.Pd, cur, guides, grid, st); break;
+            default: launch_iteration<32>(S, R.B, R.Pd, cur, guides, grid, st); break;
+            }
+        }
+        if (!bc_check(c, hipStreamEndCapture(st, &graph), "end capture")) return BC_ERR_DEVICE;
+        hipError_t e = hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (!bc_check(c, e, "graph instantiate")) { R.exec = nullptr; return BC_ERR_DEVICE; }
+        std::memcpy(R.g_key, key, sizeof key);
+        R.g_iters = iters;
+        R.g_stack = c->stack;
+        R.g_grid = grid;
+    }
+    if (!bc_check(c, hipMemsetAsync(R.B.totals, 0, 16, st), "clear totals")) return BC_ERR_DEVICE;
+    int windows = 0;
+    for (int s0 = 0; s0 < d->spp; s0 += window_ns) {
+        P.s0 = s0;
+        P.ns = std::min(window_ns, d->spp - s0);
+        *R.h_params = P;
+        if (!bc_check(c, hipMemcpyAsync(R.Pd, R.h_params, sizeof P, hipMemcpyHostToDevice, st), "params")) return BC_ERR_DEVICE;
+        hipLaunchKernelGGL(bcd::k_p_reset, dim3(1), dim3(64), 0, st, R.B);
+        const int gfill = (int)std::min<size_t>((std::max(slots, npix) + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
+        hipLaunchKernelGGL(bcd::k_p_fill, dim3(gfill), dim3(bcd::kBlock), 0, st, (const bcd::RenderParams*)R.Pd, R.B,
+                           R.acc, guides, (uint32_t)slots, s0 == 0 ? 1 : 0);
+        // every path ends within `bounces` iterations of its start, and each iteration starts
+        // at least min(slots, remaining) new samples: bound the replays accordingly
+        const size_t window_total = npix * (size_t)P.ns;
+        const size_t max_graphs = ((size_t)(d->bounces + 1) * (window_total / slots + 2)) / (size_t)iters + 4;
+        size_t g = 0;
+        for (;; g++) {
+            if (g >= max_graphs) { c->err = "persistent schedule did not drain"; return BC_ERR_DEVICE; }
+            if (!bc_check(c, hipGraphLaunch(R.exec, st), "graph launch") ||
+                !bc_check(c, hipMemcpyAsync(R.h_cnt, R.B.cnt, 4, hipMemcpyDeviceToHost, st), "queue length") ||
+                !bc_check(c, hipStreamSynchronize(st), "graph"))
+                return BC_ERR_DEVICE;
+            if (R.h_cnt[0] == 0) break;
+        }
+        hipLaunchKernelGGL(bcd::k_p_reduce, dim3(grid_pix), dim3(bcd::kBlock), 0, st, (const bcd::RenderParams*)R.Pd,
+                           R.B, R.acc);
+        if (!bc_check(c, hipGetLastError(), "reduce launch")) return BC_ERR_DEVICE;
+        windows++;
+    }
+    R.last_pool = (int)slots;
+    R.last_iters = iters;
+    R.last_windows = windows;
+    return finish(c, P, d_rgba, grid_pix, rays_out, shadow_out);
+}
+
+// Autotuner: times a short render (2 spp of the same chunk) for each pool size and graph
+// length and keeps the fastest for this image shape, scene and transport depth.
+bool autotune(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides, int& pool,
+              int& iters) {
+    RenderState& R = *c->rs;
+    for (const auto& t : R.tunes)
+        if (t.w == d->width && t.rows == rows && t.spp == d->spp && t.bounces == d->bounces && t.topo == c->topo &&
+            t.geom == c->geom) {
+            pool = t.pool;
+            iters = t.iters;
+            return true;
+        }
+    bc_render_desc trial = *d;
+    trial.spp = std::min(d->spp, 2);
+    const size_t npix = (size_t)rows * (size_t)d->width;
+    const int pools[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21};
+    const int its[] = {4, 8, 16};
+    float best = 1e30f;
+    pool = 1 << 20;
+    iters = 8;
+    for (int p : pools) {
+        if ((size_t)p > npix * (size_t)trial.spp * 2 && p != pools[0]) continue;
+        for (int it : its) {
+            if (render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it) != BC_OK) return false;
+            float ms = 0;
+            if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
+                render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it) != BC_OK ||
+                !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
+                !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
+                !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
+                return false;
+            if (ms < best) { best = ms; pool = p; iters = it; }
+        }
+    }
+    R.tunes.push_back({d->width, rows, d->spp, d->bounces, c->topo, c->geom, pool, iters});
+    return true;
+}
+
+}  // namespace
+
+void bc_render_release(bc_ctx* c) {
+    if (!c || !c->rs) return;
+    if (c->rs->exec) (void)hipGraphExecDestroy(c->rs->exec);
+    for (void* p : c->rs->buf)
+        if (p) (void)hipFree(p);
+    if (c->rs->h_cnt) (void)hipHostFree(c->rs->h_cnt);
+    delete c->rs;
+    c->rs = nullptr;
+}
+
+extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba,
+                                uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_out) {
+    if (!c || !d) return BC_ERR_ARG;
+    if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
+    if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->spp > 65536 || d->bounces < 0 || d->bounces > 1024 ||
+        y0 < 0 || rows <= 0 || y0 + rows > d->height || !d_rgba || d->sampler < 0 || d->sampler > 2 ||
+        (int64_t)d->width * d->height > ((int64_t)1 << 28) || d->mode < 0 || d->mode > 1) {
+        c->err = "invalid render description or row range";
+        return BC_ERR_ARG;
+    }
+    (void)hipSetDevice(c->device);
+    if (d->mode == 0) return render_plain(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out);
+    if (!ensure_state(c, 1, (size_t)rows * (size_t)d->width, d_guides != nullptr, 1)) return BC_ERR_NOMEM;
+    int pool = d->reserved[0], iters = d->reserved[1];
+    if (pool <= 0 || iters <= 0) {
+        if (!autotune(c, d, y0, rows, d_rgba, d_guides, pool, iters)) return BC_ERR_DEVICE;
+    }
+    return render_persistent(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out, pool, iters);
+}
+
+extern "C" int bc_render_tuning(const bc_ctx* c, int32_t* out4) {
+    if (!c || !out4) return BC_ERR_ARG;
+    out4[0] = c->rs ? c->rs->last_pool : 0;
+    out4[1] = c->rs ? c->rs->last_iters : 0;
+    out4[2] = c->stack;
+    out4[3] = c->rs ? c->rs->last_windows : 0;
+    return BC_OK;
 }
 
 extern "C" int bc_render(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rgba, uint32_t* guides,

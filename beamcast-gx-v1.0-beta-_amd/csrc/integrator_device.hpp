@@ -1,0 +1,436 @@
+// integrator_device.hpp -- device-side pieces of the wavefront integrator shared by the plain
+// (pass per sample batch) and persistent (in-place regeneration) schedules.
+//
+// Semantics follow the Go CPU integrator: trace (renderer.go:1484-1563), directLight
+// (:1427-1465), chooseLight (:1371-1393), lightDirectionalPDF (:1404-1425), the samplers
+// (:1262-1311) and the camera jitter of adaptivePixel (:1940-1950), evaluated in fp32 with the
+// operation order of the oracle so that per-sample results match it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "bc_layout.h"
+#include "trace_device.hpp"
+
+namespace bcd {
+
+constexpr int kBlock = 256;
+
+struct RenderParams {
+    int32_t width, height, spp, bounces, rr_depth, sampler, mis, power, sky_mode, want_guides;
+    float firefly, sky[3];
+    uint32_t seed;
+    float origin[3], lower[3], horiz[3], vert[3];
+    int32_t y0, npix;          // chunk: first row, pixels in chunk
+    int32_t s0, ns;            // pass: first sample, samples in this pass
+    int32_t nmat, nlight;
+};
+
+struct PathBufs {
+    uint32_t* bnc;  // per-path bounce (persistent schedule)
+    float4* res;    // per-sample result: rgb, path segments (persistent schedule)
+    uint32_t* dq;   // terminated paths (persistent schedule)
+    float4* o;      // ray origin
+    float4* d;      // ray direction
+    float4* T;      // throughput xyz, prevPdf w
+    float4* L;      // radiance
+    float4* pp;     // prevPoint xyz, w = prevDiffuse flag
+    uint4* meta;    // pixel, sample, key, packed jitter
+    uint32_t* nseg; // path segments
+    float2* hit;    // t, record index bits (0xffffffff: miss)
+    float4* so;     // shadow origin xyz, tmax w
+    float4* sd;     // shadow dir
+    float4* sc;     // shadow contribution
+    uint32_t* q[2];
+    uint32_t* sq;
+    uint32_t* cnt;  // [0],[1] queue lengths, [2] shadow queue length
+    unsigned long long* totals;  // [0] path segments, [1] shadow rays
+};
+
+struct SceneBufs {
+    const bc_node8* nodes;
+    const bc_prim* prims;
+    const bc_mat* mats;
+    const bc_light* lights;
+    const float* pdf;
+    const int32_t* light_of;
+    int* err;
+};
+
+__device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 add(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul(float3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 had(float3 a, float3 b) { return f3(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+    return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 unitv(float3 a) {  // Vec3.Unit, renderer.go:32-38
+    const float l = sqrtf(dot(a, a));
+    if (l < (float)1e-12) return f3(0.0f, 0.0f, 0.0f);
+    return mul(a, 1.0f / l);
+}
+__device__ __forceinline__ float rmax(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float rmin(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float rclamp(float x, float a, float b) { return x < a ? a : (x > b ? b : x); }
+__device__ __forceinline__ float3 xyz(float4 v) { return f3(v.x, v.y, v.z); }
+
+#define PI_F ((float)3.141592653589793)
+#define INVPI_F ((float)0.3183098861837907)
+
+// Appends the calling lanes whose pred is true to a queue with one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred, uint32_t& slot) {
+    const uint64_t m = __ballot(pred);
+    if (m == 0) return 0;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader);
+    slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    return (uint32_t)__popcll(m);
+}
+
+__device__ __forceinline__ double radical_inverse(uint64_t n, uint32_t base) {  // renderer.go:1844-1858
+    if (base == 2) return (double)__brevll(n) * 5.421010862427522e-20;
+    const double invBase = 1.0 / (double)base;
+    double inv = invBase, result = 0.0;
+    while (n > 0) {
+        const uint64_t digit = n % base;
+        result += (double)digit * inv;
+        n /= base;
+        inv *= invBase;
+    }
+    return result;
+}
+
+__device__ __forceinline__ void pixel_rotation(int x, int y, double& rx, double& ry) {  // renderer.go:1906-1915
+    uint32_t h1 = (uint32_t)x * 0x9e3779b1u ^ (uint32_t)y * 0x85ebca77u;
+    h1 ^= h1 >> 16;
+    h1 *= 0x7feb352du;
+    h1 ^= h1 >> 15;
+    uint32_t h2 = h1 ^ 0x68bc21ebu;
+    h2 ^= h2 >> 13;
+    const double inv = 1.0 / 4294967296.0;
+    rx = (double)h1 * inv;
+    ry = (double)h2 * inv;
+}
+
+__device__ __forceinline__ uint32_t pack_guide(float3 n, int mat) {  // gpu_opencl_windows.go:787-792
+    const float inv = 1.0f / (fabsf(n.x) + fabsf(n.y) + fabsf(n.z) + 1e-20f);
+    const float nx = n.x * inv, ny = n.y * inv, nz = n.z * inv;
+    float px = nx, py = ny;
+    if (nz < 0.0f) {
+        const float ax = 1.0f - fabsf(py), ay = 1.0f - fabsf(px);
+        px = copysignf(ax, px);
+        py = copysignf(ay, py);
+    }
+    px = px * 0.5f + 0.5f;
+    py = py * 0.5f + 0.5f;
+    px = px < 0.0f ? 0.0f : (px > 1.0f ? 1.0f : px);
+    py = py < 0.0f ? 0.0f : (py > 1.0f ? 1.0f : py);
+    const uint32_t qx = (uint32_t)rintf(px * 1023.0f), qy = (uint32_t)rintf(py * 1023.0f);
+    const uint32_t m = (uint32_t)(mat < 0 ? 0 : (mat > 2047 ? 2047 : mat));
+    return 0x80000000u | (m << 20) | (qy << 10) | qx;
+}
+__device__ __forceinline__ uint32_t pack_jitter(float ju, float jv) {  // gpu_opencl_windows.go:793
+    ju = ju < 0.0f ? 0.0f : (ju > 1.0f ? 1.0f : ju);
+    jv = jv < 0.0f ? 0.0f : (jv > 1.0f ? 1.0f : jv);
+    return ((uint32_t)rintf(jv * 65535.0f) << 16) | (uint32_t)rintf(ju * 65535.0f);
+}
+
+
+// Camera ray of adaptivePixel (renderer.go:1940-1950) with adaptive sampling off, written to
+// path slot j for chunk pixel pl and sample `sample`.
+__device__ __forceinline__ void init_camera_path(const RenderParams& P, const PathBufs& B, uint32_t j, int pl,
+                                                 int sample) {
+    const double g = 1.324717957244746, a1 = 1.0 / g, a2 = 1.0 / (g * g);
+    const float3 origin = f3(P.origin[0], P.origin[1], P.origin[2]);
+    const float3 lower = f3(P.lower[0], P.lower[1], P.lower[2]);
+    const float3 horiz = f3(P.horiz[0], P.horiz[1], P.horiz[2]);
+    const float3 vert = f3(P.vert[0], P.vert[1], P.vert[2]);
+    const int x = pl % P.width, y = P.y0 + pl / P.width;
+    const uint32_t pixel = (uint32_t)y * (uint32_t)P.width + (uint32_t)x;
+    const uint32_t key = bc_path_key(P.seed, pixel, (uint32_t)sample);
+    double jx, jy;
+    if (P.sampler == 1) {
+        double rx, ry;
+        pixel_rotation(x, y, rx, ry);
+        jx = radical_inverse((uint64_t)sample + 1, 2) + rx;
+        jx = jx - floor(jx);
+        jy = radical_inverse((uint64_t)sample + 1, 3) + ry;
+        jy = jy - floor(jy);
+    } else if (P.sampler == 2) {
+        double rx, ry;
+        pixel_rotation(x, y, rx, ry);
+        double sx = 0.5 + (double)(sample + 1) * a1, sy = 0.5 + (double)(sample + 1) * a2;
+        sx = sx - floor(sx);
+        sy = sy - floor(sy);
+        jx = sx + rx;
+        jx = jx - floor(jx);
+        jy = sy + ry;
+        jy = jy - floor(jy);
+    } else {
+        jx = (double)(bc_rng_u32(key, BC_DIM_CAM_X) >> 8) * (1.0 / 16777216.0);
+        jy = (double)(bc_rng_u32(key, BC_DIM_CAM_Y) >> 8) * (1.0 / 16777216.0);
+    }
+    const double uu = ((double)x + jx) / (double)(P.width - 1 > 1 ? P.width - 1 : 1);
+    const double vv = ((double)(P.height - 1 - y) + jy) / (double)(P.height - 1 > 1 ? P.height - 1 : 1);
+    const float u = (float)uu, v = (float)vv;
+    const float3 dir = sub(add(add(lower, mul(horiz, u)), mul(vert, v)), origin);
+    B.o[j] = make_float4(origin.x, origin.y, origin.z, 0.0f);
+    B.d[j] = make_float4(dir.x, dir.y, dir.z, 0.0f);
+    B.T[j] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    B.L[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    B.meta[j] = make_uint4(pixel, (uint32_t)sample, key, pack_jitter((float)jx, (float)jy));
+    B.nseg[j] = 0;
+}
+
+__device__ __forceinline__ float light_dir_pdf(const SceneBufs& S, const RenderParams& P, int prim, float3 from,
+                                               float3 lp, float3 ln) {  // renderer.go:1404-1425
+    const int li = S.light_of[prim];
+    if (li < 0) return 0.0f;
+    const float3 to = sub(lp, from);
+    const float dist2 = dot(to, to);
+    if (dist2 <= (float)1e-12) return 0.0f;
+    const float3 ld = unitv(to);
+    const float cosL = rmax(0.0f, dot(ln, mul(ld, -1.0f)));
+    if (cosL <= (float)1e-12) return 0.0f;
+    const float area = S.lights[li].area;
+    const float sel = P.power ? S.pdf[prim] : 1.0f / (float)P.nlight;
+    return sel * dist2 / (cosL * area);
+}
+
+
+// One shading step of trace for path j at `bounce` (renderer.go:1490-1560).  Returns true when
+// the path continues (a new ray is stored); an NEE shadow ray, if any, is returned in so/sd/sc.
+__device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B, const RenderParams& P, uint32_t j,
+                                           int bounce, uint32_t* __restrict__ guides, bool& shadow,
+                                           float4& shadow_o, float4& shadow_d, float4& shadow_c) {
+    const uint32_t dim = BC_DIM_BOUNCE0 + BC_DIMS_PER_BOUNCE * (uint32_t)bounce;
+    const int rr = P.rr_depth < 1 ? 4 : P.rr_depth;
+    bool alive_out = false;
+    const float2 hv = B.hit[j];
+    const uint32_t rec = __float_as_uint(hv.y);
+    const float t = hv.x;
+    float3 ro = xyz(B.o[j]), rd = xyz(B.d[j]);
+    float4 Tv = B.T[j];
+    float3 T = xyz(Tv);
+    float3 L = xyz(B.L[j]);
+    const uint4 meta = B.meta[j];
+    const uint32_t key = meta.z;
+    if (rec == 0xffffffffu) {  // background, renderer.go:1492-1495 / :1326-1331
+        float3 bg;
+        if (P.sky_mode == 1) {
+            bg = f3(P.sky[0], P.sky[1], P.sky[2]);
+        } else {
+            const float3 u = unitv(rd);
+            const float tt = (float)0.5 * (u.y + 1.0f);
+            bg = add(mul(f3((float)0.06, (float)0.08, (float)0.12), 1.0f - tt),
+                     mul(f3((float)0.32, (float)0.48, (float)0.72), tt));
+        }
+        L = add(L, had(T, bg));
+        B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
+    } else {
+        const float4* pr = reinterpret_cast<const float4*>(S.prims + rec);
+        const float4 q0 = pr[0], q1 = pr[1], q2 = pr[2];
+        const uint32_t idbits = __float_as_uint(q2.y);
+        const int id = (int)(idbits & ~BC_SPHERE_BIT);
+        const int mat = (int)__float_as_uint(q2.z);
+        float3 nrm;
+        if (idbits & BC_SPHERE_BIT) {
+            const float3 pos = add(ro, mul(rd, t));
+            nrm = mul(sub(pos, f3(q0.x, q0.y, q0.z)), q1.y);
+        } else {
+            nrm = unitv(cross(f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x)));
+        }
+        const bool front = dot(rd, nrm) < 0.0f;
+        const float3 N = front ? nrm : mul(nrm, -1.0f);
+        const float3 p = add(ro, mul(rd, t));
+        if (bounce == 0 && P.want_guides && meta.y == 0) {  // gpu_opencl_windows.go:811
+            uint32_t* gg = guides + 3 * (size_t)(meta.x - (uint32_t)P.y0 * (uint32_t)P.width);
+            gg[0] = __float_as_uint(t * sqrtf(dot(rd, rd)));
+            gg[1] = pack_guide(N, mat);
+            gg[2] = meta.w;
+        }
+        const bc_mat M = S.mats[mat];
+        const float3 color = f3(M.color[0], M.color[1], M.color[2]);
+        bool alive = true;
+        if (M.kind == 3) {  // emissive: MIS weight on BSDF-sampled hits, renderer.go:1504-1511
+            const float4 pp = B.pp[j];
+            float w = 1.0f;
+            if (P.mis && pp.w != 0.0f && id >= 0) {
+                const float lpdf = light_dir_pdf(S, P, id, xyz(pp), p, N);
+                const float a2 = Tv.w * Tv.w, b2 = lpdf * lpdf;
+                w = (a2 + b2 <= (float)1e-20) ? 0.0f : a2 / (a2 + b2);
+            }
+            L = add(L, mul(had(T, f3(M.emission[0], M.emission[1], M.emission[2])), w));
+            B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
+            alive = false;
+        } else if (M.kind == 0) {  // diffuse: NEE then cosine sampling
+            if (P.nlight > 0) {   // directLight, renderer.go:1427-1465
+                const float u = bc_rng(key, dim + BC_D_LIGHT);
+                int li;
+                float sel;
+                if (!P.power) {  // chooseLight, renderer.go:1371-1393 (Intn -> floor(u*n))
+                    li = (int)(u * (float)P.nlight);
+                    if (li >= P.nlight) li = P.nlight - 1;
+                    sel = 1.0f / (float)P.nlight;
+                } else {
+                    int lo = 0, hi = P.nlight;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        if (S.lights[mid].cdf >= u) hi = mid; else lo = mid + 1;
+                    }
+                    li = lo >= P.nlight ? P.nlight - 1 : lo;
+                    sel = S.pdf[S.lights[li].prim];
+                    if (sel <= 0.0f) sel = 1.0f / (float)P.nlight;
+                }
+                const bc_light& Lr = S.lights[li];
+                float3 qpt, lnrm;
+                const bool sph = Lr.kind == 0;
+                if (sph) {  // randUnitVector, renderer.go:1273-1285
+                    float3 ln = f3(0.0f, 0.0f, 1.0f);
+                    for (int k = 0; k < BC_NTRY_SPH; k++) {
+                        const float a = bc_rng(key, dim + BC_D_SPH + 2u * (uint32_t)k) * 2.0f - 1.0f;
+                        const float b = bc_rng(key, dim + BC_D_SPH + 2u * (uint32_t)k + 1u) * 2.0f - 1.0f;
+                        const float ss = a * a + b * b;
+                        if (ss >= 1.0f || ss <= (float)1e-18) continue;
+                        const float f = 2.0f * sqrtf(1.0f - ss);
+                        ln = f3(a * f, b * f, 1.0f - 2.0f * ss);
+                        break;
+                    }
+                    qpt = add(f3(Lr.g0[0], Lr.g0[1], Lr.g0[2]), mul(ln, Lr.g0[3]));
+                    lnrm = ln;
+                } else {  // uniform point on an emissive triangle
+                    const float u1 = bc_rng(key, dim + BC_D_TRI), u2 = bc_rng(key, dim + BC_D_TRI + 1u);
+                    const float su = sqrtf(u1);
+                    const float b1 = su * (1.0f - u2), b2 = su * u2;
+                    const float3 e1 = f3(Lr.g1[0], Lr.g1[1], Lr.g1[2]), e2 = f3(Lr.g2[0], Lr.g2[1], Lr.g2[2]);
+                    qpt = add(add(f3(Lr.g0[0], Lr.g0[1], Lr.g0[2]), mul(e1, b1)), mul(e2, b2));
+                    lnrm = unitv(cross(e1, e2));
+                }
+                const float3 to = sub(qpt, p);
+                const float dist2 = dot(to, to);
+                if (!(dist2 < (float)1e-8)) {
+                    const float3 ld = unitv(to);
+                    const float nd = rmax(0.0f, dot(N, ld));
+                    const float cl = dot(mul(lnrm, -1.0f), ld);
+                    const float cosL = sph ? rmax(0.0f, cl) : fabsf(cl);
+                    if (!(nd <= 0.0f || cosL <= 0.0f)) {
+                        const float lightPDF = sel * dist2 / (cosL * Lr.area);
+                        if (lightPDF > 0.0f) {
+                            const float bsdfPDF = nd / PI_F;
+                            float w = 1.0f;
+                            if (P.mis) {
+                                const float a2 = lightPDF * lightPDF, b2 = bsdfPDF * bsdfPDF;
+                                w = (a2 + b2 <= (float)1e-20) ? 0.0f : a2 / (a2 + b2);
+                            }
+                            const float3 f = mul(color, INVPI_F);
+                            const float3 dl = mul(had(f, f3(Lr.le[0], Lr.le[1], Lr.le[2])), nd * w / lightPDF);
+                            const float3 c = had(T, dl);  // throughput before the albedo update
+                            // visible(), renderer.go:1333-1341
+                            const float3 from = add(p, mul(N, (float)2e-4));
+                            const float3 dd = sub(qpt, from);
+                            const float dd2 = dot(dd, dd);
+                            if (dd2 < (float)1e-12) {
+                                L = add(L, c);
+                                B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
+                            } else {
+                                const float dist = sqrtf(dd2);
+                                const float3 dir = mul(dd, 1.0f / dist);
+                                shadow_o = make_float4(from.x, from.y, from.z, dist - (float)1e-4);
+                                shadow_d = make_float4(dir.x, dir.y, dir.z, 0.0f);
+                                shadow_c = make_float4(c.x, c.y, c.z, 0.0f);
+                                shadow = true;
+                            }
+                        }
+                    }
+                }
+            }
+            T = had(T, color);
+            float x = 0.0f, y = 0.0f, ss = 0.0f;  // randCosHemisphere, renderer.go:1290-1311
+            bool ok = false;
+            for (int k = 0; k < BC_NTRY_COS && !ok; k++) {
+                x = bc_rng(key, dim + BC_D_COS + 2u * (uint32_t)k) * 2.0f - 1.0f;
+                y = bc_rng(key, dim + BC_D_COS + 2u * (uint32_t)k + 1u) * 2.0f - 1.0f;
+                ss = x * x + y * y;
+                ok = ss < 1.0f;
+            }
+            if (!ok) { x = 0.0f; y = 0.0f; ss = 0.0f; }
+            const float z = sqrtf(rmax(0.0f, 1.0f - ss));
+            const float sign = N.z < 0.0f ? -1.0f : 1.0f;
+            const float a = -1.0f / (sign + N.z);
+            const float b = N.x * N.y * a;
+            const float3 tt = f3(1.0f + sign * N.x * N.x * a, sign * b, -sign * N.x);
+            const float3 bt = f3(b, sign + N.y * N.y * a, -N.y);
+            const float3 nd = add(add(mul(tt, x), mul(bt, y)), mul(N, z));
+            Tv.w = rmax(0.0f, dot(N, nd)) / PI_F;
+            B.pp[j] = make_float4(p.x, p.y, p.z, 1.0f);
+            ro = add(p, mul(N, (float)2e-4));
+            rd = nd;
+        } else if (M.kind == 1) {  // metal, renderer.go:1520-1528
+            float3 ru = f3(0.0f, 0.0f, 0.0f);
+            for (int k = 0; k < BC_NTRY_METAL; k++) {
+                const float a = bc_rng(key, dim + BC_D_METAL + 3u * (uint32_t)k) * 2.0f - 1.0f;
+                const float b = bc_rng(key, dim + BC_D_METAL + 3u * (uint32_t)k + 1u) * 2.0f - 1.0f;
+                const float c = bc_rng(key, dim + BC_D_METAL + 3u * (uint32_t)k + 2u) * 2.0f - 1.0f;
+                const float l2 = a * a + b * b + c * c;
+                if (l2 < 1.0f && l2 > (float)1e-18) { ru = f3(a, b, c); break; }
+            }
+            const float3 u = unitv(rd);
+            const float3 refl = sub(u, mul(N, 2.0f * dot(u, N)));
+            const float3 nd = unitv(add(refl, mul(ru, rclamp(M.rough, 0.0f, 1.0f))));
+            if (dot(nd, N) <= 0.0f) {
+                alive = false;
+            } else {
+                B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                T = had(T, color);
+                ro = add(p, mul(N, (float)2e-4));
+                rd = nd;
+            }
+        } else if (M.kind == 2) {  // glass, renderer.go:1529-1549
+            float eta = M.ior;
+            if (eta <= 1.0f) eta = (float)1.5;
+            const float ratio = front ? 1.0f / eta : eta;
+            const float3 u = unitv(rd);
+            const float cosT = rmin(dot(mul(u, -1.0f), N), 1.0f);
+            const float sinT = sqrtf(rmax(0.0f, 1.0f - cosT * cosT));
+            float r0 = (1.0f - ratio) / (1.0f + ratio);
+            r0 *= r0;
+            const float xx = 1.0f - cosT, x2 = xx * xx, x5 = x2 * x2 * xx;
+            const float sch = r0 + (1.0f - r0) * x5;
+            float3 nd;
+            if (ratio * sinT > 1.0f || sch > bc_rng(key, dim + BC_D_GLASS)) {
+                nd = sub(u, mul(N, 2.0f * dot(u, N)));
+            } else {
+                const float ct = rmin(dot(mul(u, -1.0f), N), 1.0f);
+                const float3 perp = mul(add(u, mul(N, ct)), ratio);
+                const float3 par = mul(N, -sqrtf(fabsf(1.0f - dot(perp, perp))));
+                nd = add(perp, par);
+            }
+            B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            T = had(T, color);
+            ro = add(p, mul(nd, (float)2e-4));
+            rd = nd;
+        } else {
+            alive = false;
+        }
+        if (alive && bounce >= rr) {  // Russian roulette, renderer.go:1550-1560
+            const float mx = rmax(T.x, rmax(T.y, T.z));
+            const float qq = rclamp(mx, (float)0.08, (float)0.95);
+            if (bc_rng(key, dim + BC_D_RR) > qq) alive = false;
+            else T = mul(T, 1.0f / qq);
+        }
+        if (alive) {
+            B.o[j] = make_float4(ro.x, ro.y, ro.z, 0.0f);
+            B.d[j] = make_float4(rd.x, rd.y, rd.z, 0.0f);
+            B.T[j] = make_float4(T.x, T.y, T.z, Tv.w);
+            alive_out = true;
+        }
+    }
+    return alive_out;
+}
+
+}  // namespace bcd

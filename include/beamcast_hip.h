@@ -93,9 +93,16 @@ typedef struct bc_render_desc {
  * between sample passes; a set flag returns BC_ERR_CANCELLED. */
 int bc_render(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* rgba,
               uint32_t* guides, uint64_t* rays_out, const volatile int32_t* cancel);
-/* Same, with rgba/guides in device memory (no host copy). shadow_rays_out may be NULL. */
+/* Same, with rgba/guides in device memory (no host copy). shadow_rays_out may be NULL.
+ * mode 1 (persistent): a pool of path slots refilled in place as paths end, the bounce loop
+ * replayed as a captured hipGraph; reserved[0] = pool slots, reserved[1] = iterations per
+ * graph, both 0 = chosen by the autotuner on first use for this image shape and scene. */
 int bc_render_device(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* d_rgba,
                      uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_rays_out);
+
+/* Schedule last used by a mode-1 render: pool slots, iterations per graph, traversal stack
+ * depth, sample windows.  No counterpart in the reference (its kernel has one fixed shape). */
+int bc_render_tuning(const bc_ctx* ctx, int32_t* out4);
 
 /* Ray queries against the uploaded scene (the traversal of gpu_opencl_windows.go:758-777
  * and of Scene.Hit / Scene.Occluded, renderer.go:777-895).  rays: n x 8 floats

@@ -121,3 +121,23 @@ def test_same_seed_fp64_and_converged_mean(dev):
     d2 = bc.render_desc(s, 48, 48, 512, 4, rr_depth=3, seed=99)
     img2, _, _ = dev.render(d2)
     assert abs(img2[..., :3].mean() / ref64[..., :3].mean() - 1) < 0.01
+
+
+@pytest.mark.parametrize("pool,iters", [(4096, 4), (50000, 8), (0, 0)])
+def test_persistent_schedule_equals_plain(dev, pool, iters):
+    """In-place regeneration, hipGraph replay and the autotuner (pool=0) change the schedule but
+    not the image: per-sample results are reduced in sample order, so rgba is bit-identical."""
+    print(f"case: persistent pool={pool} iters={iters}", flush=True)
+    s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
+    dev.upload(s)
+    d = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20)
+    plain, gp, rp = dev.render(d, guides=True)
+    dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=pool, iters=iters)
+    pers, gq, rq = dev.render(dp, guides=True)
+    assert rp == rq
+    assert np.array_equal(plain, pers) and np.array_equal(gp, gq)
+    t = dev.tuning()
+    assert t["pool"] > 0 and t["iters_per_graph"] >= 2
+    # a second call reuses the captured graph (and the tuned shape)
+    again, _, _ = dev.render(dp, guides=True)
+    assert np.array_equal(again, pers)
