@@ -70,6 +70,9 @@ def lib():
             "bc_render": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), vp]),
             "bc_render_device": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_render_tuning": (c_int, [vp, POINTER(c_int32)]),
+            "bc_forward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, POINTER(c_uint64)]),
+            "bc_backward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, vp]),
+            "bc_update_materials": (c_int, [vp, c_int32, vp, vp]),
             "bc_trace": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_trace_device": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_alloc": (vp, [vp, c_uint64]),
@@ -308,6 +311,28 @@ class Device:
         self._check(lib().bc_render(self.ctx, ctypes.byref(desc), y0, rows, _ptr(rgba), _ptr(g) if g is not None else None,
                                     ctypes.byref(rays), _ptr(cflag) if cflag is not None else None))
         return rgba, g, int(rays.value)
+
+    def update_materials(self, mat_kind, mat_data):
+        mk = np.ascontiguousarray(mat_kind, np.int32)
+        md = np.ascontiguousarray(mat_data, np.float64)
+        self._check(lib().bc_update_materials(self.ctx, len(mk), _ptr(mk), _ptr(md)))
+
+    def forward(self, desc: RenderDesc, y0=0, rows=None):
+        """Differentiable render, forward: image (rows, W, 4) and path segments."""
+        rows = desc.height - y0 if rows is None else rows
+        rgba = np.zeros((rows, desc.width, 4), np.float32)
+        rays = c_uint64()
+        self._check(lib().bc_forward(self.ctx, ctypes.byref(desc), y0, rows, _ptr(rgba), ctypes.byref(rays)))
+        return rgba, int(rays.value)
+
+    def backward(self, desc: RenderDesc, dloss_dimage, nmat, y0=0, rows=None):
+        """Path-replay backward: (d loss / d Color [nmat,3], d loss / d Emission [nmat,3])."""
+        rows = desc.height - y0 if rows is None else rows
+        dl = np.ascontiguousarray(dloss_dimage, np.float32).reshape(rows, desc.width, 3)
+        dc = np.zeros((nmat, 3), np.float32)
+        de = np.zeros((nmat, 3), np.float32)
+        self._check(lib().bc_backward(self.ctx, ctypes.byref(desc), y0, rows, _ptr(dl), _ptr(dc), _ptr(de)))
+        return dc, de
 
     def tuning(self):
         o = (c_int32 * 4)()

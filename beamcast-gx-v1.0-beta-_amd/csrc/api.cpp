@@ -113,7 +113,7 @@ void bc_destroy(bc_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                    c->tmp_in, c->tmp_out};
+                    c->tmp_in, c->tmp_out, c->out_buf};
     for (void* q : ptrs)
         if (q) hipFree(q);
     bc_render_release(c);
@@ -139,20 +139,9 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     for (int i = 0; i < v->nprim; i++) {
         if (v->prim_mat[i] < 0 || v->prim_mat[i] >= v->nmat) { c->err = "primitive material out of range"; return BC_ERR_SCENE; }
     }
-    // materials (renderer.go:93-99)
-    std::vector<bc_mat> mats((size_t)v->nmat);
-    for (int i = 0; i < v->nmat; i++) {
-        const double* d = v->mat_data + 8 * (size_t)i;
-        bc_mat& m = mats[(size_t)i];
-        std::memset(&m, 0, sizeof m);
-        m.color[0] = (float)d[0]; m.color[1] = (float)d[1]; m.color[2] = (float)d[2]; m.rough = (float)d[3];
-        m.ior = (float)d[4];
-        m.emission[0] = (float)d[5]; m.emission[1] = (float)d[6]; m.emission[2] = (float)d[7];
-        m.kind = v->mat_kind[i];
-    }
-    // light distribution (rebuildLightDistribution, renderer.go:363-407; lightWeight :1343-1357)
-    std::vector<bc_light> lights;
-    std::vector<double> w;
+    // emissive primitives: geometry and fp64 area (lightWeight, renderer.go:1343-1357)
+    c->h_lights.clear();
+    c->h_light_area.clear();
     for (int i = 0; i < v->nprim; i++) {
         const int m = v->prim_mat[i];
         if (v->mat_kind[m] != 3) continue;
@@ -175,47 +164,85 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
             }
         }
         L.area = (float)area;
-        const double* md = v->mat_data + 8 * (size_t)m;
-        L.le[0] = (float)md[5]; L.le[1] = (float)md[6]; L.le[2] = (float)md[7];
-        double wt = (0.2126 * md[5] + 0.7152 * md[6] + 0.0722 * md[7]) * area;
-        w.push_back(wt < 1e-9 ? 1e-9 : wt);
-        lights.push_back(L);
+        uint32_t mbits = (uint32_t)m;
+        std::memcpy(&L.g1[3], &mbits, 4);  // the light's material index
+        c->h_lights.push_back(L);
+        c->h_light_area.push_back(area);
     }
-    std::vector<float> pdf((size_t)v->nprim, 0.0f);
-    std::vector<int32_t> light_of((size_t)v->nprim, -1);
-    double total = 0;
-    std::vector<double> cdf(lights.size());
-    for (size_t i = 0; i < lights.size(); i++) { total += w[i]; cdf[i] = total; }
-    double prev = 0;
-    for (size_t i = 0; i < lights.size(); i++) {
-        double cv = cdf[i] / total;
-        pdf[(size_t)lights[i].prim] = (float)(cv - prev);
-        prev = cv;
-        cdf[i] = cv;
-    }
-    if (!lights.empty()) cdf.back() = 1;
-    for (size_t i = 0; i < lights.size(); i++) {
-        lights[i].cdf = (float)cdf[i];
-        light_of[(size_t)lights[i].prim] = (int32_t)i;
+    c->nprim = v->nprim;
+    c->nmat_scene = 0;
+    {
+        const int rc = bc_update_materials(c, v->nmat, v->mat_kind, v->mat_data);
+        if (rc != BC_OK) return rc;
     }
     if (!upload_vec(c, c->d_nodes, c->cap_nodes, b.nodes, "upload nodes") ||
         !upload_vec(c, c->d_prims, c->cap_prims, b.prims, "upload prims") ||
-        !upload_vec(c, c->d_mats, c->cap_mats, mats, "upload materials") ||
-        !upload_vec(c, c->d_lights, c->cap_lights, lights, "upload lights") ||
-        !upload_vec(c, c->d_pdf, c->cap_pdf, pdf, "upload light pdf") ||
-        !upload_vec(c, c->d_light_of, c->cap_light_of, light_of, "upload light map"))
+        false)
         return BC_ERR_DEVICE;
     if (!ok(c, hipStreamSynchronize(c->stream), "scene upload")) return BC_ERR_DEVICE;
     c->nnode8 = (int64_t)b.nodes.size();
     c->depth8 = b.depth;
-    c->nprim = v->nprim;
-    c->nmat = v->nmat;
-    c->nlight = (int)lights.size();
     c->stack = stack;
     c->topo = topo;
     c->geom = geom;
     c->scene_bytes = (int64_t)(b.nodes.size() * sizeof(bc_node8) + b.prims.size() * sizeof(bc_prim));
     c->has_scene = true;
+    return BC_OK;
+}
+
+// Material table and light distribution (rebuildLightDistribution, renderer.go:363-407) from
+// material data; also the cheap path for parameter updates of the differentiable render.
+int bc_update_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data) {
+    if (!c || nmat <= 0 || !mat_kind || !mat_data) return BC_ERR_ARG;
+    if (c->nmat_scene && nmat != c->nmat_scene) { c->err = "material count differs from the uploaded scene"; return BC_ERR_ARG; }
+    std::vector<bc_mat> mats((size_t)nmat);
+    for (int i = 0; i < nmat; i++) {
+        const double* d = mat_data + 8 * (size_t)i;
+        bc_mat& m = mats[(size_t)i];
+        std::memset(&m, 0, sizeof m);
+        m.color[0] = (float)d[0]; m.color[1] = (float)d[1]; m.color[2] = (float)d[2]; m.rough = (float)d[3];
+        m.ior = (float)d[4];
+        m.emission[0] = (float)d[5]; m.emission[1] = (float)d[6]; m.emission[2] = (float)d[7];
+        m.kind = mat_kind[i];
+    }
+    std::vector<bc_light> lights = c->h_lights;
+    const size_t nl = lights.size();
+    std::vector<float> pdf((size_t)c->nprim, 0.0f);
+    std::vector<int32_t> light_of((size_t)c->nprim, -1);
+    std::vector<double> cdf(nl);
+    double total = 0;
+    for (size_t i = 0; i < nl; i++) {
+        uint32_t mbits;
+        std::memcpy(&mbits, &lights[i].g1[3], 4);
+        if ((int)mbits >= nmat || mat_kind[mbits] != 3) { c->err = "a light's material is no longer emissive"; return BC_ERR_ARG; }
+        const double* md = mat_data + 8 * (size_t)mbits;
+        lights[i].le[0] = (float)md[5]; lights[i].le[1] = (float)md[6]; lights[i].le[2] = (float)md[7];
+        double wt = (0.2126 * md[5] + 0.7152 * md[6] + 0.0722 * md[7]) * c->h_light_area[i];
+        total += wt < 1e-9 ? 1e-9 : wt;
+        cdf[i] = total;
+    }
+    double prev = 0;
+    for (size_t i = 0; i < nl; i++) {
+        const double cv = cdf[i] / total;
+        pdf[(size_t)lights[i].prim] = (float)(cv - prev);
+        prev = cv;
+        cdf[i] = cv;
+    }
+    if (nl) cdf.back() = 1;
+    for (size_t i = 0; i < nl; i++) {
+        lights[i].cdf = (float)cdf[i];
+        light_of[(size_t)lights[i].prim] = (int32_t)i;
+    }
+    (void)hipSetDevice(c->device);
+    if (!upload_vec(c, c->d_mats, c->cap_mats, mats, "upload materials") ||
+        !upload_vec(c, c->d_lights, c->cap_lights, lights, "upload lights") ||
+        !upload_vec(c, c->d_pdf, c->cap_pdf, pdf, "upload light pdf") ||
+        !upload_vec(c, c->d_light_of, c->cap_light_of, light_of, "upload light map") ||
+        !ok(c, hipStreamSynchronize(c->stream), "material upload"))
+        return BC_ERR_DEVICE;
+    c->nmat = nmat;
+    c->nmat_scene = nmat;
+    c->nlight = (int)nl;
     return BC_OK;
 }
 

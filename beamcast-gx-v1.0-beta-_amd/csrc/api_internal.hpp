@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/beamcast_hip.h"
 #include "bc_layout.h"
@@ -26,12 +27,17 @@ struct bc_ctx {
     int32_t* d_light_of = nullptr;   // light index by primitive, -1 if none
     size_t cap_nodes = 0, cap_prims = 0, cap_mats = 0, cap_lights = 0, cap_pdf = 0, cap_light_of = 0;
     int64_t nnode8 = 0, scene_bytes = 0;
-    int depth8 = 0, nprim = 0, nmat = 0, nlight = 0, stack = 0;
+    int depth8 = 0, nprim = 0, nmat = 0, nlight = 0, stack = 0, nmat_scene = 0;
+    std::vector<bc_light> h_lights;  // light geometry (emission/CDF filled per material update)
+    std::vector<double> h_light_area;
     int* d_err = nullptr;
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;
     void* tmp_out = nullptr;
     size_t cap_in = 0, cap_out = 0;
+    // staging for host-pointer renders
+    void* out_buf = nullptr;
+    size_t out_cap = 0;
     // integrator
     RenderState* rs = nullptr;
 };

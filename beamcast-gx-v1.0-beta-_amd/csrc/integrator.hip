@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
         float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
         if (i < n) {
             j = q[i];
-            cont = shade_path(S, B, P, j, bounce, guides, shadow, so, sd, sc) && bounce + 1 < P.bounces;
+            cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
         }
         uint32_t slot = 0;
         wave_append(&B.cnt[cur ^ 1], cont, slot);
@@ -140,7 +140,8 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 
 // ------------------------------------------------------------------------------ accumulate
 // Per-pixel sum in sample order with clampFirefly (renderer.go:1917-1926, 1951-1953).
-__global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, PathBufs B, float4* __restrict__ acc) {
+__global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, PathBufs B, float4* __restrict__ acc,
+                                                       float4* __restrict__ keep) {
     const int stride = gridDim.x * kBlock;
     const int nloop = (P.npix + stride - 1) / stride * stride;
     for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < nloop; pl += stride) {
@@ -150,6 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, PathBufs 
             for (int sl = 0; sl < P.ns; sl++) {
                 const int j = sl * P.npix + pl;
                 const float4 l = B.L[j];
+                if (keep) keep[(size_t)(P.s0 + sl) * (size_t)P.npix + (size_t)pl] = l;
                 float cx = l.x, cy = l.y, cz = l.z;
                 if (P.firefly > 0.0f) {
                     const float lim = P.firefly;
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, con
         if (i < n) {
             j = q[i];
             const int bounce = (int)B.bnc[j];
-            cont = shade_path(S, B, P, j, bounce, guides, shadow, so, sd, sc) && bounce + 1 < P.bounces;
+            cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
             if (cont) B.bnc[j] = (uint32_t)(bounce + 1);
             done = !cont;
         }
@@ -322,12 +324,116 @@ __global__ __launch_bounds__(kBlock) void k_p_reduce(const RenderParams* __restr
     }
 }
 
+
+// ------------------------------------------------------------------------------ adjoint (path replay)
+// Backward pass of the differentiable render: every sample is replayed from its key with the
+// forward pass's total radiance, and d loss / d (Color, Emission) of each material is summed
+// per workgroup in LDS, then once per workgroup into the global gradient.
+constexpr int kAdjMaxMat = 1024;
+
+__device__ __forceinline__ void adj_lds_init(float* lds, int n) {
+    for (int i = threadIdx.x; i < n; i += kBlock) lds[i] = 0.0f;
+    __syncthreads();
+}
+__device__ __forceinline__ void adj_lds_flush(const float* lds, int n, float* __restrict__ grad) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        if (lds[i] != 0.0f) atomicAdd(&grad[i], lds[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen_adj(RenderParams P, PathBufs B, const float4* __restrict__ keep,
+                                                       float4* __restrict__ Lt) {
+    const int n = P.ns * P.npix;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        B.cnt[0] = (uint32_t)n;
+        B.cnt[1] = 0;
+        B.cnt[2] = 0;
+    }
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
+        const int sl = j / P.npix, pl = j - sl * P.npix;
+        init_camera_path(P, B, (uint32_t)j, pl, P.s0 + sl);
+        B.q[0][j] = (uint32_t)j;
+        Lt[j] = keep[(size_t)(P.s0 + sl) * (size_t)P.npix + (size_t)pl];
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade_adj(SceneBufs S, PathBufs B, RenderParams P, int cur, int bounce,
+                                                      uint32_t* __restrict__ guides, Adj A, float* __restrict__ grad) {
+    __shared__ float lds[2 * kAdjMaxMat * 3];
+    const int ng = P.nmat * 6;
+    adj_lds_init(lds, ng);
+    A.gcol = lds;
+    A.gle = lds + P.nmat * 3;
+    const uint32_t n = B.cnt[cur];
+    const uint32_t stride = gridDim.x * kBlock;
+    const uint32_t nloop = (n + stride - 1) / stride * stride;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
+        bool cont = false, shadow = false;
+        uint32_t j = 0;
+        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+        if (i < n) {
+            j = B.q[cur][i];
+            cont = shade_path<true>(S, B, P, j, bounce, guides, shadow, so, sd, sc, A) && bounce + 1 < P.bounces;
+        }
+        uint32_t slot = 0;
+        wave_append(&B.cnt[cur ^ 1], cont, slot);
+        if (cont) B.q[cur ^ 1][slot] = j;
+        uint32_t sslot = 0;
+        wave_append(&B.cnt[2], shadow, sslot);
+        if (shadow) {
+            B.sq[sslot] = j;
+            B.so[j] = so;
+            B.sd[j] = sd;
+            B.sc[j] = sc;
+        }
+    }
+    adj_lds_flush(lds, ng, grad);
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, RenderParams P, Adj A,
+                                                       float* __restrict__ grad) {
+    __shared__ uint2 stk[STACK * kBlock];
+    __shared__ float lds[kAdjMaxMat * 3];
+    const int ng = P.nmat * 3;
+    adj_lds_init(lds, ng);
+    A.gle = lds;
+    const uint32_t n = B.cnt[2];
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const uint32_t j = B.sq[i];
+        const float4 o = B.so[j], d = B.sd[j];
+        Ray r;
+        r.ox = o.x; r.oy = o.y; r.oz = o.z;
+        r.dx = d.x; r.dy = d.y; r.dz = d.z;
+        r.tmin = (float)1e-4;
+        const HitRec h = traverse<true, STACK>(S.nodes, S.prims, r, o.w, stk + threadIdx.x, kBlock, S.err);
+        if (h.id < 0) {
+            const float4 c = B.sc[j];
+            const float4 l = B.L[j];
+            B.L[j] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+            const uint32_t pl = B.meta[j].x - (uint32_t)P.y0 * (uint32_t)P.width;
+            const float3 g = mul(f3(A.dl[3 * pl], A.dl[3 * pl + 1], A.dl[3 * pl + 2]), A.inv_spp);
+            const float4 w = A.sg[j];
+            adj_emission(A, (int)__float_as_uint(w.w), g, xyz(w));
+        }
+    }
+    adj_lds_flush(lds, ng, grad + P.nmat * 3);
+}
+
 }  // namespace bcd
 
 // ------------------------------------------------------------------------------ host side
 struct RenderState {
     void* buf[24] = {};
     size_t bufcap[24] = {};
+    float4* sg = nullptr;
+    // differentiable render: per-sample radiance of the last bc_forward, its shape
+    void* keep = nullptr;
+    size_t keep_cap = 0;
+    bc_render_desc keep_desc{};
+    int keep_y0 = -1, keep_rows = 0;
+    uint64_t keep_topo = 0;
     bcd::PathBufs B{};
     float4* acc = nullptr;
     uint32_t* guide_tmp = nullptr;
@@ -368,6 +474,7 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
         {10, paths * 16}, {11, paths * 4},  {12, paths * 4}, {13, paths * 4},  {14, 64},
         {15, npix * 16 + (guides ? npix * 12 : 0) + 64},     {16, paths * 4},  {17, paths * 4},
         {18, window_samples * 16},                           {19, sizeof(bcd::RenderParams)},
+        {20, paths * 16},
     };
     for (const Want& x : w)
         if (!bc_ensure(c, R.buf[x.slot], R.bufcap[x.slot], std::max<size_t>(x.bytes, 64), "render buffers")) return false;
@@ -393,6 +500,7 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
     R.acc = (float4*)R.buf[15];
     R.guide_tmp = (uint32_t*)(R.acc + npix);
     R.Pd = (bcd::RenderParams*)R.buf[19];
+    R.sg = (float4*)R.buf[20];
     return true;
 }
 
@@ -450,7 +558,7 @@ int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, u
 
 // Plain wavefront schedule: passes of pixels x samples, three kernels per bounce.
 int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides,
-                 uint64_t* rays_out, uint64_t* shadow_out) {
+                 uint64_t* rays_out, uint64_t* shadow_out, float4* keep = nullptr) {
     const size_t npix = (size_t)rows * (size_t)d->width;
     const size_t max_paths = std::max<size_t>(npix, (size_t)1 << 22);
     int spp_pass = d->samples_per_pass > 0 ? d->samples_per_pass : (int)std::max<size_t>(1, max_paths / npix);
@@ -478,7 +586,7 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
             default: launch_bounce<32>(S, R.B, P, cur, b, guides, grid_paths, st); break;
             }
         }
-        hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, R.B, R.acc);
+        hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, R.B, R.acc, keep);
         if (!bc_check(c, hipGetLastError(), "render launch")) return BC_ERR_DEVICE;
     }
     return finish(c, P, d_rgba, grid_pix, rays_out, shadow_out);
@@ -603,6 +711,7 @@ bool autotune(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgb
 void bc_render_release(bc_ctx* c) {
     if (!c || !c->rs) return;
     if (c->rs->exec) (void)hipGraphExecDestroy(c->rs->exec);
+    if (c->rs->keep) (void)hipFree(c->rs->keep);
     for (void* p : c->rs->buf)
         if (p) (void)hipFree(p);
     if (c->rs->h_cnt) (void)hipHostFree(c->rs->h_cnt);
@@ -645,16 +754,120 @@ extern "C" int bc_render(bc_ctx* c, const bc_render_desc* d, int y0, int rows, f
     if (cancel && *cancel) { c->err = "render cancelled"; return BC_ERR_CANCELLED; }
     if (d->width <= 0 || rows <= 0) { c->err = "invalid render description or row range"; return BC_ERR_ARG; }
     const size_t npix = (size_t)rows * (size_t)d->width;
-    void* out = nullptr;
-    size_t cap = 0;
-    if (!bc_ensure(c, out, cap, npix * 16 + (guides ? npix * 12 : 0) + 64, "render output")) return BC_ERR_NOMEM;
-    uint32_t* dg = guides ? (uint32_t*)((char*)out + npix * 16) : nullptr;
-    int rc = bc_render_device(c, d, y0, rows, (float*)out, dg, rays_out, nullptr);
+    // output staging kept in the context, so repeated chunk renders reuse one buffer (and a
+    // captured graph keyed on it)
+    if (!bc_ensure(c, c->out_buf, c->out_cap, npix * 28 + 64, "render output")) return BC_ERR_NOMEM;
+    uint32_t* dg = guides ? (uint32_t*)((char*)c->out_buf + npix * 16) : nullptr;
+    int rc = bc_render_device(c, d, y0, rows, (float*)c->out_buf, dg, rays_out, nullptr);
     if (rc == BC_OK) {
-        if (!bc_check(c, hipMemcpy(rgba, out, npix * 16, hipMemcpyDeviceToHost), "copy image") ||
+        if (!bc_check(c, hipMemcpy(rgba, c->out_buf, npix * 16, hipMemcpyDeviceToHost), "copy image") ||
             (guides && !bc_check(c, hipMemcpy(guides, dg, npix * 12, hipMemcpyDeviceToHost), "copy guides")))
             rc = BC_ERR_DEVICE;
     }
-    (void)hipFree(out);
     return rc;
+}
+
+namespace {
+template <int STACK>
+void launch_bounce_adj(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur,
+                       int bounce, uint32_t* guides, const bcd::Adj& A, float* grad, int grid, hipStream_t st) {
+    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur);
+    hipLaunchKernelGGL(bcd::k_shade_adj, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, A, grad);
+    hipLaunchKernelGGL((bcd::k_shadow_adj<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, A, grad);
+}
+bool same_shape(const bc_render_desc& a, const bc_render_desc& b) {
+    return a.width == b.width && a.height == b.height && a.spp == b.spp && a.bounces == b.bounces &&
+           a.rr_depth == b.rr_depth && a.sampler == b.sampler && a.mis == b.mis && a.power_lights == b.power_lights &&
+           a.sky_mode == b.sky_mode && a.seed == b.seed && std::memcmp(a.origin, b.origin, sizeof a.origin) == 0 &&
+           std::memcmp(a.lower, b.lower, sizeof a.lower) == 0 && std::memcmp(a.horiz, b.horiz, sizeof a.horiz) == 0 &&
+           std::memcmp(a.vert, b.vert, sizeof a.vert) == 0;
+}
+}  // namespace
+
+extern "C" int bc_forward(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rgba, uint64_t* rays_out) {
+    if (!c || !d || !rgba) return BC_ERR_ARG;
+    if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
+    if (d->firefly_clamp > 0.0f) { c->err = "the differentiable render needs firefly_clamp <= 0"; return BC_ERR_ARG; }
+    if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->spp > 65536 || d->bounces < 0 || d->bounces > 1024 ||
+        y0 < 0 || rows <= 0 || y0 + rows > d->height || d->sampler < 0 || d->sampler > 2) {
+        c->err = "invalid render description or row range";
+        return BC_ERR_ARG;
+    }
+    if (c->nmat > bcd::kAdjMaxMat) { c->err = "the differentiable render supports at most 1024 materials"; return BC_ERR_ARG; }
+    (void)hipSetDevice(c->device);
+    const size_t npix = (size_t)rows * (size_t)d->width;
+    if (!ensure_state(c, 1, npix, false, 1)) return BC_ERR_NOMEM;
+    RenderState& R = *c->rs;
+    if (!bc_ensure(c, R.keep, R.keep_cap, npix * (size_t)d->spp * 16, "forward sample buffer")) return BC_ERR_NOMEM;
+    if (!bc_ensure(c, c->out_buf, c->out_cap, npix * 28 + 64, "render output")) return BC_ERR_NOMEM;
+    int rc = render_plain(c, d, y0, rows, (float*)c->out_buf, nullptr, rays_out, nullptr, (float4*)R.keep);
+    if (rc != BC_OK) return rc;
+    if (!bc_check(c, hipMemcpy(rgba, c->out_buf, npix * 16, hipMemcpyDeviceToHost), "copy image")) return BC_ERR_DEVICE;
+    R.keep_desc = *d;
+    R.keep_y0 = y0;
+    R.keep_rows = rows;
+    R.keep_topo = c->topo;
+    return BC_OK;
+}
+
+extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows, const float* dl, float* dcolor,
+                           float* demission) {
+    if (!c || !d || !dl || !dcolor || !demission) return BC_ERR_ARG;
+    RenderState* Rp = c->rs;
+    if (!Rp || !Rp->keep || Rp->keep_y0 != y0 || Rp->keep_rows != rows || Rp->keep_topo != c->topo ||
+        !same_shape(Rp->keep_desc, *d)) {
+        c->err = "bc_backward needs a bc_forward of the same description and rows first";
+        return BC_ERR_ARG;
+    }
+    (void)hipSetDevice(c->device);
+    const size_t npix = (size_t)rows * (size_t)d->width;
+    const size_t max_paths = std::max<size_t>(npix, (size_t)1 << 22);
+    const int spp_pass = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, max_paths / npix));
+    const size_t paths = npix * (size_t)spp_pass;
+    if (!ensure_state(c, paths, npix, false, paths)) return BC_ERR_NOMEM;
+    RenderState& R = *c->rs;
+    const size_t ng = (size_t)c->nmat * 6;
+    if (!bc_ensure(c, c->out_buf, c->out_cap, std::max(npix * 28 + 64, npix * 12 + ng * 4 + 64), "adjoint buffers"))
+        return BC_ERR_NOMEM;
+    float* d_dl = (float*)c->out_buf;
+    float* d_grad = d_dl + npix * 3;
+    hipStream_t st = c->stream;
+    if (!bc_check(c, hipMemcpyAsync(d_dl, dl, npix * 12, hipMemcpyHostToDevice, st), "upload d loss") ||
+        !bc_check(c, hipMemsetAsync(d_grad, 0, ng * 4, st), "clear gradient") ||
+        !bc_check(c, hipMemsetAsync(R.B.totals, 0, 16, st), "clear totals"))
+        return BC_ERR_DEVICE;
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::RenderParams P = make_params(c, d, y0, rows, false);
+    bcd::Adj A{};
+    A.dl = d_dl;
+    A.inv_spp = 1.0f / (float)d->spp;
+    A.Lt = R.B.res;
+    A.sg = R.sg;
+    const int grid = (int)std::min<size_t>((paths + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
+    for (int s0 = 0; s0 < d->spp; s0 += spp_pass) {
+        P.s0 = s0;
+        P.ns = std::min(spp_pass, d->spp - s0);
+        hipLaunchKernelGGL(bcd::k_raygen_adj, dim3(grid), dim3(bcd::kBlock), 0, st, P, R.B, (const float4*)R.keep,
+                           R.B.res);
+        for (int b = 0; b < d->bounces; b++) {
+            const int cur = b & 1;
+            switch (c->stack) {
+            case 8: launch_bounce_adj<8>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
+            case 16: launch_bounce_adj<16>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
+            case 24: launch_bounce_adj<24>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
+            default: launch_bounce_adj<32>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
+            }
+        }
+        if (!bc_check(c, hipGetLastError(), "adjoint launch")) return BC_ERR_DEVICE;
+    }
+    std::vector<float> g(ng);
+    if (!bc_check(c, hipMemcpyAsync(g.data(), d_grad, ng * 4, hipMemcpyDeviceToHost, st), "read gradient") ||
+        !bc_check(c, hipStreamSynchronize(st), "backward"))
+        return BC_ERR_DEVICE;
+    const size_t n3 = (size_t)c->nmat * 3;
+    for (size_t i = 0; i < n3; i++) {
+        dcolor[i] += g[i];
+        demission[i] += g[n3 + i];
+    }
+    return bc_check_kernel_error(c);
 }

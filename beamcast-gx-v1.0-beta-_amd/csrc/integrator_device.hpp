@@ -203,11 +203,41 @@ __device__ __forceinline__ float light_dir_pdf(const SceneBufs& S, const RenderP
 }
 
 
+// Adjoint state of the path-replay backward pass: per-pixel d loss / d image, the replayed
+// path's total radiance from the forward pass, and per-block gradient accumulators in LDS.
+struct Adj {
+    const float* dl;      // rows*W*3 d loss / d image (chunk pixels)
+    float inv_spp;
+    const float4* Lt;     // per path slot: total radiance of this sample (forward pass)
+    float4* sg;           // per path slot: d NEE contribution / d emission (rgb), light material
+    float* gcol;          // LDS: nmat*3 d loss / d Color
+    float* gle;           // LDS: nmat*3 d loss / d Emission
+};
+
+// d loss / d color at a scattering vertex: every later contribution (this vertex's NEE term
+// included) carries the factor color, so its derivative is the remaining radiance / color.
+__device__ __forceinline__ void adj_color(const Adj& A, int mat, float3 g, float3 Lt, float3 L, float3 color) {
+    const float r[3] = {Lt.x - L.x, Lt.y - L.y, Lt.z - L.z};
+    const float c[3] = {color.x, color.y, color.z};
+    const float gg[3] = {g.x, g.y, g.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        if (c[k] != 0.0f && gg[k] != 0.0f && r[k] != 0.0f) atomicAdd(&A.gcol[mat * 3 + k], gg[k] * (r[k] / c[k]));
+}
+__device__ __forceinline__ void adj_emission(const Adj& A, int mat, float3 g, float3 w) {
+    if (g.x != 0.0f && w.x != 0.0f) atomicAdd(&A.gle[mat * 3 + 0], g.x * w.x);
+    if (g.y != 0.0f && w.y != 0.0f) atomicAdd(&A.gle[mat * 3 + 1], g.y * w.y);
+    if (g.z != 0.0f && w.z != 0.0f) atomicAdd(&A.gle[mat * 3 + 2], g.z * w.z);
+}
+
 // One shading step of trace for path j at `bounce` (renderer.go:1490-1560).  Returns true when
 // the path continues (a new ray is stored); an NEE shadow ray, if any, is returned in so/sd/sc.
+// ADJ adds the path-replay gradient of the step (Russian roulette and light selection are
+// treated as constants of the parameters).
+template <bool ADJ>
 __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B, const RenderParams& P, uint32_t j,
                                            int bounce, uint32_t* __restrict__ guides, bool& shadow,
-                                           float4& shadow_o, float4& shadow_d, float4& shadow_c) {
+                                           float4& shadow_o, float4& shadow_d, float4& shadow_c, const Adj& A) {
     const uint32_t dim = BC_DIM_BOUNCE0 + BC_DIMS_PER_BOUNCE * (uint32_t)bounce;
     const int rr = P.rr_depth < 1 ? 4 : P.rr_depth;
     bool alive_out = false;
@@ -220,6 +250,12 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
     float3 L = xyz(B.L[j]);
     const uint4 meta = B.meta[j];
     const uint32_t key = meta.z;
+    float3 g = f3(0.0f, 0.0f, 0.0f), Lt = g;
+    if (ADJ) {
+        const uint32_t pl = meta.x - (uint32_t)P.y0 * (uint32_t)P.width;
+        g = mul(f3(A.dl[3 * pl], A.dl[3 * pl + 1], A.dl[3 * pl + 2]), A.inv_spp);
+        Lt = xyz(A.Lt[j]);
+    }
     if (rec == 0xffffffffu) {  // background, renderer.go:1492-1495 / :1326-1331
         float3 bg;
         if (P.sky_mode == 1) {
@@ -267,8 +303,10 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
             }
             L = add(L, mul(had(T, f3(M.emission[0], M.emission[1], M.emission[2])), w));
             B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
+            if (ADJ) adj_emission(A, mat, g, mul(T, w));
             alive = false;
         } else if (M.kind == 0) {  // diffuse: NEE then cosine sampling
+            if (ADJ) adj_color(A, mat, g, Lt, L, color);
             if (P.nlight > 0) {   // directLight, renderer.go:1427-1465
                 const float u = bc_rng(key, dim + BC_D_LIGHT);
                 int li;
@@ -330,6 +368,7 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
                             const float3 f = mul(color, INVPI_F);
                             const float3 dl = mul(had(f, f3(Lr.le[0], Lr.le[1], Lr.le[2])), nd * w / lightPDF);
                             const float3 c = had(T, dl);  // throughput before the albedo update
+                            const float3 wle = mul(had(T, f), nd * w / lightPDF);  // d c / d emission
                             // visible(), renderer.go:1333-1341
                             const float3 from = add(p, mul(N, (float)2e-4));
                             const float3 dd = sub(qpt, from);
@@ -337,7 +376,9 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
                             if (dd2 < (float)1e-12) {
                                 L = add(L, c);
                                 B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
+                                if (ADJ) adj_emission(A, (int)__float_as_uint(Lr.g1[3]), g, wle);
                             } else {
+                                if (ADJ) A.sg[j] = make_float4(wle.x, wle.y, wle.z, Lr.g1[3]);
                                 const float dist = sqrtf(dd2);
                                 const float3 dir = mul(dd, 1.0f / dist);
                                 shadow_o = make_float4(from.x, from.y, from.z, dist - (float)1e-4);
@@ -385,6 +426,7 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
             if (dot(nd, N) <= 0.0f) {
                 alive = false;
             } else {
+                if (ADJ) adj_color(A, mat, g, Lt, L, color);
                 B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                 T = had(T, color);
                 ro = add(p, mul(N, (float)2e-4));
@@ -410,6 +452,7 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
                 const float3 par = mul(N, -sqrtf(fabsf(1.0f - dot(perp, perp))));
                 nd = add(perp, par);
             }
+            if (ADJ) adj_color(A, mat, g, Lt, L, color);
             B.pp[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             T = had(T, color);
             ro = add(p, mul(nd, (float)2e-4));

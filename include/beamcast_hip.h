@@ -60,6 +60,11 @@ typedef struct bc_scene_view {
  * A change of topo_rev rebuilds the BVH8 from the binary BVH; a change of geom_rev alone
  * re-quantises the existing BVH8 topology from the new boxes. */
 int bc_scene_upload(bc_ctx* ctx, const bc_scene_view* scene, uint64_t topo_rev, uint64_t geom_rev);
+/* Replaces the material part of uploadFullSceneLocked (gpu_opencl_windows.go:384-396) and the
+ * light distribution of rebuildLightDistribution (renderer.go:363-407) without touching the
+ * BVH: new Color/Roughness/IOR/Emission for the uploaded scene (same material count and kinds
+ * of the lights).  The differentiable render updates its parameters through this call. */
+int bc_update_materials(bc_ctx* ctx, int32_t nmat, const int32_t* mat_kind, const double* mat_data);
 /* Statistics of the uploaded scene: nodes8, depth8, prims, bytes resident. */
 int bc_scene_stats(const bc_ctx* ctx, int64_t* out4);
 
@@ -99,6 +104,16 @@ int bc_render(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* 
  * graph, both 0 = chosen by the autotuner on first use for this image shape and scene. */
 int bc_render_device(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* d_rgba,
                      uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_rays_out);
+
+/* Differentiable render (path-replay backpropagation; no counterpart in the reference).
+ * bc_forward renders like bc_render (plain schedule, host rgba) and keeps each sample's path
+ * radiance; bc_backward replays every sample from its key and accumulates (+=) the gradient of
+ * the loss wrt every material's Color (nmat x 3) and Emission (nmat x 3), given
+ * dloss_dimage (rows*W*3 floats, d loss / d mean pixel radiance).  Russian roulette and the
+ * power-weighted light selection are held fixed in the derivative; firefly_clamp must be <= 0. */
+int bc_forward(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* rgba, uint64_t* rays_out);
+int bc_backward(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, const float* dloss_dimage,
+                float* dcolor, float* demission);
 
 /* Schedule last used by a mode-1 render: pool slots, iterations per graph, traversal stack
  * depth, sample windows.  No counterpart in the reference (its kernel has one fixed shape). */
