@@ -79,6 +79,7 @@ def lib():
             "bc_free": (c_int, [vp, vp]),
             "bc_upload": (c_int, [vp, vp, vp, c_uint64]),
             "bc_download": (c_int, [vp, vp, vp, c_uint64]),
+            "bc_copy_device": (c_int, [vp, vp, vp, c_uint64]),
             "bc_synchronize": (c_int, [vp]),
             "bc_timer_start": (c_int, [vp]),
             "bc_timer_stop": (c_int, [vp, POINTER(c_float)]),
@@ -285,6 +286,9 @@ class Device:
     def download(self, p, out):
         self._check(lib().bc_download(self.ctx, _ptr(out), p, out.nbytes))
         return out
+
+    def copy_device(self, d_dst, d_src, nbytes):
+        self._check(lib().bc_copy_device(self.ctx, d_dst, d_src, nbytes))
 
     def trace_device(self, d_rays, n, any_hit, d_out):
         self._check(lib().bc_trace_device(self.ctx, d_rays, n, 1 if any_hit else 0, d_out))

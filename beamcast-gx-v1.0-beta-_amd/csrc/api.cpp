@@ -128,7 +128,6 @@ const char* bc_device_name(const bc_ctx* c) { return c ? c->name.c_str() : ""; }
 
 int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t geom) {
     if (!c || !v) return BC_ERR_ARG;
-    if (c->has_scene && topo == c->topo && geom == c->geom) return BC_OK;
     if (v->nmat <= 0) { c->err = "HIP backend requires at least one material"; return BC_ERR_SCENE; }
     hipSetDevice(c->device);
     bcx::Bvh8 b;
@@ -301,6 +300,11 @@ int bc_download(bc_ctx* c, void* d, const void* s, uint64_t bytes) {
         !ok(c, hipStreamSynchronize(c->stream), "bc_download"))
         return BC_ERR_DEVICE;
     return bc_check_kernel_error(c);
+}
+int bc_copy_device(bc_ctx* c, void* d_dst, const void* d_src, uint64_t bytes) {
+    if (!c) return BC_ERR_ARG;
+    return ok(c, hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, c->stream), "bc_copy_device") ? BC_OK
+                                                                                                          : BC_ERR_DEVICE;
 }
 int bc_synchronize(bc_ctx* c) {
     if (!c) return BC_ERR_ARG;

@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-render", action="store_true", help="skip the closed-box render measurement")
+    ap.add_argument("--copy-check", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this backend runs on one GPU; the multi-GPU split (per-tile or per-sample "
@@ -134,6 +136,15 @@ def main():
             sets[(sname, any_hit)] = gen_rays(bc, set_id, args.width, any_hit, threads)
     n = args.width * args.width
     log(f"[bench] generated 4 x {n} rays: {time.time() - t0:.2f}s")
+
+    if args.copy_check:
+        gib = 1 << 30
+        a, b = dev.alloc(gib), dev.alloc(gib)
+        dev.copy_device(b, a, gib)
+        dev.sync()
+        dev.free(a)
+        dev.free(b)
+        log("[bench] copied 1 GiB device to device (counter calibration)")
 
     d_rays, d_out = {}, {}
     for k, r in sets.items():

@@ -57,8 +57,9 @@ typedef struct bc_scene_view {
 
 /* Replaces ensureSceneLocked / uploadFullSceneLocked / updateGeometryLocked
  * (gpu_opencl_windows.go:379-495) including validateGPUSceneEncoding (:470-482).
- * A change of topo_rev rebuilds the BVH8 from the binary BVH; a change of geom_rev alone
- * re-quantises the existing BVH8 topology from the new boxes. */
+ * Every call flattens and uploads the given scene; the caller decides when to call, keyed by
+ * (scene, topo_rev, geom_rev) as ensureSceneLocked does.  The revisions are recorded with the
+ * scene (bc_backward checks that the scene did not change since bc_forward). */
 int bc_scene_upload(bc_ctx* ctx, const bc_scene_view* scene, uint64_t topo_rev, uint64_t geom_rev);
 /* Replaces the material part of uploadFullSceneLocked (gpu_opencl_windows.go:384-396) and the
  * light distribution of rebuildLightDistribution (renderer.go:363-407) without touching the
@@ -132,6 +133,7 @@ void* bc_alloc(bc_ctx* ctx, uint64_t bytes);
 int bc_free(bc_ctx* ctx, void* d_ptr);
 int bc_upload(bc_ctx* ctx, void* d_dst, const void* src, uint64_t bytes);
 int bc_download(bc_ctx* ctx, void* dst, const void* d_src, uint64_t bytes);
+int bc_copy_device(bc_ctx* ctx, void* d_dst, const void* d_src, uint64_t bytes);  /* stream-ordered */
 int bc_synchronize(bc_ctx* ctx);
 /* Stream-ordered timing: record a start marker, then read the elapsed ms since it. */
 int bc_timer_start(bc_ctx* ctx);

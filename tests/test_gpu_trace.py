@@ -37,13 +37,33 @@ def check_closest(dev, host, rays, fp64_check=True):
     assert bad.size == 0, f"{bad.size} rays differ from the fp32 oracle, first {bad[:5]}"
     if fp64_check:
         p64, t64 = o.hit64(rays.astype(np.float64))
-        mism = got["prim"] != p64
-        assert mism.mean() <= 1e-4
-        m = mism & (p64 >= 0) & (got["prim"] >= 0)
-        if m.any():
-            rel = np.abs(got["t"][m] - t64[m]) / t64[m]
-            assert rel.max() < 1e-5
+        mism = np.nonzero(got["prim"] != p64)[0]
+        assert mism.size <= 1e-4 * len(rays), f"{mism.size} id mismatches vs fp64"
+        kind, _, geom, _, _ = host.export()
+        for i in mism:
+            # allowed: equal-distance ties at shared edges (|dt|/t < 1e-5), or a ray that grazes
+            # an edge of one of the two primitives within single-precision rounding
+            if got["prim"][i] >= 0 and p64[i] >= 0 and abs(got["t"][i] - t64[i]) / t64[i] < 1e-5:
+                continue
+            cands = [p for p in (got["prim"][i], p64[i]) if p >= 0]
+            assert any(kind[p] == bc.TRIANGLE and edge_distance(rays[i], geom[p]) < 1e-5 for p in cands), \
+                f"ray {i}: gpu {got['prim'][i]} t={got['t'][i]} vs fp64 {p64[i]} t={t64[i]}"
     return got
+
+
+def edge_distance(ray, abc):
+    """Smallest barycentric distance of the ray's crossing of the triangle plane from an edge."""
+    o, d = ray[:3].astype(np.float64), ray[4:7].astype(np.float64)
+    a, b, c = abc[0:3], abc[3:6], abc[6:9]
+    e1, e2 = b - a, c - a
+    h = np.cross(d, e2)
+    det = e1 @ h
+    if abs(det) < 1e-300:
+        return 0.0
+    s = o - a
+    u = (s @ h) / det
+    v = (d @ np.cross(s, e1)) / det
+    return float(min(abs(u), abs(v), abs(1 - u - v), abs(1 - u), abs(1 - v)))
 
 
 def test_soup_closest_and_any(dev):
