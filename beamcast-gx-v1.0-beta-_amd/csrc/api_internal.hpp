@@ -30,7 +30,8 @@ struct bc_ctx {
     int depth8 = 0, nprim = 0, nmat = 0, nlight = 0, stack = 0, nmat_scene = 0;
     std::vector<bc_light> h_lights;  // light geometry (emission/CDF filled per material update)
     std::vector<double> h_light_area;
-    int* d_err = nullptr;
+    int* d_err = nullptr;  // [0] kernel error flag, bytes 8-15: persistent ray counter
+    int trace_variant = 0, trace_blocks_per_cu = 0;
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;
     void* tmp_out = nullptr;

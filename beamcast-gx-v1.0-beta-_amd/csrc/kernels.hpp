@@ -17,4 +17,10 @@ int stack_for_depth(int depth);
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const bc_ray* rays,
                         int64_t n, void* out, int* err, hipStream_t st);
 
+// Persistent ray queries (variant 1: all pending primitives per step, 2: one per step);
+// `counter` is a device u64 the launch resets.
+hipError_t launch_trace_persistent(int variant, int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims,
+                                   const bc_ray* rays, int64_t n, void* out, int* err, unsigned long long* counter,
+                                   int blocks, hipStream_t st);
+
 }  // namespace bcd

@@ -226,4 +226,99 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
     return best;
 }
 
+
+// ------------------------------------------------------------------------------ stepwise form
+// The same traversal as a resumable state machine, for persistent kernels that refill a
+// lane with a new ray as soon as its ray finishes (instead of waiting for the wave's
+// slowest ray).  PPS = primitives tested per step (0 = all pending primitives).
+struct TravState {
+    Ray r;
+    float ix, iy, iz, tmax;
+    uint32_t oct;
+    HitRec best;
+    uint32_t ng_base, ng_bits, pg_base, pg_mask, node;
+    int sp;
+    bool have_node;
+};
+
+__device__ __forceinline__ void trav_init(TravState& s, const Ray& r, float tmax) {
+    s.r = r;
+    s.ix = safe_rcp(r.dx);
+    s.iy = safe_rcp(r.dy);
+    s.iz = safe_rcp(r.dz);
+    s.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+    s.tmax = tmax;
+    s.best.t = tmax;
+    s.best.id = -1;
+    s.best.rec = 0xffffffffu;
+    s.ng_base = 0;
+    s.ng_bits = 0;
+    s.pg_base = 0;
+    s.pg_mask = 0;
+    s.node = 0;
+    s.sp = 0;
+    s.have_node = true;
+}
+
+// One step; returns true when the ray is finished (result in s.best).
+template <bool ANY, int STACK, int PPS>
+__device__ __forceinline__ bool trav_step(TravState& s, const bc_node8* __restrict__ nodes,
+                                          const bc_prim* __restrict__ prims, uint2* stk, int stride,
+                                          int* __restrict__ err) {
+    if (s.have_node) {
+        visit_node(nodes, s.node, s.r, s.ix, s.iy, s.iz, s.oct, s.best.t, s.ng_base, s.ng_bits, s.pg_base, s.pg_mask);
+        s.have_node = false;
+    }
+    int tested = 0;
+    while (s.pg_mask && (PPS == 0 || tested < PPS)) {
+        tested++;
+        const uint32_t k = (uint32_t)__builtin_ctz(s.pg_mask);
+        s.pg_mask &= s.pg_mask - 1u;
+        const uint32_t ri = s.pg_base + k;
+        const float4* pq = reinterpret_cast<const float4*>(prims + ri);
+        const float4 q0 = pq[0], q1 = pq[1], q2 = pq[2];
+        const uint32_t idbits = __float_as_uint(q2.y);
+        float t;
+        bool h;
+        if (idbits & BC_SPHERE_BIT) h = sphere_test(q0, q1, s.r, s.r.tmin, s.best.t, t);
+        else h = tri_test(q0, q1, q2, s.r, s.r.tmin, s.best.t, t);
+        if (h) {
+            const int32_t id = (int32_t)(idbits & ~BC_SPHERE_BIT);
+            if (ANY) {
+                s.best.t = t;
+                s.best.id = id;
+                s.best.rec = ri;
+                return true;
+            }
+            if (s.best.id < 0 || t < s.best.t || id < s.best.id) {
+                s.best.t = t;
+                s.best.id = id;
+                s.best.rec = ri;
+            }
+        }
+    }
+    if (s.pg_mask) return false;
+    if (!(s.ng_bits & 0xffu)) {
+        if (s.sp == 0) return true;
+        s.sp--;
+        const uint2 e = stk[s.sp * stride];
+        s.ng_base = e.x;
+        s.ng_bits = e.y;
+    }
+    const uint32_t k = (uint32_t)__builtin_ctz(s.ng_bits & 0xffu);
+    s.ng_bits &= ~(1u << k);
+    const uint32_t slot = k ^ s.oct;
+    s.node = s.ng_base + (uint32_t)__builtin_popcount((s.ng_bits >> 8) & ((1u << slot) - 1u));
+    if (s.ng_bits & 0xffu) {
+        if (s.sp >= STACK) {
+            atomicOr(err, 1);
+            return true;
+        }
+        stk[s.sp * stride] = make_uint2(s.ng_base, s.ng_bits);
+        s.sp++;
+    }
+    s.have_node = true;
+    return false;
+}
+
 }  // namespace bcd

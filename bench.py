@@ -76,6 +76,19 @@ def bench_render(bc, dev, args, threads):
            "ms_per_frame": round(sec * 1e3, 3), "path_samples_per_s": round(samples / sec, 1),
            "mrays_s_segments": round(segs / sec / 1e6, 2), "mrays_s_total": round((segs + shadow) / sec / 1e6, 2),
            "segments_per_sample": round(segs / samples, 4)}
+    # batched-persistent schedule (in-place regeneration, hipGraph replay, autotuned shape)
+    dp = bc.render_desc(box, w, h, spp, bounces, rr_depth=3, sampler=1, mis=True, power=True, mode=1)
+    d_rgba = dev.alloc(w * h * 16)
+    d_g = dev.alloc(w * h * 12)
+    dev.render_device(dp, 0, h, d_rgba, d_g)  # autotunes and captures once
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dev.render_device(dp, 0, h, d_rgba, d_g)
+    psec = (time.perf_counter() - t0) / reps
+    dev.free(d_rgba)
+    dev.free(d_g)
+    out["persistent"] = {"ms_per_frame": round(psec * 1e3, 3), "path_samples_per_s": round(samples / psec, 1),
+                         "over_plain": round(sec / psec, 3), "tuning": dev.tuning()}
     if not args.no_cpu:
         import oracle as orc
 
@@ -90,6 +103,37 @@ def bench_render(bc, dev, args, threads):
                                "sample": f"rows 120-{120 + rows} of the same frame, fp64 trace restatement"}
         out["gpu_over_cpu"] = round(samples / sec / cpu_rate, 1)
     return out
+
+
+def bench_adjoint(bc, dev):
+    """Differentiable-render config: closed box retessellated to ~200k triangles with 64 materials,
+    512x512, 32 spp, 6 segments; time of one forward and one path-replay backward pass."""
+    s = bc.HostScene.workload(bc.BOX_RETESS, param=200_000)
+    kind, mat, geom, mk, md = s.export()
+    dev.upload(s, nee_triangles=True)
+    d = bc.render_desc(s, 512, 512, 32, 6, rr_depth=6, sampler=1, mis=True, power=True, seed=7)
+    target, _ = dev.forward(d)
+    md2 = md.copy()
+    md2[mk == bc.DIFFUSE, 0:3] = 0.5
+    md2[mk == bc.EMISSIVE, 5:8] *= 0.5
+    dev.update_materials(mk, md2)
+    img, _ = dev.forward(d)
+    dl = 2 * (img[..., :3] - target[..., :3])
+    dev.backward(d, dl, len(mk))
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        img, _ = dev.forward(d)
+    fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dev.backward(d, dl, len(mk))
+    bwd = (time.perf_counter() - t0) / reps
+    return {"workload": f"closed box retessellated ({s.counts()[0]} tris, {len(mk)} materials), 512x512, 32 spp, "
+                        "6 segments, L2 image loss",
+            "forward_ms": round(fwd * 1e3, 2), "backward_ms": round(bwd * 1e3, 2),
+            "adjoint_over_forward": round(bwd / fwd, 3),
+            "forward_path_samples_per_s": round(512 * 512 * 32 / fwd, 1)}
 
 
 def host_cores():
@@ -229,6 +273,7 @@ def main():
         log(f"[bench] CPU baseline: {cpu_rate:.2f} Mrays/s on {threads} threads ({secs:.1f}s)")
 
     render = None if args.no_render else bench_render(bc, dev, args, threads)
+    adjoint = None if args.no_render else bench_adjoint(bc, dev)
 
     names = {("coherent", False): "coherent_closest", ("coherent", True): "coherent_any",
              ("incoherent", False): "incoherent_closest", ("incoherent", True): "incoherent_any"}
@@ -266,6 +311,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "render": render,
+        "adjoint": adjoint,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
                                "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4)}
                     for k in launches},

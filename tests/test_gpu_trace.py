@@ -46,9 +46,22 @@ def check_closest(dev, host, rays, fp64_check=True):
             if got["prim"][i] >= 0 and p64[i] >= 0 and abs(got["t"][i] - t64[i]) / t64[i] < 1e-5:
                 continue
             cands = [p for p in (got["prim"][i], p64[i]) if p >= 0]
-            assert any(kind[p] == bc.TRIANGLE and edge_distance(rays[i], geom[p]) < 1e-5 for p in cands), \
-                f"ray {i}: gpu {got['prim'][i]} t={got['t'][i]} vs fp64 {p64[i]} t={t64[i]}"
+            assert any(kind[p] == bc.TRIANGLE and edge_distance(rays[i], geom[p]) < fp32_edge_tolerance(rays[i], geom[p])
+                       for p in cands), f"ray {i}: gpu {got['prim'][i]} t={got['t'][i]} vs fp64 {p64[i]} t={t64[i]}"
     return got
+
+
+def fp32_edge_tolerance(ray, abc):
+    """Barycentric uncertainty of the single-precision test: vertices and origin are rounded to
+    fp32 (absolute error ~ 2^-24 |coord|), which moves the barycentrics by that error over the
+    shortest edge, amplified by 1/sin of the ray's angle to the triangle plane."""
+    o, d = ray[:3].astype(np.float64), ray[4:7].astype(np.float64)
+    a, b, c = abc[0:3], abc[3:6], abc[6:9]
+    e1, e2 = b - a, c - a
+    det = abs(e1 @ np.cross(d, e2)) / (np.linalg.norm(e1) * np.linalg.norm(e2) * np.linalg.norm(d))
+    scale = max(np.abs(abc).max(), np.abs(o).max())
+    edge = min(np.linalg.norm(e1), np.linalg.norm(e2), np.linalg.norm(e2 - e1))
+    return 8 * 2.0**-23 * scale / edge / max(det, 1e-12)
 
 
 def edge_distance(ray, abc):
