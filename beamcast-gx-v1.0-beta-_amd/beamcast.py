@@ -80,7 +80,6 @@ def lib():
             "bc_upload": (c_int, [vp, vp, vp, c_uint64]),
             "bc_download": (c_int, [vp, vp, vp, c_uint64]),
             "bc_copy_device": (c_int, [vp, vp, vp, c_uint64]),
-            "bc_set_trace_schedule": (c_int, [vp, c_int32, c_int32]),
             "bc_synchronize": (c_int, [vp]),
             "bc_timer_start": (c_int, [vp]),
             "bc_timer_stop": (c_int, [vp, POINTER(c_float)]),
@@ -287,9 +286,6 @@ class Device:
     def download(self, p, out):
         self._check(lib().bc_download(self.ctx, _ptr(out), p, out.nbytes))
         return out
-
-    def set_trace_schedule(self, variant, blocks_per_cu=0):
-        self._check(lib().bc_set_trace_schedule(self.ctx, variant, blocks_per_cu))
 
     def copy_device(self, d_dst, d_src, nbytes):
         self._check(lib().bc_copy_device(self.ctx, d_dst, d_src, nbytes))

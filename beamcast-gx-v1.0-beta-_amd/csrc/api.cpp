@@ -112,7 +112,7 @@ void bc_destroy(bc_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    void* ptrs[] = {c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+    void* ptrs[] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
                     c->tmp_in, c->tmp_out, c->out_buf};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -176,6 +176,8 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     }
     if (!upload_vec(c, c->d_nodes, c->cap_nodes, b.nodes, "upload nodes") ||
         !upload_vec(c, c->d_prims, c->cap_prims, b.prims, "upload prims") ||
+        !upload_vec(c, c->d_ids, c->cap_ids, b.ids, "upload prim ids") ||
+        !upload_vec(c, c->d_pmat, c->cap_pmat, b.mats, "upload prim materials") ||
         false)
         return BC_ERR_DEVICE;
     if (!ok(c, hipStreamSynchronize(c->stream), "scene upload")) return BC_ERR_DEVICE;
@@ -184,7 +186,7 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     c->stack = stack;
     c->topo = topo;
     c->geom = geom;
-    c->scene_bytes = (int64_t)(b.nodes.size() * sizeof(bc_node8) + b.prims.size() * sizeof(bc_prim));
+    c->scene_bytes = (int64_t)(b.nodes.size() * sizeof(bc_node8) + b.prims.size() * (sizeof(bc_prim) + 8));
     c->has_scene = true;
     return BC_OK;
 }
@@ -245,13 +247,6 @@ int bc_update_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const 
     return BC_OK;
 }
 
-int bc_set_trace_schedule(bc_ctx* c, int32_t variant, int32_t blocks_per_cu) {
-    if (!c || variant < 0 || variant > 2 || blocks_per_cu < 0 || blocks_per_cu > 64) return BC_ERR_ARG;
-    c->trace_variant = variant;
-    c->trace_blocks_per_cu = blocks_per_cu;
-    return BC_OK;
-}
-
 int bc_scene_stats(const bc_ctx* c, int64_t* o) {
     if (!c || !o) return BC_ERR_ARG;
     o[0] = c->nnode8; o[1] = c->depth8; o[2] = c->nprim; o[3] = c->scene_bytes;
@@ -261,16 +256,8 @@ int bc_scene_stats(const bc_ctx* c, int64_t* o) {
 int bc_trace_device(bc_ctx* c, const void* d_rays, int64_t n, int any_hit, void* d_out) {
     if (!c || n < 0 || (n > 0 && (!d_rays || !d_out))) return BC_ERR_ARG;
     if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
-    hipError_t e;
-    if (c->trace_variant == 0) {
-        e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, (const bc_ray*)d_rays, n, d_out, c->d_err,
-                              c->stream);
-    } else {
-        const int blocks = c->num_cus * (c->trace_blocks_per_cu > 0 ? c->trace_blocks_per_cu : 8);
-        e = bcd::launch_trace_persistent(c->trace_variant, c->stack, any_hit != 0, c->d_nodes, c->d_prims,
-                                         (const bc_ray*)d_rays, n, d_out, c->d_err,
-                                         (unsigned long long*)((char*)c->d_err + 8), blocks, c->stream);
-    }
+    hipError_t e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, c->d_ids, (const bc_ray*)d_rays, n, d_out,
+                                     c->d_err, c->stream);
     return ok(c, e, "trace launch") ? BC_OK : BC_ERR_DEVICE;
 }
 

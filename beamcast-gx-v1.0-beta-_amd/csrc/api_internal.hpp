@@ -21,17 +21,19 @@ struct bc_ctx {
     uint64_t topo = 0, geom = 0;
     bc_node8* d_nodes = nullptr;
     bc_prim* d_prims = nullptr;
+    uint32_t* d_ids = nullptr;  // by record: primitive id | sphere bit
+    int32_t* d_pmat = nullptr;  // by record: material
     bc_mat* d_mats = nullptr;
     bc_light* d_lights = nullptr;
     float* d_pdf = nullptr;          // light selection pdf by primitive
     int32_t* d_light_of = nullptr;   // light index by primitive, -1 if none
+    size_t cap_ids = 0, cap_pmat = 0;
     size_t cap_nodes = 0, cap_prims = 0, cap_mats = 0, cap_lights = 0, cap_pdf = 0, cap_light_of = 0;
     int64_t nnode8 = 0, scene_bytes = 0;
     int depth8 = 0, nprim = 0, nmat = 0, nlight = 0, stack = 0, nmat_scene = 0;
     std::vector<bc_light> h_lights;  // light geometry (emission/CDF filled per material update)
     std::vector<double> h_light_area;
-    int* d_err = nullptr;  // [0] kernel error flag, bytes 8-15: persistent ray counter
-    int trace_variant = 0, trace_blocks_per_cu = 0;
+    int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;
     void* tmp_out = nullptr;

@@ -35,14 +35,18 @@ static_assert(sizeof(bc_node8) == 80, "bc_node8 must be 80 bytes");
 #define BC_META_EMPTY 0xFFu
 #define BC_SPHERE_BIT 0x80000000u
 
-// Primitive record (48 bytes, three float4), in BVH8 leaf order.
-//   triangle: q0 = (A.x, A.y, A.z, E1.x)  q1 = (E1.y, E1.z, E2.x, E2.y)  q2 = (E2.z, id, mat, 0)
-//   sphere  : q0 = (C.x, C.y, C.z, r)      q1 = (r*r, 1/r, 0, 0)          q2 = (0, id|SPHERE_BIT, mat, 0)
-// "id" is the Scene.Primitives index (renderer.go:233), stored as raw bits.
-struct alignas(16) bc_prim {
-    float q0[4], q1[4], q2[4];
+// Primitive record (36 bytes, nine floats), in BVH8 leaf order: exactly the operands of the
+// intersection test.
+//   triangle: A.xyz, E1.xyz, E2.xyz
+//   sphere  : C.xyz, r, r*r, 1/r, 0, 0, 0
+// Parallel arrays by record: id = Scene.Primitives index (renderer.go:233) | SPHERE_BIT for
+// spheres, and material.  Traversal reads the id only to resolve an accepted hit, and the
+// kind only in nodes flagged as holding spheres (bit 31 of prim_base).
+struct bc_prim {
+    float v[9];
 };
-static_assert(sizeof(bc_prim) == 48, "bc_prim must be 48 bytes");
+static_assert(sizeof(bc_prim) == 36, "bc_prim must be 36 bytes");
+#define BC_NODE_HAS_SPHERES 0x80000000u
 
 // Material record: kinds as renderer.go:86-91.
 struct alignas(16) bc_mat {

@@ -85,23 +85,22 @@ Child make_child(const bc_scene_view& v, int k) {
 
 int slots_of(const Child& c) { return c.leaf ? (c.count + 7) / 8 : 1; }
 
-void write_prim(const bc_scene_view& v, int id, bc_prim& r) {
+void write_prim(const bc_scene_view& v, int id, Bvh8& out) {
+    bc_prim r;
     std::memset(&r, 0, sizeof r);
     const double* g = v.prim_geom + 10 * (size_t)id;
     uint32_t idbits = (uint32_t)id;
     if (v.prim_kind[id] == 0) {
-        double rad = g[3];
-        r.q0[0] = (float)g[0]; r.q0[1] = (float)g[1]; r.q0[2] = (float)g[2]; r.q0[3] = (float)rad;
-        r.q1[0] = (float)(rad * rad); r.q1[1] = (float)(1.0 / rad);
+        const double rad = g[3];
+        r.v[0] = (float)g[0]; r.v[1] = (float)g[1]; r.v[2] = (float)g[2];
+        r.v[3] = (float)rad; r.v[4] = (float)(rad * rad); r.v[5] = (float)(1.0 / rad);
         idbits |= BC_SPHERE_BIT;
     } else {
-        r.q0[0] = (float)g[0]; r.q0[1] = (float)g[1]; r.q0[2] = (float)g[2]; r.q0[3] = (float)g[3];
-        r.q1[0] = (float)g[4]; r.q1[1] = (float)g[5]; r.q1[2] = (float)g[6]; r.q1[3] = (float)g[7];
-        r.q2[0] = (float)g[8];
+        for (int k = 0; k < 9; k++) r.v[k] = (float)g[k];
     }
-    std::memcpy(&r.q2[1], &idbits, 4);
-    int32_t m = v.prim_mat[id];
-    std::memcpy(&r.q2[2], &m, 4);
+    out.prims.push_back(r);
+    out.ids.push_back(idbits);
+    out.mats.push_back(v.prim_mat[id]);
 }
 
 }  // namespace
@@ -109,6 +108,8 @@ void write_prim(const bc_scene_view& v, int id, bc_prim& r) {
 bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
     out.nodes.clear();
     out.prims.clear();
+    out.ids.clear();
+    out.mats.clear();
     out.depth = 0;
     if (v.nnode <= 0 || v.nprim <= 0) { err = "scene has no BVH nodes or primitives"; return false; }
     // validateGPUSceneEncoding analogue (gpu_opencl_windows.go:470-482)
@@ -122,6 +123,8 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
         if (l[3] > 0 && (l[2] < 0 || (int64_t)l[2] + l[3] > v.nprim)) { err = "BVH leaf range out of bounds"; return false; }
     }
     out.prims.reserve((size_t)v.nprim);
+    out.ids.reserve((size_t)v.nprim);
+    out.mats.reserve((size_t)v.nprim);
     out.nodes.reserve((size_t)v.nnode / 3 + 1);
     out.nodes.emplace_back();
     struct Work { int node2, i8, depth; };
@@ -250,9 +253,9 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
             if (c.leaf) {
                 nd.meta[s] = (uint8_t)(((c.count - 1) << 5) | poff);
                 for (int k = 0; k < c.count; k++) {
-                    bc_prim r;
-                    write_prim(v, v.order[c.start + k], r);
-                    out.prims.push_back(r);
+                    const int id = v.order[c.start + k];
+                    write_prim(v, id, out);
+                    if (v.prim_kind[id] == 0) nd.prim_base |= BC_NODE_HAS_SPHERES;
                 }
                 poff += c.count;
             } else {

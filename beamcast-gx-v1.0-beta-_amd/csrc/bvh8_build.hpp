@@ -11,6 +11,8 @@ namespace bcx {
 struct Bvh8 {
     std::vector<bc_node8> nodes;
     std::vector<bc_prim> prims;  // leaf order
+    std::vector<uint32_t> ids;   // by record: Scene.Primitives index | BC_SPHERE_BIT
+    std::vector<int32_t> mats;   // by record: material
     int depth = 0;               // levels of bc_node8 (root = 1)
 };
 

@@ -72,7 +72,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int 
         r.ox = o.x; r.oy = o.y; r.oz = o.z;
         r.dx = d.x; r.dy = d.y; r.dz = d.z;
         r.tmin = 1e-4f;
-        const HitRec h = traverse<false, STACK>(S.nodes, S.prims, r, 1e30f, stk + threadIdx.x, kBlock, S.err);
+        const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, r, 1e30f, stk + threadIdx.x, kBlock, S.err);
         B.hit[j] = make_float2(h.t, __uint_as_float(h.id >= 0 ? h.rec : 0xffffffffu));
         B.nseg[j] += 1u;
     }
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B) {
         r.ox = o.x; r.oy = o.y; r.oz = o.z;
         r.dx = d.x; r.dy = d.y; r.dz = d.z;
         r.tmin = (float)1e-4;
-        const HitRec h = traverse<true, STACK>(S.nodes, S.prims, r, o.w, stk + threadIdx.x, kBlock, S.err);
+        const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
         if (h.id < 0) {
             const float4 c = B.sc[j];
             const float4 l = B.L[j];
@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, 
         r.ox = o.x; r.oy = o.y; r.oz = o.z;
         r.dx = d.x; r.dy = d.y; r.dz = d.z;
         r.tmin = (float)1e-4;
-        const HitRec h = traverse<true, STACK>(S.nodes, S.prims, r, o.w, stk + threadIdx.x, kBlock, S.err);
+        const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
         if (h.id < 0) {
             const float4 c = B.sc[j];
             const float4 l = B.L[j];
@@ -442,7 +442,7 @@ struct RenderState {
     uint32_t* h_cnt = nullptr;              // pinned
     // captured iteration graph of the persistent schedule
     hipGraphExec_t exec = nullptr;
-    const void* g_key[11] = {};
+    const void* g_key[13] = {};
     int g_iters = 0, g_stack = 0, g_grid = 0;
     // autotuner cache: (shape) -> (pool, iterations per graph)
     struct Tune {
@@ -566,7 +566,7 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     const size_t paths = npix * (size_t)spp_pass;
     if (!ensure_state(c, paths, npix, d_guides != nullptr, 1)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
-    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
     const int grid_paths = (int)std::min<size_t>((paths + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
     const int grid_pix = (int)std::min<size_t>((npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
@@ -601,14 +601,14 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
     const size_t slots = std::min<size_t>((size_t)pool, total_first);
     if (!ensure_state(c, slots, npix, d_guides != nullptr, total_first)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
-    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
     const int grid = (int)std::min<size_t>((slots + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
     const int grid_pix = (int)std::min<size_t>((npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
     hipStream_t st = c->stream;
     uint32_t* guides = d_guides ? d_guides : R.guide_tmp;
     iters = std::max(2, iters & ~1);
-    const void* key[11] = {c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+    const void* key[13] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
                            R.B.o, R.B.res, R.Pd, guides};
     if (!R.exec || R.g_iters != iters || R.g_stack != c->stack || R.g_grid != grid ||
         std::memcmp(key, R.g_key, sizeof key) != 0) {
@@ -836,7 +836,7 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
         !bc_check(c, hipMemsetAsync(d_grad, 0, ng * 4, st), "clear gradient") ||
         !bc_check(c, hipMemsetAsync(R.B.totals, 0, 16, st), "clear totals"))
         return BC_ERR_DEVICE;
-    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, false);
     bcd::Adj A{};
     A.dl = d_dl;

@@ -49,6 +49,8 @@ struct PathBufs {
 struct SceneBufs {
     const bc_node8* nodes;
     const bc_prim* prims;
+    const uint32_t* ids;
+    const int32_t* pmat;
     const bc_mat* mats;
     const bc_light* lights;
     const float* pdf;
@@ -269,17 +271,16 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
         L = add(L, had(T, bg));
         B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
     } else {
-        const float4* pr = reinterpret_cast<const float4*>(S.prims + rec);
-        const float4 q0 = pr[0], q1 = pr[1], q2 = pr[2];
-        const uint32_t idbits = __float_as_uint(q2.y);
+        const float* gq = S.prims[rec].v;
+        const uint32_t idbits = S.ids[rec];
         const int id = (int)(idbits & ~BC_SPHERE_BIT);
-        const int mat = (int)__float_as_uint(q2.z);
+        const int mat = S.pmat[rec];
         float3 nrm;
         if (idbits & BC_SPHERE_BIT) {
             const float3 pos = add(ro, mul(rd, t));
-            nrm = mul(sub(pos, f3(q0.x, q0.y, q0.z)), q1.y);
+            nrm = mul(sub(pos, f3(gq[0], gq[1], gq[2])), gq[5]);
         } else {
-            nrm = unitv(cross(f3(q0.w, q1.x, q1.y), f3(q1.z, q1.w, q2.x)));
+            nrm = unitv(cross(f3(gq[3], gq[4], gq[5]), f3(gq[6], gq[7], gq[8])));
         }
         const bool front = dot(rd, nrm) < 0.0f;
         const float3 N = front ? nrm : mul(nrm, -1.0f);

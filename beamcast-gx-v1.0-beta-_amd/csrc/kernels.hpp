@@ -14,13 +14,8 @@ int stack_for_depth(int depth);
 
 // Ray queries: any_hit writes int32 occlusion flags, else bc_hit records.  *err is set
 // non-zero by the kernel if a traversal stack would overflow.
-hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const bc_ray* rays,
+hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids,
+                        const bc_ray* rays,
                         int64_t n, void* out, int* err, hipStream_t st);
-
-// Persistent ray queries (variant 1: all pending primitives per step, 2: one per step);
-// `counter` is a device u64 the launch resets.
-hipError_t launch_trace_persistent(int variant, int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims,
-                                   const bc_ray* rays, int64_t n, void* out, int* err, unsigned long long* counter,
-                                   int blocks, hipStream_t st);
 
 }  // namespace bcd

@@ -37,17 +37,18 @@ struct HitRec {
 
 __device__ __forceinline__ float u8f(uint32_t w, uint32_t sh) { return (float)((w >> sh) & 0xffu); }
 
-// triHit in fp32 (renderer.go:684-711); t returned through tout.
-__device__ __forceinline__ bool tri_test(const float4 q0, const float4 q1, const float4 q2, const Ray& r, float tmin,
-                                         float tmax, float& tout) {
-    const float e1x = q0.w, e1y = q1.x, e1z = q1.y, e2x = q1.z, e2y = q1.w, e2z = q2.x;
+// triHit in fp32 (renderer.go:684-711) on a record g = A.xyz E1.xyz E2.xyz; t through tout.
+__device__ __forceinline__ bool tri_test(const float* __restrict__ g, const Ray& r, float tmin, float tmax,
+                                         float& tout) {
+    const float ax = g[0], ay = g[1], az = g[2];
+    const float e1x = g[3], e1y = g[4], e1z = g[5], e2x = g[6], e2y = g[7], e2z = g[8];
     const float hx = r.dy * e2z - r.dz * e2y;
     const float hy = r.dz * e2x - r.dx * e2z;
     const float hz = r.dx * e2y - r.dy * e2x;
     const float det = e1x * hx + e1y * hy + e1z * hz;
     if (fabsf(det) < 1e-10f) return false;
     const float f = 1.0f / det;
-    const float sx = r.ox - q0.x, sy = r.oy - q0.y, sz = r.oz - q0.z;
+    const float sx = r.ox - ax, sy = r.oy - ay, sz = r.oz - az;
     const float u = f * (sx * hx + sy * hy + sz * hz);
     if (u < 0.0f || u > 1.0f) return false;
     const float qx = sy * e1z - sz * e1y;
@@ -63,16 +64,16 @@ __device__ __forceinline__ bool tri_test(const float4 q0, const float4 q1, const
 
 // sphereHitAccel (renderer.go:650-678) with the line-distance discriminant of Haines et al.,
 // "Precision Improvements for Ray/Sphere Intersection" (Ray Tracing Gems, 2019).
-__device__ __forceinline__ bool sphere_test(const float4 q0, const float4 q1, const Ray& r, float tmin, float tmax,
+__device__ __forceinline__ bool sphere_test(const float* __restrict__ g, const Ray& r, float tmin, float tmax,
                                             float& tout) {
     const float a = r.dx * r.dx + r.dy * r.dy + r.dz * r.dz;
     if (!(a > 1e-18f)) return false;
     const float inva = 1.0f / a;
-    const float ocx = r.ox - q0.x, ocy = r.oy - q0.y, ocz = r.oz - q0.z;
+    const float ocx = r.ox - g[0], ocy = r.oy - g[1], ocz = r.oz - g[2];
     const float hb = ocx * r.dx + ocy * r.dy + ocz * r.dz;
     const float k = hb * inva;
     const float lx = ocx - r.dx * k, ly = ocy - r.dy * k, lz = ocz - r.dz * k;
-    const float disc = a * (q1.x - (lx * lx + ly * ly + lz * lz));
+    const float disc = a * (g[4] - (lx * lx + ly * ly + lz * lz));
     if (disc < 0.0f) return false;
     const float sq = sqrtf(disc);
     float root = (-hb - sq) * inva;
@@ -117,6 +118,7 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
     const uint32_t nz0 = nz ? hiz0 : loz0, nz1 = nz ? hiz1 : loz1, fz0 = nz ? loz0 : hiz0, fz1 = nz ? loz1 : hiz1;
     const uint32_t meta0 = w1.z, meta1 = w1.w;
     uint32_t hits = 0;
+    float tnmin = 3.0e38f;  // smallest entry distance among hit internal children
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const uint32_t sh = (uint32_t)(s & 3) * 8u;
@@ -132,6 +134,7 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
         const uint32_t meta = ((hiw ? meta1 : meta0) >> sh) & 0xffu;
         const bool h = (tn <= tf) && (meta != BC_META_EMPTY);
         hits |= (h ? 1u : 0u) << s;
+        if (h && ((imask >> s) & 1u)) tnmin = fminf(tnmin, tn);
     }
     // internal children: reorder hit bits so that bit k is slot k ^ oct
     const uint32_t ih = hits & imask;
@@ -139,7 +142,9 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
 #pragma unroll
     for (int k = 0; k < 8; k++) ord |= ((ih >> ((uint32_t)k ^ oct)) & 1u) << k;
     ng_base = w1.x;
-    ng_bits = ord | (imask << 8);
+    // bits 16-31: upper half of the (non-negative) smallest entry distance; truncation
+    // rounds towards zero, so it is a lower bound usable to cull the group after a pop
+    ng_bits = ord | (imask << 8) | (__float_as_uint(fmaxf(tnmin, 0.0f)) & 0xffff0000u);
     // leaf children: union of their primitive runs
     uint32_t lh = hits & ~imask & 0xffu, pm = 0;
     while (lh) {
@@ -157,7 +162,8 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
 // stk points at this lane's column of the LDS stack (entry i at stk[i * stride]).
 template <bool ANY, int STACK>
 __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, const bc_prim* __restrict__ prims,
-                                           const Ray& r, float tmax, uint2* stk, int stride, int* __restrict__ err) {
+                                           const uint32_t* __restrict__ ids, const Ray& r, float tmax, uint2* stk,
+                                           int stride, int* __restrict__ err) {
     const float ix = safe_rcp(r.dx), iy = safe_rcp(r.dy), iz = safe_rcp(r.dz);
     const uint32_t oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
     HitRec best;
@@ -173,19 +179,19 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
             visit_node(nodes, node, r, ix, iy, iz, oct, best.t, ng_base, ng_bits, pg_base, pg_mask);
             have_node = false;
         }
+        const bool sph_node = (pg_base & BC_NODE_HAS_SPHERES) != 0;
+        const uint32_t pbase = pg_base & ~BC_NODE_HAS_SPHERES;
         while (pg_mask) {
             const uint32_t k = (uint32_t)__builtin_ctz(pg_mask);
             pg_mask &= pg_mask - 1u;
-            const uint32_t ri = pg_base + k;
-            const float4* pq = reinterpret_cast<const float4*>(prims + ri);
-            const float4 q0 = pq[0], q1 = pq[1], q2 = pq[2];
-            const uint32_t idbits = __float_as_uint(q2.y);
+            const uint32_t ri = pbase + k;
+            const float* g = prims[ri].v;
             float t;
             bool h;
-            if (idbits & BC_SPHERE_BIT) h = sphere_test(q0, q1, r, r.tmin, best.t, t);
-            else h = tri_test(q0, q1, q2, r, r.tmin, best.t, t);
+            if (sph_node && (ids[ri] & BC_SPHERE_BIT)) h = sphere_test(g, r, r.tmin, best.t, t);
+            else h = tri_test(g, r, r.tmin, best.t, t);
             if (h) {
-                const int32_t id = (int32_t)(idbits & ~BC_SPHERE_BIT);
+                const int32_t id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
                 if (ANY) {
                     best.t = t;
                     best.id = id;
@@ -203,7 +209,7 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
             const uint32_t k = (uint32_t)__builtin_ctz(ng_bits & 0xffu);
             ng_bits &= ~(1u << k);
             const uint32_t slot = k ^ oct;
-            const uint32_t imask = ng_bits >> 8;
+            const uint32_t imask = (ng_bits >> 8) & 0xffu;
             node = ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
             if (ng_bits & 0xffu) {
                 if (sp >= STACK) {  // host guarantees depth <= STACK; report instead of overflowing
@@ -219,6 +225,8 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
             const uint2 e = stk[sp * stride];
             ng_base = e.x;
             ng_bits = e.y;
+            // closest hit: a group whose nearest child starts beyond the current hit is done
+            if (!ANY && __uint_as_float(e.y & 0xffff0000u) > best.t) ng_bits = 0;
         } else {
             break;
         }
@@ -226,99 +234,5 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
     return best;
 }
 
-
-// ------------------------------------------------------------------------------ stepwise form
-// The same traversal as a resumable state machine, for persistent kernels that refill a
-// lane with a new ray as soon as its ray finishes (instead of waiting for the wave's
-// slowest ray).  PPS = primitives tested per step (0 = all pending primitives).
-struct TravState {
-    Ray r;
-    float ix, iy, iz, tmax;
-    uint32_t oct;
-    HitRec best;
-    uint32_t ng_base, ng_bits, pg_base, pg_mask, node;
-    int sp;
-    bool have_node;
-};
-
-__device__ __forceinline__ void trav_init(TravState& s, const Ray& r, float tmax) {
-    s.r = r;
-    s.ix = safe_rcp(r.dx);
-    s.iy = safe_rcp(r.dy);
-    s.iz = safe_rcp(r.dz);
-    s.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
-    s.tmax = tmax;
-    s.best.t = tmax;
-    s.best.id = -1;
-    s.best.rec = 0xffffffffu;
-    s.ng_base = 0;
-    s.ng_bits = 0;
-    s.pg_base = 0;
-    s.pg_mask = 0;
-    s.node = 0;
-    s.sp = 0;
-    s.have_node = true;
-}
-
-// One step; returns true when the ray is finished (result in s.best).
-template <bool ANY, int STACK, int PPS>
-__device__ __forceinline__ bool trav_step(TravState& s, const bc_node8* __restrict__ nodes,
-                                          const bc_prim* __restrict__ prims, uint2* stk, int stride,
-                                          int* __restrict__ err) {
-    if (s.have_node) {
-        visit_node(nodes, s.node, s.r, s.ix, s.iy, s.iz, s.oct, s.best.t, s.ng_base, s.ng_bits, s.pg_base, s.pg_mask);
-        s.have_node = false;
-    }
-    int tested = 0;
-    while (s.pg_mask && (PPS == 0 || tested < PPS)) {
-        tested++;
-        const uint32_t k = (uint32_t)__builtin_ctz(s.pg_mask);
-        s.pg_mask &= s.pg_mask - 1u;
-        const uint32_t ri = s.pg_base + k;
-        const float4* pq = reinterpret_cast<const float4*>(prims + ri);
-        const float4 q0 = pq[0], q1 = pq[1], q2 = pq[2];
-        const uint32_t idbits = __float_as_uint(q2.y);
-        float t;
-        bool h;
-        if (idbits & BC_SPHERE_BIT) h = sphere_test(q0, q1, s.r, s.r.tmin, s.best.t, t);
-        else h = tri_test(q0, q1, q2, s.r, s.r.tmin, s.best.t, t);
-        if (h) {
-            const int32_t id = (int32_t)(idbits & ~BC_SPHERE_BIT);
-            if (ANY) {
-                s.best.t = t;
-                s.best.id = id;
-                s.best.rec = ri;
-                return true;
-            }
-            if (s.best.id < 0 || t < s.best.t || id < s.best.id) {
-                s.best.t = t;
-                s.best.id = id;
-                s.best.rec = ri;
-            }
-        }
-    }
-    if (s.pg_mask) return false;
-    if (!(s.ng_bits & 0xffu)) {
-        if (s.sp == 0) return true;
-        s.sp--;
-        const uint2 e = stk[s.sp * stride];
-        s.ng_base = e.x;
-        s.ng_bits = e.y;
-    }
-    const uint32_t k = (uint32_t)__builtin_ctz(s.ng_bits & 0xffu);
-    s.ng_bits &= ~(1u << k);
-    const uint32_t slot = k ^ s.oct;
-    s.node = s.ng_base + (uint32_t)__builtin_popcount((s.ng_bits >> 8) & ((1u << slot) - 1u));
-    if (s.ng_bits & 0xffu) {
-        if (s.sp >= STACK) {
-            atomicOr(err, 1);
-            return true;
-        }
-        stk[s.sp * stride] = make_uint2(s.ng_base, s.ng_bits);
-        s.sp++;
-    }
-    s.have_node = true;
-    return false;
-}
 
 }  // namespace bcd

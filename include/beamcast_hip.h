@@ -128,11 +128,6 @@ int bc_trace(bc_ctx* ctx, const float* rays, int64_t n, int any_hit, void* out);
 /* Same with device pointers, enqueued on the context stream (asynchronous). */
 int bc_trace_device(bc_ctx* ctx, const void* d_rays, int64_t n, int any_hit, void* d_out);
 
-/* Schedule of the ray-query kernels (no counterpart in the reference): variant 0 = one ray per
- * thread; 1 = persistent lanes refilled from a ray counter, all pending primitives per step;
- * 2 = persistent, one primitive per step.  blocks_per_cu sizes persistent grids (0 = 8). */
-int bc_set_trace_schedule(bc_ctx* ctx, int32_t variant, int32_t blocks_per_cu);
-
 /* Device memory and stream helpers for callers that keep inputs resident in HBM. */
 void* bc_alloc(bc_ctx* ctx, uint64_t bytes);
 int bc_free(bc_ctx* ctx, void* d_ptr);
