@@ -28,82 +28,108 @@
 
 namespace bcd {
 
+constexpr uint32_t kIdle = 0xffffffffu;
+
+// Slots of segment `seg` among the first `total` slots.
+__device__ __forceinline__ uint32_t seg_slots(uint32_t seg, uint32_t total) {
+    const uint32_t first = seg * kSeg;
+    return first >= total ? 0u : (total - first < kSeg ? total - first : kSeg);
+}
+
+// Sum over a workgroup of NW waves; the result is valid in thread 0.
+template <int NW = kWaves>
+__device__ __forceinline__ unsigned long long block_sum(unsigned long long v, unsigned long long* lds) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += (unsigned long long)__shfl_xor((long long)v, off);
+    if (__lane_id() == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    unsigned long long t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < NW; k++) t += lds[k];
+    return t;
+}
+
 // ------------------------------------------------------------------------------ raygen
+// Slot j = (sample - s0) * npix + pixel; the pass's segments start full.
 __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, float* __restrict__ acc,
-                                                   uint32_t* __restrict__ guides) {
-    const int n = P.ns * P.npix;
-    const int stride = gridDim.x * kBlock;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        B.cnt[0] = (uint32_t)n;
-        B.cnt[1] = 0;
-        B.cnt[2] = 0;
-    }
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
-        const int sl = j / P.npix, pl = j - sl * P.npix;
-        init_camera_path(P, B, (uint32_t)j, pl, P.s0 + sl);
-        B.q[0][j] = (uint32_t)j;
-        if (P.s0 == 0 && sl == 0) {
-            float* a = acc + 4 * (size_t)pl;
-            a[0] = 0.0f; a[1] = 0.0f; a[2] = 0.0f; a[3] = 0.0f;
-            if (P.want_guides) {
-                uint32_t* gg = guides + 3 * (size_t)pl;
-                gg[0] = 0u; gg[1] = 0u; gg[2] = 0u;
+                                                   uint32_t* __restrict__ guides, uint32_t nseg) {
+    const uint32_t n = (uint32_t)P.ns * (uint32_t)P.npix;
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t cnt = seg_slots(seg, n), base = seg * kSeg;
+        for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+            const uint32_t j = base + k;
+            const uint32_t sl = j / (uint32_t)P.npix, pl = j - sl * (uint32_t)P.npix;
+            init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
+            B.q[0][j] = j;
+            if (P.s0 == 0 && sl == 0) {
+                float* a = acc + 4 * (size_t)pl;
+                a[0] = 0.0f; a[1] = 0.0f; a[2] = 0.0f; a[3] = 0.0f;
+                if (P.want_guides) {
+                    uint32_t* gg = guides + 3 * (size_t)pl;
+                    gg[0] = 0u; gg[1] = 0u; gg[2] = 0u;
+                }
             }
         }
+        if (threadIdx.x == 0) B.qc[0][seg] = cnt;
     }
 }
 
 // ------------------------------------------------------------------------------ extend
 template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int cur) {
+__global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg) {
     __shared__ uint2 stk[STACK * kBlock];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        B.cnt[cur ^ 1] = 0;  // next queue, filled by shade of this bounce
-        B.cnt[2] = 0;        // shadow queue of this bounce
-        B.cnt[3] = 0;        // terminated paths (persistent schedule)
-    }
-    const uint32_t n = B.cnt[cur];
-    const uint32_t stride = gridDim.x * kBlock;
     const uint32_t* q = B.q[cur];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const uint32_t j = q[i];
-        const float4 o = B.o[j], d = B.d[j];
-        Ray r;
-        r.ox = o.x; r.oy = o.y; r.oz = o.z;
-        r.dx = d.x; r.dy = d.y; r.dz = d.z;
-        r.tmin = 1e-4f;
-        const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, r, 1e30f, stk + threadIdx.x, kBlock, S.err);
-        B.hit[j] = make_float2(h.t, __uint_as_float(h.id >= 0 ? h.rec : 0xffffffffu));
-        B.nseg[j] += 1u;
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
+            const uint32_t j = q[base + k];
+            const float4 o = B.o[j], d = B.d[j];
+            Ray r;
+            r.ox = o.x; r.oy = o.y; r.oz = o.z;
+            r.dx = d.x; r.dy = d.y; r.dz = d.z;
+            r.tmin = 1e-4f;
+            const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, r, 1e30f, stk + threadIdx.x, kBlock, S.err);
+            B.hit[j] = make_float2(h.t, __uint_as_float(h.rec));
+            B.nseg[j] += 1u;
+        }
     }
 }
 
 // ------------------------------------------------------------------------------ shade
 __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, RenderParams P, int cur, int bounce,
-                                                  uint32_t* __restrict__ guides) {
-    const uint32_t n = B.cnt[cur];
-    const uint32_t stride = gridDim.x * kBlock;
+                                                  uint32_t* __restrict__ guides, uint32_t nseg) {
+    __shared__ uint32_t lds[2 * 2 * kWaves];
+    uint32_t parity = 0;
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
-    const uint32_t nloop = (n + stride - 1) / stride * stride;  // uniform trip count for wave ops
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
-        bool cont = false, shadow = false;
-        uint32_t j = 0;
-        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
-        if (i < n) {
-            j = q[i];
-            cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        uint32_t bc = 0, bs = 0;
+        for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+            const uint32_t k = r0 + threadIdx.x;
+            bool cont = false, shadow = false;
+            uint32_t j = 0;
+            float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+            if (k < n) {
+                j = q[base + k];
+                cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
+            }
+            const bool pr[2] = {cont, shadow};
+            uint32_t rk[2], tot[2];
+            block_rank<2>(pr, rk, tot, lds, parity);
+            if (cont) qn[base + bc + rk[0]] = j;
+            if (shadow) {
+                B.sq[base + bs + rk[1]] = j;
+                B.so[j] = so;
+                B.sd[j] = sd;
+                B.sc[j] = sc;
+            }
+            bc += tot[0];
+            bs += tot[1];
         }
-        uint32_t slot = 0;
-        wave_append(&B.cnt[cur ^ 1], cont, slot);
-        if (cont) qn[slot] = j;
-        uint32_t sslot = 0;
-        wave_append(&B.cnt[2], shadow, sslot);
-        if (shadow) {
-            B.sq[sslot] = j;
-            B.so[j] = so;
-            B.sd[j] = sd;
-            B.sc[j] = sc;
+        if (threadIdx.x == 0) {
+            B.qc[cur ^ 1][seg] = bc;
+            B.sqc[seg] = bs;
         }
     }
 }
@@ -111,66 +137,62 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
 // ------------------------------------------------------------------------------ shadow
 // Occluded(p + N*2e-4 -> q, 1e-4, dist - 1e-4) of visible() (renderer.go:1333-1341, 851-895).
 template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B) {
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint32_t nseg) {
     __shared__ uint2 stk[STACK * kBlock];
-    const uint32_t n = B.cnt[2];
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&B.totals[1], (unsigned long long)n);
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const uint32_t j = B.sq[i];
-        const float4 o = B.so[j], d = B.sd[j];
-        Ray r;
-        r.ox = o.x; r.oy = o.y; r.oz = o.z;
-        r.dx = d.x; r.dy = d.y; r.dz = d.z;
-        r.tmin = (float)1e-4;
-        const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
-        if (h.id < 0) {
-            const float4 c = B.sc[j];
-            const float4 l = B.L[j];
-            B.L[j] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t n = B.sqc[seg], base = seg * kSeg;
+        if (threadIdx.x == 0) B.tally[kMaxGrid + seg] += n;
+        for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
+            const uint32_t j = B.sq[base + k];
+            const float4 o = B.so[j], d = B.sd[j];
+            Ray r;
+            r.ox = o.x; r.oy = o.y; r.oz = o.z;
+            r.dx = d.x; r.dy = d.y; r.dz = d.z;
+            r.tmin = (float)1e-4;
+            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
+            if (h.rec == 0xffffffffu) {
+                const float4 c = B.sc[j];
+                const float4 l = B.L[j];
+                B.L[j] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+            }
         }
     }
-}
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off);
-    return v;
 }
 
 // ------------------------------------------------------------------------------ accumulate
 // Per-pixel sum in sample order with clampFirefly (renderer.go:1917-1926, 1951-1953).
+__device__ __forceinline__ float4 clamp_sample(const RenderParams& P, float4 l) {
+    if (P.firefly > 0.0f) {
+        const float lim = P.firefly;
+        const float m = rmax(l.x, rmax(l.y, l.z));
+        if (!(m <= lim || m <= 0.0f)) {
+            const float k = lim / m;
+            l.x = l.x * k; l.y = l.y * k; l.z = l.z * k;
+        }
+    }
+    return l;
+}
+
 __global__ __launch_bounds__(kBlock) void k_accumulate(RenderParams P, PathBufs B, float4* __restrict__ acc,
                                                        float4* __restrict__ keep) {
-    const int stride = gridDim.x * kBlock;
-    const int nloop = (P.npix + stride - 1) / stride * stride;
-    for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < nloop; pl += stride) {
-        uint32_t segs = 0;
-        if (pl < P.npix) {
-            float4 a = acc[pl];
-            for (int sl = 0; sl < P.ns; sl++) {
-                const int j = sl * P.npix + pl;
-                const float4 l = B.L[j];
-                if (keep) keep[(size_t)(P.s0 + sl) * (size_t)P.npix + (size_t)pl] = l;
-                float cx = l.x, cy = l.y, cz = l.z;
-                if (P.firefly > 0.0f) {
-                    const float lim = P.firefly;
-                    const float m = rmax(cx, rmax(cy, cz));
-                    if (!(m <= lim || m <= 0.0f)) {
-                        const float k = lim / m;
-                        cx = cx * k; cy = cy * k; cz = cz * k;
-                    }
-                }
-                a.x = a.x + cx; a.y = a.y + cy; a.z = a.z + cz;
-                const uint32_t ns = B.nseg[j];
-                a.w = a.w + (float)ns;
-                segs += ns;
-            }
-            acc[pl] = a;
+    __shared__ unsigned long long lds[kWaves];
+    unsigned long long segs = 0;
+    for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < P.npix; pl += gridDim.x * kBlock) {
+        float4 a = acc[pl];
+        for (int sl = 0; sl < P.ns; sl++) {
+            const int j = sl * P.npix + pl;
+            const float4 l = B.L[j];
+            if (keep) keep[(size_t)(P.s0 + sl) * (size_t)P.npix + (size_t)pl] = l;
+            const float4 c = clamp_sample(P, l);
+            a.x = a.x + c.x; a.y = a.y + c.y; a.z = a.z + c.z;
+            const uint32_t ns = B.nseg[j];
+            a.w = a.w + (float)ns;
+            segs += ns;
         }
-        const uint32_t tot = wave_sum(segs);
-        if (__lane_id() == 0 && tot) atomicAdd(&B.totals[0], (unsigned long long)tot);
+        acc[pl] = a;
     }
+    const unsigned long long t = block_sum(segs, lds);
+    if (threadIdx.x == 0) B.tally[blockIdx.x] += t;
 }
 
 __global__ __launch_bounds__(kBlock) void k_finalize(RenderParams P, const float4* __restrict__ acc,
@@ -182,27 +204,41 @@ __global__ __launch_bounds__(kBlock) void k_finalize(RenderParams P, const float
     }
 }
 
-
-// ------------------------------------------------------------------------------ persistent schedule
-// A fixed pool of path slots.  Each iteration: extend -> shade -> shadow -> regen.  Paths that
-// end leave their radiance in a per-sample slot and their path slot is refilled in place with
-// the next (pixel, sample) of the window, claimed from a device counter.  Samples are keyed by
-// (pixel, sample), never by slot, so the image does not depend on the schedule; the per-sample
-// results are reduced in sample order afterwards.  Parameters live in device memory so the
-// iteration sequence can be captured once as a hipGraph and replayed.
-
-__global__ void k_p_reset(PathBufs B) {
-    if (threadIdx.x < 5) B.cnt[threadIdx.x] = 0;
+// Reduces the per-block tallies into totals[0] (path segments) and totals[1] (shadow rays).
+constexpr int kTallyBlock = 1024;
+__global__ __launch_bounds__(kTallyBlock) void k_tally(PathBufs B, uint32_t nseg) {
+    __shared__ unsigned long long lds[kTallyBlock / 64];
+    unsigned long long a = 0, b = 0;
+    for (uint32_t i = threadIdx.x; i < (uint32_t)kMaxGrid; i += kTallyBlock) a += B.tally[i];
+    for (uint32_t i = threadIdx.x; i < nseg; i += kTallyBlock) b += B.tally[kMaxGrid + i];
+    const unsigned long long ta = block_sum<kTallyBlock / 64>(a, lds);
+    __syncthreads();
+    const unsigned long long tb = block_sum<kTallyBlock / 64>(b, lds);
+    if (threadIdx.x == 0) {
+        B.totals[0] = ta;
+        B.totals[1] = tb;
+    }
 }
 
-__global__ __launch_bounds__(kBlock) void k_p_fill(const RenderParams* __restrict__ Pd, PathBufs B,
-                                                   float4* __restrict__ acc, uint32_t* __restrict__ guides,
-                                                   uint32_t slots, int clear_pixels) {
+
+// ------------------------------------------------------------------------------ persistent schedule
+// A fixed pool of path slots in segments.  Each iteration: extend -> shade -> shadow.  A path
+// that ends is listed in its segment's terminated list; the next extend of that segment stores
+// its radiance in the per-sample result slot and restarts the path slot in place with the
+// segment's next sample, then traces it with the segment's continuing paths.  The window's
+// samples are cut into chunks of kSeg consecutive (pixel, sample) indices and segment s owns
+// chunks s, s + nseg, s + 2 nseg, ..., so a segment restarts runs of neighbouring pixels and
+// its camera rays stay coherent.  Samples are keyed by (pixel, sample), never by
+// slot, so the image does not depend on the schedule; per-sample results are reduced in sample
+// order afterwards.  Parameters live in device memory so that a run of iterations can be
+// captured once as a hipGraph and replayed.
+
+__global__ __launch_bounds__(kBlock) void k_p_start(const RenderParams* __restrict__ Pd, PathBufs B,
+                                                    float4* __restrict__ acc, uint32_t* __restrict__ guides,
+                                                    uint32_t nseg, int clear_pixels) {
     const RenderParams P = *Pd;
-    const uint32_t stride = gridDim.x * kBlock;
-    const uint32_t total = (uint32_t)P.ns * (uint32_t)P.npix;
     if (clear_pixels) {
-        for (uint32_t pl = blockIdx.x * kBlock + threadIdx.x; pl < (uint32_t)P.npix; pl += stride) {
+        for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < P.npix; pl += gridDim.x * kBlock) {
             acc[pl] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             if (P.want_guides) {
                 uint32_t* gg = guides + 3 * (size_t)pl;
@@ -210,215 +246,276 @@ __global__ __launch_bounds__(kBlock) void k_p_fill(const RenderParams* __restric
             }
         }
     }
-    const uint32_t nloop = (slots + stride - 1) / stride * stride;
-    for (uint32_t j = blockIdx.x * kBlock + threadIdx.x; j < nloop; j += stride) {
-        uint32_t idx = 0xffffffffu;
-        wave_append(&B.cnt[4], j < slots, idx);
-        const bool start = j < slots && idx < total;
-        if (start) {
-            const uint32_t sl = idx / (uint32_t)P.npix, pl = idx - sl * (uint32_t)P.npix;
-            init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
-            B.bnc[j] = 0;
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t base = seg * kSeg;
+        for (uint32_t k = threadIdx.x; k < kSeg; k += kBlock) {
+            B.dq[base + k] = base + k;
+            B.gid[base + k] = kIdle;
         }
-        uint32_t slot = 0;
-        wave_append(&B.cnt[0], start, slot);
-        if (start) B.q[0][slot] = j;
+        if (threadIdx.x == 0) {
+            B.qc[0][seg] = 0u;
+            B.qc[1][seg] = 0u;
+            B.sqc[seg] = 0u;
+            B.dqc[seg] = kSeg;
+            B.snext[seg] = 0u;
+        }
     }
+}
+
+// Samples of the window (W of them, in chunks of kSeg) owned by segment `seg` of `nseg`.
+__device__ __forceinline__ uint32_t seg_samples(uint32_t seg, uint32_t nseg, uint32_t W) {
+    const uint32_t chunks = (W + kSeg - 1) / kSeg;
+    if (seg >= chunks) return 0u;
+    const uint32_t last = chunks - 1u;
+    uint32_t n = ((last - seg) / nseg + 1u) * kSeg;
+    if ((last - seg) % nseg == 0u) n -= chunks * kSeg - W;  // the short final chunk is this segment's
+    return n;
+}
+
+__global__ void k_p_clear_flag(PathBufs B) {
+    if (threadIdx.x == 0) B.flag[0] = 0u;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
+                                                     int cur, uint32_t nseg) {
+    __shared__ uint2 stk[STACK * kBlock];
+    const RenderParams P = *Pd;
+    const uint32_t W = (uint32_t)P.ns * (uint32_t)P.npix;
+    uint32_t* q = B.q[cur];
+    bool work = false;
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t n = B.qc[cur][seg], nd = B.dqc[seg], m0 = B.snext[seg], base = seg * kSeg;
+        const uint32_t owned = seg_samples(seg, nseg, W);
+        const uint32_t avail = owned > m0 ? owned - m0 : 0u;
+        const uint32_t nr = nd < avail ? nd : avail;
+        for (uint32_t k = threadIdx.x; k < n + nd; k += kBlock) {
+            uint32_t j;
+            bool live = true;
+            if (k < n) {
+                j = q[base + k];
+            } else {
+                const uint32_t kk = k - n;
+                j = B.dq[base + kk];
+                const uint32_t g = B.gid[j];
+                if (g != kIdle) {
+                    const float4 l = B.L[j];
+                    B.res[g] = make_float4(l.x, l.y, l.z, (float)B.nseg[j]);
+                }
+                if (kk < nr) {
+                    const uint32_t m = m0 + kk;
+                    const uint32_t gn = ((m / kSeg) * nseg + seg) * kSeg + m % kSeg;
+                    const uint32_t sl = gn / (uint32_t)P.npix, pl = gn - sl * (uint32_t)P.npix;
+                    init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
+                    B.gid[j] = gn;
+                    B.bnc[j] = 0u;
+                    q[base + n + kk] = j;
+                } else {
+                    B.gid[j] = kIdle;
+                    live = false;
+                }
+            }
+            if (live) {
+                const float4 o = B.o[j], d = B.d[j];
+                Ray r;
+                r.ox = o.x; r.oy = o.y; r.oz = o.z;
+                r.dx = d.x; r.dy = d.y; r.dz = d.z;
+                r.tmin = 1e-4f;
+                const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, r, 1e30f, stk + threadIdx.x, kBlock, S.err);
+                B.hit[j] = make_float2(h.t, __uint_as_float(h.rec));
+                B.nseg[j] += 1u;
+            }
+        }
+        work = work || (n + nd) > 0u;
+        __syncthreads();  // every lane has read this segment's lengths
+        if (threadIdx.x == 0) {
+            B.qc[cur][seg] = n + nr;
+            B.dqc[seg] = 0u;
+            B.snext[seg] = m0 + nr;
+        }
+    }
+    if (work && threadIdx.x == 0) B.flag[0] = 1u;
 }
 
 __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
-                                                    int cur, uint32_t* __restrict__ guides) {
+                                                    int cur, uint32_t* __restrict__ guides, uint32_t nseg) {
+    __shared__ uint32_t lds[2 * 3 * kWaves];
+    uint32_t parity = 0;
     const RenderParams P = *Pd;
-    const uint32_t n = B.cnt[cur];
-    const uint32_t stride = gridDim.x * kBlock;
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
-    const uint32_t nloop = (n + stride - 1) / stride * stride;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
-        bool cont = false, shadow = false, done = false;
-        uint32_t j = 0;
-        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
-        if (i < n) {
-            j = q[i];
-            const int bounce = (int)B.bnc[j];
-            cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
-            if (cont) B.bnc[j] = (uint32_t)(bounce + 1);
-            done = !cont;
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        uint32_t bc = 0, bd = 0, bs = 0;
+        for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+            const uint32_t k = r0 + threadIdx.x;
+            bool cont = false, shadow = false, done = false;
+            uint32_t j = 0;
+            float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+            if (k < n) {
+                j = q[base + k];
+                const int bounce = (int)B.bnc[j];
+                cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
+                if (cont) B.bnc[j] = (uint32_t)(bounce + 1);
+                done = !cont;
+            }
+            const bool pr[3] = {cont, done, shadow};
+            uint32_t rk[3], tot[3];
+            block_rank<3>(pr, rk, tot, lds, parity);
+            if (cont) qn[base + bc + rk[0]] = j;
+            if (done) B.dq[base + bd + rk[1]] = j;
+            if (shadow) {
+                B.sq[base + bs + rk[2]] = j;
+                B.so[j] = so;
+                B.sd[j] = sd;
+                B.sc[j] = sc;
+            }
+            bc += tot[0];
+            bd += tot[1];
+            bs += tot[2];
         }
-        uint32_t slot = 0;
-        wave_append(&B.cnt[cur ^ 1], cont, slot);
-        if (cont) qn[slot] = j;
-        uint32_t dslot = 0;
-        wave_append(&B.cnt[3], done, dslot);
-        if (done) B.dq[dslot] = j;
-        uint32_t sslot = 0;
-        wave_append(&B.cnt[2], shadow, sslot);
-        if (shadow) {
-            B.sq[sslot] = j;
-            B.so[j] = so;
-            B.sd[j] = sd;
-            B.sc[j] = sc;
+        if (threadIdx.x == 0) {
+            B.qc[cur ^ 1][seg] = bc;
+            B.dqc[seg] = bd;
+            B.sqc[seg] = bs;
         }
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_p_regen(PathBufs B, const RenderParams* __restrict__ Pd, int cur) {
-    const RenderParams P = *Pd;
-    const uint32_t n = B.cnt[3];
-    const uint32_t total = (uint32_t)P.ns * (uint32_t)P.npix;
-    const uint32_t stride = gridDim.x * kBlock;
-    uint32_t* qn = B.q[cur ^ 1];
-    const uint32_t nloop = (n + stride - 1) / stride * stride;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
-        uint32_t j = 0;
-        if (i < n) {
-            j = B.dq[i];
-            const uint4 meta = B.meta[j];
-            const uint32_t pl = meta.x - (uint32_t)P.y0 * (uint32_t)P.width;
-            const uint32_t g = (meta.y - (uint32_t)P.s0) * (uint32_t)P.npix + pl;
-            const float4 l = B.L[j];
-            B.res[g] = make_float4(l.x, l.y, l.z, (float)B.nseg[j]);
-        }
-        uint32_t idx = 0xffffffffu;
-        wave_append(&B.cnt[4], i < n, idx);
-        const bool start = i < n && idx < total;
-        if (start) {
-            const uint32_t sl = idx / (uint32_t)P.npix, pl = idx - sl * (uint32_t)P.npix;
-            init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
-            B.bnc[j] = 0;
-        }
-        uint32_t slot = 0;
-        wave_append(&B.cnt[cur ^ 1], start, slot);
-        if (start) qn[slot] = j;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_p_reduce(const RenderParams* __restrict__ Pd, PathBufs B,
                                                      float4* __restrict__ acc) {
+    __shared__ unsigned long long lds[kWaves];
     const RenderParams P = *Pd;
-    const int stride = gridDim.x * kBlock;
-    const int nloop = (P.npix + stride - 1) / stride * stride;
-    for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < nloop; pl += stride) {
-        uint32_t segs = 0;
-        if (pl < P.npix) {
-            float4 a = acc[pl];
-            for (int sl = 0; sl < P.ns; sl++) {
-                const float4 r = B.res[(size_t)sl * (size_t)P.npix + (size_t)pl];
-                float cx = r.x, cy = r.y, cz = r.z;
-                if (P.firefly > 0.0f) {
-                    const float lim = P.firefly;
-                    const float m = rmax(cx, rmax(cy, cz));
-                    if (!(m <= lim || m <= 0.0f)) {
-                        const float k = lim / m;
-                        cx = cx * k; cy = cy * k; cz = cz * k;
-                    }
-                }
-                a.x = a.x + cx; a.y = a.y + cy; a.z = a.z + cz;
-                a.w = a.w + r.w;
-                segs += (uint32_t)r.w;
-            }
-            acc[pl] = a;
+    unsigned long long segs = 0;
+    for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < P.npix; pl += gridDim.x * kBlock) {
+        float4 a = acc[pl];
+        for (int sl = 0; sl < P.ns; sl++) {
+            const float4 r = B.res[(size_t)sl * (size_t)P.npix + (size_t)pl];
+            const float4 c = clamp_sample(P, r);
+            a.x = a.x + c.x; a.y = a.y + c.y; a.z = a.z + c.z;
+            a.w = a.w + r.w;
+            segs += (uint32_t)r.w;
         }
-        const uint32_t tot = wave_sum(segs);
-        if (__lane_id() == 0 && tot) atomicAdd(&B.totals[0], (unsigned long long)tot);
+        acc[pl] = a;
     }
+    const unsigned long long t = block_sum(segs, lds);
+    if (threadIdx.x == 0) B.tally[blockIdx.x] += t;
 }
 
 
 // ------------------------------------------------------------------------------ adjoint (path replay)
 // Backward pass of the differentiable render: every sample is replayed from its key with the
 // forward pass's total radiance, and d loss / d (Color, Emission) of each material is summed
-// per workgroup in LDS, then once per workgroup into the global gradient.
+// per workgroup in LDS, then once per workgroup into one of kGradCopies copies of the global
+// gradient (chosen by workgroup index, i.e. by XCD), which the host adds up: eight copies
+// divide the same-address atomic traffic of the flush by eight.
 constexpr int kAdjMaxMat = 1024;
+constexpr int kGradCopies = 8;
 
 __device__ __forceinline__ void adj_lds_init(float* lds, int n) {
     for (int i = threadIdx.x; i < n; i += kBlock) lds[i] = 0.0f;
     __syncthreads();
 }
-__device__ __forceinline__ void adj_lds_flush(const float* lds, int n, float* __restrict__ grad) {
+// grad: copy 0 of the gradient (nmat x 6 floats: colour rgb then emission rgb); `off`
+// selects the part that lds holds.
+__device__ __forceinline__ void adj_lds_flush(const float* lds, int n, float* __restrict__ grad, int nmat, int off) {
     __syncthreads();
+    float* g = grad + (size_t)(blockIdx.x % kGradCopies) * (size_t)(nmat * 6) + off;
     for (int i = threadIdx.x; i < n; i += kBlock)
-        if (lds[i] != 0.0f) atomicAdd(&grad[i], lds[i]);
+        if (lds[i] != 0.0f) atomicAdd(&g[i], lds[i]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_raygen_adj(RenderParams P, PathBufs B, const float4* __restrict__ keep,
-                                                       float4* __restrict__ Lt) {
-    const int n = P.ns * P.npix;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        B.cnt[0] = (uint32_t)n;
-        B.cnt[1] = 0;
-        B.cnt[2] = 0;
-    }
-    for (int j = blockIdx.x * kBlock + threadIdx.x; j < n; j += gridDim.x * kBlock) {
-        const int sl = j / P.npix, pl = j - sl * P.npix;
-        init_camera_path(P, B, (uint32_t)j, pl, P.s0 + sl);
-        B.q[0][j] = (uint32_t)j;
-        Lt[j] = keep[(size_t)(P.s0 + sl) * (size_t)P.npix + (size_t)pl];
+                                                       float4* __restrict__ Lt, uint32_t nseg) {
+    const uint32_t n = (uint32_t)P.ns * (uint32_t)P.npix;
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t cnt = seg_slots(seg, n), base = seg * kSeg;
+        for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
+            const uint32_t j = base + k;
+            const uint32_t sl = j / (uint32_t)P.npix, pl = j - sl * (uint32_t)P.npix;
+            init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
+            B.q[0][j] = j;
+            Lt[j] = keep[(size_t)(P.s0 + (int)sl) * (size_t)P.npix + (size_t)pl];
+        }
+        if (threadIdx.x == 0) B.qc[0][seg] = cnt;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneBufs S, PathBufs B, RenderParams P, int cur, int bounce,
-                                                      uint32_t* __restrict__ guides, Adj A, float* __restrict__ grad) {
+                                                      uint32_t* __restrict__ guides, Adj A, float* __restrict__ grad,
+                                                      uint32_t nseg) {
     __shared__ float lds[2 * kAdjMaxMat * 3];
+    __shared__ uint32_t rl[2 * 2 * kWaves];
+    uint32_t parity = 0;
     const int ng = P.nmat * 6;
     adj_lds_init(lds, ng);
     A.gcol = lds;
     A.gle = lds + P.nmat * 3;
-    const uint32_t n = B.cnt[cur];
-    const uint32_t stride = gridDim.x * kBlock;
-    const uint32_t nloop = (n + stride - 1) / stride * stride;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nloop; i += stride) {
-        bool cont = false, shadow = false;
-        uint32_t j = 0;
-        float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
-        if (i < n) {
-            j = B.q[cur][i];
-            cont = shade_path<true>(S, B, P, j, bounce, guides, shadow, so, sd, sc, A) && bounce + 1 < P.bounces;
+    const uint32_t* q = B.q[cur];
+    uint32_t* qn = B.q[cur ^ 1];
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        uint32_t bc = 0, bs = 0;
+        for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+            const uint32_t k = r0 + threadIdx.x;
+            bool cont = false, shadow = false;
+            uint32_t j = 0;
+            float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
+            if (k < n) {
+                j = q[base + k];
+                cont = shade_path<true>(S, B, P, j, bounce, guides, shadow, so, sd, sc, A) && bounce + 1 < P.bounces;
+            }
+            const bool pr[2] = {cont, shadow};
+            uint32_t rk[2], tot[2];
+            block_rank<2>(pr, rk, tot, rl, parity);
+            if (cont) qn[base + bc + rk[0]] = j;
+            if (shadow) {
+                B.sq[base + bs + rk[1]] = j;
+                B.so[j] = so;
+                B.sd[j] = sd;
+                B.sc[j] = sc;
+            }
+            bc += tot[0];
+            bs += tot[1];
         }
-        uint32_t slot = 0;
-        wave_append(&B.cnt[cur ^ 1], cont, slot);
-        if (cont) B.q[cur ^ 1][slot] = j;
-        uint32_t sslot = 0;
-        wave_append(&B.cnt[2], shadow, sslot);
-        if (shadow) {
-            B.sq[sslot] = j;
-            B.so[j] = so;
-            B.sd[j] = sd;
-            B.sc[j] = sc;
+        if (threadIdx.x == 0) {
+            B.qc[cur ^ 1][seg] = bc;
+            B.sqc[seg] = bs;
         }
     }
-    adj_lds_flush(lds, ng, grad);
+    adj_lds_flush(lds, ng, grad, P.nmat, 0);
 }
 
 template <int STACK>
 __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, RenderParams P, Adj A,
-                                                       float* __restrict__ grad) {
+                                                       float* __restrict__ grad, uint32_t nseg) {
     __shared__ uint2 stk[STACK * kBlock];
     __shared__ float lds[kAdjMaxMat * 3];
     const int ng = P.nmat * 3;
     adj_lds_init(lds, ng);
     A.gle = lds;
-    const uint32_t n = B.cnt[2];
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const uint32_t j = B.sq[i];
-        const float4 o = B.so[j], d = B.sd[j];
-        Ray r;
-        r.ox = o.x; r.oy = o.y; r.oz = o.z;
-        r.dx = d.x; r.dy = d.y; r.dz = d.z;
-        r.tmin = (float)1e-4;
-        const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
-        if (h.id < 0) {
-            const float4 c = B.sc[j];
-            const float4 l = B.L[j];
-            B.L[j] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
-            const uint32_t pl = B.meta[j].x - (uint32_t)P.y0 * (uint32_t)P.width;
-            const float3 g = mul(f3(A.dl[3 * pl], A.dl[3 * pl + 1], A.dl[3 * pl + 2]), A.inv_spp);
-            const float4 w = A.sg[j];
-            adj_emission(A, (int)__float_as_uint(w.w), g, xyz(w));
+    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+        const uint32_t n = B.sqc[seg], base = seg * kSeg;
+        for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
+            const uint32_t j = B.sq[base + k];
+            const float4 o = B.so[j], d = B.sd[j];
+            Ray r;
+            r.ox = o.x; r.oy = o.y; r.oz = o.z;
+            r.dx = d.x; r.dy = d.y; r.dz = d.z;
+            r.tmin = (float)1e-4;
+            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
+            if (h.rec == 0xffffffffu) {
+                const float4 c = B.sc[j];
+                const float4 l = B.L[j];
+                B.L[j] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+                const uint32_t pl = B.meta[j].x - (uint32_t)P.y0 * (uint32_t)P.width;
+                const float3 g = mul(f3(A.dl[3 * pl], A.dl[3 * pl + 1], A.dl[3 * pl + 2]), A.inv_spp);
+                const float4 w = A.sg[j];
+                adj_emission(A, (int)__float_as_uint(w.w), g, xyz(w));
+            }
         }
     }
-    adj_lds_flush(lds, ng, grad + P.nmat * 3);
+    adj_lds_flush(lds, ng, grad, P.nmat, P.nmat * 3);
 }
 
 }  // namespace bcd
@@ -435,15 +532,15 @@ struct RenderState {
     int keep_y0 = -1, keep_rows = 0;
     uint64_t keep_topo = 0;
     bcd::PathBufs B{};
+    uint32_t nseg = 0;  // segments of the current allocation
     float4* acc = nullptr;
     uint32_t* guide_tmp = nullptr;
     bcd::RenderParams* Pd = nullptr;
     bcd::RenderParams* h_params = nullptr;  // pinned staging for Pd
     uint32_t* h_cnt = nullptr;              // pinned
-    // captured iteration graph of the persistent schedule
+    // captured iteration graph of the persistent schedule and what its kernel arguments hold
     hipGraphExec_t exec = nullptr;
-    const void* g_key[13] = {};
-    int g_iters = 0, g_stack = 0, g_grid = 0;
+    std::vector<const void*> g_key;
     // autotuner cache: (shape) -> (pool, iterations per graph)
     struct Tune {
         int w, rows, spp, bounces;
@@ -458,6 +555,9 @@ namespace {
 
 constexpr size_t kResultCap = (size_t)1 << 25;  // per-sample results held per window (512 MB)
 
+size_t segments_for(size_t paths) { return (paths + bcd::kSeg - 1) / bcd::kSeg; }
+
+// Allocates (or keeps) path state for `paths` slots in whole segments.
 bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t window_samples) {
     if (!c->rs) c->rs = new RenderState();
     RenderState& R = *c->rs;
@@ -467,17 +567,19 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
         R.h_cnt = (uint32_t*)h;
         R.h_params = (bcd::RenderParams*)((char*)h + 256);
     }
+    const size_t nseg = segments_for(paths), slots = nseg * bcd::kSeg;
     struct Want { int slot; size_t bytes; };
     const Want w[] = {
-        {0, paths * 16},  {1, paths * 16},  {2, paths * 16}, {3, paths * 16},  {4, paths * 16},
-        {5, paths * 16},  {6, paths * 4},   {7, paths * 8},  {8, paths * 16},  {9, paths * 16},
-        {10, paths * 16}, {11, paths * 4},  {12, paths * 4}, {13, paths * 4},  {14, 64},
-        {15, npix * 16 + (guides ? npix * 12 : 0) + 64},     {16, paths * 4},  {17, paths * 4},
+        {0, slots * 16},  {1, slots * 16},  {2, slots * 16}, {3, slots * 16},  {4, slots * 16},
+        {5, slots * 16},  {6, slots * 4},   {7, slots * 8},  {8, slots * 16},  {9, slots * 16},
+        {10, slots * 16}, {11, slots * 4},  {12, slots * 4}, {13, slots * 4},  {14, 64},
+        {15, npix * 16 + (guides ? npix * 12 : 0) + 64},     {16, slots * 4},  {17, slots * 4},
         {18, window_samples * 16},                           {19, sizeof(bcd::RenderParams)},
-        {20, paths * 16},
+        {20, slots * 16}, {21, slots * 4}, {22, nseg * 5 * 4}, {23, ((size_t)bcd::kMaxGrid + nseg) * 8},
     };
     for (const Want& x : w)
         if (!bc_ensure(c, R.buf[x.slot], R.bufcap[x.slot], std::max<size_t>(x.bytes, 64), "render buffers")) return false;
+    R.nseg = (uint32_t)nseg;
     R.B.o = (float4*)R.buf[0];
     R.B.d = (float4*)R.buf[1];
     R.B.T = (float4*)R.buf[2];
@@ -492,8 +594,8 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
     R.B.q[0] = (uint32_t*)R.buf[11];
     R.B.q[1] = (uint32_t*)R.buf[12];
     R.B.sq = (uint32_t*)R.buf[13];
-    R.B.cnt = (uint32_t*)R.buf[14];
-    R.B.totals = (unsigned long long*)((char*)R.buf[14] + 32);
+    R.B.totals = (unsigned long long*)R.buf[14];
+    R.B.flag = (uint32_t*)((char*)R.buf[14] + 32);
     R.B.bnc = (uint32_t*)R.buf[16];
     R.B.dq = (uint32_t*)R.buf[17];
     R.B.res = (float4*)R.buf[18];
@@ -501,7 +603,22 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
     R.guide_tmp = (uint32_t*)(R.acc + npix);
     R.Pd = (bcd::RenderParams*)R.buf[19];
     R.sg = (float4*)R.buf[20];
+    R.B.gid = (uint32_t*)R.buf[21];
+    uint32_t* sc = (uint32_t*)R.buf[22];
+    R.B.qc[0] = sc;
+    R.B.qc[1] = sc + nseg;
+    R.B.sqc = sc + 2 * nseg;
+    R.B.dqc = sc + 3 * nseg;
+    R.B.snext = sc + 4 * nseg;
+    R.B.tally = (unsigned long long*)R.buf[23];
     return true;
+}
+
+// One workgroup per segment: a segment's live paths are packed at its front, so a sparse
+// segment leaves the later waves of its workgroup idle, and they exit at once.
+int seg_grid(const bc_ctx*, size_t nseg) { return (int)nseg; }
+int pix_grid(const bc_ctx* c, size_t npix) {
+    return (int)std::min<size_t>({(npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8, (size_t)bcd::kMaxGrid});
 }
 
 bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, int rows, bool guides) {
@@ -520,24 +637,35 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
     P.npix = rows * d->width;
     P.nmat = c->nmat;
     P.nlight = c->nlight;
+    P.restir = d->restir_candidates;
+    for (int k = 0; k < 3; k++) {
+        P.sun_dir[k] = d->sun_dir[k];
+        P.sun[k] = d->sun_irradiance[k];
+    }
     return P;
 }
 
 template <int STACK>
 void launch_bounce(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur, int bounce,
-                   uint32_t* guides, int grid, hipStream_t st) {
-    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur);
-    hipLaunchKernelGGL(bcd::k_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides);
-    hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B);
+                   uint32_t* guides, int grid, uint32_t nseg, hipStream_t st) {
+    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg);
+    hipLaunchKernelGGL(bcd::k_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, nseg);
+    hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg);
 }
 
 template <int STACK>
 void launch_iteration(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd, int cur,
-                      uint32_t* guides, int grid, hipStream_t st) {
-    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur);
-    hipLaunchKernelGGL(bcd::k_p_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides);
-    hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B);
-    hipLaunchKernelGGL(bcd::k_p_regen, dim3(grid), dim3(bcd::kBlock), 0, st, B, Pd, cur);
+                      uint32_t* guides, int grid, uint32_t nseg, bool last, hipStream_t st) {
+    // the drain test reads the flag written by the last iteration of a replay only
+    if (last) hipLaunchKernelGGL(bcd::k_p_clear_flag, dim3(1), dim3(64), 0, st, B);
+    hipLaunchKernelGGL((bcd::k_p_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, nseg);
+    hipLaunchKernelGGL(bcd::k_p_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
+    hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg);
+}
+
+bool clear_tally(bc_ctx* c) {
+    RenderState& R = *c->rs;
+    return bc_check(c, hipMemsetAsync(R.B.tally, 0, ((size_t)bcd::kMaxGrid + R.nseg) * 8, c->stream), "clear tally");
 }
 
 int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, uint64_t* rays_out, uint64_t* shadow_out) {
@@ -545,6 +673,7 @@ int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, u
     hipStream_t st = c->stream;
     hipLaunchKernelGGL(bcd::k_finalize, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, (const float4*)R.acc,
                        (float4*)d_rgba);
+    hipLaunchKernelGGL(bcd::k_tally, dim3(1), dim3(bcd::kTallyBlock), 0, st, R.B, R.nseg);
     if (!bc_check(c, hipGetLastError(), "finalize launch") ||
         !bc_check(c, hipMemcpyAsync(R.h_cnt + 16, R.B.totals, 16, hipMemcpyDeviceToHost, st), "read totals") ||
         !bc_check(c, hipStreamSynchronize(st), "render"))
@@ -568,22 +697,23 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     RenderState& R = *c->rs;
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
-    const int grid_paths = (int)std::min<size_t>((paths + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
-    const int grid_pix = (int)std::min<size_t>((npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
+    const uint32_t nseg = (uint32_t)segments_for(paths);
+    const int grid = seg_grid(c, nseg);
+    const int grid_pix = pix_grid(c, npix);
     hipStream_t st = c->stream;
-    if (!bc_check(c, hipMemsetAsync(R.B.totals, 0, 16, st), "clear totals")) return BC_ERR_DEVICE;
+    if (!clear_tally(c)) return BC_ERR_DEVICE;
     uint32_t* guides = d_guides ? d_guides : R.guide_tmp;
     for (int s0 = 0; s0 < d->spp; s0 += spp_pass) {
         P.s0 = s0;
         P.ns = std::min(spp_pass, d->spp - s0);
-        hipLaunchKernelGGL(bcd::k_raygen, dim3(grid_paths), dim3(bcd::kBlock), 0, st, P, R.B, (float*)R.acc, guides);
+        hipLaunchKernelGGL(bcd::k_raygen, dim3(grid), dim3(bcd::kBlock), 0, st, P, R.B, (float*)R.acc, guides, nseg);
         for (int b = 0; b < d->bounces; b++) {
             const int cur = b & 1;
             switch (c->stack) {
-            case 8: launch_bounce<8>(S, R.B, P, cur, b, guides, grid_paths, st); break;
-            case 16: launch_bounce<16>(S, R.B, P, cur, b, guides, grid_paths, st); break;
-            case 24: launch_bounce<24>(S, R.B, P, cur, b, guides, grid_paths, st); break;
-            default: launch_bounce<32>(S, R.B, P, cur, b, guides, grid_paths, st); break;
+            case 8: launch_bounce<8>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
+            case 16: launch_bounce<16>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
+            case 24: launch_bounce<24>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
+            default: launch_bounce<32>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
             }
         }
         hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, R.B, R.acc, keep);
@@ -598,61 +728,61 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
     const size_t npix = (size_t)rows * (size_t)d->width;
     const int window_ns = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, kResultCap / npix));
     const size_t total_first = npix * (size_t)window_ns;
-    const size_t slots = std::min<size_t>((size_t)pool, total_first);
+    const size_t slots = segments_for(std::min<size_t>((size_t)std::max(pool, 1), total_first)) * bcd::kSeg;
     if (!ensure_state(c, slots, npix, d_guides != nullptr, total_first)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
-    const int grid = (int)std::min<size_t>((slots + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
-    const int grid_pix = (int)std::min<size_t>((npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
+    const uint32_t nseg = (uint32_t)segments_for(slots);
+    const int grid = seg_grid(c, nseg);
+    const int grid_pix = pix_grid(c, npix);
     hipStream_t st = c->stream;
     uint32_t* guides = d_guides ? d_guides : R.guide_tmp;
     iters = std::max(2, iters & ~1);
-    const void* key[13] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                           R.B.o, R.B.res, R.Pd, guides};
-    if (!R.exec || R.g_iters != iters || R.g_stack != c->stack || R.g_grid != grid ||
-        std::memcmp(key, R.g_key, sizeof key) != 0) {
+    std::vector<const void*> key = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf,
+                                    c->d_light_of, c->d_err, R.Pd, guides,
+                                    (const void*)(uintptr_t)nseg, (const void*)(uintptr_t)grid,
+                                    (const void*)(uintptr_t)iters, (const void*)(uintptr_t)c->stack};
+    for (void* p : R.buf) key.push_back(p);
+    if (!R.exec || key != R.g_key) {
         if (R.exec) { (void)hipGraphExecDestroy(R.exec); R.exec = nullptr; }
         hipGraph_t graph = nullptr;
         if (!bc_check(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture")) return BC_ERR_DEVICE;
         for (int it = 0; it < iters; it++) {
             const int cur = it & 1;
+            const bool last = it == iters - 1;
             switch (c->stack) {
-            case 8: launch_iteration<8>(S, R.B, R.Pd, cur, guides, grid, st); break;
-            case 16: launch_iteration<16>(S, R.B, R.Pd, cur, guides, grid, st); break;
-            case 24: launch_iteration<24>(S, R.B, R.Pd, cur, guides, grid, st); break;
-            default: launch_iteration<32>(S, R.B, R.Pd, cur, guides, grid, st); break;
+            case 8: launch_iteration<8>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
+            case 16: launch_iteration<16>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
+            case 24: launch_iteration<24>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
+            default: launch_iteration<32>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
             }
         }
         if (!bc_check(c, hipStreamEndCapture(st, &graph), "end capture")) return BC_ERR_DEVICE;
         hipError_t e = hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         if (!bc_check(c, e, "graph instantiate")) { R.exec = nullptr; return BC_ERR_DEVICE; }
-        std::memcpy(R.g_key, key, sizeof key);
-        R.g_iters = iters;
-        R.g_stack = c->stack;
-        R.g_grid = grid;
+        R.g_key = key;
     }
-    if (!bc_check(c, hipMemsetAsync(R.B.totals, 0, 16, st), "clear totals")) return BC_ERR_DEVICE;
+    if (!clear_tally(c)) return BC_ERR_DEVICE;
     int windows = 0;
     for (int s0 = 0; s0 < d->spp; s0 += window_ns) {
         P.s0 = s0;
         P.ns = std::min(window_ns, d->spp - s0);
         *R.h_params = P;
         if (!bc_check(c, hipMemcpyAsync(R.Pd, R.h_params, sizeof P, hipMemcpyHostToDevice, st), "params")) return BC_ERR_DEVICE;
-        hipLaunchKernelGGL(bcd::k_p_reset, dim3(1), dim3(64), 0, st, R.B);
-        const int gfill = (int)std::min<size_t>((std::max(slots, npix) + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
-        hipLaunchKernelGGL(bcd::k_p_fill, dim3(gfill), dim3(bcd::kBlock), 0, st, (const bcd::RenderParams*)R.Pd, R.B,
-                           R.acc, guides, (uint32_t)slots, s0 == 0 ? 1 : 0);
-        // every path ends within `bounces` iterations of its start, and each iteration starts
-        // at least min(slots, remaining) new samples: bound the replays accordingly
+        hipLaunchKernelGGL(bcd::k_p_start, dim3(std::max(grid, grid_pix)), dim3(bcd::kBlock), 0, st,
+                           (const bcd::RenderParams*)R.Pd, R.B, R.acc, guides, nseg, s0 == 0 ? 1 : 0);
+        // a slot holds one path for at most bounces + 1 iterations (the last one records it and
+        // restarts the slot), and every segment restarts all of its free slots while it has
+        // samples: bound the replays accordingly
         const size_t window_total = npix * (size_t)P.ns;
-        const size_t max_graphs = ((size_t)(d->bounces + 1) * (window_total / slots + 2)) / (size_t)iters + 4;
-        size_t g = 0;
-        for (;; g++) {
+        const size_t max_iters = (size_t)(d->bounces + 2) * (window_total / slots + 2) + 2;
+        const size_t max_graphs = max_iters / (size_t)iters + 4;
+        for (size_t g = 0;; g++) {
             if (g >= max_graphs) { c->err = "persistent schedule did not drain"; return BC_ERR_DEVICE; }
             if (!bc_check(c, hipGraphLaunch(R.exec, st), "graph launch") ||
-                !bc_check(c, hipMemcpyAsync(R.h_cnt, R.B.cnt, 4, hipMemcpyDeviceToHost, st), "queue length") ||
+                !bc_check(c, hipMemcpyAsync(R.h_cnt, R.B.flag, 4, hipMemcpyDeviceToHost, st), "drain flag") ||
                 !bc_check(c, hipStreamSynchronize(st), "graph"))
                 return BC_ERR_DEVICE;
             if (R.h_cnt[0] == 0) break;
@@ -730,7 +860,8 @@ extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int 
         return BC_ERR_ARG;
     }
     (void)hipSetDevice(c->device);
-    if (d->mode == 0) return render_plain(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out);
+    // zero transport segments render black on either schedule; the plain one does it without tracing
+    if (d->mode == 0 || d->bounces == 0) return render_plain(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out);
     if (!ensure_state(c, 1, (size_t)rows * (size_t)d->width, d_guides != nullptr, 1)) return BC_ERR_NOMEM;
     int pool = d->reserved[0], iters = d->reserved[1];
     if (pool <= 0 || iters <= 0) {
@@ -770,10 +901,12 @@ extern "C" int bc_render(bc_ctx* c, const bc_render_desc* d, int y0, int rows, f
 namespace {
 template <int STACK>
 void launch_bounce_adj(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur,
-                       int bounce, uint32_t* guides, const bcd::Adj& A, float* grad, int grid, hipStream_t st) {
-    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur);
-    hipLaunchKernelGGL(bcd::k_shade_adj, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, A, grad);
-    hipLaunchKernelGGL((bcd::k_shadow_adj<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, A, grad);
+                       int bounce, uint32_t* guides, const bcd::Adj& A, float* grad, int grid, int grid_adj,
+                       uint32_t nseg, hipStream_t st) {
+    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg);
+    hipLaunchKernelGGL(bcd::k_shade_adj, dim3(grid_adj), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, A, grad,
+                       nseg);
+    hipLaunchKernelGGL((bcd::k_shadow_adj<STACK>), dim3(grid_adj), dim3(bcd::kBlock), 0, st, S, B, P, A, grad, nseg);
 }
 bool same_shape(const bc_render_desc& a, const bc_render_desc& b) {
     return a.width == b.width && a.height == b.height && a.spp == b.spp && a.bounces == b.bounces &&
@@ -827,14 +960,14 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
     if (!ensure_state(c, paths, npix, false, paths)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     const size_t ng = (size_t)c->nmat * 6;
-    if (!bc_ensure(c, c->out_buf, c->out_cap, std::max(npix * 28 + 64, npix * 12 + ng * 4 + 64), "adjoint buffers"))
+    const size_t ngc = ng * bcd::kGradCopies;
+    if (!bc_ensure(c, c->out_buf, c->out_cap, std::max(npix * 28 + 64, npix * 12 + ngc * 4 + 64), "adjoint buffers"))
         return BC_ERR_NOMEM;
     float* d_dl = (float*)c->out_buf;
     float* d_grad = d_dl + npix * 3;
     hipStream_t st = c->stream;
     if (!bc_check(c, hipMemcpyAsync(d_dl, dl, npix * 12, hipMemcpyHostToDevice, st), "upload d loss") ||
-        !bc_check(c, hipMemsetAsync(d_grad, 0, ng * 4, st), "clear gradient") ||
-        !bc_check(c, hipMemsetAsync(R.B.totals, 0, 16, st), "clear totals"))
+        !bc_check(c, hipMemsetAsync(d_grad, 0, ngc * 4, st), "clear gradient"))
         return BC_ERR_DEVICE;
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, false);
@@ -843,31 +976,37 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
     A.inv_spp = 1.0f / (float)d->spp;
     A.Lt = R.B.res;
     A.sg = R.sg;
-    const int grid = (int)std::min<size_t>((paths + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8);
+    const uint32_t nseg = (uint32_t)segments_for(paths);
+    const int grid = seg_grid(c, nseg);
+    // the kernels that flush per-workgroup gradients walk several segments per workgroup
+    const int grid_adj = (int)std::min<size_t>(nseg, (size_t)c->num_cus * 16);
     for (int s0 = 0; s0 < d->spp; s0 += spp_pass) {
         P.s0 = s0;
         P.ns = std::min(spp_pass, d->spp - s0);
         hipLaunchKernelGGL(bcd::k_raygen_adj, dim3(grid), dim3(bcd::kBlock), 0, st, P, R.B, (const float4*)R.keep,
-                           R.B.res);
+                           R.B.res, nseg);
         for (int b = 0; b < d->bounces; b++) {
             const int cur = b & 1;
             switch (c->stack) {
-            case 8: launch_bounce_adj<8>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
-            case 16: launch_bounce_adj<16>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
-            case 24: launch_bounce_adj<24>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
-            default: launch_bounce_adj<32>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, st); break;
+            case 8: launch_bounce_adj<8>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
+            case 16: launch_bounce_adj<16>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
+            case 24: launch_bounce_adj<24>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
+            default: launch_bounce_adj<32>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
             }
         }
         if (!bc_check(c, hipGetLastError(), "adjoint launch")) return BC_ERR_DEVICE;
     }
-    std::vector<float> g(ng);
-    if (!bc_check(c, hipMemcpyAsync(g.data(), d_grad, ng * 4, hipMemcpyDeviceToHost, st), "read gradient") ||
+    std::vector<float> g(ngc);
+    if (!bc_check(c, hipMemcpyAsync(g.data(), d_grad, ngc * 4, hipMemcpyDeviceToHost, st), "read gradient") ||
         !bc_check(c, hipStreamSynchronize(st), "backward"))
         return BC_ERR_DEVICE;
     const size_t n3 = (size_t)c->nmat * 3;
-    for (size_t i = 0; i < n3; i++) {
-        dcolor[i] += g[i];
-        demission[i] += g[n3 + i];
+    for (int k = 0; k < bcd::kGradCopies; k++) {
+        const float* gk = g.data() + (size_t)k * ng;
+        for (size_t i = 0; i < n3; i++) {
+            dcolor[i] += gk[i];
+            demission[i] += gk[n3 + i];
+        }
     }
     return bc_check_kernel_error(c);
 }

@@ -23,10 +23,24 @@ struct RenderParams {
     int32_t y0, npix;          // chunk: first row, pixels in chunk
     int32_t s0, ns;            // pass: first sample, samples in this pass
     int32_t nmat, nlight;
+    int32_t restir;            // candidates of the resampled direct lighting (0 = one-sample NEE)
+    float sun_dir[3], sun[3];  // directional light (restir only)
 };
+
+// Path slots are grouped in segments of kSeg consecutive slots.  One workgroup owns a segment
+// in every kernel: it walks the segment's queues, and compacts the paths that continue (and the
+// shadow rays, and the terminated paths) into the same segment of the output queues, writing
+// the segment's lengths itself.  No queue needs a device-wide counter: a same-address atomic
+// from all eight XCDs costs ~10 ns on MI355X, which made one-atomic-per-wave appends the
+// dominant cost of shading.
+constexpr int kWaves = kBlock / 64;
+constexpr int kSegRounds = 1;  // workgroup rounds per segment: one path slot per lane
+constexpr uint32_t kSeg = (uint32_t)kBlock * kSegRounds;
+constexpr int kMaxGrid = 4096;  // largest grid of the per-pixel kernels (one tally word each)
 
 struct PathBufs {
     uint32_t* bnc;  // per-path bounce (persistent schedule)
+    uint32_t* gid;  // per-path window sample index, 0xffffffff = idle (persistent schedule)
     float4* res;    // per-sample result: rgb, path segments (persistent schedule)
     uint32_t* dq;   // terminated paths (persistent schedule)
     float4* o;      // ray origin
@@ -40,9 +54,14 @@ struct PathBufs {
     float4* so;     // shadow origin xyz, tmax w
     float4* sd;     // shadow dir
     float4* sc;     // shadow contribution
-    uint32_t* q[2];
-    uint32_t* sq;
-    uint32_t* cnt;  // [0],[1] queue lengths, [2] shadow queue length
+    uint32_t* q[2];  // path queues (segment s holds entries [s*kSeg, s*kSeg + qc[s]))
+    uint32_t* sq;    // shadow-ray queue
+    uint32_t* qc[2]; // per-segment queue lengths
+    uint32_t* sqc;   // per-segment shadow queue length
+    uint32_t* dqc;   // per-segment terminated-path count (persistent schedule)
+    uint32_t* snext; // per-segment next sample ordinal (persistent schedule)
+    uint32_t* flag;  // [0] != 0: some segment had work (persistent drain test)
+    unsigned long long* tally;   // path segments per pixel-kernel block [0, kMaxGrid), then shadow rays per segment
     unsigned long long* totals;  // [0] path segments, [1] shadow rays
 };
 
@@ -80,17 +99,36 @@ __device__ __forceinline__ float3 xyz(float4 v) { return f3(v.x, v.y, v.z); }
 #define PI_F ((float)3.141592653589793)
 #define INVPI_F ((float)0.3183098861837907)
 
-// Appends the calling lanes whose pred is true to a queue with one atomic per wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred, uint32_t& slot) {
-    const uint64_t m = __ballot(pred);
-    if (m == 0) return 0;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, (int)leader);
-    slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    return (uint32_t)__popcll(m);
+// Block-wide ranks of NQ predicates for one round of a segment walk: rank[q] is the number
+// of lanes of the workgroup below this lane whose pred[q] holds, total[q] the round's count.
+// lds holds 2 x NQ x kWaves words; consecutive rounds alternate halves (parity), so a single
+// barrier per round suffices.  Every thread of the workgroup must call it the same number of times.
+template <int NQ>
+__device__ __forceinline__ void block_rank(const bool (&pred)[NQ], uint32_t (&rank)[NQ], uint32_t (&total)[NQ],
+                                           uint32_t* lds, uint32_t& parity) {
+    const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+    uint32_t* L = lds + parity * (NQ * kWaves);
+    uint32_t below[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const uint64_t m = __ballot(pred[q]);
+        below[q] = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) L[q * kWaves + w] = (uint32_t)__popcll(m);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (int k = 0; k < kWaves; k++) {
+            const uint32_t c = L[q * kWaves + k];
+            before += (uint32_t)k < w ? c : 0u;
+            tot += c;
+        }
+        rank[q] = before + below[q];
+        total[q] = tot;
+    }
+    parity ^= 1u;
 }
 
 __device__ __forceinline__ double radical_inverse(uint64_t n, uint32_t base) {  // renderer.go:1844-1858

@@ -142,9 +142,11 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
 #pragma unroll
     for (int k = 0; k < 8; k++) ord |= ((ih >> ((uint32_t)k ^ oct)) & 1u) << k;
     ng_base = w1.x;
-    // bits 16-31: upper half of the (non-negative) smallest entry distance; truncation
-    // rounds towards zero, so it is a lower bound usable to cull the group after a pop
-    ng_bits = ord | (imask << 8) | (__float_as_uint(fmaxf(tnmin, 0.0f)) & 0xffff0000u);
+    // bits 16-31: upper half of the smallest entry distance.  Truncating a non-negative
+    // float rounds towards zero, so it is a lower bound usable to cull the group after a
+    // pop; a negative distance (only with tmin < 0) is stored as -inf, which never culls.
+    const uint32_t tnb = tnmin >= 0.0f ? (__float_as_uint(tnmin) & 0xffff0000u) : 0xff800000u;
+    ng_bits = ord | (imask << 8) | tnb;
     // leaf children: union of their primitive runs
     uint32_t lh = hits & ~imask & 0xffu, pm = 0;
     while (lh) {
@@ -191,16 +193,17 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
             if (sph_node && (ids[ri] & BC_SPHERE_BIT)) h = sphere_test(g, r, r.tmin, best.t, t);
             else h = tri_test(g, r, r.tmin, best.t, t);
             if (h) {
-                const int32_t id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
                 if (ANY) {
                     best.t = t;
-                    best.id = id;
+                    best.id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
                     best.rec = ri;
                     return best;
                 }
-                if (best.id < 0 || t < best.t || id < best.id) {
+                // the test admits t <= best.t; only an exact tie needs the primitive
+                // indices (smaller wins), so the id table is read in that rare case only
+                if (best.rec == 0xffffffffu || t < best.t ||
+                    (ids[ri] & ~BC_SPHERE_BIT) < (ids[best.rec] & ~BC_SPHERE_BIT)) {
                     best.t = t;
-                    best.id = id;
                     best.rec = ri;
                 }
             }
@@ -231,6 +234,7 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
             break;
         }
     }
+    if (!ANY && best.rec != 0xffffffffu) best.id = (int32_t)(ids[best.rec] & ~BC_SPHERE_BIT);
     return best;
 }
 

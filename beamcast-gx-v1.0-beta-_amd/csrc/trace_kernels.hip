@@ -23,7 +23,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const bc_node8* __restric
     r.dx = b.x; r.dy = b.y; r.dz = b.z;
     const HitRec h = traverse<ANY, STACK>(nodes, prims, ids, r, b.w, stk + threadIdx.x, kTraceBlock, err);
     if (ANY) {
-        reinterpret_cast<int32_t*>(out)[i] = h.id >= 0 ? 1 : 0;
+        reinterpret_cast<int32_t*>(out)[i] = h.rec != 0xffffffffu ? 1 : 0;
     } else {
         bc_hit o;
         o.t = h.id >= 0 ? h.t : 0.0f;

@@ -87,7 +87,13 @@ typedef struct bc_render_desc {
     double origin[3], lower[3], horiz[3], vert[3];   /* Camera.basis (renderer.go:947-960) */
     int32_t mode;               /* 0 plain wavefront, 1 persistent + graph */
     int32_t samples_per_pass;   /* 0 = automatic */
-    int32_t reserved[6];
+    int32_t reserved[6];        /* mode 1: [0] pool slots, [1] iterations per graph (0 = autotune) */
+    /* Direct lighting by resampled importance sampling (ReSTIR DI initial candidates; an
+     * extension, the Go integrator has one-sample NEE): restir_candidates > 0 draws that many
+     * candidates per diffuse vertex from {sun, sky, emitters} and keeps one in a reservoir. */
+    int32_t restir_candidates;
+    float sun_dir[3];           /* direction towards the sun */
+    float sun_irradiance[3];    /* rgb; 0 = no sun (needs restir_candidates > 0) */
 } bc_render_desc;
 
 /* Replaces renderOpenCL's chunk loop (gpu_opencl_windows.go:565-723): renders rows

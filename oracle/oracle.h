@@ -110,6 +110,9 @@ typedef struct orc_render_desc {
     uint32_t seed;
     float   sky_color[3];
     double  origin[3], lower[3], horiz[3], vert[3];
+    int32_t restir_candidates;  /* >0: resampled direct lighting over {sun, sky, emitters} */
+    float   sun_dir[3];         /* unit direction towards the sun */
+    float   sun_irradiance[3];
 } orc_render_desc;
 
 /* Renders rows [y0, y0+rows).  rgba: rows*W*4 (rgb mean radiance, w = path segments),

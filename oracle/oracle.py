@@ -22,6 +22,7 @@ class RenderDesc(ctypes.Structure):
         ("sampler", c_int32), ("mis", c_int32), ("power_lights", c_int32), ("sky_mode", c_int32),
         ("firefly_clamp", c_float), ("seed", c_uint32), ("sky_color", c_float * 3),
         ("origin", c_double * 3), ("lower", c_double * 3), ("horiz", c_double * 3), ("vert", c_double * 3),
+        ("restir_candidates", c_int32), ("sun_dir", c_float * 3), ("sun_irradiance", c_float * 3),
     ]
 
 
