@@ -30,10 +30,10 @@ namespace bcd {
 
 constexpr uint32_t kIdle = 0xffffffffu;
 
-// Slots of segment `seg` among the first `total` slots.
-__device__ __forceinline__ uint32_t seg_slots(uint32_t seg, uint32_t total) {
-    const uint32_t first = seg * kSeg;
-    return first >= total ? 0u : (total - first < kSeg ? total - first : kSeg);
+// Slots of segment `seg` (of `ss` slots each) among the first `total` slots.
+__device__ __forceinline__ uint32_t seg_slots(uint32_t seg, uint32_t ss, uint32_t total) {
+    const uint32_t first = seg * ss;
+    return first >= total ? 0u : (total - first < ss ? total - first : ss);
 }
 
 // Sum over a workgroup of NW waves; the result is valid in thread 0.
@@ -55,7 +55,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, f
                                                    uint32_t* __restrict__ guides, uint32_t nseg) {
     const uint32_t n = (uint32_t)P.ns * (uint32_t)P.npix;
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t cnt = seg_slots(seg, n), base = seg * kSeg;
+        const uint32_t cnt = seg_slots(seg, B.seg, n), base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
             const uint32_t j = base + k;
             const uint32_t sl = j / (uint32_t)P.npix, pl = j - sl * (uint32_t)P.npix;
@@ -80,7 +80,7 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int 
     __shared__ uint2 stk[STACK * kBlock];
     const uint32_t* q = B.q[cur];
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
             const uint32_t j = q[base + k];
             const float4 o = B.o[j], d = B.d[j];
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         uint32_t bc = 0, bs = 0;
         for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
             const uint32_t k = r0 + threadIdx.x;
@@ -140,7 +140,7 @@ template <int STACK>
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint32_t nseg) {
     __shared__ uint2 stk[STACK * kBlock];
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t n = B.sqc[seg], base = seg * kSeg;
+        const uint32_t n = B.sqc[seg], base = seg * B.seg;
         if (threadIdx.x == 0) B.tally[kMaxGrid + seg] += n;
         for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
             const uint32_t j = B.sq[base + k];
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kTallyBlock) void k_tally(PathBufs B, uint32_t nseg
 // that ends is listed in its segment's terminated list; the next extend of that segment stores
 // its radiance in the per-sample result slot and restarts the path slot in place with the
 // segment's next sample, then traces it with the segment's continuing paths.  The window's
-// samples are cut into chunks of kSeg consecutive (pixel, sample) indices and segment s owns
+// samples are cut into chunks of B.seg consecutive (pixel, sample) indices and segment s owns
 // chunks s, s + nseg, s + 2 nseg, ..., so a segment restarts runs of neighbouring pixels and
 // its camera rays stay coherent.  Samples are keyed by (pixel, sample), never by
 // slot, so the image does not depend on the schedule; per-sample results are reduced in sample
@@ -247,8 +247,8 @@ __global__ __launch_bounds__(kBlock) void k_p_start(const RenderParams* __restri
         }
     }
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t base = seg * kSeg;
-        for (uint32_t k = threadIdx.x; k < kSeg; k += kBlock) {
+        const uint32_t base = seg * B.seg;
+        for (uint32_t k = threadIdx.x; k < B.seg; k += kBlock) {
             B.dq[base + k] = base + k;
             B.gid[base + k] = kIdle;
         }
@@ -256,19 +256,19 @@ __global__ __launch_bounds__(kBlock) void k_p_start(const RenderParams* __restri
             B.qc[0][seg] = 0u;
             B.qc[1][seg] = 0u;
             B.sqc[seg] = 0u;
-            B.dqc[seg] = kSeg;
+            B.dqc[seg] = B.seg;
             B.snext[seg] = 0u;
         }
     }
 }
 
-// Samples of the window (W of them, in chunks of kSeg) owned by segment `seg` of `nseg`.
-__device__ __forceinline__ uint32_t seg_samples(uint32_t seg, uint32_t nseg, uint32_t W) {
-    const uint32_t chunks = (W + kSeg - 1) / kSeg;
+// Samples of the window (W of them, in chunks of ss) owned by segment `seg` of `nseg`.
+__device__ __forceinline__ uint32_t seg_samples(uint32_t seg, uint32_t nseg, uint32_t ss, uint32_t W) {
+    const uint32_t chunks = (W + ss - 1) / ss;
     if (seg >= chunks) return 0u;
     const uint32_t last = chunks - 1u;
-    uint32_t n = ((last - seg) / nseg + 1u) * kSeg;
-    if ((last - seg) % nseg == 0u) n -= chunks * kSeg - W;  // the short final chunk is this segment's
+    uint32_t n = ((last - seg) / nseg + 1u) * ss;
+    if ((last - seg) % nseg == 0u) n -= chunks * ss - W;  // the short final chunk is this segment's
     return n;
 }
 
@@ -285,8 +285,8 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
     uint32_t* q = B.q[cur];
     bool work = false;
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t n = B.qc[cur][seg], nd = B.dqc[seg], m0 = B.snext[seg], base = seg * kSeg;
-        const uint32_t owned = seg_samples(seg, nseg, W);
+        const uint32_t n = B.qc[cur][seg], nd = B.dqc[seg], m0 = B.snext[seg], base = seg * B.seg;
+        const uint32_t owned = seg_samples(seg, nseg, B.seg, W);
         const uint32_t avail = owned > m0 ? owned - m0 : 0u;
         const uint32_t nr = nd < avail ? nd : avail;
         for (uint32_t k = threadIdx.x; k < n + nd; k += kBlock) {
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
                 }
                 if (kk < nr) {
                     const uint32_t m = m0 + kk;
-                    const uint32_t gn = ((m / kSeg) * nseg + seg) * kSeg + m % kSeg;
+                    const uint32_t gn = ((m / B.seg) * nseg + seg) * B.seg + m % B.seg;
                     const uint32_t sl = gn / (uint32_t)P.npix, pl = gn - sl * (uint32_t)P.npix;
                     init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
                     B.gid[j] = gn;
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, con
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         uint32_t bc = 0, bd = 0, bs = 0;
         for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
             const uint32_t k = r0 + threadIdx.x;
@@ -429,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen_adj(RenderParams P, PathBufs 
                                                        float4* __restrict__ Lt, uint32_t nseg) {
     const uint32_t n = (uint32_t)P.ns * (uint32_t)P.npix;
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t cnt = seg_slots(seg, n), base = seg * kSeg;
+        const uint32_t cnt = seg_slots(seg, B.seg, n), base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
             const uint32_t j = base + k;
             const uint32_t sl = j / (uint32_t)P.npix, pl = j - sl * (uint32_t)P.npix;
@@ -454,7 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneBufs S, PathBufs B, R
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t n = B.qc[cur][seg], base = seg * kSeg;
+        const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         uint32_t bc = 0, bs = 0;
         for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
             const uint32_t k = r0 + threadIdx.x;
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, 
     adj_lds_init(lds, ng);
     A.gle = lds;
     for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
-        const uint32_t n = B.sqc[seg], base = seg * kSeg;
+        const uint32_t n = B.sqc[seg], base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
             const uint32_t j = B.sq[base + k];
             const float4 o = B.so[j], d = B.sd[j];
@@ -545,20 +545,20 @@ struct RenderState {
     struct Tune {
         int w, rows, spp, bounces;
         uint64_t topo, geom;
-        int pool, iters;
+        int pool, iters, seg;
     };
     std::vector<Tune> tunes;
-    int last_pool = 0, last_iters = 0, last_windows = 0;
+    int last_pool = 0, last_iters = 0, last_windows = 0, last_seg = 0;
 };
 
 namespace {
 
 constexpr size_t kResultCap = (size_t)1 << 25;  // per-sample results held per window (512 MB)
 
-size_t segments_for(size_t paths) { return (paths + bcd::kSeg - 1) / bcd::kSeg; }
+size_t segments_for(size_t paths, uint32_t ss) { return (paths + ss - 1) / ss; }
 
-// Allocates (or keeps) path state for `paths` slots in whole segments.
-bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t window_samples) {
+// Allocates (or keeps) path state for `paths` slots in whole segments of `ss` slots.
+bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t window_samples, uint32_t ss) {
     if (!c->rs) c->rs = new RenderState();
     RenderState& R = *c->rs;
     if (!R.h_cnt) {
@@ -567,7 +567,7 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
         R.h_cnt = (uint32_t*)h;
         R.h_params = (bcd::RenderParams*)((char*)h + 256);
     }
-    const size_t nseg = segments_for(paths), slots = nseg * bcd::kSeg;
+    const size_t nseg = segments_for(paths, ss), slots = nseg * ss;
     struct Want { int slot; size_t bytes; };
     const Want w[] = {
         {0, slots * 16},  {1, slots * 16},  {2, slots * 16}, {3, slots * 16},  {4, slots * 16},
@@ -580,6 +580,7 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
     for (const Want& x : w)
         if (!bc_ensure(c, R.buf[x.slot], R.bufcap[x.slot], std::max<size_t>(x.bytes, 64), "render buffers")) return false;
     R.nseg = (uint32_t)nseg;
+    R.B.seg = ss;
     R.B.o = (float4*)R.buf[0];
     R.B.d = (float4*)R.buf[1];
     R.B.T = (float4*)R.buf[2];
@@ -617,6 +618,14 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
 // One workgroup per segment: a segment's live paths are packed at its front, so a sparse
 // segment leaves the later waves of its workgroup idle, and they exit at once.
 int seg_grid(const bc_ctx*, size_t nseg) { return (int)nseg; }
+
+// Plain schedule: the largest segment (up to kSegPlain) that still gives every CU eight
+// workgroups, so late, sparse bounces are packed into few waves without starving the chip.
+uint32_t plain_segment(const bc_ctx* c, size_t paths) {
+    uint32_t ss = bcd::kSegPlain;
+    while (ss > (uint32_t)bcd::kBlock && paths / ss < (size_t)c->num_cus * 8) ss >>= 1;
+    return ss;
+}
 int pix_grid(const bc_ctx* c, size_t npix) {
     return (int)std::min<size_t>({(npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8, (size_t)bcd::kMaxGrid});
 }
@@ -638,10 +647,13 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
     P.nmat = c->nmat;
     P.nlight = c->nlight;
     P.restir = d->restir_candidates;
-    for (int k = 0; k < 3; k++) {
-        P.sun_dir[k] = d->sun_dir[k];
-        P.sun[k] = d->sun_irradiance[k];
-    }
+    // sun direction normalised in double precision, then rounded
+    const double sx = d->sun_dir[0], sy = d->sun_dir[1], sz = d->sun_dir[2];
+    const double sl = std::sqrt(sx * sx + sy * sy + sz * sz);
+    P.sun_dir[0] = sl > 0 ? (float)(sx / sl) : 0.0f;
+    P.sun_dir[1] = sl > 0 ? (float)(sy / sl) : 0.0f;
+    P.sun_dir[2] = sl > 0 ? (float)(sz / sl) : 0.0f;
+    for (int k = 0; k < 3; k++) P.sun[k] = d->sun_irradiance[k];
     return P;
 }
 
@@ -693,11 +705,11 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     int spp_pass = d->samples_per_pass > 0 ? d->samples_per_pass : (int)std::max<size_t>(1, max_paths / npix);
     spp_pass = std::min(spp_pass, d->spp);
     const size_t paths = npix * (size_t)spp_pass;
-    if (!ensure_state(c, paths, npix, d_guides != nullptr, 1)) return BC_ERR_NOMEM;
+    if (!ensure_state(c, paths, npix, d_guides != nullptr, 1, plain_segment(c, paths))) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
-    const uint32_t nseg = (uint32_t)segments_for(paths);
+    const uint32_t nseg = R.nseg;
     const int grid = seg_grid(c, nseg);
     const int grid_pix = pix_grid(c, npix);
     hipStream_t st = c->stream;
@@ -724,16 +736,16 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
 
 // Persistent schedule with in-place regeneration; `iters` iterations are one captured graph.
 int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides,
-                      uint64_t* rays_out, uint64_t* shadow_out, int pool, int iters) {
+                      uint64_t* rays_out, uint64_t* shadow_out, int pool, int iters, uint32_t ss) {
     const size_t npix = (size_t)rows * (size_t)d->width;
     const int window_ns = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, kResultCap / npix));
     const size_t total_first = npix * (size_t)window_ns;
-    const size_t slots = segments_for(std::min<size_t>((size_t)std::max(pool, 1), total_first)) * bcd::kSeg;
-    if (!ensure_state(c, slots, npix, d_guides != nullptr, total_first)) return BC_ERR_NOMEM;
+    const size_t slots = segments_for(std::min<size_t>((size_t)std::max(pool, 1), total_first), ss) * ss;
+    if (!ensure_state(c, slots, npix, d_guides != nullptr, total_first, ss)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
-    const uint32_t nseg = (uint32_t)segments_for(slots);
+    const uint32_t nseg = R.nseg;
     const int grid = seg_grid(c, nseg);
     const int grid_pix = pix_grid(c, npix);
     hipStream_t st = c->stream;
@@ -742,7 +754,7 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
     std::vector<const void*> key = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf,
                                     c->d_light_of, c->d_err, R.Pd, guides,
                                     (const void*)(uintptr_t)nseg, (const void*)(uintptr_t)grid,
-                                    (const void*)(uintptr_t)iters, (const void*)(uintptr_t)c->stack};
+                                    (const void*)(uintptr_t)iters, (const void*)(uintptr_t)c->stack, (const void*)(uintptr_t)ss};
     for (void* p : R.buf) key.push_back(p);
     if (!R.exec || key != R.g_key) {
         if (R.exec) { (void)hipGraphExecDestroy(R.exec); R.exec = nullptr; }
@@ -794,45 +806,53 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
     }
     R.last_pool = (int)slots;
     R.last_iters = iters;
+    R.last_seg = (int)ss;
     R.last_windows = windows;
     return finish(c, P, d_rgba, grid_pix, rays_out, shadow_out);
 }
 
-// Autotuner: times a short render (2 spp of the same chunk) for each pool size and graph
-// length and keeps the fastest for this image shape, scene and transport depth.
+// Autotuner: for each pool size, segment size and graph length, times a short render of the
+// same chunk with enough samples to refill the pool about three times (at least 2 spp), and
+// keeps the fastest per sample for this image shape, scene and transport depth.
 bool autotune(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides, int& pool,
-              int& iters) {
+              int& iters, uint32_t& seg) {
     RenderState& R = *c->rs;
     for (const auto& t : R.tunes)
         if (t.w == d->width && t.rows == rows && t.spp == d->spp && t.bounces == d->bounces && t.topo == c->topo &&
             t.geom == c->geom) {
             pool = t.pool;
             iters = t.iters;
+            seg = (uint32_t)t.seg;
             return true;
         }
     bc_render_desc trial = *d;
-    trial.spp = std::min(d->spp, 2);
     const size_t npix = (size_t)rows * (size_t)d->width;
     const int pools[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21};
     const int its[] = {4, 8, 16};
+    const uint32_t segs[] = {256, 1024};
     float best = 1e30f;
     pool = 1 << 20;
     iters = 8;
+    seg = bcd::kSegPersist;
     for (int p : pools) {
-        if ((size_t)p > npix * (size_t)trial.spp * 2 && p != pools[0]) continue;
-        for (int it : its) {
-            if (render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it) != BC_OK) return false;
-            float ms = 0;
-            if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
-                render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it) != BC_OK ||
-                !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
-                !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
-                !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
-                return false;
-            if (ms < best) { best = ms; pool = p; iters = it; }
-        }
+        if ((size_t)p > npix * (size_t)d->spp && p != pools[0]) continue;
+        trial.spp = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(2, (3 * (size_t)p + npix - 1) / npix));
+        for (uint32_t ss : segs)
+            for (int it : its) {
+                if (render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it, ss) != BC_OK)
+                    return false;
+                float ms = 0;
+                if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
+                    render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it, ss) != BC_OK ||
+                    !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
+                    !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
+                    !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
+                    return false;
+                ms /= (float)trial.spp;
+                if (ms < best) { best = ms; pool = p; iters = it; seg = ss; }
+            }
     }
-    R.tunes.push_back({d->width, rows, d->spp, d->bounces, c->topo, c->geom, pool, iters});
+    R.tunes.push_back({d->width, rows, d->spp, d->bounces, c->topo, c->geom, pool, iters, (int)seg});
     return true;
 }
 
@@ -859,23 +879,39 @@ extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int 
         c->err = "invalid render description or row range";
         return BC_ERR_ARG;
     }
+    if (d->restir_candidates < 0 || d->restir_candidates > BC_RIS_MAX) {
+        c->err = "restir_candidates must be in [0, 64]";
+        return BC_ERR_ARG;
+    }
+    if (d->restir_candidates == 0 && (d->sun_irradiance[0] != 0.0f || d->sun_irradiance[1] != 0.0f ||
+                                      d->sun_irradiance[2] != 0.0f)) {
+        c->err = "a sun (sun_irradiance != 0) needs restir_candidates > 0";
+        return BC_ERR_ARG;
+    }
     (void)hipSetDevice(c->device);
     // zero transport segments render black on either schedule; the plain one does it without tracing
     if (d->mode == 0 || d->bounces == 0) return render_plain(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out);
-    if (!ensure_state(c, 1, (size_t)rows * (size_t)d->width, d_guides != nullptr, 1)) return BC_ERR_NOMEM;
+    if (!ensure_state(c, 1, (size_t)rows * (size_t)d->width, d_guides != nullptr, 1, bcd::kSegPersist)) return BC_ERR_NOMEM;
     int pool = d->reserved[0], iters = d->reserved[1];
-    if (pool <= 0 || iters <= 0) {
-        if (!autotune(c, d, y0, rows, d_rgba, d_guides, pool, iters)) return BC_ERR_DEVICE;
+    uint32_t seg = d->reserved[2] > 0 ? (uint32_t)d->reserved[2] : bcd::kSegPersist;
+    if (seg % bcd::kBlock != 0 || seg > bcd::kSegMax) {
+        c->err = "segment size (reserved[2]) must be a multiple of 256 up to 4096";
+        return BC_ERR_ARG;
     }
-    return render_persistent(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out, pool, iters);
+    if (pool <= 0 || iters <= 0) {
+        if (!autotune(c, d, y0, rows, d_rgba, d_guides, pool, iters, seg)) return BC_ERR_DEVICE;
+    }
+    return render_persistent(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out, pool, iters, seg);
 }
 
-extern "C" int bc_render_tuning(const bc_ctx* c, int32_t* out4) {
-    if (!c || !out4) return BC_ERR_ARG;
-    out4[0] = c->rs ? c->rs->last_pool : 0;
-    out4[1] = c->rs ? c->rs->last_iters : 0;
-    out4[2] = c->stack;
-    out4[3] = c->rs ? c->rs->last_windows : 0;
+extern "C" int bc_render_tuning(const bc_ctx* c, int32_t* out8) {
+    if (!c || !out8) return BC_ERR_ARG;
+    out8[0] = c->rs ? c->rs->last_pool : 0;
+    out8[1] = c->rs ? c->rs->last_iters : 0;
+    out8[2] = c->stack;
+    out8[3] = c->rs ? c->rs->last_windows : 0;
+    out8[4] = c->rs ? c->rs->last_seg : 0;
+    for (int k = 5; k < 8; k++) out8[k] = 0;
     return BC_OK;
 }
 
@@ -913,7 +949,7 @@ bool same_shape(const bc_render_desc& a, const bc_render_desc& b) {
            a.rr_depth == b.rr_depth && a.sampler == b.sampler && a.mis == b.mis && a.power_lights == b.power_lights &&
            a.sky_mode == b.sky_mode && a.seed == b.seed && std::memcmp(a.origin, b.origin, sizeof a.origin) == 0 &&
            std::memcmp(a.lower, b.lower, sizeof a.lower) == 0 && std::memcmp(a.horiz, b.horiz, sizeof a.horiz) == 0 &&
-           std::memcmp(a.vert, b.vert, sizeof a.vert) == 0;
+           std::memcmp(a.vert, b.vert, sizeof a.vert) == 0 && a.restir_candidates == b.restir_candidates;
 }
 }  // namespace
 
@@ -927,9 +963,14 @@ extern "C" int bc_forward(bc_ctx* c, const bc_render_desc* d, int y0, int rows, 
         return BC_ERR_ARG;
     }
     if (c->nmat > bcd::kAdjMaxMat) { c->err = "the differentiable render supports at most 1024 materials"; return BC_ERR_ARG; }
+    if (d->restir_candidates != 0 || d->sun_irradiance[0] != 0.0f || d->sun_irradiance[1] != 0.0f ||
+        d->sun_irradiance[2] != 0.0f) {
+        c->err = "the differentiable render supports one-sample light sampling only (restir_candidates = 0, no sun)";
+        return BC_ERR_ARG;
+    }
     (void)hipSetDevice(c->device);
     const size_t npix = (size_t)rows * (size_t)d->width;
-    if (!ensure_state(c, 1, npix, false, 1)) return BC_ERR_NOMEM;
+    if (!ensure_state(c, 1, npix, false, 1, bcd::kSegPlain)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     if (!bc_ensure(c, R.keep, R.keep_cap, npix * (size_t)d->spp * 16, "forward sample buffer")) return BC_ERR_NOMEM;
     if (!bc_ensure(c, c->out_buf, c->out_cap, npix * 28 + 64, "render output")) return BC_ERR_NOMEM;
@@ -957,7 +998,7 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
     const size_t max_paths = std::max<size_t>(npix, (size_t)1 << 22);
     const int spp_pass = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, max_paths / npix));
     const size_t paths = npix * (size_t)spp_pass;
-    if (!ensure_state(c, paths, npix, false, paths)) return BC_ERR_NOMEM;
+    if (!ensure_state(c, paths, npix, false, paths, plain_segment(c, paths))) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     const size_t ng = (size_t)c->nmat * 6;
     const size_t ngc = ng * bcd::kGradCopies;
@@ -976,7 +1017,7 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
     A.inv_spp = 1.0f / (float)d->spp;
     A.Lt = R.B.res;
     A.sg = R.sg;
-    const uint32_t nseg = (uint32_t)segments_for(paths);
+    const uint32_t nseg = R.nseg;
     const int grid = seg_grid(c, nseg);
     // the kernels that flush per-workgroup gradients walk several segments per workgroup
     const int grid_adj = (int)std::min<size_t>(nseg, (size_t)c->num_cus * 16);

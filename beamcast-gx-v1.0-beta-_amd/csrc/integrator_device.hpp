@@ -27,18 +27,21 @@ struct RenderParams {
     float sun_dir[3], sun[3];  // directional light (restir only)
 };
 
-// Path slots are grouped in segments of kSeg consecutive slots.  One workgroup owns a segment
-// in every kernel: it walks the segment's queues, and compacts the paths that continue (and the
-// shadow rays, and the terminated paths) into the same segment of the output queues, writing
-// the segment's lengths itself.  No queue needs a device-wide counter: a same-address atomic
-// from all eight XCDs costs ~10 ns on MI355X, which made one-atomic-per-wave appends the
-// dominant cost of shading.
+// Path slots are grouped in segments of B.seg consecutive slots (a multiple of the workgroup
+// size).  One workgroup owns a segment in every kernel: it walks the segment's queues, and
+// compacts the paths that continue (and the shadow rays, and the terminated paths) into the
+// same segment of the output queues, writing the segment's lengths itself.  No queue needs a
+// device-wide counter: a same-address atomic from all eight XCDs costs ~10 ns on MI355X, which
+// made one-atomic-per-wave appends the dominant cost of shading.  Larger segments pack the
+// survivors of late bounces into fewer, fuller waves; smaller ones give more workgroups.
 constexpr int kWaves = kBlock / 64;
-constexpr int kSegRounds = 1;  // workgroup rounds per segment: one path slot per lane
-constexpr uint32_t kSeg = (uint32_t)kBlock * kSegRounds;
+constexpr uint32_t kSegPlain = 1024;     // segment of the plain schedule
+constexpr uint32_t kSegPersist = 256;    // default segment of the persistent schedule
+constexpr uint32_t kSegMax = 4096;
 constexpr int kMaxGrid = 4096;  // largest grid of the per-pixel kernels (one tally word each)
 
 struct PathBufs {
+    uint32_t seg;   // slots per segment
     uint32_t* bnc;  // per-path bounce (persistent schedule)
     uint32_t* gid;  // per-path window sample index, 0xffffffff = idle (persistent schedule)
     float4* res;    // per-sample result: rgb, path segments (persistent schedule)
@@ -54,7 +57,7 @@ struct PathBufs {
     float4* so;     // shadow origin xyz, tmax w
     float4* sd;     // shadow dir
     float4* sc;     // shadow contribution
-    uint32_t* q[2];  // path queues (segment s holds entries [s*kSeg, s*kSeg + qc[s]))
+    uint32_t* q[2];  // path queues (segment s holds entries [s*seg, s*seg + qc[s]))
     uint32_t* sq;    // shadow-ray queue
     uint32_t* qc[2]; // per-segment queue lengths
     uint32_t* sqc;   // per-segment shadow queue length
@@ -243,6 +246,168 @@ __device__ __forceinline__ float light_dir_pdf(const SceneBufs& S, const RenderP
 }
 
 
+// Background radiance seen along d (renderer.go:1326-1331): constant sky or the gradient.
+__device__ __forceinline__ float3 sky_radiance(const RenderParams& P, float3 d) {
+    if (P.sky_mode == 1) return f3(P.sky[0], P.sky[1], P.sky[2]);
+    const float3 u = unitv(d);
+    const float tt = (float)0.5 * (u.y + 1.0f);
+    return add(mul(f3((float)0.06, (float)0.08, (float)0.12), 1.0f - tt), mul(f3((float)0.32, (float)0.48, (float)0.72), tt));
+}
+
+__device__ __forceinline__ float lum(float3 v) { return (float)0.2126 * v.x + (float)0.7152 * v.y + (float)0.0722 * v.z; }
+
+// Direction of the cosine-weighted hemisphere sample (x, y, z) about N (the frame of
+// randCosHemisphere, renderer.go:1290-1311).
+__device__ __forceinline__ float3 hemi_dir(float3 N, float x, float y, float z) {
+    const float sign = N.z < 0.0f ? -1.0f : 1.0f;
+    const float a = -1.0f / (sign + N.z);
+    const float b = N.x * N.y * a;
+    const float3 tt = f3(1.0f + sign * N.x * N.x * a, sign * b, -sign * N.x);
+    const float3 bt = f3(b, sign + N.y * N.y * a, -N.y);
+    return add(add(mul(tt, x), mul(bt, y)), mul(N, z));
+}
+
+// Resampled direct lighting at a diffuse vertex (an extension; the Go integrator has
+// one-sample NEE): RIS over P.restir candidates drawn uniformly from the components {sun
+// (delta), emitters (chooseLight + point sampling of directLight), sky (cosine hemisphere)},
+// target = luminance of the unshadowed contribution, one streaming reservoir.  Each candidate
+// weighs target / (component probability x its own pdf); the survivor y carries
+// W = sum(w) / (M target(y)).  Returns false when nothing survives; otherwise the unshadowed
+// contribution F(y) W (before throughput) and the visibility ray: towards the point y when
+// `point`, else along the direction y.
+__device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParams& P, float3 p, float3 N,
+                                           float3 color, uint32_t key, int bounce, float3& contrib, float3& y_out,
+                                           bool& point_out) {
+    const int M = P.restir < BC_RIS_MAX ? P.restir : BC_RIS_MAX;
+    const float3 sd = f3(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
+    const float3 E = f3(P.sun[0], P.sun[1], P.sun[2]);
+    const bool sun_on = (E.x > 0.0f || E.y > 0.0f || E.z > 0.0f) && (sd.x != 0.0f || sd.y != 0.0f || sd.z != 0.0f);
+    const bool area_on = P.nlight > 0;
+    const int nc = 1 + (sun_on ? 1 : 0) + (area_on ? 1 : 0);
+    const float pc = 1.0f / (float)nc;
+    const float3 f = mul(color, INVPI_F);
+    float wsum = 0.0f, ph_s = 0.0f;
+    float3 F_s = f3(0.0f, 0.0f, 0.0f), y_s = F_s;
+    bool have = false, point_s = false;
+    for (int k = 0; k < M; k++) {
+        const uint32_t base = BC_DIM_RIS0 + ((uint32_t)bounce * 64u + (uint32_t)k) * BC_RIS_DIMS;
+        int ci = (int)(bc_rng(key, base) * (float)nc);
+        if (ci >= nc) ci = nc - 1;
+        int comp;  // 0 sun, 1 emitters, 2 sky
+        if (sun_on) comp = ci == 0 ? 0 : ((area_on && ci == 1) ? 1 : 2);
+        else comp = (area_on && ci == 0) ? 1 : 2;
+        float w = 0.0f, ph = 0.0f;
+        float3 F = f3(0.0f, 0.0f, 0.0f), y = F;
+        bool point = false;
+        if (comp == 0) {
+            const float cs = dot(N, sd);
+            if (cs > 0.0f) {
+                F = mul(had(f, E), cs);
+                ph = lum(F);
+                w = ph / pc;
+                y = sd;
+            }
+        } else if (comp == 1) {
+            const float u = bc_rng(key, base + 1u);
+            int li;
+            float sel;
+            if (!P.power) {
+                li = (int)(u * (float)P.nlight);
+                if (li >= P.nlight) li = P.nlight - 1;
+                sel = 1.0f / (float)P.nlight;
+            } else {
+                int lo = 0, hi = P.nlight;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (S.lights[mid].cdf >= u) hi = mid; else lo = mid + 1;
+                }
+                li = lo >= P.nlight ? P.nlight - 1 : lo;
+                sel = S.pdf[S.lights[li].prim];
+                if (sel <= 0.0f) sel = 1.0f / (float)P.nlight;
+            }
+            const bc_light& Lr = S.lights[li];
+            float3 qpt, lnrm;
+            const bool sph = Lr.kind == 0;
+            if (sph) {
+                float3 ln = f3(0.0f, 0.0f, 1.0f);
+                for (int t = 0; t < BC_NTRY_SPH; t++) {
+                    const float a = bc_rng(key, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                    const float b = bc_rng(key, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                    const float ss = a * a + b * b;
+                    if (ss >= 1.0f || ss <= (float)1e-18) continue;
+                    const float ff = 2.0f * sqrtf(1.0f - ss);
+                    ln = f3(a * ff, b * ff, 1.0f - 2.0f * ss);
+                    break;
+                }
+                qpt = add(f3(Lr.g0[0], Lr.g0[1], Lr.g0[2]), mul(ln, Lr.g0[3]));
+                lnrm = ln;
+            } else {
+                const float u1 = bc_rng(key, base + 2u), u2 = bc_rng(key, base + 3u);
+                const float su = sqrtf(u1);
+                const float b1 = su * (1.0f - u2), b2 = su * u2;
+                const float3 e1 = f3(Lr.g1[0], Lr.g1[1], Lr.g1[2]), e2 = f3(Lr.g2[0], Lr.g2[1], Lr.g2[2]);
+                qpt = add(add(f3(Lr.g0[0], Lr.g0[1], Lr.g0[2]), mul(e1, b1)), mul(e2, b2));
+                lnrm = unitv(cross(e1, e2));
+            }
+            const float3 to = sub(qpt, p);
+            const float dist2 = dot(to, to);
+            if (!(dist2 < (float)1e-8)) {
+                const float3 ld = unitv(to);
+                const float nd = rmax(0.0f, dot(N, ld));
+                const float cl = dot(mul(lnrm, -1.0f), ld);
+                const float cosL = sph ? rmax(0.0f, cl) : fabsf(cl);
+                if (!(nd <= 0.0f || cosL <= 0.0f)) {
+                    const float pin = sel * dist2 / (cosL * Lr.area);
+                    if (pin > 0.0f) {
+                        F = mul(had(f, f3(Lr.le[0], Lr.le[1], Lr.le[2])), nd);
+                        ph = lum(F);
+                        w = ph / (pc * pin);
+                        y = qpt;
+                        point = true;
+                    }
+                }
+            }
+        } else {
+            float x = 0.0f, yy = 0.0f, ss = 0.0f;
+            bool ok = false;
+            for (int t = 0; t < BC_NTRY_COS && !ok; t++) {
+                x = bc_rng(key, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                yy = bc_rng(key, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                ss = x * x + yy * yy;
+                ok = ss < 1.0f;
+            }
+            if (ok) {
+                const float z = sqrtf(rmax(0.0f, 1.0f - ss));
+                const float3 dir = hemi_dir(N, x, yy, z);
+                const float cv = rmax(0.0f, dot(N, dir));
+                if (cv > 0.0f) {
+                    const float pin = cv / PI_F;
+                    F = mul(had(f, sky_radiance(P, dir)), cv);
+                    ph = lum(F);
+                    w = ph / (pc * pin);
+                    y = dir;
+                }
+            }
+        }
+        if (w > 0.0f) {
+            wsum = wsum + w;
+            if (bc_rng(key, base + 20u) * wsum < w) {
+                have = true;
+                F_s = F;
+                ph_s = ph;
+                y_s = y;
+                point_s = point;
+            }
+        }
+    }
+    if (!have) return false;
+    const float W = wsum / ((float)M * ph_s);
+    contrib = mul(F_s, W);
+    y_out = y_s;
+    point_out = point_s;
+    return true;
+}
+
 // Adjoint state of the path-replay backward pass: per-pixel d loss / d image, the replayed
 // path's total radiance from the forward pass, and per-block gradient accumulators in LDS.
 struct Adj {
@@ -297,17 +462,11 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
         Lt = xyz(A.Lt[j]);
     }
     if (rec == 0xffffffffu) {  // background, renderer.go:1492-1495 / :1326-1331
-        float3 bg;
-        if (P.sky_mode == 1) {
-            bg = f3(P.sky[0], P.sky[1], P.sky[2]);
-        } else {
-            const float3 u = unitv(rd);
-            const float tt = (float)0.5 * (u.y + 1.0f);
-            bg = add(mul(f3((float)0.06, (float)0.08, (float)0.12), 1.0f - tt),
-                     mul(f3((float)0.32, (float)0.48, (float)0.72), tt));
+        // with resampled direct lighting the sky seen from a diffuse vertex is a candidate there
+        if (!(P.restir > 0 && B.pp[j].w != 0.0f)) {
+            L = add(L, had(T, sky_radiance(P, rd)));
+            B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
         }
-        L = add(L, had(T, bg));
-        B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
     } else {
         const float* gq = S.prims[rec].v;
         const uint32_t idbits = S.ids[rec];
@@ -335,7 +494,10 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
         if (M.kind == 3) {  // emissive: MIS weight on BSDF-sampled hits, renderer.go:1504-1511
             const float4 pp = B.pp[j];
             float w = 1.0f;
-            if (P.mis && pp.w != 0.0f && id >= 0) {
+            if (P.restir > 0 && pp.w != 0.0f) {
+                // registered emitters seen from a diffuse vertex are resampling candidates there
+                w = (id >= 0 && S.light_of[id] >= 0) ? 0.0f : 1.0f;
+            } else if (P.mis && pp.w != 0.0f && id >= 0) {
                 const float lpdf = light_dir_pdf(S, P, id, xyz(pp), p, N);
                 const float a2 = Tv.w * Tv.w, b2 = lpdf * lpdf;
                 w = (a2 + b2 <= (float)1e-20) ? 0.0f : a2 / (a2 + b2);
@@ -346,7 +508,34 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
             alive = false;
         } else if (M.kind == 0) {  // diffuse: NEE then cosine sampling
             if (ADJ) adj_color(A, mat, g, Lt, L, color);
-            if (P.nlight > 0) {   // directLight, renderer.go:1427-1465
+            if (!ADJ && P.restir > 0) {
+                float3 dl, y;
+                bool point;
+                if (ris_direct(S, P, p, N, color, key, bounce, dl, y, point)) {
+                    const float3 c = had(T, dl);
+                    const float3 from = add(p, mul(N, (float)2e-4));
+                    if (point) {  // visible(), renderer.go:1333-1341
+                        const float3 dd = sub(y, from);
+                        const float dd2 = dot(dd, dd);
+                        if (dd2 < (float)1e-12) {
+                            L = add(L, c);
+                            B.L[j] = make_float4(L.x, L.y, L.z, 0.0f);
+                        } else {
+                            const float dist = sqrtf(dd2);
+                            const float3 dir = mul(dd, 1.0f / dist);
+                            shadow_o = make_float4(from.x, from.y, from.z, dist - (float)1e-4);
+                            shadow_d = make_float4(dir.x, dir.y, dir.z, 0.0f);
+                            shadow_c = make_float4(c.x, c.y, c.z, 0.0f);
+                            shadow = true;
+                        }
+                    } else {
+                        shadow_o = make_float4(from.x, from.y, from.z, (float)1e30);
+                        shadow_d = make_float4(y.x, y.y, y.z, 0.0f);
+                        shadow_c = make_float4(c.x, c.y, c.z, 0.0f);
+                        shadow = true;
+                    }
+                }
+            } else if (P.nlight > 0) {   // directLight, renderer.go:1427-1465
                 const float u = bc_rng(key, dim + BC_D_LIGHT);
                 int li;
                 float sel;
@@ -440,12 +629,7 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
             }
             if (!ok) { x = 0.0f; y = 0.0f; ss = 0.0f; }
             const float z = sqrtf(rmax(0.0f, 1.0f - ss));
-            const float sign = N.z < 0.0f ? -1.0f : 1.0f;
-            const float a = -1.0f / (sign + N.z);
-            const float b = N.x * N.y * a;
-            const float3 tt = f3(1.0f + sign * N.x * N.x * a, sign * b, -sign * N.x);
-            const float3 bt = f3(b, sign + N.y * N.y * a, -N.y);
-            const float3 nd = add(add(mul(tt, x), mul(bt, y)), mul(N, z));
+            const float3 nd = hemi_dir(N, x, y, z);
             Tv.w = rmax(0.0f, dot(N, nd)) / PI_F;
             B.pp[j] = make_float4(p.x, p.y, p.z, 1.0f);
             ro = add(p, mul(N, (float)2e-4));

@@ -87,7 +87,8 @@ typedef struct bc_render_desc {
     double origin[3], lower[3], horiz[3], vert[3];   /* Camera.basis (renderer.go:947-960) */
     int32_t mode;               /* 0 plain wavefront, 1 persistent + graph */
     int32_t samples_per_pass;   /* 0 = automatic */
-    int32_t reserved[6];        /* mode 1: [0] pool slots, [1] iterations per graph (0 = autotune) */
+    int32_t reserved[6];        /* mode 1: [0] pool slots, [1] iterations per graph (0 for either =
+                                   autotune), [2] slots per segment (0 = 256; multiple of 256) */
     /* Direct lighting by resampled importance sampling (ReSTIR DI initial candidates; an
      * extension, the Go integrator has one-sample NEE): restir_candidates > 0 draws that many
      * candidates per diffuse vertex from {sun, sky, emitters} and keeps one in a reservoir. */
@@ -122,9 +123,10 @@ int bc_forward(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float*
 int bc_backward(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, const float* dloss_dimage,
                 float* dcolor, float* demission);
 
-/* Schedule last used by a mode-1 render: pool slots, iterations per graph, traversal stack
- * depth, sample windows.  No counterpart in the reference (its kernel has one fixed shape). */
-int bc_render_tuning(const bc_ctx* ctx, int32_t* out4);
+/* Schedule last used by a mode-1 render, 8 ints: pool slots, iterations per graph, traversal
+ * stack depth, sample windows, slots per segment, 0, 0, 0.  No counterpart in the reference
+ * (its kernel has one fixed shape). */
+int bc_render_tuning(const bc_ctx* ctx, int32_t* out8);
 
 /* Ray queries against the uploaded scene (the traversal of gpu_opencl_windows.go:758-777
  * and of Scene.Hit / Scene.Occluded, renderer.go:777-895).  rays: n x 8 floats

@@ -1007,6 +1007,20 @@ void orc_camera_basis(const double* pos, const double* target, const double* up,
 #define NTRY_SPH 8
 #define NTRY_COS 8
 #define NTRY_METAL 9
+/* resampled direct lighting: candidate k of bounce b draws dims DIM_RIS0 + (64 b + k) * RIS_DIMS + o,
+ * o = 0 component, 1 light pick, 2-3 triangle point, 4-19 sphere point / sky direction tries,
+ * 20 reservoir update */
+#define DIM_RIS0 0x100000u
+#define RIS_DIMS 32u
+#define RIS_MAX 64
+
+/* sun direction of a render description, normalised in double precision and rounded to float */
+static inline void orc_sun_dir(const orc_render_desc* d, float out[3]) {
+    double x = d->sun_dir[0], y = d->sun_dir[1], z = d->sun_dir[2];
+    double l = sqrt(x * x + y * y + z * z);
+    if (!(l > 0)) { out[0] = out[1] = out[2] = 0.0f; return; }
+    out[0] = (float)(x / l); out[1] = (float)(y / l); out[2] = (float)(z / l);
+}
 
 static inline double rnd64(uint32_t key, uint32_t dim) { return (double)(orc_rng_u32(key, dim) >> 8) * (1.0 / 16777216.0); }
 static inline float rnd32(uint32_t key, uint32_t dim) { return (float)(orc_rng_u32(key, dim) >> 8) * (1.0f / 16777216.0f); }

@@ -123,21 +123,63 @@ def test_same_seed_fp64_and_converged_mean(dev):
     assert abs(img2[..., :3].mean() / ref64[..., :3].mean() - 1) < 0.01
 
 
-@pytest.mark.parametrize("pool,iters", [(4096, 4), (50000, 8), (0, 0)])
-def test_persistent_schedule_equals_plain(dev, pool, iters):
+@pytest.mark.parametrize("pool,iters,segment", [(4096, 4, 0), (50000, 8, 1024), (3000, 6, 512), (0, 0, 0)])
+def test_persistent_schedule_equals_plain(dev, pool, iters, segment):
     """In-place regeneration, hipGraph replay and the autotuner (pool=0) change the schedule but
     not the image: per-sample results are reduced in sample order, so rgba is bit-identical."""
-    print(f"case: persistent pool={pool} iters={iters}", flush=True)
+    print(f"case: persistent pool={pool} iters={iters} segment={segment}", flush=True)
     s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
     dev.upload(s)
     d = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20)
     plain, gp, rp = dev.render(d, guides=True)
-    dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=pool, iters=iters)
+    dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=pool, iters=iters,
+                        segment=segment)
     pers, gq, rq = dev.render(dp, guides=True)
     assert rp == rq
     assert np.array_equal(plain, pers) and np.array_equal(gp, gq)
     t = dev.tuning()
-    assert t["pool"] > 0 and t["iters_per_graph"] >= 2
+    assert t["pool"] > 0 and t["iters_per_graph"] >= 2 and t["pool"] % t["segment"] == 0
+    if pool:
+        assert t["segment"] == (segment or 256)
     # a second call reuses the captured graph (and the tuned shape)
     again, _, _ = dev.render(dp, guides=True)
     assert np.array_equal(again, pers)
+
+
+SUN = dict(sun_dir=(0.3, 0.8, 0.2), sun=(3.0, 3.0, 3.0))
+
+
+@pytest.mark.parametrize("kind,param,sun,nee", [(bc.CLOSED_BOX, 0, False, True), (bc.TERRAIN, 48, True, False),
+                                                (bc.SHOWCASE, 0, True, False)])
+def test_restir_parity(dev, kind, param, sun, nee):
+    """Resampled direct lighting (32 candidates over sun, emitters and sky) matches the oracle per
+    pixel, including metal and glass vertices (Showcase) and the skipped BSDF-sampled light."""
+    print(f"case: restir scene {kind} param {param} sun {sun}", flush=True)
+    s = bc.HostScene.workload(kind, param=param, bins=16, leaf=4)
+    extra = SUN if sun else {}
+    d = bc.render_desc(s, 48, 32, 4, 6, rr_depth=3, sampler=1, sky_mode=1, sky_color=(0.3, 0.3, 0.3), restir=32,
+                       **extra)
+    img, _, _ = parity(dev, s, d, nee=nee, bins=16)
+    assert img[..., :3].mean() > 0.02
+
+
+def test_restir_persistent_equals_plain(dev):
+    print("case: restir persistent terrain 32 cells", flush=True)
+    s = bc.HostScene.workload(bc.TERRAIN, param=32, bins=16, leaf=4)
+    dev.upload(s)
+    kw = dict(rr_depth=3, sampler=1, sky_mode=1, sky_color=(0.3, 0.3, 0.3), restir=16, **SUN)
+    plain, _, rp = dev.render(bc.render_desc(s, 64, 48, 5, 8, **kw))
+    pers, _, rq = dev.render(bc.render_desc(s, 64, 48, 5, 8, mode=1, pool=5000, iters=4, **kw))
+    assert rp == rq and np.array_equal(plain, pers)
+
+
+def test_restir_argument_errors(dev):
+    print("case: restir argument errors", flush=True)
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s, nee_triangles=True)
+    with pytest.raises(bc.BeamcastError):
+        dev.render(bc.render_desc(s, 16, 16, 1, 2, **SUN))          # a sun needs restir > 0
+    with pytest.raises(bc.BeamcastError):
+        dev.render(bc.render_desc(s, 16, 16, 1, 2, restir=65))       # at most 64 candidates
+    with pytest.raises(bc.BeamcastError):
+        dev.forward(bc.render_desc(s, 16, 16, 1, 2, restir=8))       # adjoint: one-sample NEE only

@@ -313,3 +313,53 @@ def test_mis_is_unbiased():
     d = orc.desc_from(bc.render_desc(s, 48, 32, 256, 6, rr_depth=3, sampler=1, mis=True, power=False, seed=7))
     img2, _, _, _ = o.render(d, precision=64)
     assert abs(img2[..., :3].mean() / means[0] - 1) < 0.03
+
+
+def sunlit_plane_scene(albedo):
+    s = bc.HostScene()
+    m = s.add_material(bc.DIFFUSE, (albedo, albedo, albedo))
+    h = 60.0
+    s.add_triangle((-h, 0, -h), (-h, 0, h), (h, 0, h), m)
+    s.add_triangle((-h, 0, -h), (h, 0, h), (h, 0, -h), m)
+    s.build()
+    s.camera = np.array([0, 3, 3, 0, 0, 0, 0, 1, 0, 40.0])
+    return s
+
+
+@pytest.mark.parametrize("bounces", [1, 3])
+def test_ris_plane_closed_form(bounces):
+    """Resampled direct lighting on a diffuse plane under a constant sky Ls and a sun E from
+    direction s: the expected radiance is albedo * (Ls + E (n.s) / pi) whatever the candidate
+    split; the BSDF continuation escapes to the sky, which the resampled estimate already holds,
+    so more bounces add nothing."""
+    a, ls, e = 0.5, float(np.float32(0.3)), 3.0
+    sd = np.array([0.3, 0.8, 0.2]) / np.linalg.norm([0.3, 0.8, 0.2])
+    s = sunlit_plane_scene(a)
+    o = orc.Scene.from_host(s)
+    d = orc.desc_from(bc.render_desc(s, 24, 24, 16, bounces, rr_depth=8, sky_mode=1, sky_color=(ls, ls, ls),
+                                     restir=32, sun_dir=sd, sun=(e, e, e)))
+    img, _, _, shadow = o.render(d, precision=64)
+    want = np.float32(a) * (ls + e * sd[1] / np.pi)
+    got = img[..., :3].mean()
+    assert abs(got / want - 1) < 0.01
+    assert shadow == 24 * 24 * 16  # one visibility ray per diffuse vertex
+    # the sun alone (black sky): sky candidates weigh zero but still count in M, so each sample
+    # is (suns drawn / M) x 2 x the sun term, unbiased, and every rgb channel is equal
+    d.sky_color[0] = d.sky_color[1] = d.sky_color[2] = 0.0
+    img2, _, _, _ = o.render(d, precision=64)
+    sun = np.float32(a) * e * sd[1] / np.pi
+    assert abs(img2[..., :3].mean() / sun - 1) < 0.01
+    assert np.all(img2[..., 0] == img2[..., 2]) and img2[..., 0].min() > 0
+
+
+def test_ris_matches_one_sample_nee():
+    """Resampled direct lighting and NEE+MIS estimate the same image in the mean (closed box:
+    quad light, gradient sky through the open side)."""
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    o = orc.Scene.from_host(s, nee_triangles=True)
+    means = []
+    for restir in (0, 32):
+        d = orc.desc_from(bc.render_desc(s, 40, 40, 96, 4, rr_depth=3, sampler=1, restir=restir, seed=5))
+        img, _, _, _ = o.render(d, precision=64)
+        means.append(img[..., :3].reshape(-1, 3).mean(axis=0))
+    assert np.all(np.abs(means[1] / means[0] - 1) < 0.02)
