@@ -54,7 +54,7 @@ __device__ __forceinline__ unsigned long long block_sum(unsigned long long v, un
 __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, float* __restrict__ acc,
                                                    uint32_t* __restrict__ guides, uint32_t nseg) {
     const uint32_t n = (uint32_t)P.ns * (uint32_t)P.npix;
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t cnt = seg_slots(seg, B.seg, n), base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
             const uint32_t j = base + k;
@@ -79,7 +79,7 @@ template <int STACK>
 __global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg) {
     __shared__ uint2 stk[STACK * kBlock];
     const uint32_t* q = B.q[cur];
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
             const uint32_t j = q[base + k];
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
     uint32_t parity = 0;
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         uint32_t bc = 0, bs = 0;
         for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
@@ -139,7 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
 template <int STACK>
 __global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint32_t nseg) {
     __shared__ uint2 stk[STACK * kBlock];
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.sqc[seg], base = seg * B.seg;
         if (threadIdx.x == 0) B.tally[kMaxGrid + seg] += n;
         for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_p_start(const RenderParams* __restri
             }
         }
     }
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < B.seg; k += kBlock) {
             B.dq[base + k] = base + k;
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
     const uint32_t W = (uint32_t)P.ns * (uint32_t)P.npix;
     uint32_t* q = B.q[cur];
     bool work = false;
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.qc[cur][seg], nd = B.dqc[seg], m0 = B.snext[seg], base = seg * B.seg;
         const uint32_t owned = seg_samples(seg, nseg, B.seg, W);
         const uint32_t avail = owned > m0 ? owned - m0 : 0u;
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, con
     const RenderParams P = *Pd;
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         uint32_t bc = 0, bd = 0, bs = 0;
         for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
@@ -428,7 +428,7 @@ __device__ __forceinline__ void adj_lds_flush(const float* lds, int n, float* __
 __global__ __launch_bounds__(kBlock) void k_raygen_adj(RenderParams P, PathBufs B, const float4* __restrict__ keep,
                                                        float4* __restrict__ Lt, uint32_t nseg) {
     const uint32_t n = (uint32_t)P.ns * (uint32_t)P.npix;
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t cnt = seg_slots(seg, B.seg, n), base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < cnt; k += kBlock) {
             const uint32_t j = base + k;
@@ -453,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneBufs S, PathBufs B, R
     A.gle = lds + P.nmat * 3;
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
         uint32_t bc = 0, bs = 0;
         for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
@@ -494,7 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, 
     const int ng = P.nmat * 3;
     adj_lds_init(lds, ng);
     A.gle = lds;
-    for (uint32_t seg = blockIdx.x; seg < nseg; seg += gridDim.x) {
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.sqc[seg], base = seg * B.seg;
         for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
             const uint32_t j = B.sq[base + k];
@@ -619,6 +619,9 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
 // segment leaves the later waves of its workgroup idle, and they exit at once.
 int seg_grid(const bc_ctx*, size_t nseg) { return (int)nseg; }
 
+// Segments per XCD turn of the workgroup mapping (reserved[3]; 0 = 1, plain round-robin).
+uint32_t xcd_group(const bc_render_desc* d) { return d->reserved[3] > 0 ? (uint32_t)d->reserved[3] : 1u; }
+
 // Plain schedule: the largest segment (up to kSegPlain) that still gives every CU eight
 // workgroups, so late, sparse bounces are packed into few waves without starving the chip.
 uint32_t plain_segment(const bc_ctx* c, size_t paths) {
@@ -707,6 +710,7 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     const size_t paths = npix * (size_t)spp_pass;
     if (!ensure_state(c, paths, npix, d_guides != nullptr, 1, plain_segment(c, paths))) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
+    R.B.xg = xcd_group(d);
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
     const uint32_t nseg = R.nseg;
@@ -743,6 +747,7 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
     const size_t slots = segments_for(std::min<size_t>((size_t)std::max(pool, 1), total_first), ss) * ss;
     if (!ensure_state(c, slots, npix, d_guides != nullptr, total_first, ss)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
+    R.B.xg = xcd_group(d);
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
     const uint32_t nseg = R.nseg;
@@ -754,7 +759,8 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
     std::vector<const void*> key = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf,
                                     c->d_light_of, c->d_err, R.Pd, guides,
                                     (const void*)(uintptr_t)nseg, (const void*)(uintptr_t)grid,
-                                    (const void*)(uintptr_t)iters, (const void*)(uintptr_t)c->stack, (const void*)(uintptr_t)ss};
+                                    (const void*)(uintptr_t)iters, (const void*)(uintptr_t)c->stack, (const void*)(uintptr_t)ss,
+                                    (const void*)(uintptr_t)R.B.xg};
     for (void* p : R.buf) key.push_back(p);
     if (!R.exec || key != R.g_key) {
         if (R.exec) { (void)hipGraphExecDestroy(R.exec); R.exec = nullptr; }
@@ -1000,6 +1006,7 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
     const size_t paths = npix * (size_t)spp_pass;
     if (!ensure_state(c, paths, npix, false, paths, plain_segment(c, paths))) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
+    R.B.xg = xcd_group(d);
     const size_t ng = (size_t)c->nmat * 6;
     const size_t ngc = ng * bcd::kGradCopies;
     if (!bc_ensure(c, c->out_buf, c->out_cap, std::max(npix * 28 + 64, npix * 12 + ngc * 4 + 64), "adjoint buffers"))

@@ -42,6 +42,7 @@ constexpr int kMaxGrid = 4096;  // largest grid of the per-pixel kernels (one ta
 
 struct PathBufs {
     uint32_t seg;   // slots per segment
+    uint32_t xg;    // segments per XCD turn (see xcd_chunk)
     uint32_t* bnc;  // per-path bounce (persistent schedule)
     uint32_t* gid;  // per-path window sample index, 0xffffffff = idle (persistent schedule)
     float4* res;    // per-sample result: rgb, path segments (persistent schedule)

@@ -37,6 +37,18 @@ struct HitRec {
 
 __device__ __forceinline__ float u8f(uint32_t w, uint32_t sh) { return (float)((w >> sh) & 0xffu); }
 
+// Workgroups are dispatched to the eight XCDs of MI355X round-robin (workgroup b runs on XCD
+// b mod 8), and each XCD has its own L2.  This maps workgroup b to a work chunk so that the
+// eight XCDs take turns on runs of K consecutive chunks: the workgroups resident on one XCD at
+// a time then work on neighbouring rays (pixels, path segments) and share that XCD's L2, while
+// every region of the work is still spread over all XCDs (whole contiguous eighths per XCD
+// would leave XCDs idle when cost varies across the image).  A bijection on [0, nb).
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t b, uint32_t nb, uint32_t K) {
+    const uint32_t full = nb - nb % (8u * K);
+    if (b >= full) return b;
+    return (b / (8u * K)) * (8u * K) + (b & 7u) * K + (b >> 3) % K;
+}
+
 // triHit in fp32 (renderer.go:684-711) on a record g = A.xyz E1.xyz E2.xyz; t through tout.
 __device__ __forceinline__ bool tri_test(const float* __restrict__ g, const Ray& r, float tmin, float tmax,
                                          float& tout) {

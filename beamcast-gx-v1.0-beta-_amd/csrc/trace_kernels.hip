@@ -14,7 +14,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const bc_node8* __restric
                                                        const bc_ray* __restrict__ rays, int64_t n,
                                                        void* __restrict__ out, int* __restrict__ err) {
     __shared__ uint2 stk[STACK * kTraceBlock];
-    const int64_t i = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x, 64u) * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     const float4 a = rp[0], b = rp[1];

@@ -105,6 +105,72 @@ def bench_render(bc, dev, args, threads):
     return out
 
 
+def bench_terrain(bc, dev, args, threads):
+    """Procedural terrain: a (cells x cells) fBm heightfield (2 x 2047^2 = 8,380,418 triangles at the
+    default), 64 diffuse materials, sun from (0.3, 0.8, 0.2) with irradiance 3 plus a constant sky 0.3;
+    1920x1080, 64 spp, 8 segments, resampled direct lighting with 32 candidates.  Plain wavefront
+    schedule vs the persistent schedule (in-place regeneration, hipGraph replay, autotuned pool,
+    segment and graph length), and the oracle's fp64 integrator on a bounded band of rows."""
+    t0 = time.time()
+    s = bc.HostScene.workload(bc.TERRAIN, param=args.terrain_cells, bins=32, leaf=4)
+    build = time.time() - t0
+    t0 = time.time()
+    dev.upload(s)
+    upload = time.time() - t0
+    log(f"[bench] terrain: host SAH build {build:.1f}s, BVH8 flatten + upload {upload:.1f}s, {dev.stats()}")
+    w, h, spp, bounces = 1920, 1080, args.terrain_spp, 8
+    kw = dict(rr_depth=3, sampler=1, sky_mode=1, sky_color=(0.3, 0.3, 0.3), restir=32, sun_dir=(0.3, 0.8, 0.2),
+              sun=(3.0, 3.0, 3.0))
+    d = bc.render_desc(s, w, h, spp, bounces, **kw)
+    d_rgba = dev.alloc(w * h * 16)
+    dev.render_device(d, 0, h, d_rgba)
+    t0 = time.perf_counter()
+    segs, shadow = dev.render_device(d, 0, h, d_rgba)
+    sec = time.perf_counter() - t0
+    dp = bc.render_desc(s, w, h, spp, bounces, mode=1, **kw)
+    t0 = time.perf_counter()
+    dev.render_device(dp, 0, h, d_rgba)  # autotunes and captures once
+    tune = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    psegs, _ = dev.render_device(dp, 0, h, d_rgba)
+    psec = time.perf_counter() - t0
+    dev.free(d_rgba)
+    samples = w * h * spp
+    assert psegs == segs
+    log(f"[bench] terrain: plain {sec * 1e3:.1f} ms, persistent {psec * 1e3:.1f} ms (autotune + first {tune:.1f}s)")
+    out = {"workload": f"terrain {args.terrain_cells}^2 cells ({s.counts()[0]} tris, 64 materials), sun + sky 0.3, "
+                       f"{w}x{h}, {spp} spp Halton, {bounces} segments, RR from bounce 3, resampled direct "
+                       "lighting 32 candidates",
+           "ms_per_frame": round(sec * 1e3, 2), "path_samples_per_s": round(samples / sec, 1),
+           "mrays_s_segments": round(segs / sec / 1e6, 2), "mrays_s_total": round((segs + shadow) / sec / 1e6, 2),
+           "segments_per_sample": round(segs / samples, 4),
+           "persistent": {"ms_per_frame": round(psec * 1e3, 2), "path_samples_per_s": round(samples / psec, 1),
+                          "over_plain": round(sec / psec, 3), "tuning": dev.tuning(),
+                          "autotune_and_first_frame_s": round(tune, 2)}}
+    if not args.no_cpu:
+        import oracle as orc
+
+        t0 = time.time()
+        o = orc.Scene.from_host(s, bins=32, leaf=4)
+        log(f"[bench] terrain oracle build {time.time() - t0:.1f}s")
+        od = orc.desc_from(d)
+        y0, rows = h // 2, 1
+        t0 = time.perf_counter()
+        o.render(od, precision=64, y0=y0, rows=rows, threads=threads)
+        one = time.perf_counter() - t0
+        rows = int(max(1, min(64, args.cpu_seconds / 2 / max(one, 1e-3))))
+        t0 = time.perf_counter()
+        o.render(od, precision=64, y0=y0, rows=rows, threads=threads)
+        csec = time.perf_counter() - t0
+        cpu_rate = w * rows * spp / csec
+        out["cpu_baseline"] = {"path_samples_per_s": round(cpu_rate, 1), "cores": threads, "kind": "port",
+                               "sample": f"rows {y0}-{y0 + rows} of the same frame, fp64 trace restatement, "
+                                         f"{csec:.1f} s"}
+        out["gpu_over_cpu"] = round(samples / sec / cpu_rate, 1)
+        del o
+    return out
+
+
 def bench_adjoint(bc, dev):
     """Differentiable-render config: closed box retessellated to ~200k triangles with 64 materials,
     512x512, 32 spp, 6 segments; time of one forward and one path-replay backward pass."""
@@ -152,7 +218,10 @@ def main():
     ap.add_argument("--tris", type=int, default=1_000_000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline duration")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-render", action="store_true", help="skip the closed-box render measurement")
+    ap.add_argument("--no-render", action="store_true", help="skip the render, terrain and adjoint measurements")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the terrain render measurement")
+    ap.add_argument("--terrain-cells", type=int, default=2047)
+    ap.add_argument("--terrain-spp", type=int, default=64)
     ap.add_argument("--copy-check", action="store_true",
                     help="also run one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -274,6 +343,7 @@ def main():
 
     render = None if args.no_render else bench_render(bc, dev, args, threads)
     adjoint = None if args.no_render else bench_adjoint(bc, dev)
+    terrain = None if (args.no_render or args.no_terrain) else bench_terrain(bc, dev, args, threads)
 
     names = {("coherent", False): "coherent_closest", ("coherent", True): "coherent_any",
              ("incoherent", False): "incoherent_closest", ("incoherent", True): "incoherent_any"}
@@ -312,6 +382,7 @@ def main():
         "cpu_baseline": cpu,
         "render": render,
         "adjoint": adjoint,
+        "terrain": terrain,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
                                "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4)}
                     for k in launches},

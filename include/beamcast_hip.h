@@ -88,7 +88,8 @@ typedef struct bc_render_desc {
     int32_t mode;               /* 0 plain wavefront, 1 persistent + graph */
     int32_t samples_per_pass;   /* 0 = automatic */
     int32_t reserved[6];        /* mode 1: [0] pool slots, [1] iterations per graph (0 for either =
-                                   autotune), [2] slots per segment (0 = 256; multiple of 256) */
+                                   autotune), [2] slots per segment (0 = 256; multiple of 256);
+                                   any mode: [3] path segments per XCD turn (0 = 1) */
     /* Direct lighting by resampled importance sampling (ReSTIR DI initial candidates; an
      * extension, the Go integrator has one-sample NEE): restir_candidates > 0 draws that many
      * candidates per diffuse vertex from {sun, sky, emitters} and keeps one in a reservoir. */
