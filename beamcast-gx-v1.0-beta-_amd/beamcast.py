@@ -24,7 +24,7 @@ DIFFUSE, METAL, GLASS, EMISSIVE = 0, 1, 2, 3
 SPHERE, TRIANGLE = 0, 1
 
 # workload kinds of bch_workload_scene
-CLOSED_BOX, SOUP, SHOWCASE, CORNELL, STRESS, TERRAIN, BOX_RETESS = 1, 2, 3, 4, 5, 6, 7
+CLOSED_BOX, SOUP, SHOWCASE, CORNELL, STRESS, TERRAIN, BOX_RETESS, BATCH_SCENE = 1, 2, 3, 4, 5, 6, 7, 8
 
 
 class SceneView(ctypes.Structure):
@@ -71,6 +71,8 @@ def lib():
             "bc_render": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), vp]),
             "bc_render_device": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_render_tuning": (c_int, [vp, POINTER(c_int32)]),
+            "bc_batch_upload": (c_int, [vp, c_int, vp]),
+            "bc_batch_render_device": (c_int, [vp, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_forward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, POINTER(c_uint64)]),
             "bc_backward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, vp]),
             "bc_update_materials": (c_int, [vp, c_int32, vp, vp]),
@@ -349,6 +351,34 @@ class Device:
         self._check(lib().bc_render_device(self.ctx, ctypes.byref(desc), y0, rows, d_rgba, d_guides,
                                            ctypes.byref(rays), ctypes.byref(shadow)))
         return int(rays.value), int(shadow.value)
+
+    def upload_batch(self, scenes, nee_triangles=False):
+        """Packs several host scenes into one device arena (bc_batch_upload)."""
+        views = (SceneView * len(scenes))()
+        for i, sc in enumerate(scenes):
+            views[i] = sc.view(nee_triangles).contents
+        self._check(lib().bc_batch_upload(self.ctx, len(scenes), views))
+
+    def render_batch_device(self, descs, d_rgba):
+        """Renders every scene of the resident batch into device buffers; descs[0].mode picks the
+        schedule (0: scene after scene, plain; 1: one persistent schedule over all scenes)."""
+        n = len(descs)
+        da = (RenderDesc * n)(*descs)
+        ptrs = (c_void_p * n)(*[c_void_p(p) if not isinstance(p, c_void_p) else p for p in d_rgba])
+        rays, shadow = c_uint64(), c_uint64()
+        self._check(lib().bc_batch_render_device(self.ctx, n, da, ptrs, ctypes.byref(rays), ctypes.byref(shadow)))
+        return int(rays.value), int(shadow.value)
+
+    def render_batch(self, descs):
+        """Host-side convenience: list of rgba images (H x W x 4) and the path-segment count."""
+        bufs = [self.alloc(d.width * d.height * 16) for d in descs]
+        try:
+            rays, _ = self.render_batch_device(descs, bufs)
+            imgs = [self.download(b, np.empty((d.height, d.width, 4), np.float32)) for b, d in zip(bufs, descs)]
+        finally:
+            for b in bufs:
+                self.free(b)
+        return imgs, rays
 
 
 def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampler=1, mis=True, power=True,

@@ -3,10 +3,13 @@
 // context with a stream, scene buffers cached by revision, errors returned as status codes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/beamcast_hip.h"
@@ -40,6 +43,46 @@ bool upload_vec(bc_ctx* c, T*& dst, size_t& cap_bytes, const std::vector<T>& v, 
     }
     if (!v.empty() && !ok(c, hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), what))
         return false;
+    return true;
+}
+// Emissive primitives of a scene view as NEE lights: geometry and fp64 area (lightWeight,
+// renderer.go:1343-1357).  pbase / mbase offset the primitive and material indices (packed
+// scenes).
+void collect_lights(const bc_scene_view& v, int pbase, int mbase, std::vector<bc_light>& out,
+                    std::vector<double>& area_out) {
+    for (int i = 0; i < v.nprim; i++) {
+        const int m = v.prim_mat[i];
+        if (v.mat_kind[m] != 3) continue;
+        const double* g = v.prim_geom + 10 * (size_t)i;
+        bc_light L;
+        std::memset(&L, 0, sizeof L);
+        L.prim = pbase + i;
+        double area;
+        if (v.prim_kind[i] == 0) {
+            L.kind = 0;
+            area = 4 * M_PI * g[3] * g[3];
+            L.g0[0] = (float)g[0]; L.g0[1] = (float)g[1]; L.g0[2] = (float)g[2]; L.g0[3] = (float)g[3];
+        } else {
+            if (!v.nee_triangles) continue;
+            L.kind = 1;
+            const double cx = g[4] * g[8] - g[5] * g[7], cy = g[5] * g[6] - g[3] * g[8], cz = g[3] * g[7] - g[4] * g[6];
+            area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            for (int k = 0; k < 3; k++) {
+                L.g0[k] = (float)g[k]; L.g1[k] = (float)g[3 + k]; L.g2[k] = (float)g[6 + k];
+            }
+        }
+        L.area = (float)area;
+        uint32_t mbits = (uint32_t)(mbase + m);
+        std::memcpy(&L.g1[3], &mbits, 4);  // the light's material index
+        out.push_back(L);
+        area_out.push_back(area);
+    }
+}
+
+bool check_view(bc_ctx* c, const bc_scene_view& v) {
+    if (v.nmat <= 0) { c->err = "HIP backend requires at least one material"; return false; }
+    for (int i = 0; i < v.nprim; i++)
+        if (v.prim_mat[i] < 0 || v.prim_mat[i] >= v.nmat) { c->err = "primitive material out of range"; return false; }
     return true;
 }
 }  // namespace
@@ -128,46 +171,19 @@ const char* bc_device_name(const bc_ctx* c) { return c ? c->name.c_str() : ""; }
 
 int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t geom) {
     if (!c || !v) return BC_ERR_ARG;
-    if (v->nmat <= 0) { c->err = "HIP backend requires at least one material"; return BC_ERR_SCENE; }
+    if (!check_view(c, *v)) return BC_ERR_SCENE;
     hipSetDevice(c->device);
     bcx::Bvh8 b;
     std::string err;
     if (!bcx::build_bvh8(*v, b, err)) { c->err = err; return BC_ERR_SCENE; }
     const int stack = bcd::stack_for_depth(b.depth);
     if (stack < 0) { c->err = "BVH8 depth " + std::to_string(b.depth) + " exceeds the traversal stack (33 levels)"; return BC_ERR_SCENE; }
-    for (int i = 0; i < v->nprim; i++) {
-        if (v->prim_mat[i] < 0 || v->prim_mat[i] >= v->nmat) { c->err = "primitive material out of range"; return BC_ERR_SCENE; }
-    }
-    // emissive primitives: geometry and fp64 area (lightWeight, renderer.go:1343-1357)
+    c->has_scene = false;
+    c->batch.clear();
     c->h_lights.clear();
     c->h_light_area.clear();
-    for (int i = 0; i < v->nprim; i++) {
-        const int m = v->prim_mat[i];
-        if (v->mat_kind[m] != 3) continue;
-        const double* g = v->prim_geom + 10 * (size_t)i;
-        bc_light L;
-        std::memset(&L, 0, sizeof L);
-        L.prim = i;
-        double area;
-        if (v->prim_kind[i] == 0) {
-            L.kind = 0;
-            area = 4 * M_PI * g[3] * g[3];
-            L.g0[0] = (float)g[0]; L.g0[1] = (float)g[1]; L.g0[2] = (float)g[2]; L.g0[3] = (float)g[3];
-        } else {
-            if (!v->nee_triangles) continue;
-            L.kind = 1;
-            const double cx = g[4] * g[8] - g[5] * g[7], cy = g[5] * g[6] - g[3] * g[8], cz = g[3] * g[7] - g[4] * g[6];
-            area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-            for (int k = 0; k < 3; k++) {
-                L.g0[k] = (float)g[k]; L.g1[k] = (float)g[3 + k]; L.g2[k] = (float)g[6 + k];
-            }
-        }
-        L.area = (float)area;
-        uint32_t mbits = (uint32_t)m;
-        std::memcpy(&L.g1[3], &mbits, 4);  // the light's material index
-        c->h_lights.push_back(L);
-        c->h_light_area.push_back(area);
-    }
+    collect_lights(*v, 0, 0, c->h_lights, c->h_light_area);
+    c->light_groups.assign(1, std::make_pair(0, (int)c->h_lights.size()));
     c->nprim = v->nprim;
     c->nmat_scene = 0;
     {
@@ -195,7 +211,14 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
 // material data; also the cheap path for parameter updates of the differentiable render.
 int bc_update_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data) {
     if (!c || nmat <= 0 || !mat_kind || !mat_data) return BC_ERR_ARG;
+    if (!c->batch.empty()) { c->err = "material updates apply to a single resident scene, not a batch"; return BC_ERR_ARG; }
     if (c->nmat_scene && nmat != c->nmat_scene) { c->err = "material count differs from the uploaded scene"; return BC_ERR_ARG; }
+    return bc_set_materials(c, nmat, mat_kind, mat_data);
+}
+
+}  // extern "C"
+
+int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data) {
     std::vector<bc_mat> mats((size_t)nmat);
     for (int i = 0; i < nmat; i++) {
         const double* d = mat_data + 8 * (size_t)i;
@@ -211,25 +234,27 @@ int bc_update_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const 
     std::vector<float> pdf((size_t)c->nprim, 0.0f);
     std::vector<int32_t> light_of((size_t)c->nprim, -1);
     std::vector<double> cdf(nl);
-    double total = 0;
-    for (size_t i = 0; i < nl; i++) {
-        uint32_t mbits;
-        std::memcpy(&mbits, &lights[i].g1[3], 4);
-        if ((int)mbits >= nmat || mat_kind[mbits] != 3) { c->err = "a light's material is no longer emissive"; return BC_ERR_ARG; }
-        const double* md = mat_data + 8 * (size_t)mbits;
-        lights[i].le[0] = (float)md[5]; lights[i].le[1] = (float)md[6]; lights[i].le[2] = (float)md[7];
-        double wt = (0.2126 * md[5] + 0.7152 * md[6] + 0.0722 * md[7]) * c->h_light_area[i];
-        total += wt < 1e-9 ? 1e-9 : wt;
-        cdf[i] = total;
+    for (const auto& grp : c->light_groups) {  // one normalised distribution per scene
+        double total = 0;
+        for (int i = grp.first; i < grp.second; i++) {
+            uint32_t mbits;
+            std::memcpy(&mbits, &lights[(size_t)i].g1[3], 4);
+            if ((int)mbits >= nmat || mat_kind[mbits] != 3) { c->err = "a light's material is no longer emissive"; return BC_ERR_ARG; }
+            const double* md = mat_data + 8 * (size_t)mbits;
+            lights[(size_t)i].le[0] = (float)md[5]; lights[(size_t)i].le[1] = (float)md[6]; lights[(size_t)i].le[2] = (float)md[7];
+            double wt = (0.2126 * md[5] + 0.7152 * md[6] + 0.0722 * md[7]) * c->h_light_area[(size_t)i];
+            total += wt < 1e-9 ? 1e-9 : wt;
+            cdf[(size_t)i] = total;
+        }
+        double prev = 0;
+        for (int i = grp.first; i < grp.second; i++) {
+            const double cv = cdf[(size_t)i] / total;
+            pdf[(size_t)lights[(size_t)i].prim] = (float)(cv - prev);
+            prev = cv;
+            cdf[(size_t)i] = cv;
+        }
+        if (grp.second > grp.first) cdf[(size_t)grp.second - 1] = 1;
     }
-    double prev = 0;
-    for (size_t i = 0; i < nl; i++) {
-        const double cv = cdf[i] / total;
-        pdf[(size_t)lights[i].prim] = (float)(cv - prev);
-        prev = cv;
-        cdf[i] = cv;
-    }
-    if (nl) cdf.back() = 1;
     for (size_t i = 0; i < nl; i++) {
         lights[i].cdf = (float)cdf[i];
         light_of[(size_t)lights[i].prim] = (int32_t)i;
@@ -244,6 +269,92 @@ int bc_update_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const 
     c->nmat = nmat;
     c->nmat_scene = nmat;
     c->nlight = (int)nl;
+    return BC_OK;
+}
+
+extern "C" {
+
+int bc_batch_upload(bc_ctx* c, int n, const bc_scene_view* views) {
+    if (!c || n <= 0 || !views) return BC_ERR_ARG;
+    for (int s = 0; s < n; s++)
+        if (!check_view(c, views[s])) { c->err = "batch scene " + std::to_string(s) + ": " + c->err; return BC_ERR_SCENE; }
+    hipSetDevice(c->device);
+    // the scenes' BVH8 builds are independent: spread them over host threads
+    std::vector<bcx::Bvh8> bv((size_t)n);
+    std::vector<std::string> errs((size_t)n);
+    std::vector<char> okb((size_t)n, 0);
+    std::atomic<int> next{0};
+    const unsigned nth = std::max(1u, std::min<unsigned>((unsigned)n, std::min(16u, std::thread::hardware_concurrency())));
+    auto work = [&]() {
+        for (int s; (s = next.fetch_add(1)) < n;) okb[(size_t)s] = bcx::build_bvh8(views[s], bv[(size_t)s], errs[(size_t)s]);
+    };
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < nth; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    int depth = 0;
+    for (int s = 0; s < n; s++) {
+        if (!okb[(size_t)s]) { c->err = "batch scene " + std::to_string(s) + ": " + errs[(size_t)s]; return BC_ERR_SCENE; }
+        depth = std::max(depth, bv[(size_t)s].depth);
+    }
+    const int stack = bcd::stack_for_depth(depth);
+    if (stack < 0) { c->err = "BVH8 depth " + std::to_string(depth) + " exceeds the traversal stack (33 levels)"; return BC_ERR_SCENE; }
+    // concatenate: node, record, primitive-id and material indices rebased per scene
+    bcx::Bvh8 all;
+    std::vector<int32_t> mkind;
+    std::vector<double> mdata;
+    std::vector<bc_ctx::BatchScene> refs;
+    c->has_scene = false;
+    c->h_lights.clear();
+    c->h_light_area.clear();
+    c->light_groups.clear();
+    uint32_t pbase = 0;
+    for (int s = 0; s < n; s++) {
+        const bcx::Bvh8& b = bv[(size_t)s];
+        const bc_scene_view& v = views[s];
+        const uint32_t nbase = (uint32_t)all.nodes.size(), rbase = (uint32_t)all.prims.size();
+        const int mbase = (int)mkind.size();
+        if ((uint64_t)nbase + b.nodes.size() >= 0x80000000ull || (uint64_t)rbase + b.prims.size() >= 0x80000000ull ||
+            (uint64_t)pbase + (uint64_t)v.nprim >= 0x80000000ull) {
+            c->err = "batch exceeds 2^31 nodes or primitives";
+            return BC_ERR_SCENE;
+        }
+        for (bc_node8 nd : b.nodes) {
+            nd.child_base += nbase;
+            nd.prim_base = ((nd.prim_base & ~BC_NODE_HAS_SPHERES) + rbase) | (nd.prim_base & BC_NODE_HAS_SPHERES);
+            all.nodes.push_back(nd);
+        }
+        all.prims.insert(all.prims.end(), b.prims.begin(), b.prims.end());
+        for (uint32_t id : b.ids) all.ids.push_back(((id & ~BC_SPHERE_BIT) + pbase) | (id & BC_SPHERE_BIT));
+        for (int32_t m : b.mats) all.mats.push_back(m + mbase);
+        mkind.insert(mkind.end(), v.mat_kind, v.mat_kind + v.nmat);
+        mdata.insert(mdata.end(), v.mat_data, v.mat_data + 8 * (size_t)v.nmat);
+        const int l0 = (int)c->h_lights.size();
+        collect_lights(v, (int)pbase, mbase, c->h_lights, c->h_light_area);
+        c->light_groups.emplace_back(l0, (int)c->h_lights.size());
+        refs.push_back({nbase, (uint32_t)l0, (int32_t)c->h_lights.size() - l0, v.nprim, v.nmat, b.depth,
+                        (int64_t)b.nodes.size()});
+        pbase += (uint32_t)v.nprim;
+    }
+    c->nprim = (int)pbase;
+    c->nmat_scene = 0;
+    c->batch.clear();
+    {
+        const int rc = bc_set_materials(c, (int32_t)mkind.size(), mkind.data(), mdata.data());
+        if (rc != BC_OK) return rc;
+    }
+    if (!upload_vec(c, c->d_nodes, c->cap_nodes, all.nodes, "upload nodes") ||
+        !upload_vec(c, c->d_prims, c->cap_prims, all.prims, "upload prims") ||
+        !upload_vec(c, c->d_ids, c->cap_ids, all.ids, "upload prim ids") ||
+        !upload_vec(c, c->d_pmat, c->cap_pmat, all.mats, "upload prim materials") ||
+        !ok(c, hipStreamSynchronize(c->stream), "batch upload"))
+        return BC_ERR_DEVICE;
+    c->nnode8 = (int64_t)all.nodes.size();
+    c->depth8 = depth;
+    c->stack = stack;
+    c->topo = c->geom = 0;
+    c->scene_bytes = (int64_t)(all.nodes.size() * sizeof(bc_node8) + all.prims.size() * (sizeof(bc_prim) + 8));
+    c->batch = refs;
     return BC_OK;
 }
 

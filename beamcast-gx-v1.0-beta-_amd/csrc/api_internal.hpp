@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/beamcast_hip.h"
@@ -33,6 +34,15 @@ struct bc_ctx {
     int depth8 = 0, nprim = 0, nmat = 0, nlight = 0, stack = 0, nmat_scene = 0;
     std::vector<bc_light> h_lights;  // light geometry (emission/CDF filled per material update)
     std::vector<double> h_light_area;
+    // light table groups normalised separately ([begin, end) into h_lights): one per scene
+    std::vector<std::pair<int, int>> light_groups;
+    // packed multi-scene batch (bc_batch_upload); empty when a single scene is resident
+    struct BatchScene {
+        uint32_t root, lbase;  // root node, first light
+        int32_t nlight, nprim, nmat, depth;
+        int64_t nnode;
+    };
+    std::vector<BatchScene> batch;
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;
@@ -47,5 +57,7 @@ struct bc_ctx {
 
 bool bc_ensure(bc_ctx* c, void*& p, size_t& cap, size_t bytes, const char* what);
 bool bc_check(bc_ctx* c, hipError_t e, const char* what);
+// Material table and per-group light distributions from the material data (api.cpp).
+int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data);
 int bc_check_kernel_error(bc_ctx* c);
 void bc_render_release(bc_ctx* c);

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, f
 
 // ------------------------------------------------------------------------------ extend
 template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg) {
+__global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg, uint32_t root) {
     __shared__ uint2 stk[STACK * kBlock];
     const uint32_t* q = B.q[cur];
     for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
@@ -88,7 +88,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int 
             r.ox = o.x; r.oy = o.y; r.oz = o.z;
             r.dx = d.x; r.dy = d.y; r.dz = d.z;
             r.tmin = 1e-4f;
-            const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, r, 1e30f, stk + threadIdx.x, kBlock, S.err);
+            const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, root, r, 1e30f, stk + threadIdx.x, kBlock,
+                                                    S.err);
             B.hit[j] = make_float2(h.t, __uint_as_float(h.rec));
             B.nseg[j] += 1u;
         }
@@ -136,8 +137,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
 
 // ------------------------------------------------------------------------------ shadow
 // Occluded(p + N*2e-4 -> q, 1e-4, dist - 1e-4) of visible() (renderer.go:1333-1341, 851-895).
-template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint32_t nseg) {
+// MULTI: packed scenes, the root of each path's scene from its scene index.
+template <int STACK, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint32_t nseg, uint32_t root,
+                                                   const RenderParams* __restrict__ Ps) {
     __shared__ uint2 stk[STACK * kBlock];
     for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.sqc[seg], base = seg * B.seg;
@@ -149,7 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint
             r.ox = o.x; r.oy = o.y; r.oz = o.z;
             r.dx = d.x; r.dy = d.y; r.dz = d.z;
             r.tmin = (float)1e-4;
-            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
+            const uint32_t rt = MULTI ? Ps[B.sid[j]].root : root;
+            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, rt, r, o.w, stk + threadIdx.x, kBlock, S.err);
             if (h.rec == 0xffffffffu) {
                 const float4 c = B.sc[j];
                 const float4 l = B.L[j];
@@ -276,12 +280,26 @@ __global__ void k_p_clear_flag(PathBufs B) {
     if (threadIdx.x == 0) B.flag[0] = 0u;
 }
 
-template <int STACK>
+// Scene of window sample g among packed scenes: the last s with gpre[s] <= g (gpre: prefix
+// sums of the scenes' window samples, nscene + 1 entries, g < gpre[nscene]).
+__device__ __forceinline__ uint32_t scene_of(const uint32_t* __restrict__ gpre, uint32_t nscene, uint32_t g) {
+    uint32_t lo = 0, hi = nscene;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (gpre[mid] <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// MULTI: the window's samples are those of several packed scenes (Pd[s], s < nscene); each
+// restarted path records its scene and is traced from that scene's root.
+template <int STACK, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
-                                                     int cur, uint32_t nseg) {
+                                                     int cur, uint32_t nseg, const uint32_t* __restrict__ gpre,
+                                                     uint32_t nscene) {
     __shared__ uint2 stk[STACK * kBlock];
     const RenderParams P = *Pd;
-    const uint32_t W = (uint32_t)P.ns * (uint32_t)P.npix;
+    const uint32_t W = MULTI ? gpre[nscene] : (uint32_t)P.ns * (uint32_t)P.npix;
     uint32_t* q = B.q[cur];
     bool work = false;
     for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
@@ -305,8 +323,17 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
                 if (kk < nr) {
                     const uint32_t m = m0 + kk;
                     const uint32_t gn = ((m / B.seg) * nseg + seg) * B.seg + m % B.seg;
-                    const uint32_t sl = gn / (uint32_t)P.npix, pl = gn - sl * (uint32_t)P.npix;
-                    init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
+                    if constexpr (MULTI) {
+                        const uint32_t sc = scene_of(gpre, nscene, gn);
+                        const RenderParams& Q = Pd[sc];
+                        const uint32_t gl = gn - gpre[sc];
+                        const uint32_t sl = gl / (uint32_t)Q.npix, pl = gl - sl * (uint32_t)Q.npix;
+                        init_camera_path(Q, B, j, (int)pl, Q.s0 + (int)sl);
+                        B.sid[j] = sc;
+                    } else {
+                        const uint32_t sl = gn / (uint32_t)P.npix, pl = gn - sl * (uint32_t)P.npix;
+                        init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
+                    }
                     B.gid[j] = gn;
                     B.bnc[j] = 0u;
                     q[base + n + kk] = j;
@@ -321,7 +348,9 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
                 r.ox = o.x; r.oy = o.y; r.oz = o.z;
                 r.dx = d.x; r.dy = d.y; r.dz = d.z;
                 r.tmin = 1e-4f;
-                const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, r, 1e30f, stk + threadIdx.x, kBlock, S.err);
+                const uint32_t rt = MULTI ? Pd[B.sid[j]].root : P.root;
+                const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, rt, r, 1e30f, stk + threadIdx.x, kBlock,
+                                                        S.err);
                 B.hit[j] = make_float2(h.t, __uint_as_float(h.rec));
                 B.nseg[j] += 1u;
             }
@@ -337,6 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
     if (work && threadIdx.x == 0) B.flag[0] = 1u;
 }
 
+template <bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
                                                     int cur, uint32_t* __restrict__ guides, uint32_t nseg) {
     __shared__ uint32_t lds[2 * 3 * kWaves];
@@ -355,7 +385,12 @@ __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, con
             if (k < n) {
                 j = q[base + k];
                 const int bounce = (int)B.bnc[j];
-                cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
+                if constexpr (MULTI) {
+                    const RenderParams& Q = Pd[B.sid[j]];
+                    cont = shade_path<false>(S, B, Q, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < Q.bounces;
+                } else {
+                    cont = shade_path<false>(S, B, P, j, bounce, guides, shadow, so, sd, sc, Adj{}) && bounce + 1 < P.bounces;
+                }
                 if (cont) B.bnc[j] = (uint32_t)(bounce + 1);
                 done = !cont;
             }
@@ -503,7 +538,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, 
             r.ox = o.x; r.oy = o.y; r.oz = o.z;
             r.dx = d.x; r.dy = d.y; r.dz = d.z;
             r.tmin = (float)1e-4;
-            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, r, o.w, stk + threadIdx.x, kBlock, S.err);
+            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, P.root, r, o.w, stk + threadIdx.x, kBlock,
+                                                   S.err);
             if (h.rec == 0xffffffffu) {
                 const float4 c = B.sc[j];
                 const float4 l = B.L[j];
@@ -522,8 +558,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, 
 
 // ------------------------------------------------------------------------------ host side
 struct RenderState {
-    void* buf[24] = {};
-    size_t bufcap[24] = {};
+    void* buf[25] = {};
+    size_t bufcap[25] = {};
     float4* sg = nullptr;
     // differentiable render: per-sample radiance of the last bc_forward, its shape
     void* keep = nullptr;
@@ -535,8 +571,9 @@ struct RenderState {
     uint32_t nseg = 0;  // segments of the current allocation
     float4* acc = nullptr;
     uint32_t* guide_tmp = nullptr;
-    bcd::RenderParams* Pd = nullptr;
-    bcd::RenderParams* h_params = nullptr;  // pinned staging for Pd
+    bcd::RenderParams* Pd = nullptr;        // per-job parameters (device)
+    uint32_t* gpre = nullptr;               // per-job window sample prefix sums (device)
+    bcd::RenderParams* h_params = nullptr;  // pinned staging for Pd, then gpre
     uint32_t* h_cnt = nullptr;              // pinned
     // captured iteration graph of the persistent schedule and what its kernel arguments hold
     hipGraphExec_t exec = nullptr;
@@ -544,7 +581,7 @@ struct RenderState {
     // autotuner cache: (shape) -> (pool, iterations per graph)
     struct Tune {
         int w, rows, spp, bounces;
-        uint64_t topo, geom;
+        uint64_t topo, geom, shape;  // shape: hash of a batch's frame sizes (0: one scene)
         int pool, iters, seg;
     };
     std::vector<Tune> tunes;
@@ -553,17 +590,21 @@ struct RenderState {
 
 namespace {
 
-constexpr size_t kResultCap = (size_t)1 << 25;  // per-sample results held per window (512 MB)
+constexpr size_t kResultCap = (size_t)1 << 25;   // per-sample results held per window (512 MB)
+constexpr size_t kBatchCap = (size_t)1 << 27;    // the same for a batch of scenes (2 GB)
+constexpr int kMaxJobs = 256;                    // scenes of one batch render
 
 size_t segments_for(size_t paths, uint32_t ss) { return (paths + ss - 1) / ss; }
 
 // Allocates (or keeps) path state for `paths` slots in whole segments of `ss` slots.
-bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t window_samples, uint32_t ss) {
+bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t window_samples, uint32_t ss,
+                  int njobs = 1) {
     if (!c->rs) c->rs = new RenderState();
     RenderState& R = *c->rs;
     if (!R.h_cnt) {
+        const size_t pbytes = (size_t)kMaxJobs * sizeof(bcd::RenderParams) + (size_t)(kMaxJobs + 1) * 4;
         void* h = nullptr;
-        if (!bc_check(c, hipHostMalloc(&h, 4096, hipHostMallocDefault), "pinned staging")) return false;
+        if (!bc_check(c, hipHostMalloc(&h, 256 + pbytes, hipHostMallocDefault), "pinned staging")) return false;
         R.h_cnt = (uint32_t*)h;
         R.h_params = (bcd::RenderParams*)((char*)h + 256);
     }
@@ -574,8 +615,10 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
         {5, slots * 16},  {6, slots * 4},   {7, slots * 8},  {8, slots * 16},  {9, slots * 16},
         {10, slots * 16}, {11, slots * 4},  {12, slots * 4}, {13, slots * 4},  {14, 64},
         {15, npix * 16 + (guides ? npix * 12 : 0) + 64},     {16, slots * 4},  {17, slots * 4},
-        {18, window_samples * 16},                           {19, sizeof(bcd::RenderParams)},
+        {18, window_samples * 16},
+        {19, (size_t)njobs * sizeof(bcd::RenderParams) + (size_t)(njobs + 1) * 4 + 16},
         {20, slots * 16}, {21, slots * 4}, {22, nseg * 5 * 4}, {23, ((size_t)bcd::kMaxGrid + nseg) * 8},
+        {24, slots * 4},
     };
     for (const Want& x : w)
         if (!bc_ensure(c, R.buf[x.slot], R.bufcap[x.slot], std::max<size_t>(x.bytes, 64), "render buffers")) return false;
@@ -603,6 +646,8 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
     R.acc = (float4*)R.buf[15];
     R.guide_tmp = (uint32_t*)(R.acc + npix);
     R.Pd = (bcd::RenderParams*)R.buf[19];
+    R.gpre = (uint32_t*)(R.Pd + njobs);
+    R.B.sid = (uint32_t*)R.buf[24];
     R.sg = (float4*)R.buf[20];
     R.B.gid = (uint32_t*)R.buf[21];
     uint32_t* sc = (uint32_t*)R.buf[22];
@@ -633,7 +678,8 @@ int pix_grid(const bc_ctx* c, size_t npix) {
     return (int)std::min<size_t>({(npix + bcd::kBlock - 1) / bcd::kBlock, (size_t)c->num_cus * 8, (size_t)bcd::kMaxGrid});
 }
 
-bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, int rows, bool guides) {
+// scene: index into a resident batch, or -1 for the resident single scene.
+bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, int rows, bool guides, int scene = -1) {
     bcd::RenderParams P{};
     P.width = d->width; P.height = d->height; P.spp = d->spp; P.bounces = d->bounces; P.rr_depth = d->rr_depth;
     P.sampler = d->sampler; P.mis = d->mis != 0; P.power = d->power_lights != 0; P.sky_mode = d->sky_mode;
@@ -649,6 +695,14 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
     P.npix = rows * d->width;
     P.nmat = c->nmat;
     P.nlight = c->nlight;
+    P.root = 0;
+    P.lbase = 0;
+    if (scene >= 0) {
+        const bc_ctx::BatchScene& b = c->batch[(size_t)scene];
+        P.nlight = b.nlight;
+        P.root = b.root;
+        P.lbase = b.lbase;
+    }
     P.restir = d->restir_candidates;
     // sun direction normalised in double precision, then rounded
     const double sx = d->sun_dir[0], sy = d->sun_dir[1], sz = d->sun_dir[2];
@@ -663,19 +717,23 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
 template <int STACK>
 void launch_bounce(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur, int bounce,
                    uint32_t* guides, int grid, uint32_t nseg, hipStream_t st) {
-    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg);
+    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg, P.root);
     hipLaunchKernelGGL(bcd::k_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, nseg);
-    hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg);
+    hipLaunchKernelGGL((bcd::k_shadow<STACK, false>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg, P.root,
+                       (const bcd::RenderParams*)nullptr);
 }
 
-template <int STACK>
-void launch_iteration(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd, int cur,
-                      uint32_t* guides, int grid, uint32_t nseg, bool last, hipStream_t st) {
+// One iteration of the persistent schedule; MULTI: the jobs are packed scenes of a batch.
+template <int STACK, bool MULTI>
+void launch_iteration(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd,
+                      const uint32_t* gpre, uint32_t njobs, int cur, uint32_t* guides, int grid, uint32_t nseg,
+                      bool last, hipStream_t st) {
     // the drain test reads the flag written by the last iteration of a replay only
     if (last) hipLaunchKernelGGL(bcd::k_p_clear_flag, dim3(1), dim3(64), 0, st, B);
-    hipLaunchKernelGGL((bcd::k_p_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, nseg);
-    hipLaunchKernelGGL(bcd::k_p_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
-    hipLaunchKernelGGL((bcd::k_shadow<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg);
+    hipLaunchKernelGGL((bcd::k_p_extend<STACK, MULTI>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, nseg,
+                       gpre, njobs);
+    hipLaunchKernelGGL(bcd::k_p_shade<MULTI>, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
+    hipLaunchKernelGGL((bcd::k_shadow<STACK, MULTI>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg, 0u, Pd);
 }
 
 bool clear_tally(bc_ctx* c) {
@@ -683,11 +741,10 @@ bool clear_tally(bc_ctx* c) {
     return bc_check(c, hipMemsetAsync(R.B.tally, 0, ((size_t)bcd::kMaxGrid + R.nseg) * 8, c->stream), "clear tally");
 }
 
-int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, uint64_t* rays_out, uint64_t* shadow_out) {
+// Finalise: reduced pixel sums -> rgba (already done per job by the caller), then the tallies.
+int finish_tally(bc_ctx* c, uint64_t* rays_out, uint64_t* shadow_out) {
     RenderState& R = *c->rs;
     hipStream_t st = c->stream;
-    hipLaunchKernelGGL(bcd::k_finalize, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, (const float4*)R.acc,
-                       (float4*)d_rgba);
     hipLaunchKernelGGL(bcd::k_tally, dim3(1), dim3(bcd::kTallyBlock), 0, st, R.B, R.nseg);
     if (!bc_check(c, hipGetLastError(), "finalize launch") ||
         !bc_check(c, hipMemcpyAsync(R.h_cnt + 16, R.B.totals, 16, hipMemcpyDeviceToHost, st), "read totals") ||
@@ -700,9 +757,17 @@ int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, u
     return bc_check_kernel_error(c);
 }
 
-// Plain wavefront schedule: passes of pixels x samples, three kernels per bounce.
+int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, uint64_t* rays_out, uint64_t* shadow_out) {
+    RenderState& R = *c->rs;
+    hipLaunchKernelGGL(bcd::k_finalize, dim3(grid_pix), dim3(bcd::kBlock), 0, c->stream, P, (const float4*)R.acc,
+                       (float4*)d_rgba);
+    return finish_tally(c, rays_out, shadow_out);
+}
+
+// Plain wavefront schedule: passes of pixels x samples, three kernels per bounce.  scene: batch
+// scene index (-1: the single resident scene).
 int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides,
-                 uint64_t* rays_out, uint64_t* shadow_out, float4* keep = nullptr) {
+                 uint64_t* rays_out, uint64_t* shadow_out, float4* keep = nullptr, int scene = -1) {
     const size_t npix = (size_t)rows * (size_t)d->width;
     const size_t max_paths = std::max<size_t>(npix, (size_t)1 << 22);
     int spp_pass = d->samples_per_pass > 0 ? d->samples_per_pass : (int)std::max<size_t>(1, max_paths / npix);
@@ -712,7 +777,7 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     RenderState& R = *c->rs;
     R.B.xg = xcd_group(d);
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
-    bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
+    bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr, scene);
     const uint32_t nseg = R.nseg;
     const int grid = seg_grid(c, nseg);
     const int grid_pix = pix_grid(c, npix);
@@ -738,43 +803,75 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     return finish(c, P, d_rgba, grid_pix, rays_out, shadow_out);
 }
 
-// Persistent schedule with in-place regeneration; `iters` iterations are one captured graph.
-int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides,
-                      uint64_t* rays_out, uint64_t* shadow_out, int pool, int iters, uint32_t ss) {
-    const size_t npix = (size_t)rows * (size_t)d->width;
-    const int window_ns = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, kResultCap / npix));
-    const size_t total_first = npix * (size_t)window_ns;
+// One frame (or row chunk) of the persistent schedule: a single scene, or one scene of a batch.
+struct Job {
+    const bc_render_desc* d;
+    int y0, rows, scene;
+    float* d_rgba;
+    uint32_t* d_guides;
+};
+
+template <int STACK>
+void capture_iterations(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd,
+                        const uint32_t* gpre, uint32_t njobs, bool multi, uint32_t* guides, int grid, uint32_t nseg,
+                        int iters, hipStream_t st) {
+    for (int it = 0; it < iters; it++) {
+        if (multi) launch_iteration<STACK, true>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, st);
+        else launch_iteration<STACK, false>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, st);
+    }
+}
+
+// Persistent schedule with in-place regeneration over the samples of all jobs (one job: a
+// scene; several: the packed scenes of a batch, one launch sequence for all of them); `iters`
+// iterations are one captured graph.  Jobs are rendered in windows of samples [s0, s0 + ns)
+// of every job whose per-sample results fit the result buffer.
+int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* rays_out, uint64_t* shadow_out, int pool,
+                           int iters, uint32_t ss) {
+    const bool multi = !(jobs.size() == 1 && jobs[0].scene < 0);
+    const uint32_t njobs = (uint32_t)jobs.size();
+    size_t npix_all = 0;
+    int max_spp = 0;
+    std::vector<size_t> accoff;
+    for (const Job& jb : jobs) {
+        accoff.push_back(npix_all);
+        npix_all += (size_t)jb.rows * (size_t)jb.d->width;
+        max_spp = std::max(max_spp, jb.d->spp);
+    }
+    const size_t cap = multi ? kBatchCap : kResultCap;
+    const int window_ns = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(1, cap / npix_all));
+    size_t total_first = 0;
+    for (const Job& jb : jobs)
+        total_first += (size_t)jb.rows * (size_t)jb.d->width * (size_t)std::min(window_ns, jb.d->spp);
     const size_t slots = segments_for(std::min<size_t>((size_t)std::max(pool, 1), total_first), ss) * ss;
-    if (!ensure_state(c, slots, npix, d_guides != nullptr, total_first, ss)) return BC_ERR_NOMEM;
+    const bool guides_on = !multi && jobs[0].d_guides != nullptr;
+    if (!ensure_state(c, slots, npix_all, guides_on, total_first, ss, (int)njobs)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
-    R.B.xg = xcd_group(d);
+    R.B.xg = xcd_group(jobs[0].d);
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
-    bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
+    std::vector<bcd::RenderParams> P(njobs);
+    for (uint32_t k = 0; k < njobs; k++)
+        P[k] = make_params(c, jobs[k].d, jobs[k].y0, jobs[k].rows, guides_on, jobs[k].scene);
     const uint32_t nseg = R.nseg;
     const int grid = seg_grid(c, nseg);
-    const int grid_pix = pix_grid(c, npix);
     hipStream_t st = c->stream;
-    uint32_t* guides = d_guides ? d_guides : R.guide_tmp;
+    uint32_t* guides = guides_on ? jobs[0].d_guides : R.guide_tmp;
     iters = std::max(2, iters & ~1);
     std::vector<const void*> key = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf,
                                     c->d_light_of, c->d_err, R.Pd, guides,
                                     (const void*)(uintptr_t)nseg, (const void*)(uintptr_t)grid,
                                     (const void*)(uintptr_t)iters, (const void*)(uintptr_t)c->stack, (const void*)(uintptr_t)ss,
-                                    (const void*)(uintptr_t)R.B.xg};
+                                    (const void*)(uintptr_t)R.B.xg, (const void*)(uintptr_t)multi,
+                                    (const void*)(uintptr_t)njobs};
     for (void* p : R.buf) key.push_back(p);
     if (!R.exec || key != R.g_key) {
         if (R.exec) { (void)hipGraphExecDestroy(R.exec); R.exec = nullptr; }
         hipGraph_t graph = nullptr;
         if (!bc_check(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture")) return BC_ERR_DEVICE;
-        for (int it = 0; it < iters; it++) {
-            const int cur = it & 1;
-            const bool last = it == iters - 1;
-            switch (c->stack) {
-            case 8: launch_iteration<8>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
-            case 16: launch_iteration<16>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
-            case 24: launch_iteration<24>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
-            default: launch_iteration<32>(S, R.B, R.Pd, cur, guides, grid, nseg, last, st); break;
-            }
+        switch (c->stack) {
+        case 8: capture_iterations<8>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
+        case 16: capture_iterations<16>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
+        case 24: capture_iterations<24>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
+        default: capture_iterations<32>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
         }
         if (!bc_check(c, hipStreamEndCapture(st, &graph), "end capture")) return BC_ERR_DEVICE;
         hipError_t e = hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0);
@@ -783,19 +880,31 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
         R.g_key = key;
     }
     if (!clear_tally(c)) return BC_ERR_DEVICE;
+    if (multi && !bc_check(c, hipMemsetAsync(R.acc, 0, npix_all * 16, st), "clear pixel sums")) return BC_ERR_DEVICE;
     int windows = 0;
-    for (int s0 = 0; s0 < d->spp; s0 += window_ns) {
-        P.s0 = s0;
-        P.ns = std::min(window_ns, d->spp - s0);
-        *R.h_params = P;
-        if (!bc_check(c, hipMemcpyAsync(R.Pd, R.h_params, sizeof P, hipMemcpyHostToDevice, st), "params")) return BC_ERR_DEVICE;
-        hipLaunchKernelGGL(bcd::k_p_start, dim3(std::max(grid, grid_pix)), dim3(bcd::kBlock), 0, st,
-                           (const bcd::RenderParams*)R.Pd, R.B, R.acc, guides, nseg, s0 == 0 ? 1 : 0);
+    std::vector<uint32_t> gpre(njobs + 1);
+    for (int s0 = 0; s0 < max_spp; s0 += window_ns) {
+        gpre[0] = 0;
+        for (uint32_t k = 0; k < njobs; k++) {
+            P[k].s0 = s0;
+            P[k].ns = std::max(0, std::min(window_ns, jobs[k].d->spp - s0));
+            gpre[k + 1] = gpre[k] + (uint32_t)P[k].npix * (uint32_t)P[k].ns;
+        }
+        const size_t window_total = gpre[njobs];
+        std::memcpy(R.h_params, P.data(), njobs * sizeof(bcd::RenderParams));
+        std::memcpy((char*)R.h_params + njobs * sizeof(bcd::RenderParams), gpre.data(), (njobs + 1) * 4);
+        if (!bc_check(c, hipMemcpyAsync(R.Pd, R.h_params, njobs * sizeof(bcd::RenderParams) + (njobs + 1) * 4,
+                                        hipMemcpyHostToDevice, st), "params"))
+            return BC_ERR_DEVICE;
+        const int gstart = std::max(grid, multi ? 1 : pix_grid(c, (size_t)P[0].npix));
+        hipLaunchKernelGGL(bcd::k_p_start, dim3(gstart), dim3(bcd::kBlock), 0, st, (const bcd::RenderParams*)R.Pd, R.B,
+                           R.acc, guides, nseg, (!multi && s0 == 0) ? 1 : 0);
         // a slot holds one path for at most bounces + 1 iterations (the last one records it and
         // restarts the slot), and every segment restarts all of its free slots while it has
         // samples: bound the replays accordingly
-        const size_t window_total = npix * (size_t)P.ns;
-        const size_t max_iters = (size_t)(d->bounces + 2) * (window_total / slots + 2) + 2;
+        int max_b = 0;
+        for (const Job& jb : jobs) max_b = std::max(max_b, jb.d->bounces);
+        const size_t max_iters = (size_t)(max_b + 2) * (window_total / slots + 2) + 2;
         const size_t max_graphs = max_iters / (size_t)iters + 4;
         for (size_t g = 0;; g++) {
             if (g >= max_graphs) { c->err = "persistent schedule did not drain"; return BC_ERR_DEVICE; }
@@ -805,34 +914,60 @@ int render_persistent(bc_ctx* c, const bc_render_desc* d, int y0, int rows, floa
                 return BC_ERR_DEVICE;
             if (R.h_cnt[0] == 0) break;
         }
-        hipLaunchKernelGGL(bcd::k_p_reduce, dim3(grid_pix), dim3(bcd::kBlock), 0, st, (const bcd::RenderParams*)R.Pd,
-                           R.B, R.acc);
+        for (uint32_t k = 0; k < njobs; k++) {
+            if (P[k].ns == 0) continue;
+            bcd::PathBufs Bk = R.B;
+            Bk.res = R.B.res + gpre[k];
+            hipLaunchKernelGGL(bcd::k_p_reduce, dim3(pix_grid(c, (size_t)P[k].npix)), dim3(bcd::kBlock), 0, st,
+                               (const bcd::RenderParams*)(R.Pd + k), Bk, R.acc + accoff[k]);
+        }
         if (!bc_check(c, hipGetLastError(), "reduce launch")) return BC_ERR_DEVICE;
         windows++;
+    }
+    for (uint32_t k = 0; k < njobs; k++) {
+        hipLaunchKernelGGL(bcd::k_finalize, dim3(pix_grid(c, (size_t)P[k].npix)), dim3(bcd::kBlock), 0, st, P[k],
+                           (const float4*)(R.acc + accoff[k]), (float4*)jobs[k].d_rgba);
     }
     R.last_pool = (int)slots;
     R.last_iters = iters;
     R.last_seg = (int)ss;
     R.last_windows = windows;
-    return finish(c, P, d_rgba, grid_pix, rays_out, shadow_out);
+    return finish_tally(c, rays_out, shadow_out);
+}
+
+uint64_t jobs_shape(const std::vector<Job>& jobs) {
+    if (jobs.size() == 1 && jobs[0].scene < 0) return 0;
+    uint64_t h = 1469598103934665603ull;
+    for (const Job& jb : jobs) {
+        const uint64_t v[4] = {(uint64_t)jb.d->width, (uint64_t)jb.rows, (uint64_t)jb.d->spp, (uint64_t)jb.d->bounces};
+        for (uint64_t x : v) h = (h ^ x) * 1099511628211ull;
+    }
+    return h;
 }
 
 // Autotuner: for each pool size, segment size and graph length, times a short render of the
-// same chunk with enough samples to refill the pool about three times (at least 2 spp), and
+// same jobs with enough samples to refill the pool about three times (at least 2 spp), and
 // keeps the fastest per sample for this image shape, scene and transport depth.
-bool autotune(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides, int& pool,
-              int& iters, uint32_t& seg) {
+bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg) {
     RenderState& R = *c->rs;
+    const bc_render_desc* d = jobs[0].d;
+    const uint64_t shape = jobs_shape(jobs);
     for (const auto& t : R.tunes)
-        if (t.w == d->width && t.rows == rows && t.spp == d->spp && t.bounces == d->bounces && t.topo == c->topo &&
-            t.geom == c->geom) {
+        if (t.w == d->width && t.rows == jobs[0].rows && t.spp == d->spp && t.bounces == d->bounces && t.topo == c->topo &&
+            t.geom == c->geom && t.shape == shape) {
             pool = t.pool;
             iters = t.iters;
             seg = (uint32_t)t.seg;
             return true;
         }
-    bc_render_desc trial = *d;
-    const size_t npix = (size_t)rows * (size_t)d->width;
+    size_t npix = 0;
+    int max_spp = 0;
+    for (const Job& jb : jobs) {
+        npix += (size_t)jb.rows * (size_t)jb.d->width;
+        max_spp = std::max(max_spp, jb.d->spp);
+    }
+    std::vector<bc_render_desc> trial(jobs.size());
+    std::vector<Job> tj = jobs;
     const int pools[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21};
     const int its[] = {4, 8, 16};
     const uint32_t segs[] = {256, 1024};
@@ -841,24 +976,63 @@ bool autotune(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgb
     iters = 8;
     seg = bcd::kSegPersist;
     for (int p : pools) {
-        if ((size_t)p > npix * (size_t)d->spp && p != pools[0]) continue;
-        trial.spp = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(2, (3 * (size_t)p + npix - 1) / npix));
+        if ((size_t)p > npix * (size_t)max_spp && p != pools[0]) continue;
+        const int tspp = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(2, (3 * (size_t)p + npix - 1) / npix));
+        for (size_t k = 0; k < jobs.size(); k++) {
+            trial[k] = *jobs[k].d;
+            trial[k].spp = std::min(trial[k].spp, tspp);
+            tj[k].d = &trial[k];
+        }
         for (uint32_t ss : segs)
             for (int it : its) {
-                if (render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it, ss) != BC_OK)
-                    return false;
+                if (render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK) return false;
                 float ms = 0;
                 if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
-                    render_persistent(c, &trial, y0, rows, d_rgba, d_guides, nullptr, nullptr, p, it, ss) != BC_OK ||
+                    render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK ||
                     !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
                     !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
                     !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
                     return false;
-                ms /= (float)trial.spp;
+                ms /= (float)tspp;
                 if (ms < best) { best = ms; pool = p; iters = it; seg = ss; }
             }
     }
-    R.tunes.push_back({d->width, rows, d->spp, d->bounces, c->topo, c->geom, pool, iters, (int)seg});
+    R.tunes.push_back({d->width, jobs[0].rows, d->spp, d->bounces, c->topo, c->geom, shape, pool, iters, (int)seg});
+    return true;
+}
+
+// Schedule shape of a mode-1 render from the first description: explicit or autotuned.
+int persistent_shape(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg) {
+    const bc_render_desc* d = jobs[0].d;
+    pool = d->reserved[0];
+    iters = d->reserved[1];
+    seg = d->reserved[2] > 0 ? (uint32_t)d->reserved[2] : bcd::kSegPersist;
+    if (seg % bcd::kBlock != 0 || seg > bcd::kSegMax) {
+        c->err = "segment size (reserved[2]) must be a multiple of 256 up to 4096";
+        return BC_ERR_ARG;
+    }
+    if (pool <= 0 || iters <= 0) {
+        if (!autotune(c, jobs, pool, iters, seg)) return BC_ERR_DEVICE;
+    }
+    return BC_OK;
+}
+
+bool valid_desc(bc_ctx* c, const bc_render_desc* d, int y0, int rows) {
+    if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->spp > 65536 || d->bounces < 0 || d->bounces > 1024 ||
+        y0 < 0 || rows <= 0 || y0 + rows > d->height || d->sampler < 0 || d->sampler > 2 ||
+        (int64_t)d->width * d->height > ((int64_t)1 << 28) || d->mode < 0 || d->mode > 1) {
+        c->err = "invalid render description or row range";
+        return false;
+    }
+    if (d->restir_candidates < 0 || d->restir_candidates > BC_RIS_MAX) {
+        c->err = "restir_candidates must be in [0, 64]";
+        return false;
+    }
+    if (d->restir_candidates == 0 && (d->sun_irradiance[0] != 0.0f || d->sun_irradiance[1] != 0.0f ||
+                                      d->sun_irradiance[2] != 0.0f)) {
+        c->err = "a sun (sun_irradiance != 0) needs restir_candidates > 0";
+        return false;
+    }
     return true;
 }
 
@@ -879,35 +1053,67 @@ extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int 
                                 uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_out) {
     if (!c || !d) return BC_ERR_ARG;
     if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
-    if (d->width <= 0 || d->height <= 0 || d->spp <= 0 || d->spp > 65536 || d->bounces < 0 || d->bounces > 1024 ||
-        y0 < 0 || rows <= 0 || y0 + rows > d->height || !d_rgba || d->sampler < 0 || d->sampler > 2 ||
-        (int64_t)d->width * d->height > ((int64_t)1 << 28) || d->mode < 0 || d->mode > 1) {
-        c->err = "invalid render description or row range";
-        return BC_ERR_ARG;
-    }
-    if (d->restir_candidates < 0 || d->restir_candidates > BC_RIS_MAX) {
-        c->err = "restir_candidates must be in [0, 64]";
-        return BC_ERR_ARG;
-    }
-    if (d->restir_candidates == 0 && (d->sun_irradiance[0] != 0.0f || d->sun_irradiance[1] != 0.0f ||
-                                      d->sun_irradiance[2] != 0.0f)) {
-        c->err = "a sun (sun_irradiance != 0) needs restir_candidates > 0";
-        return BC_ERR_ARG;
-    }
+    if (!d_rgba) { c->err = "invalid render description or row range"; return BC_ERR_ARG; }
+    if (!valid_desc(c, d, y0, rows)) return BC_ERR_ARG;
     (void)hipSetDevice(c->device);
     // zero transport segments render black on either schedule; the plain one does it without tracing
     if (d->mode == 0 || d->bounces == 0) return render_plain(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out);
     if (!ensure_state(c, 1, (size_t)rows * (size_t)d->width, d_guides != nullptr, 1, bcd::kSegPersist)) return BC_ERR_NOMEM;
-    int pool = d->reserved[0], iters = d->reserved[1];
-    uint32_t seg = d->reserved[2] > 0 ? (uint32_t)d->reserved[2] : bcd::kSegPersist;
-    if (seg % bcd::kBlock != 0 || seg > bcd::kSegMax) {
-        c->err = "segment size (reserved[2]) must be a multiple of 256 up to 4096";
+    const std::vector<Job> jobs = {Job{d, y0, rows, -1, d_rgba, d_guides}};
+    int pool, iters;
+    uint32_t seg;
+    const int rc = persistent_shape(c, jobs, pool, iters, seg);
+    if (rc != BC_OK) return rc;
+    return render_persistent_jobs(c, jobs, rays_out, shadow_out, pool, iters, seg);
+}
+
+extern "C" int bc_batch_render_device(bc_ctx* c, int n, const bc_render_desc* descs, float* const* d_rgba,
+                                      uint64_t* rays_out, uint64_t* shadow_out) {
+    if (!c || n <= 0 || !descs || !d_rgba) return BC_ERR_ARG;
+    if (c->batch.empty()) { c->err = "no batch uploaded"; return BC_ERR_SCENE; }
+    if ((size_t)n != c->batch.size() || n > kMaxJobs) {
+        c->err = "one description per batch scene is needed (at most 256 scenes)";
         return BC_ERR_ARG;
     }
-    if (pool <= 0 || iters <= 0) {
-        if (!autotune(c, d, y0, rows, d_rgba, d_guides, pool, iters, seg)) return BC_ERR_DEVICE;
+    for (int k = 0; k < n; k++) {
+        if (!d_rgba[k]) { c->err = "invalid render description or row range"; return BC_ERR_ARG; }
+        if (!valid_desc(c, &descs[k], 0, descs[k].height)) {
+            c->err = "batch scene " + std::to_string(k) + ": " + c->err;
+            return BC_ERR_ARG;
+        }
     }
-    return render_persistent(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out, pool, iters, seg);
+    (void)hipSetDevice(c->device);
+    uint64_t rays = 0, shadow = 0;
+    if (descs[0].mode == 0) {  // the scenes one after another, plain schedule
+        for (int k = 0; k < n; k++) {
+            uint64_t r = 0, sh = 0;
+            const int rc = render_plain(c, &descs[k], 0, descs[k].height, d_rgba[k], nullptr, &r, &sh, nullptr, k);
+            if (rc != BC_OK) return rc;
+            rays += r;
+            shadow += sh;
+        }
+    } else {  // one persistent schedule over the samples of every scene
+        std::vector<Job> jobs;
+        int max_b = 0;
+        for (int k = 0; k < n; k++) {
+            jobs.push_back(Job{&descs[k], 0, descs[k].height, k, d_rgba[k], nullptr});
+            max_b = std::max(max_b, descs[k].bounces);
+        }
+        if (max_b == 0) {
+            c->err = "a batch render needs at least one transport segment";
+            return BC_ERR_ARG;
+        }
+        if (!ensure_state(c, 1, 1, false, 1, bcd::kSegPersist, n)) return BC_ERR_NOMEM;
+        int pool, iters;
+        uint32_t seg;
+        int rc = persistent_shape(c, jobs, pool, iters, seg);
+        if (rc != BC_OK) return rc;
+        rc = render_persistent_jobs(c, jobs, &rays, &shadow, pool, iters, seg);
+        if (rc != BC_OK) return rc;
+    }
+    if (rays_out) *rays_out = rays;
+    if (shadow_out) *shadow_out = shadow;
+    return BC_OK;
 }
 
 extern "C" int bc_render_tuning(const bc_ctx* c, int32_t* out8) {
@@ -945,7 +1151,7 @@ template <int STACK>
 void launch_bounce_adj(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur,
                        int bounce, uint32_t* guides, const bcd::Adj& A, float* grad, int grid, int grid_adj,
                        uint32_t nseg, hipStream_t st) {
-    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg);
+    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg, P.root);
     hipLaunchKernelGGL(bcd::k_shade_adj, dim3(grid_adj), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, A, grad,
                        nseg);
     hipLaunchKernelGGL((bcd::k_shadow_adj<STACK>), dim3(grid_adj), dim3(bcd::kBlock), 0, st, S, B, P, A, grad, nseg);

@@ -23,6 +23,7 @@ struct RenderParams {
     int32_t y0, npix;          // chunk: first row, pixels in chunk
     int32_t s0, ns;            // pass: first sample, samples in this pass
     int32_t nmat, nlight;
+    uint32_t root, lbase;      // scene root node and first light of the scene (packed scenes)
     int32_t restir;            // candidates of the resampled direct lighting (0 = one-sample NEE)
     float sun_dir[3], sun[3];  // directional light (restir only)
 };
@@ -45,6 +46,7 @@ struct PathBufs {
     uint32_t xg;    // segments per XCD turn (see xcd_chunk)
     uint32_t* bnc;  // per-path bounce (persistent schedule)
     uint32_t* gid;  // per-path window sample index, 0xffffffff = idle (persistent schedule)
+    uint32_t* sid;  // per-path scene index (packed scenes)
     float4* res;    // per-sample result: rgb, path segments (persistent schedule)
     uint32_t* dq;   // terminated paths (persistent schedule)
     float4* o;      // ray origin
@@ -320,13 +322,13 @@ __device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParam
                 int lo = 0, hi = P.nlight;
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
-                    if (S.lights[mid].cdf >= u) hi = mid; else lo = mid + 1;
+                    if (S.lights[P.lbase + mid].cdf >= u) hi = mid; else lo = mid + 1;
                 }
                 li = lo >= P.nlight ? P.nlight - 1 : lo;
-                sel = S.pdf[S.lights[li].prim];
+                sel = S.pdf[S.lights[P.lbase + li].prim];
                 if (sel <= 0.0f) sel = 1.0f / (float)P.nlight;
             }
-            const bc_light& Lr = S.lights[li];
+            const bc_light& Lr = S.lights[P.lbase + li];
             float3 qpt, lnrm;
             const bool sph = Lr.kind == 0;
             if (sph) {
@@ -548,13 +550,13 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
                     int lo = 0, hi = P.nlight;
                     while (lo < hi) {
                         const int mid = (lo + hi) >> 1;
-                        if (S.lights[mid].cdf >= u) hi = mid; else lo = mid + 1;
+                        if (S.lights[P.lbase + mid].cdf >= u) hi = mid; else lo = mid + 1;
                     }
                     li = lo >= P.nlight ? P.nlight - 1 : lo;
-                    sel = S.pdf[S.lights[li].prim];
+                    sel = S.pdf[S.lights[P.lbase + li].prim];
                     if (sel <= 0.0f) sel = 1.0f / (float)P.nlight;
                 }
-                const bc_light& Lr = S.lights[li];
+                const bc_light& Lr = S.lights[P.lbase + li];
                 float3 qpt, lnrm;
                 const bool sph = Lr.kind == 0;
                 if (sph) {  // randUnitVector, renderer.go:1273-1285

@@ -174,10 +174,12 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
 
 // Traverses the BVH8.  ANY=true stops at the first primitive hit in [tmin, tmax].
 // stk points at this lane's column of the LDS stack (entry i at stk[i * stride]).
+// root: index of the scene's root node (0 for a single scene; scenes packed in one arena
+// each have their own).
 template <bool ANY, int STACK>
 __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, const bc_prim* __restrict__ prims,
-                                           const uint32_t* __restrict__ ids, const Ray& r, float tmax, uint2* stk,
-                                           int stride, int* __restrict__ err) {
+                                           const uint32_t* __restrict__ ids, uint32_t root, const Ray& r, float tmax,
+                                           uint2* stk, int stride, int* __restrict__ err) {
     const float ix = safe_rcp(r.dx), iy = safe_rcp(r.dy), iz = safe_rcp(r.dz);
     const uint32_t oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
     HitRec best;
@@ -185,7 +187,7 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
     best.id = -1;
     best.rec = 0xffffffffu;
     uint32_t ng_base = 0, ng_bits = 0, pg_base = 0, pg_mask = 0;
-    uint32_t node = 0;
+    uint32_t node = root;
     bool have_node = true;
     int sp = 0;
     for (;;) {

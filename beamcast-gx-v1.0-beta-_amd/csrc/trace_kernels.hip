@@ -21,7 +21,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const bc_node8* __restric
     Ray r;
     r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
     r.dx = b.x; r.dy = b.y; r.dz = b.z;
-    const HitRec h = traverse<ANY, STACK>(nodes, prims, ids, r, b.w, stk + threadIdx.x, kTraceBlock, err);
+    const HitRec h = traverse<ANY, STACK>(nodes, prims, ids, 0u, r, b.w, stk + threadIdx.x, kTraceBlock, err);
     if (ANY) {
         reinterpret_cast<int32_t*>(out)[i] = h.rec != 0xffffffffu ? 1 : 0;
     } else {

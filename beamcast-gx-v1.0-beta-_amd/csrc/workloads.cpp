@@ -318,6 +318,63 @@ bch_scene* bch_workload_scene(int kind, uint64_t seed, int64_t param, int bins, 
         set_cam(cam, V(.5, .6, -.35), V(.5, .1, .5), V(0, 1, 0), 50);
         break;
     }
+    case 8: {  // scene `param` of the multi-scene batch: boxes, spheres and 2-16 quad lights
+        s.Name = "batch scene";
+        const uint64_t i = (uint64_t)std::max<int64_t>(0, param);
+        // triangle budget log-uniform in [1e4, 2e6]
+        const double tris = std::exp(std::log(1e4) + u01(seed, i, 0) * std::log(200.0));
+        std::vector<int> pal;
+        for (int k = 0; k < 6; k++)
+            pal.push_back(mat_diffuse(s, V(.15 + .7 * u01(seed, i * 64 + (uint64_t)k, 1), .15 + .7 * u01(seed, i * 64 + (uint64_t)k, 2),
+                                           .15 + .7 * u01(seed, i * 64 + (uint64_t)k, 3))));
+        bcx::Material metal;
+        metal.Kind = bcx::MatMetal;
+        metal.Color = V(.85, .8, .7);
+        metal.Roughness = .15;
+        pal.push_back(s.AddMaterial(metal));
+        bcx::Material glass;
+        glass.Kind = bcx::MatGlass;
+        glass.Color = V(.95, .97, 1);
+        glass.IOR = 1.5;
+        pal.push_back(s.AddMaterial(glass));
+        const int floor_m = mat_diffuse(s, V(.6, .6, .6));
+        add_quad(s, V(-30, 0, -30), V(-30, 0, 30), V(30, 0, 30), V(30, 0, -30), floor_m);
+        // instanced boxes: 12 triangles each, about 90% of the budget, over a 20 x 20 lot
+        const int64_t nbox = std::max<int64_t>(1, (int64_t)(tris * .9 / 12));
+        const double lot = 10;
+        for (int64_t b = 0; b < nbox; b++) {
+            const uint64_t it = i * 0x100000000ull + (uint64_t)b;
+            const double x = -lot + 2 * lot * u01(seed, it, 4), z = -lot + 2 * lot * u01(seed, it, 5);
+            const double w = .05 + .5 * u01(seed, it, 6), d = .05 + .5 * u01(seed, it, 7), h = .05 + 1.5 * u01(seed, it, 8);
+            const int m = pal[(size_t)(sm64(seed ^ it) % pal.size())];
+            add_box(s, V(x, 0, z), w, h, d, 360 * u01(seed, it, 9), m);
+        }
+        // analytic spheres
+        const int64_t nsph = std::max<int64_t>(4, (int64_t)(tris / 200));
+        for (int64_t b = 0; b < nsph; b++) {
+            const uint64_t it = i * 0x100000000ull + 0x80000000ull + (uint64_t)b;
+            const double r = .1 + .4 * u01(seed, it, 10);
+            const double x = -lot + 2 * lot * u01(seed, it, 11), z = -lot + 2 * lot * u01(seed, it, 12);
+            s.AddSphere(V(x, r + 2 * u01(seed, it, 13), z), r, pal[(size_t)(sm64(seed ^ it ^ 0x5555) % pal.size())]);
+        }
+        // 2-16 emissive quads overhead
+        const int nl = 2 + (int)(u01(seed, i, 14) * 15);
+        for (int l = 0; l < nl; l++) {
+            const uint64_t it = i * 64 + (uint64_t)l;
+            bcx::Material lm;
+            lm.Kind = bcx::MatEmissive;
+            const double warm = u01(seed, it, 15);
+            const double e = 6 + 10 * u01(seed, it, 16);
+            lm.Emission = V(e, e * (.75 + .2 * warm), e * (.55 + .45 * (1 - warm)));
+            const int lmat = s.AddMaterial(lm);
+            const double x = -8 + 16 * u01(seed, it, 17), z = -8 + 16 * u01(seed, it, 18), y = 6 + 3 * u01(seed, it, 19);
+            const double hs = .3 + .6 * u01(seed, it, 20);
+            add_quad(s, V(x - hs, y, z - hs), V(x + hs, y, z - hs), V(x + hs, y, z + hs), V(x - hs, y, z + hs), lmat);
+        }
+        const double az = 2 * M_PI * u01(seed, i, 21), dist = 14 + 6 * u01(seed, i, 22);
+        set_cam(cam, V(dist * std::cos(az), 4 + 4 * u01(seed, i, 23), dist * std::sin(az)), V(0, .5, 0), V(0, 1, 0), 45);
+        break;
+    }
     case 7: {  // closed box retessellated to about `param` triangles with 64 materials
         s.Name = "closed box (retessellated)";
         int sub = (int)std::max<double>(1.0, std::floor(std::sqrt((double)param / 10.0)));

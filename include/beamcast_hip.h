@@ -124,6 +124,19 @@ int bc_forward(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float*
 int bc_backward(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, const float* dloss_dimage,
                 float* dcolor, float* demission);
 
+/* Multi-scene batch (no counterpart in the reference, whose backend holds one scene): packs n
+ * scenes into one device arena (BVH8 nodes, records, materials and light tables concatenated and
+ * rebased; each scene keeps its own light distribution).  Replaces the resident scene or batch;
+ * bc_render / bc_trace need a single scene (bc_scene_upload) again afterwards. */
+int bc_batch_upload(bc_ctx* ctx, int n, const bc_scene_view* scenes);
+/* Renders every scene of the resident batch: descs[i] (full frames) into d_rgba[i] (device,
+ * W_i x H_i x 4 floats, the layout of bc_render).  descs[0].mode selects the schedule: 0 renders
+ * the scenes one after another with the plain wavefront schedule, 1 runs one persistent schedule
+ * (pool, graph and segment from descs[0].reserved, or autotuned) over the samples of all scenes.
+ * Images equal those of rendering each scene alone.  rays_out / shadow_out: totals (may be NULL). */
+int bc_batch_render_device(bc_ctx* ctx, int n, const bc_render_desc* descs, float* const* d_rgba,
+                           uint64_t* rays_out, uint64_t* shadow_out);
+
 /* Schedule last used by a mode-1 render, 8 ints: pool slots, iterations per graph, traversal
  * stack depth, sample windows, slots per segment, 0, 0, 0.  No counterpart in the reference
  * (its kernel has one fixed shape). */
