@@ -945,9 +945,11 @@ uint64_t jobs_shape(const std::vector<Job>& jobs) {
     return h;
 }
 
-// Autotuner: for each pool size, segment size and graph length, times a short render of the
-// same jobs with enough samples to refill the pool about three times (at least 2 spp), and
-// keeps the fastest per sample for this image shape, scene and transport depth.
+// Autotuner: times short renders of the same jobs, with enough samples to refill the pool about
+// three times (at least 1 spp), and keeps the fastest per sample for this image shape, scene and
+// transport depth.  Pool size and segment size are swept at 8 iterations per graph, then the
+// graph length for the best pair.  Each trial is timed after one untimed run (which captures the
+// graph) unless the trial itself holds more than 2^24 path samples.
 bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg) {
     RenderState& R = *c->rs;
     const bc_render_desc* d = jobs[0].d;
@@ -968,34 +970,44 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
     }
     std::vector<bc_render_desc> trial(jobs.size());
     std::vector<Job> tj = jobs;
-    const int pools[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21};
-    const int its[] = {4, 8, 16};
+    const int pools[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23};
     const uint32_t segs[] = {256, 1024};
     float best = 1e30f;
     pool = 1 << 20;
     iters = 8;
     seg = bcd::kSegPersist;
-    for (int p : pools) {
-        if ((size_t)p > npix * (size_t)max_spp && p != pools[0]) continue;
-        const int tspp = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(2, (3 * (size_t)p + npix - 1) / npix));
+    auto time_trial = [&](int p, int it, uint32_t ss, float& per_sample) -> bool {
+        const int tspp = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(1, (3 * (size_t)p + npix - 1) / npix));
         for (size_t k = 0; k < jobs.size(); k++) {
             trial[k] = *jobs[k].d;
             trial[k].spp = std::min(trial[k].spp, tspp);
             tj[k].d = &trial[k];
         }
-        for (uint32_t ss : segs)
-            for (int it : its) {
-                if (render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK) return false;
-                float ms = 0;
-                if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
-                    render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK ||
-                    !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
-                    !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
-                    !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
-                    return false;
-                ms /= (float)tspp;
-                if (ms < best) { best = ms; pool = p; iters = it; seg = ss; }
-            }
+        if (npix * (size_t)tspp <= ((size_t)1 << 24) &&
+            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK)
+            return false;
+        float ms = 0;
+        if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
+            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK ||
+            !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
+            !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
+            !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
+            return false;
+        per_sample = ms / (float)tspp;
+        return true;
+    };
+    for (int p : pools) {
+        if ((size_t)p > npix * (size_t)max_spp && p != pools[0]) continue;
+        for (uint32_t ss : segs) {
+            float ms;
+            if (!time_trial(p, 8, ss, ms)) return false;
+            if (ms < best) { best = ms; pool = p; seg = ss; }
+        }
+    }
+    for (int it : {4, 16}) {
+        float ms;
+        if (!time_trial(pool, it, seg, ms)) return false;
+        if (ms < best) { best = ms; iters = it; }
     }
     R.tunes.push_back({d->width, jobs[0].rows, d->spp, d->bounces, c->topo, c->geom, shape, pool, iters, (int)seg});
     return true;

@@ -171,6 +171,64 @@ def bench_terrain(bc, dev, args, threads):
     return out
 
 
+BATCH_RES = ((640, 360), (1280, 720), (2560, 1440))
+
+
+def batch_resolution(i, seed=0xC5):
+    """Resolution of batch scene i: one of 640x360, 1280x720, 2560x1440 by a seeded hash."""
+    h = (seed * 0x9E3779B97F4A7C15 + (i + 1) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    h ^= h >> 31
+    return BATCH_RES[(h * 0x94D049BB133111EB >> 61) % 3]
+
+
+def bench_batch(bc, dev, args, threads):
+    """Multi-scene batch: 32 seeded scenes (log-uniform 1e4..2e6 triangles of instanced boxes and
+    spheres, 2-16 quad lights, resolutions from {640x360, 1280x720, 2560x1440}), Ultra transport
+    (16 segments, RR from bounce 6, no clamp) at --batch-spp samples per pixel.  Plain: the scenes
+    one after another; persistent: one schedule over all scenes' samples (autotuned)."""
+    n = args.batch_scenes
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(threads) as ex:
+        scenes = list(ex.map(lambda i: bc.HostScene.workload(bc.BATCH_SCENE, seed=0xC5, param=i), range(n)))
+    build = time.time() - t0
+    t0 = time.time()
+    dev.upload_batch(scenes, nee_triangles=True)
+    upload = time.time() - t0
+    tris = sum(sc.counts()[0] for sc in scenes)
+    log(f"[bench] batch: {n} scenes, {tris} primitives, host build {build:.1f}s, pack + upload {upload:.1f}s")
+    descs = []
+    for i, sc in enumerate(scenes):
+        w, h = batch_resolution(i)
+        descs.append(bc.render_desc(sc, w, h, args.batch_spp, 16, rr_depth=6, sampler=1, mis=True, power=True))
+    pix = sum(d.width * d.height for d in descs)
+    bufs = [dev.alloc(d.width * d.height * 16) for d in descs]
+    out = {"workload": f"{n} batch scenes ({tris} primitives, {pix} pixels), {args.batch_spp} spp Halton, 16 segments, "
+                       "RR from bounce 6, no clamp, MIS + power NEE over emissive quads"}
+    for mode in (0, 1):
+        for d in descs:
+            d.mode = mode
+        first = None
+        if mode == 1 or not args.batch_no_warmup:
+            t0 = time.perf_counter()
+            dev.render_batch_device(descs, bufs)  # mode 1: autotunes and captures once
+            first = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        segs, shadow = dev.render_batch_device(descs, bufs)
+        sec = time.perf_counter() - t0
+        key = "plain" if mode == 0 else "persistent"
+        out[key] = {"seconds": round(sec, 4), "path_samples_per_s": round(pix * args.batch_spp / sec, 1),
+                    "mrays_s_total": round((segs + shadow) / sec / 1e6, 2),
+                    "segments_per_sample": round(segs / (pix * args.batch_spp), 4)}
+        if mode == 1:
+            out[key]["tuning"] = dev.tuning()
+            out[key]["autotune_and_first_s"] = round(first, 2)
+        log(f"[bench] batch {key}: {sec:.3f}s")
+    out["persistent_over_plain"] = round(out["plain"]["seconds"] / out["persistent"]["seconds"], 3)
+    for b in bufs:
+        dev.free(b)
+    return out
+
+
 def bench_adjoint(bc, dev):
     """Differentiable-render config: closed box retessellated to ~200k triangles with 64 materials,
     512x512, 32 spp, 6 segments; time of one forward and one path-replay backward pass."""
@@ -202,6 +260,29 @@ def bench_adjoint(bc, dev):
             "forward_path_samples_per_s": round(512 * 512 * 32 / fwd, 1)}
 
 
+PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_soup1M_16Mrays.json")
+
+
+def roofline_entry(kernel, achieved, alg_bytes):
+    """Roofline of the dominant kernel.  traffic: HBM-side bytes per launch of that kernel from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE passes, scaled by a 1 GiB copy),
+    with the VALU-busy fraction of the same runs as the secondary roofline."""
+    out = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_ray": round(alg_bytes, 1)}
+    try:
+        with open(PROFILE_SUMMARY) as f:
+            prof = json.load(f)
+        k = prof["launches"].get(kernel[len("k_trace ("):-1])
+        if k:
+            out["traffic"] = k["traffic_bytes"]
+            out["traffic_source"] = os.path.relpath(PROFILE_SUMMARY, ROOT)
+            out["valu_busy"] = k.get("valu_busy")
+            out["l2_hit"] = k.get("l2_hit")
+    except (OSError, ValueError, KeyError):
+        pass
+    return out
+
+
 def host_cores():
     try:
         return len(os.sched_getaffinity(0))
@@ -222,6 +303,12 @@ def main():
     ap.add_argument("--no-terrain", action="store_true", help="skip the terrain render measurement")
     ap.add_argument("--terrain-cells", type=int, default=2047)
     ap.add_argument("--terrain-spp", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true", help="skip the multi-scene batch measurement")
+    ap.add_argument("--only", choices=("render", "terrain", "batch", "adjoint"),
+                    help="run this render leg alone (for profiling) and print its result")
+    ap.add_argument("--batch-scenes", type=int, default=32)
+    ap.add_argument("--batch-spp", type=int, default=8, help="samples per pixel of the batch leg (Ultra: 256)")
+    ap.add_argument("--batch-no-warmup", action="store_true", help="time the plain batch render without a warm-up")
     ap.add_argument("--copy-check", action="store_true",
                     help="also run one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -233,6 +320,14 @@ def main():
     import beamcast as bc
 
     threads = min(16, host_cores())
+    if args.only:
+        dev = bc.Device(0)
+        legs = {"render": lambda: bench_render(bc, dev, args, threads), "terrain": lambda: bench_terrain(bc, dev, args, threads),
+                "batch": lambda: bench_batch(bc, dev, args, threads), "adjoint": lambda: bench_adjoint(bc, dev)}
+        out = legs[args.only]()
+        dev.close()
+        print(json.dumps({"leg": args.only, args.only: out}))
+        return
     t0 = time.time()
     scene = bc.HostScene.workload(bc.SOUP, seed=0x5EED, param=args.tris, bins=32, leaf=4)
     log(f"[bench] host SAH build of {args.tris} triangles: {time.time() - t0:.2f}s")
@@ -344,6 +439,7 @@ def main():
     render = None if args.no_render else bench_render(bc, dev, args, threads)
     adjoint = None if args.no_render else bench_adjoint(bc, dev)
     terrain = None if (args.no_render or args.no_terrain) else bench_terrain(bc, dev, args, threads)
+    batch = None if (args.no_render or args.no_batch) else bench_batch(bc, dev, args, threads)
 
     names = {("coherent", False): "coherent_closest", ("coherent", True): "coherent_any",
              ("incoherent", False): "incoherent_closest", ("incoherent", True): "incoherent_any"}
@@ -369,20 +465,12 @@ def main():
             "bvh8_depth": int(st["depth8"]),
             "scene_bytes": int(st["bytes"]),
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"k_trace ({names[dominant]})",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "alg_bytes_per_ray": round(alg[dominant], 1),
-        },
+        "roofline": roofline_entry(f"k_trace ({names[dominant]})", achieved, alg[dominant]),
         "cpu_baseline": cpu,
         "render": render,
         "adjoint": adjoint,
         "terrain": terrain,
+        "batch": batch,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
                                "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4)}
                     for k in launches},
