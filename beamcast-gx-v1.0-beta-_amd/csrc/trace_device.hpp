@@ -174,6 +174,12 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
 
 // Traverses the BVH8.  ANY=true stops at the first primitive hit in [tmin, tmax].
 // stk points at this lane's column of the LDS stack (entry i at stk[i * stride]).
+// Primitive records fetched together in a leaf (see the leaf loop of traverse).
+#ifndef BC_PRIM_BATCH
+#define BC_PRIM_BATCH 2
+#endif
+constexpr int kPrimBatch = BC_PRIM_BATCH;
+
 // root: index of the scene's root node (0 for a single scene; scenes packed in one arena
 // each have their own).
 template <bool ANY, int STACK>
@@ -197,28 +203,47 @@ __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, c
         }
         const bool sph_node = (pg_base & BC_NODE_HAS_SPHERES) != 0;
         const uint32_t pbase = pg_base & ~BC_NODE_HAS_SPHERES;
+        // primitives kPrimBatch at a time: their records are loaded before any is tested, so
+        // the memory round trips overlap (the loop is latency-bound); tests stay in mask order
         while (pg_mask) {
-            const uint32_t k = (uint32_t)__builtin_ctz(pg_mask);
-            pg_mask &= pg_mask - 1u;
-            const uint32_t ri = pbase + k;
-            const float* g = prims[ri].v;
-            float t;
-            bool h;
-            if (sph_node && (ids[ri] & BC_SPHERE_BIT)) h = sphere_test(g, r, r.tmin, best.t, t);
-            else h = tri_test(g, r, r.tmin, best.t, t);
-            if (h) {
-                if (ANY) {
-                    best.t = t;
-                    best.id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
-                    best.rec = ri;
-                    return best;
-                }
-                // the test admits t <= best.t; only an exact tie needs the primitive
-                // indices (smaller wins), so the id table is read in that rare case only
-                if (best.rec == 0xffffffffu || t < best.t ||
-                    (ids[ri] & ~BC_SPHERE_BIT) < (ids[best.rec] & ~BC_SPHERE_BIT)) {
-                    best.t = t;
-                    best.rec = ri;
+            uint32_t rr[kPrimBatch];
+            int cnt = 0;
+#pragma unroll
+            for (int u = 0; u < kPrimBatch; u++) {
+                const bool any = pg_mask != 0u;
+                rr[u] = pbase + (any ? (uint32_t)__builtin_ctz(pg_mask) : 0u);
+                if (!any) rr[u] = rr[0];
+                cnt += any ? 1 : 0;
+                pg_mask &= pg_mask - 1u;
+            }
+            float g[kPrimBatch][9];
+#pragma unroll
+            for (int u = 0; u < kPrimBatch; u++)
+#pragma unroll
+                for (int q = 0; q < 9; q++) g[u][q] = prims[rr[u]].v[q];
+            bool sp[kPrimBatch];
+#pragma unroll
+            for (int u = 0; u < kPrimBatch; u++) sp[u] = sph_node && (ids[rr[u]] & BC_SPHERE_BIT);
+#pragma unroll
+            for (int u = 0; u < kPrimBatch; u++) {
+                if (u >= cnt) break;
+                const uint32_t ri = rr[u];
+                float t;
+                const bool h = sp[u] ? sphere_test(g[u], r, r.tmin, best.t, t) : tri_test(g[u], r, r.tmin, best.t, t);
+                if (h) {
+                    if (ANY) {
+                        best.t = t;
+                        best.id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
+                        best.rec = ri;
+                        return best;
+                    }
+                    // the test admits t <= best.t; only an exact tie needs the primitive
+                    // indices (smaller wins), so the id table is read in that rare case only
+                    if (best.rec == 0xffffffffu || t < best.t ||
+                        (ids[ri] & ~BC_SPHERE_BIT) < (ids[best.rec] & ~BC_SPHERE_BIT)) {
+                        best.t = t;
+                        best.rec = ri;
+                    }
                 }
             }
         }
