@@ -1,12 +1,17 @@
 // bvh8_build.cpp -- flattens the host-built binary SAH BVH into quantised 8-wide nodes.
 //
 // Input is the Go Scene's BVH exactly as Scene.Build leaves it (renderer.go:409-633:
-// Nodes in DFS pre-order, leaves as [Start, Start+Count) ranges of Order).  The binary
-// tree is collapsed top-down: a node's child set starts as its two children and the
-// internal child with the largest surface area is repeatedly replaced by its own two
-// children while the set still fits 8 slots and 32 primitives.  Binary leaves are kept
-// whole (split into runs of at most 8 primitives when larger), so every primitive the
-// reference would test in a leaf is tested here too; only culling boxes change.
+// Nodes in DFS pre-order, leaves as [Start, Start+Count) ranges of Order).  The binary tree is
+// collapsed into 8-wide nodes by a surface-area cost optimisation over the binary tree (the
+// dynamic program of Ylitie, Karras and Laine, "Efficient Incoherent Ray Traversal on GPUs
+// Through Compressed Wide BVHs", HPG 2017): for every binary node and every slot budget
+// 1..8 it keeps the cheapest way to represent the subtree as at most that many children of
+// one wide node -- a leaf of at most kLeafMax primitives, a wide internal node, or the budget
+// split between the two binary children -- with cost = area x (kNodeCost per node visited,
+// kPrimCost per primitive tested).  Leaves hold at most kLeafMax primitives so that eight of
+// them fit the 32-primitive run of a node; larger binary leaves become several leaf slots.
+// Every primitive appears in exactly one leaf; which leaf only changes culling, never a hit
+// result (equal distances resolve to the smaller primitive index).
 #include "bvh8_build.hpp"
 
 #include <algorithm>
@@ -83,8 +88,6 @@ Child make_child(const bc_scene_view& v, int k) {
     return c;
 }
 
-int slots_of(const Child& c) { return c.leaf ? (c.count + 7) / 8 : 1; }
-
 void write_prim(const bc_scene_view& v, int id, Bvh8& out) {
     bc_prim r;
     std::memset(&r, 0, sizeof r);
@@ -101,6 +104,106 @@ void write_prim(const bc_scene_view& v, int id, Bvh8& out) {
     out.prims.push_back(r);
     out.ids.push_back(idbits);
     out.mats.push_back(v.prim_mat[id]);
+}
+
+#ifndef BC_LEAF_MAX
+#define BC_LEAF_MAX 2
+#endif
+#ifndef BC_PRIM_COST
+#define BC_PRIM_COST 0.3
+#endif
+constexpr int kLeafMax = BC_LEAF_MAX;       // primitives per leaf slot (8 slots x kLeafMax <= 32)
+constexpr double kNodeCost = 1.0;           // relative cost of visiting one wide node
+constexpr double kPrimCost = BC_PRIM_COST;  // relative cost of one primitive test
+
+// Per binary node: cost of representing its subtree with at most i (1..8) child slots of the
+// enclosing wide node, and how.
+struct Plan {
+    float cost[8];
+    uint8_t how[8];  // 0: use the plan for one slot fewer; 1: leaf; 2: internal wide node;
+                     // 3..9: split k = how-2 slots to the left child, the rest to the right
+    int32_t start, count;  // primitive range of the subtree in Order (count -1: not contiguous)
+};
+
+// Child items of wide nodes, produced top-down from the plans.
+void expand(const bc_scene_view& v, const std::vector<Plan>& plan, int n, int slots, std::vector<Child>& out) {
+    const Plan& P = plan[(size_t)n];
+    int i = slots;
+    while (i > 1 && P.how[i - 1] == 0) i--;
+    const uint8_t h = P.how[i - 1];
+    const int32_t* l = v.node_link + 4 * (size_t)n;
+    if (h == 1) {  // leaf: the subtree's primitives, in runs of at most kLeafMax
+        if (P.count <= kLeafMax) {
+            Child c;
+            c.leaf = true; c.node2 = -1; c.start = P.start; c.count = P.count;
+            c.box = node_box(v, n);
+            out.push_back(c);
+        } else {
+            for (int s = 0; s < P.count; s += kLeafMax) {
+                Child c;
+                c.leaf = true; c.node2 = -1; c.start = P.start + s; c.count = std::min(kLeafMax, P.count - s);
+                c.box.empty();
+                for (int k = 0; k < c.count; k++) c.box.grow(prim_box(v, v.order[c.start + k]));
+                out.push_back(c);
+            }
+        }
+    } else if (h == 2) {
+        out.push_back(make_child(v, n));
+    } else {
+        const int k = h - 2;
+        expand(v, plan, l[0], k, out);
+        expand(v, plan, l[1], i - k, out);
+    }
+}
+
+// Bottom-up cost plans of every binary node (children have larger indices than parents).
+void make_plans(const bc_scene_view& v, std::vector<Plan>& plan) {
+    const float inf = INFINITY;
+    plan.resize((size_t)v.nnode);
+    for (int n = v.nnode - 1; n >= 0; n--) {
+        Plan& P = plan[(size_t)n];
+        const int32_t* l = v.node_link + 4 * (size_t)n;
+        const double A = node_box(v, n).area();
+        if (l[3] > 0) {  // binary leaf: ceil(count / kLeafMax) leaf slots
+            P.start = l[2];
+            P.count = l[3];
+            const int need = (l[3] + kLeafMax - 1) / kLeafMax;
+            const float c = (float)(A * kPrimCost * l[3]);
+            for (int i = 1; i <= 8; i++) {
+                P.cost[i - 1] = i >= need ? c : inf;
+                P.how[i - 1] = 1;
+            }
+            continue;
+        }
+        const Plan& L = plan[(size_t)l[0]];
+        const Plan& R = plan[(size_t)l[1]];
+        const bool contiguous = L.count >= 0 && R.count >= 0 && R.start == L.start + L.count;
+        P.start = L.start;
+        P.count = contiguous ? L.count + R.count : -1;
+        float dist[9];
+        uint8_t dk[9];
+        for (int j = 2; j <= 8; j++) {
+            dist[j] = inf;
+            dk[j] = 0;
+            for (int k = 1; k < j; k++) {
+                const float c = L.cost[k - 1] + R.cost[j - k - 1];
+                if (c < dist[j]) { dist[j] = c; dk[j] = (uint8_t)k; }
+            }
+        }
+        const float c_int = dist[8] + (float)(A * kNodeCost);
+        const float c_leaf = (contiguous && P.count <= kLeafMax) ? (float)(A * kPrimCost * P.count) : inf;
+        P.cost[0] = c_leaf <= c_int ? c_leaf : c_int;
+        P.how[0] = c_leaf <= c_int ? 1 : 2;
+        for (int i = 2; i <= 8; i++) {
+            if (dist[i] < P.cost[i - 2]) {
+                P.cost[i - 1] = dist[i];
+                P.how[i - 1] = (uint8_t)(2 + dk[i]);
+            } else {
+                P.cost[i - 1] = P.cost[i - 2];
+                P.how[i - 1] = 0;
+            }
+        }
+    }
 }
 
 }  // namespace
@@ -126,6 +229,9 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
     out.ids.reserve((size_t)v.nprim);
     out.mats.reserve((size_t)v.nprim);
     out.nodes.reserve((size_t)v.nnode / 3 + 1);
+    std::vector<Plan> plan;
+    make_plans(v, plan);
+    if (!(plan[0].cost[7] < INFINITY)) { err = "internal: no BVH8 plan for the scene"; return false; }
     out.nodes.emplace_back();
     struct Work { int node2, i8, depth; };
     std::vector<Work> work;
@@ -134,46 +240,23 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
         Work w = work.back();
         work.pop_back();
         out.depth = std::max(out.depth, w.depth);
-        std::vector<Child> ch;
+        // the wide node's children: the root's (or an internal item's) best split of 8 slots
+        std::vector<Child> fin;
         const int32_t* l0 = v.node_link + 4 * (size_t)w.node2;
         if (l0[3] > 0) {
-            ch.push_back(make_child(v, w.node2));
+            expand(v, plan, w.node2, 8, fin);  // a single binary leaf as the whole scene
         } else {
-            ch.push_back(make_child(v, l0[0]));
-            ch.push_back(make_child(v, l0[1]));
-        }
-        for (;;) {  // greedy expansion by surface area
-            int slots = 0, prims = 0;
-            for (auto& c : ch) { slots += slots_of(c); prims += c.leaf ? c.count : 0; }
-            int best = -1;
-            double best_area = -1;
-            for (size_t i = 0; i < ch.size(); i++) {
-                if (ch[i].leaf) continue;
-                const int32_t* l = v.node_link + 4 * (size_t)ch[i].node2;
-                Child a = make_child(v, l[0]), b = make_child(v, l[1]);
-                int ns = slots - 1 + slots_of(a) + slots_of(b);
-                int np = prims + (a.leaf ? a.count : 0) + (b.leaf ? b.count : 0);
-                if (ns > 8 || np > 32) continue;
-                double ar = ch[i].box.area();
-                if (ar > best_area) { best_area = ar; best = (int)i; }
+            // distribute 8 slots over the two binary children (the node itself is internal)
+            const Plan& L = plan[(size_t)l0[0]];
+            const Plan& R = plan[(size_t)l0[1]];
+            float best = INFINITY;
+            int bk = 1;
+            for (int k = 1; k < 8; k++) {
+                const float c = L.cost[k - 1] + R.cost[8 - k - 1];
+                if (c < best) { best = c; bk = k; }
             }
-            if (best < 0) break;
-            const int32_t* l = v.node_link + 4 * (size_t)ch[best].node2;
-            Child a = make_child(v, l[0]), b = make_child(v, l[1]);
-            ch[best] = a;
-            ch.push_back(b);
-        }
-        // split leaves larger than 8 primitives into runs with their own boxes
-        std::vector<Child> fin;
-        for (auto& c : ch) {
-            if (!c.leaf || c.count <= 8) { fin.push_back(c); continue; }
-            for (int s = 0; s < c.count; s += 8) {
-                Child p;
-                p.leaf = true; p.node2 = -1; p.start = c.start + s; p.count = std::min(8, c.count - s);
-                p.box.empty();
-                for (int k = 0; k < p.count; k++) p.box.grow(prim_box(v, v.order[p.start + k]));
-                fin.push_back(p);
-            }
+            expand(v, plan, l0[0], bk, fin);
+            expand(v, plan, l0[1], 8 - bk, fin);
         }
         if (fin.size() > 8) { err = "internal: BVH8 node overflow"; return false; }
         Box nb;
