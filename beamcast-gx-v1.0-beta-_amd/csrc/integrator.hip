@@ -366,16 +366,64 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
     if (work && threadIdx.x == 0) B.flag[0] = 1u;
 }
 
-template <bool MULTI>
+// Material class of a path's hit: 0 escaped, 1 + material kind (diffuse, metal, glass, emissive).
+__device__ __forceinline__ uint32_t hit_class(const SceneBufs& S, const PathBufs& B, uint32_t j) {
+    const uint32_t rec = __float_as_uint(B.hit[j].y);
+    if (rec == 0xffffffffu) return 0u;
+    return 1u + (uint32_t)S.mats[S.pmat[rec]].kind;
+}
+
+// SORT: the segment's paths are first ordered by material class (a counting sort in LDS), so
+// that the waves of the shading loop take one BSDF branch each.  The order does not change
+// any result: paths are independent and results are keyed by sample.
+template <bool MULTI, bool SORT>
 __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
                                                     int cur, uint32_t* __restrict__ guides, uint32_t nseg) {
+    // the sort's ranks keep their own buffer and parity: block_rank alternates the halves of a
+    // buffer between consecutive calls, which is race-free only for calls of one width
     __shared__ uint32_t lds[2 * 3 * kWaves];
-    uint32_t parity = 0;
+    __shared__ uint32_t lds_sort[SORT ? 2 * 5 * kWaves : 1];
+    __shared__ uint32_t order[SORT ? kSegMax : 1];
+    __shared__ uint32_t place[SORT ? kSegMax : 1];
+    uint32_t parity = 0, parity_sort = 0;
     const RenderParams P = *Pd;
     const uint32_t* q = B.q[cur];
     uint32_t* qn = B.q[cur ^ 1];
     for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
+        if constexpr (SORT) {
+            // place[k] = (class << 24) | rank of item k within its class
+            uint32_t run[5] = {0u, 0u, 0u, 0u, 0u};
+            for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
+                const uint32_t k = r0 + threadIdx.x;
+                const uint32_t c = k < n ? hit_class(S, B, q[base + k]) : 5u;
+                const bool pr[5] = {c == 0u, c == 1u, c == 2u, c == 3u, c == 4u};
+                uint32_t rk[5], tot[5];
+                block_rank<5>(pr, rk, tot, lds_sort, parity_sort);
+                if (k < n) {
+                    uint32_t pos = 0;
+#pragma unroll
+                    for (int cc = 0; cc < 5; cc++)
+                        if ((uint32_t)cc == c) pos = run[cc] + rk[cc];
+                    place[k] = (c << 24) | pos;
+                }
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++) run[cc] += tot[cc];
+            }
+            __syncthreads();
+            uint32_t off[5], acc = 0;
+#pragma unroll
+            for (int cc = 0; cc < 5; cc++) { off[cc] = acc; acc += run[cc]; }
+            for (uint32_t k = threadIdx.x; k < n; k += kBlock) {
+                const uint32_t pc = place[k], c = pc >> 24;
+                uint32_t o = 0;
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++)
+                    if ((uint32_t)cc == c) o = off[cc];
+                order[o + (pc & 0xffffffu)] = q[base + k];
+            }
+            __syncthreads();
+        }
         uint32_t bc = 0, bd = 0, bs = 0;
         for (uint32_t r0 = 0; r0 < n; r0 += kBlock) {
             const uint32_t k = r0 + threadIdx.x;
@@ -383,7 +431,7 @@ __global__ __launch_bounds__(kBlock) void k_p_shade(SceneBufs S, PathBufs B, con
             uint32_t j = 0;
             float4 so = make_float4(0, 0, 0, 0), sd = so, sc = so;
             if (k < n) {
-                j = q[base + k];
+                j = SORT ? order[k] : q[base + k];
                 const int bounce = (int)B.bnc[j];
                 if constexpr (MULTI) {
                     const RenderParams& Q = Pd[B.sid[j]];
@@ -582,10 +630,10 @@ struct RenderState {
     struct Tune {
         int w, rows, spp, bounces;
         uint64_t topo, geom, shape;  // shape: hash of a batch's frame sizes (0: one scene)
-        int pool, iters, seg;
+        int pool, iters, seg, sort;
     };
     std::vector<Tune> tunes;
-    int last_pool = 0, last_iters = 0, last_windows = 0, last_seg = 0;
+    int last_pool = 0, last_iters = 0, last_windows = 0, last_seg = 0, last_sort = 0;
 };
 
 namespace {
@@ -727,12 +775,15 @@ void launch_bounce(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::R
 template <int STACK, bool MULTI>
 void launch_iteration(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd,
                       const uint32_t* gpre, uint32_t njobs, int cur, uint32_t* guides, int grid, uint32_t nseg,
-                      bool last, hipStream_t st) {
+                      bool last, bool sort, hipStream_t st) {
     // the drain test reads the flag written by the last iteration of a replay only
     if (last) hipLaunchKernelGGL(bcd::k_p_clear_flag, dim3(1), dim3(64), 0, st, B);
     hipLaunchKernelGGL((bcd::k_p_extend<STACK, MULTI>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, nseg,
                        gpre, njobs);
-    hipLaunchKernelGGL(bcd::k_p_shade<MULTI>, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
+    if (sort)
+        hipLaunchKernelGGL((bcd::k_p_shade<MULTI, true>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
+    else
+        hipLaunchKernelGGL((bcd::k_p_shade<MULTI, false>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
     hipLaunchKernelGGL((bcd::k_shadow<STACK, MULTI>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg, 0u, Pd);
 }
 
@@ -814,10 +865,10 @@ struct Job {
 template <int STACK>
 void capture_iterations(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd,
                         const uint32_t* gpre, uint32_t njobs, bool multi, uint32_t* guides, int grid, uint32_t nseg,
-                        int iters, hipStream_t st) {
+                        int iters, bool sort, hipStream_t st) {
     for (int it = 0; it < iters; it++) {
-        if (multi) launch_iteration<STACK, true>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, st);
-        else launch_iteration<STACK, false>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, st);
+        if (multi) launch_iteration<STACK, true>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, sort, st);
+        else launch_iteration<STACK, false>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, sort, st);
     }
 }
 
@@ -826,7 +877,7 @@ void capture_iterations(const bcd::SceneBufs& S, const bcd::PathBufs& B, const b
 // iterations are one captured graph.  Jobs are rendered in windows of samples [s0, s0 + ns)
 // of every job whose per-sample results fit the result buffer.
 int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* rays_out, uint64_t* shadow_out, int pool,
-                           int iters, uint32_t ss) {
+                           int iters, uint32_t ss, bool sort) {
     const bool multi = !(jobs.size() == 1 && jobs[0].scene < 0);
     const uint32_t njobs = (uint32_t)jobs.size();
     size_t npix_all = 0;
@@ -860,7 +911,7 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
                                     c->d_light_of, c->d_err, R.Pd, guides,
                                     (const void*)(uintptr_t)nseg, (const void*)(uintptr_t)grid,
                                     (const void*)(uintptr_t)iters, (const void*)(uintptr_t)c->stack, (const void*)(uintptr_t)ss,
-                                    (const void*)(uintptr_t)R.B.xg, (const void*)(uintptr_t)multi,
+                                    (const void*)(uintptr_t)R.B.xg, (const void*)(uintptr_t)multi, (const void*)(uintptr_t)sort,
                                     (const void*)(uintptr_t)njobs};
     for (void* p : R.buf) key.push_back(p);
     if (!R.exec || key != R.g_key) {
@@ -868,10 +919,10 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
         hipGraph_t graph = nullptr;
         if (!bc_check(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture")) return BC_ERR_DEVICE;
         switch (c->stack) {
-        case 8: capture_iterations<8>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
-        case 16: capture_iterations<16>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
-        case 24: capture_iterations<24>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
-        default: capture_iterations<32>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, st); break;
+        case 8: capture_iterations<8>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
+        case 16: capture_iterations<16>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
+        case 24: capture_iterations<24>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
+        default: capture_iterations<32>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
         }
         if (!bc_check(c, hipStreamEndCapture(st, &graph), "end capture")) return BC_ERR_DEVICE;
         hipError_t e = hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0);
@@ -931,6 +982,7 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
     R.last_pool = (int)slots;
     R.last_iters = iters;
     R.last_seg = (int)ss;
+    R.last_sort = sort ? 1 : 0;
     R.last_windows = windows;
     return finish_tally(c, rays_out, shadow_out);
 }
@@ -948,9 +1000,10 @@ uint64_t jobs_shape(const std::vector<Job>& jobs) {
 // Autotuner: times short renders of the same jobs, with enough samples to refill the pool about
 // three times (at least 1 spp), and keeps the fastest per sample for this image shape, scene and
 // transport depth.  Pool size and segment size are swept at 8 iterations per graph, then the
-// graph length for the best pair.  Each trial is timed after one untimed run (which captures the
-// graph) unless the trial itself holds more than 2^24 path samples.
-bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg) {
+// graph length for the best pair, then material-sorted shading for the best triple.  Each trial
+// is timed after one untimed run (which captures the graph) unless the trial itself holds more
+// than 2^24 path samples.
+bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg, bool& sort) {
     RenderState& R = *c->rs;
     const bc_render_desc* d = jobs[0].d;
     const uint64_t shape = jobs_shape(jobs);
@@ -960,6 +1013,7 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
             pool = t.pool;
             iters = t.iters;
             seg = (uint32_t)t.seg;
+            sort = t.sort != 0;
             return true;
         }
     size_t npix = 0;
@@ -976,7 +1030,8 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
     pool = 1 << 20;
     iters = 8;
     seg = bcd::kSegPersist;
-    auto time_trial = [&](int p, int it, uint32_t ss, float& per_sample) -> bool {
+    sort = false;
+    auto time_trial = [&](int p, int it, uint32_t ss, bool so, float& per_sample) -> bool {
         const int tspp = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(1, (3 * (size_t)p + npix - 1) / npix));
         for (size_t k = 0; k < jobs.size(); k++) {
             trial[k] = *jobs[k].d;
@@ -984,11 +1039,11 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
             tj[k].d = &trial[k];
         }
         if (npix * (size_t)tspp <= ((size_t)1 << 24) &&
-            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK)
+            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss, so) != BC_OK)
             return false;
         float ms = 0;
         if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
-            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss) != BC_OK ||
+            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss, so) != BC_OK ||
             !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
             !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
             !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
@@ -1000,31 +1055,38 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
         if ((size_t)p > npix * (size_t)max_spp && p != pools[0]) continue;
         for (uint32_t ss : segs) {
             float ms;
-            if (!time_trial(p, 8, ss, ms)) return false;
+            if (!time_trial(p, 8, ss, false, ms)) return false;
             if (ms < best) { best = ms; pool = p; seg = ss; }
         }
     }
     for (int it : {4, 16}) {
         float ms;
-        if (!time_trial(pool, it, seg, ms)) return false;
+        if (!time_trial(pool, it, seg, false, ms)) return false;
         if (ms < best) { best = ms; iters = it; }
     }
-    R.tunes.push_back({d->width, jobs[0].rows, d->spp, d->bounces, c->topo, c->geom, shape, pool, iters, (int)seg});
+    {
+        float ms;
+        if (!time_trial(pool, iters, seg, true, ms)) return false;
+        if (ms < best) { best = ms; sort = true; }
+    }
+    R.tunes.push_back({d->width, jobs[0].rows, d->spp, d->bounces, c->topo, c->geom, shape, pool, iters, (int)seg,
+                       sort ? 1 : 0});
     return true;
 }
 
 // Schedule shape of a mode-1 render from the first description: explicit or autotuned.
-int persistent_shape(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg) {
+int persistent_shape(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg, bool& sort) {
     const bc_render_desc* d = jobs[0].d;
     pool = d->reserved[0];
     iters = d->reserved[1];
     seg = d->reserved[2] > 0 ? (uint32_t)d->reserved[2] : bcd::kSegPersist;
+    sort = d->reserved[4] != 0;
     if (seg % bcd::kBlock != 0 || seg > bcd::kSegMax) {
         c->err = "segment size (reserved[2]) must be a multiple of 256 up to 4096";
         return BC_ERR_ARG;
     }
     if (pool <= 0 || iters <= 0) {
-        if (!autotune(c, jobs, pool, iters, seg)) return BC_ERR_DEVICE;
+        if (!autotune(c, jobs, pool, iters, seg, sort)) return BC_ERR_DEVICE;
     }
     return BC_OK;
 }
@@ -1074,9 +1136,10 @@ extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int 
     const std::vector<Job> jobs = {Job{d, y0, rows, -1, d_rgba, d_guides}};
     int pool, iters;
     uint32_t seg;
-    const int rc = persistent_shape(c, jobs, pool, iters, seg);
+    bool sort;
+    const int rc = persistent_shape(c, jobs, pool, iters, seg, sort);
     if (rc != BC_OK) return rc;
-    return render_persistent_jobs(c, jobs, rays_out, shadow_out, pool, iters, seg);
+    return render_persistent_jobs(c, jobs, rays_out, shadow_out, pool, iters, seg, sort);
 }
 
 extern "C" int bc_batch_render_device(bc_ctx* c, int n, const bc_render_desc* descs, float* const* d_rgba,
@@ -1118,9 +1181,10 @@ extern "C" int bc_batch_render_device(bc_ctx* c, int n, const bc_render_desc* de
         if (!ensure_state(c, 1, 1, false, 1, bcd::kSegPersist, n)) return BC_ERR_NOMEM;
         int pool, iters;
         uint32_t seg;
-        int rc = persistent_shape(c, jobs, pool, iters, seg);
+        bool sort;
+        int rc = persistent_shape(c, jobs, pool, iters, seg, sort);
         if (rc != BC_OK) return rc;
-        rc = render_persistent_jobs(c, jobs, &rays, &shadow, pool, iters, seg);
+        rc = render_persistent_jobs(c, jobs, &rays, &shadow, pool, iters, seg, sort);
         if (rc != BC_OK) return rc;
     }
     if (rays_out) *rays_out = rays;
@@ -1135,7 +1199,8 @@ extern "C" int bc_render_tuning(const bc_ctx* c, int32_t* out8) {
     out8[2] = c->stack;
     out8[3] = c->rs ? c->rs->last_windows : 0;
     out8[4] = c->rs ? c->rs->last_seg : 0;
-    for (int k = 5; k < 8; k++) out8[k] = 0;
+    out8[5] = c->rs ? c->rs->last_sort : 0;
+    for (int k = 6; k < 8; k++) out8[k] = 0;
     return BC_OK;
 }
 

@@ -89,7 +89,8 @@ typedef struct bc_render_desc {
     int32_t samples_per_pass;   /* 0 = automatic */
     int32_t reserved[6];        /* mode 1: [0] pool slots, [1] iterations per graph (0 for either =
                                    autotune), [2] slots per segment (0 = 256; multiple of 256);
-                                   any mode: [3] path segments per XCD turn (0 = 1) */
+                                   any mode: [3] path segments per XCD turn (0 = 1);
+                                   mode 1: [4] 1 = shade each segment's paths sorted by material */
     /* Direct lighting by resampled importance sampling (ReSTIR DI initial candidates; an
      * extension, the Go integrator has one-sample NEE): restir_candidates > 0 draws that many
      * candidates per diffuse vertex from {sun, sky, emitters} and keeps one in a reservoir. */
@@ -138,7 +139,7 @@ int bc_batch_render_device(bc_ctx* ctx, int n, const bc_render_desc* descs, floa
                            uint64_t* rays_out, uint64_t* shadow_out);
 
 /* Schedule last used by a mode-1 render, 8 ints: pool slots, iterations per graph, traversal
- * stack depth, sample windows, slots per segment, 0, 0, 0.  No counterpart in the reference
+ * stack depth, sample windows, slots per segment, material-sorted shading (0/1), 0, 0.  No counterpart in the reference
  * (its kernel has one fixed shape). */
 int bc_render_tuning(const bc_ctx* ctx, int32_t* out8);
 

@@ -36,9 +36,10 @@ def scenes_and_descs():
     return [box, show, b1, terr], descs
 
 
-@pytest.mark.parametrize("mode,pool,iters,segment", [(0, 0, 0, 0), (1, 3000, 4, 0), (1, 20000, 8, 1024), (1, 0, 0, 0)])
-def test_batch_equals_single_scene_renders(dev, mode, pool, iters, segment):
-    print(f"case: batch mode {mode} pool {pool} iters {iters} segment {segment}", flush=True)
+@pytest.mark.parametrize("mode,pool,iters,segment,msort", [(0, 0, 0, 0, 0), (1, 3000, 4, 0, 0), (1, 20000, 8, 1024, 1),
+                                                           (1, 0, 0, 0, 0)])
+def test_batch_equals_single_scene_renders(dev, mode, pool, iters, segment, msort):
+    print(f"case: batch mode {mode} pool {pool} iters {iters} segment {segment} material sort {msort}", flush=True)
     scenes, descs = scenes_and_descs()
     singles, srays = [], 0
     for s, d in zip(scenes, descs):
@@ -49,7 +50,7 @@ def test_batch_equals_single_scene_renders(dev, mode, pool, iters, segment):
     dev.upload_batch(scenes, nee_triangles=True)
     for d in descs:
         d.mode = mode
-        d.reserved[0], d.reserved[1], d.reserved[2] = pool, iters, segment
+        d.reserved[0], d.reserved[1], d.reserved[2], d.reserved[4] = pool, iters, segment, msort
     imgs, rays = dev.render_batch(descs)
     assert rays == srays
     for k, (a, b) in enumerate(zip(imgs, singles)):

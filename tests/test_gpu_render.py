@@ -123,17 +123,18 @@ def test_same_seed_fp64_and_converged_mean(dev):
     assert abs(img2[..., :3].mean() / ref64[..., :3].mean() - 1) < 0.01
 
 
-@pytest.mark.parametrize("pool,iters,segment", [(4096, 4, 0), (50000, 8, 1024), (3000, 6, 512), (0, 0, 0)])
-def test_persistent_schedule_equals_plain(dev, pool, iters, segment):
+@pytest.mark.parametrize("pool,iters,segment,msort", [(4096, 4, 0, 0), (50000, 8, 1024, 0), (3000, 6, 512, 1),
+                                                      (20000, 4, 256, 1), (0, 0, 0, 0)])
+def test_persistent_schedule_equals_plain(dev, pool, iters, segment, msort):
     """In-place regeneration, hipGraph replay and the autotuner (pool=0) change the schedule but
     not the image: per-sample results are reduced in sample order, so rgba is bit-identical."""
-    print(f"case: persistent pool={pool} iters={iters} segment={segment}", flush=True)
+    print(f"case: persistent pool={pool} iters={iters} segment={segment} material sort={msort}", flush=True)
     s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
     dev.upload(s)
     d = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20)
     plain, gp, rp = dev.render(d, guides=True)
     dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=pool, iters=iters,
-                        segment=segment)
+                        segment=segment, material_sort=msort)
     pers, gq, rq = dev.render(dp, guides=True)
     assert rp == rq
     assert np.array_equal(plain, pers) and np.array_equal(gp, gq)
