@@ -37,6 +37,19 @@ class SceneView(ctypes.Structure):
     ]
 
 
+class PostDesc(ctypes.Structure):
+    _fields_ = [
+        ("denoise_passes", c_int32), ("fxaa", c_int32), ("txaa", c_int32), ("hdr", c_int32),
+        ("exposure", c_float), ("gamma", c_float), ("reserved", c_int32 * 4),
+    ]
+
+
+def post_desc(denoise=0, fxaa=0, txaa=0, hdr=2, exposure=0.0, gamma=2.2):
+    p = PostDesc()
+    p.denoise_passes, p.fxaa, p.txaa, p.hdr, p.exposure, p.gamma = denoise, fxaa, txaa, hdr, exposure, gamma
+    return p
+
+
 class RenderDesc(ctypes.Structure):
     _fields_ = [
         ("width", c_int32), ("height", c_int32), ("spp", c_int32), ("bounces", c_int32),
@@ -72,6 +85,8 @@ def lib():
             "bc_render_device": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_render_tuning": (c_int, [vp, POINTER(c_int32)]),
             "bc_batch_upload": (c_int, [vp, c_int, vp]),
+            "bc_postprocess_device": (c_int, [vp, POINTER(PostDesc), c_int, c_int, vp, vp, vp, vp]),
+            "bc_render_post": (c_int, [vp, POINTER(RenderDesc), POINTER(PostDesc), vp, vp, POINTER(c_uint64)]),
             "bc_batch_render_device": (c_int, [vp, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_forward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, POINTER(c_uint64)]),
             "bc_backward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, vp]),
@@ -352,6 +367,19 @@ class Device:
         self._check(lib().bc_render_device(self.ctx, ctypes.byref(desc), y0, rows, d_rgba, d_guides,
                                            ctypes.byref(rays), ctypes.byref(shadow)))
         return int(rays.value), int(shadow.value)
+
+    def postprocess_device(self, post, width, height, d_rgba, d_guides, d_pixels, d_linear=None):
+        self._check(lib().bc_postprocess_device(self.ctx, ctypes.byref(post), width, height, d_rgba, d_guides,
+                                                d_pixels, d_linear))
+
+    def render_post(self, desc, post, want_linear=False):
+        """Render + device post stack: (RGBA8 H x W uint32, linear H x W x 4 or None, segments)."""
+        px = np.zeros((desc.height, desc.width), np.uint32)
+        lin = np.zeros((desc.height, desc.width, 4), np.float32) if want_linear else None
+        rays = c_uint64()
+        self._check(lib().bc_render_post(self.ctx, ctypes.byref(desc), ctypes.byref(post), _ptr(px),
+                                         _ptr(lin) if lin is not None else None, ctypes.byref(rays)))
+        return px, lin, int(rays.value)
 
     def upload_batch(self, scenes, nee_triangles=False):
         """Packs several host scenes into one device arena (bc_batch_upload)."""

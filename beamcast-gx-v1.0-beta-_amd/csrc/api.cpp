@@ -160,6 +160,7 @@ void bc_destroy(bc_ctx* c) {
     for (void* q : ptrs)
         if (q) hipFree(q);
     bc_render_release(c);
+    bc_post_release(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);

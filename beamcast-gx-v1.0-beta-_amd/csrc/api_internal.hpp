@@ -10,6 +10,7 @@
 #include "bc_layout.h"
 
 struct RenderState;  // integrator buffers (integrator.hip)
+struct PostState;    // post-process buffers (post.hip)
 
 struct bc_ctx {
     int device = 0;
@@ -53,6 +54,8 @@ struct bc_ctx {
     size_t out_cap = 0;
     // integrator
     RenderState* rs = nullptr;
+    // post stack
+    PostState* ps = nullptr;
 };
 
 bool bc_ensure(bc_ctx* c, void*& p, size_t& cap, size_t bytes, const char* what);
@@ -61,3 +64,4 @@ bool bc_check(bc_ctx* c, hipError_t e, const char* what);
 int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data);
 int bc_check_kernel_error(bc_ctx* c);
 void bc_render_release(bc_ctx* c);
+void bc_post_release(bc_ctx* c);

@@ -123,10 +123,21 @@ def bench_terrain(bc, dev, args, threads):
               sun=(3.0, 3.0, 3.0))
     d = bc.render_desc(s, w, h, spp, bounces, **kw)
     d_rgba = dev.alloc(w * h * 16)
-    dev.render_device(d, 0, h, d_rgba)
+    d_g = dev.alloc(w * h * 12)
+    dev.render_device(d, 0, h, d_rgba, d_g)
     t0 = time.perf_counter()
     segs, shadow = dev.render_device(d, 0, h, d_rgba)
     sec = time.perf_counter() - t0
+    # device post stack on the rendered frame: denoise (3 passes), FXAA 2, TXAA 1, toRGBA
+    post = bc.post_desc(denoise=3, fxaa=2, txaa=1, hdr=2)
+    d_px = dev.alloc(w * h * 4)
+    dev.postprocess_device(post, w, h, d_rgba, d_g, d_px)
+    dev.timer_start()
+    for _ in range(5):
+        dev.postprocess_device(post, w, h, d_rgba, d_g, d_px)
+    post_ms = dev.timer_stop() / 5
+    dev.free(d_px)
+    dev.free(d_g)
     dp = bc.render_desc(s, w, h, spp, bounces, mode=1, **kw)
     t0 = time.perf_counter()
     dev.render_device(dp, 0, h, d_rgba)  # autotunes and captures once
@@ -146,7 +157,9 @@ def bench_terrain(bc, dev, args, threads):
            "segments_per_sample": round(segs / samples, 4),
            "persistent": {"ms_per_frame": round(psec * 1e3, 2), "path_samples_per_s": round(samples / psec, 1),
                           "over_plain": round(sec / psec, 3), "tuning": dev.tuning(),
-                          "autotune_and_first_frame_s": round(tune, 2)}}
+                          "autotune_and_first_frame_s": round(tune, 2)},
+           "post": {"ms_per_frame": round(post_ms, 3), "stages": "a-trous denoise 3 passes, FXAA 2, TXAA 1, toRGBA",
+                    "readback_bytes_per_pixel": {"with_device_post": 4, "without": 28}}}
     if not args.no_cpu:
         import oracle as orc
 

@@ -138,6 +138,30 @@ int bc_batch_upload(bc_ctx* ctx, int n, const bc_scene_view* scenes);
 int bc_batch_render_device(bc_ctx* ctx, int n, const bc_render_desc* descs, float* const* d_rgba,
                            uint64_t* rays_out, uint64_t* shadow_out);
 
+/* Post stack on the device (SURVEY "next" row: guide + post stack): restates, in fp32, the part of
+ * RenderState.Start after the render (renderer.go:2842-2895) with HBAO, temporal reuse, bloom,
+ * debug views and upscaling off: atrousDenoise with the compact guides (renderer.go:2228-2287,
+ * guide decode gpu_opencl_windows.go:552-562, 686-716), applyFXAA (:2557-2586),
+ * applyTXAAResolve (:2588-2595, edgeAwareSharpen :2146-2183), toRGBA (:2055-2064). */
+typedef struct bc_post_desc {
+    int32_t denoise_passes;  /* atrousDenoise iterations (Start: 2 + min(3, max(TAA, TXAA))); 0 = off */
+    int32_t fxaa;            /* applyFXAA level, 0 = off */
+    int32_t txaa;            /* applyTXAAResolve level, 0 = off */
+    int32_t hdr;             /* toneMapHDR mode (0 clamp, 1 Reinhard, 3 ACES fit, other filmic) */
+    float exposure, gamma;   /* toRGBA */
+    int32_t reserved[4];
+} bc_post_desc;
+/* d_rgba: W*H*4 floats as bc_render_device writes them; d_guides: its compact guides (needed
+ * when denoise_passes > 0).  Writes RGBA8 words (0xAABBGGRR, toRGBA's packing) to d_pixels and/or
+ * the processed linear frame (same float4 layout) to d_linear; either may be NULL, not both. */
+int bc_postprocess_device(bc_ctx* ctx, const bc_post_desc* post, int width, int height, const float* d_rgba,
+                          const uint32_t* d_guides, uint32_t* d_pixels, float* d_linear);
+/* Whole frame: bc_render_device + bc_postprocess_device, then only the RGBA8 pixels (4 B/px;
+ * 28 B/px without the device post stack) and, if linear is not NULL, the processed linear frame
+ * come back to the host. */
+int bc_render_post(bc_ctx* ctx, const bc_render_desc* desc, const bc_post_desc* post, uint32_t* pixels,
+                   float* linear, uint64_t* rays_out);
+
 /* Schedule last used by a mode-1 render, 8 ints: pool slots, iterations per graph, traversal
  * stack depth, sample windows, slots per segment, material-sorted shading (0/1), 0, 0.  No counterpart in the reference
  * (its kernel has one fixed shape). */

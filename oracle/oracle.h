@@ -124,6 +124,15 @@ int64_t orc_render64(const orc_scene* s, const orc_render_desc* d, int y0, int r
 int64_t orc_render32(const orc_scene* s, const orc_render_desc* d, int y0, int rows,
                      float* rgba, uint32_t* guides, int64_t* shadow_rays, int nthreads);
 
+/* Post stack (oracle_post.c): denoise -> FXAA -> TXAA -> toRGBA on a rendered frame.
+ * rgba: W*H*4 floats (bc_render layout); guides: W*H*3 compact guide words (may be NULL when
+ * denoise_passes == 0).  linear_out: W*H*3 doubles (may be NULL); pixels_out: W*H RGBA8 words
+ * (may be NULL).  Returns 0, or -1 when out of memory. */
+int orc_post(int w, int h, const float* rgba, const uint32_t* guides, int denoise_passes, int fxaa_level,
+             int txaa_level, double exposure, double gamma, int hdr, double* linear_out, uint32_t* pixels_out);
+uint32_t orc_to_rgba(const double* c, double exposure, double gamma, int hdr);
+void orc_decode_oct_normal(uint32_t qx, uint32_t qy, double* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,9 @@ def lib():
             "orc_camera_basis": (None, [vp, vp, vp, c_double, c_double, vp]),
             "orc_render64": (c_int64, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_int64), c_int]),
             "orc_render32": (c_int64, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_int64), c_int]),
+            "orc_post": (c_int, [c_int, c_int, vp, vp, c_int, c_int, c_int, c_double, c_double, c_int, vp, vp]),
+            "orc_to_rgba": (c_uint32, [vp, c_double, c_double, c_int]),
+            "orc_decode_oct_normal": (None, [c_uint32, c_uint32, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -225,3 +228,26 @@ def aabb_hit(box6, ray6, tmin=1e-4, tmax=1e30):
     r = np.asarray(ray6, np.float64)
     near = c_double()
     return bool(lib().orc_aabb_hit(_p(b), _p(r), tmin, tmax, ctypes.byref(near))), near.value
+
+
+def post(rgba, guides=None, denoise=0, fxaa=0, txaa=0, exposure=0.0, gamma=2.2, hdr=2):
+    """Post stack of the renderer on a frame: (linear H x W x 3 float64, RGBA8 H x W uint32)."""
+    rgba = np.ascontiguousarray(rgba, np.float32)
+    h, w = rgba.shape[:2]
+    g = None if guides is None else np.ascontiguousarray(guides, np.uint32)
+    lin = np.zeros((h, w, 3))
+    pix = np.zeros((h, w), np.uint32)
+    if lib().orc_post(w, h, _p(rgba), _p(g), denoise, fxaa, txaa, exposure, gamma, hdr, _p(lin), _p(pix)) != 0:
+        raise MemoryError("orc_post")
+    return lin, pix
+
+
+def to_rgba(c, exposure=0.0, gamma=2.2, hdr=2):
+    v = np.ascontiguousarray(c, np.float64)
+    return int(lib().orc_to_rgba(_p(v), exposure, gamma, hdr))
+
+
+def decode_oct_normal(qx, qy):
+    n = np.zeros(3)
+    lib().orc_decode_oct_normal(qx, qy, _p(n))
+    return n

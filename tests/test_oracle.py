@@ -363,3 +363,88 @@ def test_ris_matches_one_sample_nee():
         img, _, _, _ = o.render(d, precision=64)
         means.append(img[..., :3].reshape(-1, 3).mean(axis=0))
     assert np.all(np.abs(means[1] / means[0] - 1) < 0.02)
+
+
+# ---------------------------------------------------------------- post stack (oracle_post.c)
+
+def rgba_bytes(word):
+    return word & 255, (word >> 8) & 255, (word >> 16) & 255, word >> 24
+
+
+def test_to_rgba_special_values():
+    """toRGBA (renderer.go:2055-2064): identity tone curve and gamma 1 quantise as floor(256 c);
+    exposure doubles per stop; gamma 2 is a square root; Reinhard maps 1 -> 1/2 and 3 -> 3/4;
+    every curve maps 0 to 0 and saturates at 255; alpha is 0xff, red in the low byte."""
+    assert rgba_bytes(orc.to_rgba([0.5, 0.25, 0.75], 0.0, 1.0, 0)) == (128, 64, 192, 255)
+    assert rgba_bytes(orc.to_rgba([0.25, 0.125, 0.0], 1.0, 1.0, 0)) == (128, 64, 0, 255)
+    assert rgba_bytes(orc.to_rgba([0.25, 0.0625, 1.0], 0.0, 2.0, 0)) == (128, 64, 255, 255)
+    assert rgba_bytes(orc.to_rgba([1.0, 3.0, 0.0], 0.0, 1.0, 1)) == (128, 192, 0, 255)
+    for hdr in (0, 1, 2, 3):
+        assert rgba_bytes(orc.to_rgba([0.0, 0.0, 0.0], 0.0, 2.2, hdr)) == (0, 0, 0, 255)
+    for hdr in (0, 2, 3):
+        assert rgba_bytes(orc.to_rgba([1e6, 1e6, 1e6], 0.0, 2.2, hdr))[:3] == (255, 255, 255)
+    # gamma is floored at 0.1: 0.5 ** 10 * 256 < 1
+    assert rgba_bytes(orc.to_rgba([0.5, 0.5, 0.5], 0.0, 0.01, 0))[:3] == (0, 0, 0)
+    # monotone in radiance for every curve
+    for hdr in (0, 1, 2, 3):
+        vals = [rgba_bytes(orc.to_rgba([v, v, v], 0.0, 2.2, hdr))[0] for v in np.linspace(0, 4, 200)]
+        assert all(a <= b for a, b in zip(vals, vals[1:]))
+
+
+def test_decode_oct_normal():
+    """decodeOctNormal (gpu_opencl_windows.go:552-562): unit length; the folded corners are -z."""
+    for q in [(0, 0), (1023, 1023), (0, 1023), (1023, 0)]:
+        assert np.allclose(orc.decode_oct_normal(*q), [0, 0, -1])
+    rng = np.random.default_rng(3)
+    for qx, qy in rng.integers(0, 1024, size=(200, 2)):
+        n = orc.decode_oct_normal(int(qx), int(qy))
+        assert abs(np.linalg.norm(n) - 1) < 1e-12
+        x, y = qx / 1023 * 2 - 1, qy / 1023 * 2 - 1
+        assert (n[2] >= 0) == (abs(x) + abs(y) <= 1)
+
+
+def guides_uniform(h, w, depth=3.0, valid=None, packed_normal=(511, 511)):
+    g = np.zeros((h, w, 3), np.uint32)
+    g[..., 0] = np.float32(depth).view(np.uint32)
+    g[..., 1] = 0x80000000 | (packed_normal[1] << 10) | packed_normal[0]
+    if valid is not None:
+        g[..., 1][~valid] = 0
+        g[..., 0][~valid] = 0
+    return g
+
+
+def test_atrous_properties():
+    """atrousDenoise (renderer.go:2228-2287): a constant image is a fixed point; pixels with and
+    without a primary hit never mix; one pass at an interior pixel with equal guides is the
+    weighted mean (1.5 a + 8 e b) / (1.5 + 8 e), e = exp(-3.5 |lum(a) - lum(b)|)."""
+    h, w = 9, 11
+    rgba = np.zeros((h, w, 4), np.float32)
+    rgba[..., :3] = (0.3, 0.6, 0.2)
+    lin, _ = orc.post(rgba, guides_uniform(h, w), denoise=3, fxaa=0)
+    assert np.allclose(lin, np.float32([0.3, 0.6, 0.2]), rtol=1e-12, atol=0)
+    valid = np.zeros((h, w), bool)
+    valid[:, :5] = True
+    rgba[..., :3] = np.where(valid[..., None], 1.0, 2.0)
+    lin, _ = orc.post(rgba, guides_uniform(h, w, valid=valid), denoise=2)
+    assert np.allclose(lin[valid], 1.0) and np.allclose(lin[~valid], 2.0)
+    rgba[..., :3] = 0.2
+    rgba[4, 5, :3] = 1.0
+    lin, _ = orc.post(rgba, guides_uniform(h, w), denoise=1)
+    e = np.exp(-3.5 * 0.8)
+    assert np.isclose(lin[4, 5, 0], (1.5 * 1.0 + 8 * e * 0.2) / (1.5 + 8 * e), rtol=1e-12)
+
+
+def test_fxaa_and_txaa_properties():
+    """applyFXAA (renderer.go:2557-2586): borders and low-contrast pixels are untouched; a lone
+    bright pixel blends with the mean of its 4 neighbours (level 2: blend 0.5).  TXAA leaves a
+    constant image constant."""
+    h, w = 7, 8
+    rgba = np.full((h, w, 4), 0.1, np.float32)
+    lin, _ = orc.post(rgba, fxaa=3, txaa=2)
+    assert np.allclose(lin, np.float32(0.1), rtol=1e-12)
+    rgba[3, 4, :3] = 2.0
+    lin, _ = orc.post(rgba, fxaa=2)
+    a, b = 2.0, float(np.float32(0.1))  # the frame is fp32; the stack computes in fp64
+    assert np.isclose(lin[3, 4, 0], a * 0.5 + b * 0.5, rtol=1e-12)
+    assert np.isclose(lin[2, 4, 0], b * 0.5 + (a + 3 * b) / 4 * 0.5, rtol=1e-12)
+    assert np.allclose(lin[0], b) and np.allclose(lin[:, 0], b)
