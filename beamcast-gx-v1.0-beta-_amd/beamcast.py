@@ -91,6 +91,8 @@ def lib():
             "bc_forward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, POINTER(c_uint64)]),
             "bc_backward": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, vp]),
             "bc_update_materials": (c_int, [vp, c_int32, vp, vp]),
+            "bc_scene_refit": (c_int, [vp, vp, c_uint64]),
+            "bc_scene_arrays": (c_int, [vp, vp, vp, vp]),
             "bc_trace": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_trace_device": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_alloc": (vp, [vp, c_uint64]),
@@ -255,6 +257,12 @@ class Device:
     def __del__(self):
         self.close()
 
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
     def _check(self, rc):
         if rc != BC_OK:
             raise BeamcastError(f"status {rc}: {lib().bc_last_error(self.ctx).decode()}")
@@ -267,6 +275,12 @@ class Device:
         topo, geom = revisions if revisions is not None else scene.revisions()
         self._check(lib().bc_scene_upload(self.ctx, scene.view(nee_triangles), topo, geom))
 
+    def refit(self, scene: HostScene, nee_triangles=False, geom_revision=None):
+        """Geometry-only update (bc_scene_refit): primitives of the resident scene moved, same
+        topology.  Call scene.refit() (Scene.Refit) first so the host BVH matches."""
+        geom = geom_revision if geom_revision is not None else scene.revisions()[1]
+        self._check(lib().bc_scene_refit(self.ctx, scene.view(nee_triangles), geom))
+
     def upload_view(self, view, topo, geom):
         self._check(lib().bc_scene_upload(self.ctx, view, topo, geom))
 
@@ -274,6 +288,18 @@ class Device:
         o = (c_int64 * 4)()
         self._check(lib().bc_scene_stats(self.ctx, o))
         return {"nodes8": o[0], "depth8": o[1], "prims": o[2], "bytes": o[3]}
+
+    NODE8 = np.dtype([("o", "<f4", 3), ("e", "u1", 3), ("imask", "u1"), ("child_base", "<u4"), ("prim_base", "<u4"),
+                      ("meta", "u1", 8), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
+
+    def scene_arrays(self):
+        """The resident BVH8: (nodes [NODE8], records [n, 9] float32, ids [n] uint32)."""
+        st = self.stats()
+        nodes = np.zeros(st["nodes8"], self.NODE8)
+        recs = np.zeros((st["prims"], 9), np.float32)
+        ids = np.zeros(st["prims"], np.uint32)
+        self._check(lib().bc_scene_arrays(self.ctx, _ptr(nodes), _ptr(recs), _ptr(ids)))
+        return nodes, recs, ids
 
     def trace(self, rays, any_hit=False):
         rays = np.ascontiguousarray(rays, dtype=np.float32)

@@ -156,7 +156,7 @@ void bc_destroy(bc_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                    c->tmp_in, c->tmp_out, c->out_buf};
+                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox};
     for (void* q : ptrs)
         if (q) hipFree(q);
     bc_render_release(c);
@@ -197,6 +197,19 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
         !upload_vec(c, c->d_pmat, c->cap_pmat, b.mats, "upload prim materials") ||
         false)
         return BC_ERR_DEVICE;
+    // node lists by level for device refits (deepest level first)
+    {
+        std::vector<std::vector<uint32_t>> lv((size_t)b.depth + 1);
+        for (size_t i = 0; i < b.node_depth.size(); i++) lv[b.node_depth[i]].push_back((uint32_t)i);
+        std::vector<uint32_t> flat;
+        c->level_off.assign(1, 0u);
+        for (int d = b.depth; d >= 1; d--) {
+            flat.insert(flat.end(), lv[(size_t)d].begin(), lv[(size_t)d].end());
+            c->level_off.push_back((uint32_t)flat.size());
+        }
+        if (!upload_vec(c, c->d_levels, c->cap_levels, flat, "upload node levels")) return BC_ERR_DEVICE;
+        c->h_ids = b.ids;
+    }
     if (!ok(c, hipStreamSynchronize(c->stream), "scene upload")) return BC_ERR_DEVICE;
     c->nnode8 = (int64_t)b.nodes.size();
     c->depth8 = b.depth;
@@ -274,6 +287,49 @@ int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const dou
 }
 
 extern "C" {
+
+int bc_scene_refit(bc_ctx* c, const bc_scene_view* v, uint64_t geom) {
+    if (!c || !v) return BC_ERR_ARG;
+    if (!c->has_scene || !c->batch.empty()) { c->err = "bc_scene_refit needs a resident single scene"; return BC_ERR_SCENE; }
+    if (v->nprim != c->nprim || (int64_t)c->h_ids.size() != (int64_t)v->nprim) {
+        c->err = "bc_scene_refit: the primitive count differs from the resident scene (topology changed)";
+        return BC_ERR_ARG;
+    }
+    if (!check_view(c, *v)) return BC_ERR_SCENE;
+    for (size_t r = 0; r < c->h_ids.size(); r++) {
+        const int id = (int)(c->h_ids[r] & ~BC_SPHERE_BIT);
+        if ((v->prim_kind[id] == 0) != ((c->h_ids[r] & BC_SPHERE_BIT) != 0)) {
+            c->err = "bc_scene_refit: a primitive changed kind (topology changed)";
+            return BC_ERR_ARG;
+        }
+    }
+    (void)hipSetDevice(c->device);
+    // moved primitive records in leaf order
+    std::vector<bc_prim> recs(c->h_ids.size());
+    for (size_t r = 0; r < recs.size(); r++) recs[r] = bcx::prim_record(*v, (int)(c->h_ids[r] & ~BC_SPHERE_BIT));
+    if (!upload_vec(c, c->d_prims, c->cap_prims, recs, "upload primitive records")) return BC_ERR_DEVICE;
+    void* nb = c->d_nbox;
+    if (!bc_ensure(c, nb, c->cap_nbox, (size_t)c->nnode8 * 32, "refit boxes")) return BC_ERR_NOMEM;
+    c->d_nbox = (float4*)nb;
+    for (size_t k = 0; k + 1 < c->level_off.size(); k++) {
+        const uint32_t a = c->level_off[k], n = c->level_off[k + 1] - a;
+        if (!ok(c, bcd::launch_refit_level(c->d_nodes, c->d_prims, c->d_ids, c->d_nbox, c->d_levels + a, n, c->d_err,
+                                           c->stream), "refit launch"))
+            return BC_ERR_DEVICE;
+    }
+    // light geometry and its distribution (areas change with the primitives)
+    c->h_lights.clear();
+    c->h_light_area.clear();
+    collect_lights(*v, 0, 0, c->h_lights, c->h_light_area);
+    c->light_groups.assign(1, std::make_pair(0, (int)c->h_lights.size()));
+    const int rc = bc_set_materials(c, v->nmat, v->mat_kind, v->mat_data);
+    if (rc != BC_OK) return rc;
+    if (!ok(c, hipStreamSynchronize(c->stream), "refit")) return BC_ERR_DEVICE;
+    const int krc = bc_check_kernel_error(c);
+    if (krc != BC_OK) return krc;
+    c->geom = geom;
+    return BC_OK;
+}
 
 int bc_batch_upload(bc_ctx* c, int n, const bc_scene_view* views) {
     if (!c || n <= 0 || !views) return BC_ERR_ARG;
@@ -362,6 +418,19 @@ int bc_batch_upload(bc_ctx* c, int n, const bc_scene_view* views) {
 int bc_scene_stats(const bc_ctx* c, int64_t* o) {
     if (!c || !o) return BC_ERR_ARG;
     o[0] = c->nnode8; o[1] = c->depth8; o[2] = c->nprim; o[3] = c->scene_bytes;
+    return BC_OK;
+}
+
+int bc_scene_arrays(bc_ctx* c, void* nodes, void* prims, uint32_t* ids) {
+    if (!c) return BC_ERR_ARG;
+    if (!c->has_scene && c->batch.empty()) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
+    (void)hipSetDevice(c->device);
+    if (nodes && !ok(c, hipMemcpy(nodes, c->d_nodes, (size_t)c->nnode8 * sizeof(bc_node8), hipMemcpyDeviceToHost), "read nodes"))
+        return BC_ERR_DEVICE;
+    if (prims && !ok(c, hipMemcpy(prims, c->d_prims, (size_t)c->nprim * sizeof(bc_prim), hipMemcpyDeviceToHost), "read records"))
+        return BC_ERR_DEVICE;
+    if (ids && !ok(c, hipMemcpy(ids, c->d_ids, (size_t)c->nprim * sizeof(uint32_t), hipMemcpyDeviceToHost), "read ids"))
+        return BC_ERR_DEVICE;
     return BC_OK;
 }
 

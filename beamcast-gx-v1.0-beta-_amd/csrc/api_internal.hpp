@@ -44,6 +44,13 @@ struct bc_ctx {
         int64_t nnode;
     };
     std::vector<BatchScene> batch;
+    // device refit of the resident single scene: record ids, node lists by level (deepest
+    // first; level_off[k] .. level_off[k+1] is one level), per-node union boxes
+    std::vector<uint32_t> h_ids;
+    std::vector<uint32_t> level_off;
+    uint32_t* d_levels = nullptr;
+    float4* d_nbox = nullptr;
+    size_t cap_levels = 0, cap_nbox = 0;
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;

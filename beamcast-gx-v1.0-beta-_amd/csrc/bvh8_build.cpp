@@ -89,20 +89,8 @@ Child make_child(const bc_scene_view& v, int k) {
 }
 
 void write_prim(const bc_scene_view& v, int id, Bvh8& out) {
-    bc_prim r;
-    std::memset(&r, 0, sizeof r);
-    const double* g = v.prim_geom + 10 * (size_t)id;
-    uint32_t idbits = (uint32_t)id;
-    if (v.prim_kind[id] == 0) {
-        const double rad = g[3];
-        r.v[0] = (float)g[0]; r.v[1] = (float)g[1]; r.v[2] = (float)g[2];
-        r.v[3] = (float)rad; r.v[4] = (float)(rad * rad); r.v[5] = (float)(1.0 / rad);
-        idbits |= BC_SPHERE_BIT;
-    } else {
-        for (int k = 0; k < 9; k++) r.v[k] = (float)g[k];
-    }
-    out.prims.push_back(r);
-    out.ids.push_back(idbits);
+    out.prims.push_back(prim_record(v, id));
+    out.ids.push_back((uint32_t)id | (v.prim_kind[id] == 0 ? BC_SPHERE_BIT : 0u));
     out.mats.push_back(v.prim_mat[id]);
 }
 
@@ -208,11 +196,26 @@ void make_plans(const bc_scene_view& v, std::vector<Plan>& plan) {
 
 }  // namespace
 
+bc_prim prim_record(const bc_scene_view& v, int id) {
+    bc_prim r;
+    std::memset(&r, 0, sizeof r);
+    const double* g = v.prim_geom + 10 * (size_t)id;
+    if (v.prim_kind[id] == 0) {
+        const double rad = g[3];
+        r.v[0] = (float)g[0]; r.v[1] = (float)g[1]; r.v[2] = (float)g[2];
+        r.v[3] = (float)rad; r.v[4] = (float)(rad * rad); r.v[5] = (float)(1.0 / rad);
+    } else {
+        for (int k = 0; k < 9; k++) r.v[k] = (float)g[k];
+    }
+    return r;
+}
+
 bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
     out.nodes.clear();
     out.prims.clear();
     out.ids.clear();
     out.mats.clear();
+    out.node_depth.clear();
     out.depth = 0;
     if (v.nnode <= 0 || v.nprim <= 0) { err = "scene has no BVH nodes or primitives"; return false; }
     // validateGPUSceneEncoding analogue (gpu_opencl_windows.go:470-482)
@@ -349,6 +352,8 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
             }
         }
         out.nodes[(size_t)w.i8] = nd;
+        if (out.node_depth.size() < out.nodes.size()) out.node_depth.resize(out.nodes.size(), 0);
+        out.node_depth[(size_t)w.i8] = w.depth;
     }
     if ((int64_t)out.prims.size() != v.nprim) {
         err = "BVH leaves do not cover every primitive exactly once";

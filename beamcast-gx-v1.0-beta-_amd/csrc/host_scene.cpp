@@ -359,7 +359,15 @@ int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, s
     }
     bc_ctx* ctx = g_rt.ctx;
     deviceName = bc_device_name(ctx);
-    if (g_rt.scene != &scene || g_rt.topo != scene.GPUTopologyRevision || g_rt.geom != scene.GPUGeometryRevision) {
+    // ensureSceneLocked (gpu_opencl_windows.go:379-445): a geometry-only change refits the
+    // resident BVH on the device (updateGeometryLocked); anything else uploads the scene
+    bool refitted = false;
+    if (g_rt.scene == &scene && g_rt.topo == scene.GPUTopologyRevision && g_rt.geom != scene.GPUGeometryRevision) {
+        refitted = bc_scene_refit(ctx, &scene.View(false), scene.GPUGeometryRevision) == BC_OK;
+        if (refitted) g_rt.geom = scene.GPUGeometryRevision;
+    }
+    if (!refitted &&
+        (g_rt.scene != &scene || g_rt.topo != scene.GPUTopologyRevision || g_rt.geom != scene.GPUGeometryRevision)) {
         int rc = bc_scene_upload(ctx, &scene.View(false), scene.GPUTopologyRevision, scene.GPUGeometryRevision);
         if (rc != BC_OK) { err = bc_last_error(ctx); return rc; }
         g_rt.scene = &scene;

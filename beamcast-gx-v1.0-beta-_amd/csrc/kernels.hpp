@@ -18,4 +18,9 @@ hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc
                         const bc_ray* rays,
                         int64_t n, void* out, int* err, hipStream_t st);
 
+// One level of the device BVH8 refit: recomputes the boxes of the `n` nodes in `list`
+// (whose children are already refit) and stores each node's union box in nbox[2 i .. 2 i + 1].
+hipError_t launch_refit_level(bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, float4* nbox,
+                              const uint32_t* list, uint32_t n, int* err, hipStream_t st);
+
 }  // namespace bcd

@@ -113,6 +113,13 @@ func (rt *hipRuntime) ensureSceneLocked(scene *Scene) error {
 		nprim: C.int32_t(np), order: &order[0], prim_kind: &kind[0], prim_mat: &pmat[0], prim_geom: &geom[0],
 		nmat: C.int32_t(nm), mat_kind: &mkind[0], mat_data: &mdata[0], nee_triangles: 0,
 	}
+	// geometry-only change (updateGeometryLocked): refit the resident BVH8 on the device
+	if rt.scene == scene && rt.topologyRev == scene.GPUTopologyRevision {
+		if C.bc_scene_refit(rt.ctx, view, C.uint64_t(scene.GPUGeometryRevision)) == 0 {
+			rt.geometryRev = scene.GPUGeometryRevision
+			return nil
+		}
+	}
 	if rc := C.bc_scene_upload(rt.ctx, view, C.uint64_t(scene.GPUTopologyRevision), C.uint64_t(scene.GPUGeometryRevision)); rc != 0 {
 		return hipErr(rt.ctx)
 	}

@@ -66,8 +66,21 @@ int bc_scene_upload(bc_ctx* ctx, const bc_scene_view* scene, uint64_t topo_rev, 
  * BVH: new Color/Roughness/IOR/Emission for the uploaded scene (same material count and kinds
  * of the lights).  The differentiable render updates its parameters through this call. */
 int bc_update_materials(bc_ctx* ctx, int32_t nmat, const int32_t* mat_kind, const double* mat_data);
+/* Geometry-only update of the resident single scene: primitives moved, topology unchanged (same
+ * primitive count, kinds and Go BVH leaf membership).  Replaces updateGeometryLocked
+ * (gpu_opencl_windows.go:447-468), which re-packs and re-uploads every node after Scene.Refit
+ * (renderer.go:451-473): here only the primitive records are uploaded and the BVH8 node boxes are
+ * refit on the device bottom-up, one launch per level, with the same conservative quantisation as
+ * bc_scene_upload.  Lights and their distribution are rebuilt from the new geometry.  Traversal
+ * stays exact (the hit set of a ray does not depend on the box tightness); geom_rev is recorded. */
+int bc_scene_refit(bc_ctx* ctx, const bc_scene_view* scene, uint64_t geom_rev);
 /* Statistics of the uploaded scene: nodes8, depth8, prims, bytes resident. */
 int bc_scene_stats(const bc_ctx* ctx, int64_t* out4);
+/* Copies the resident BVH8 to host memory for inspection (no counterpart in the reference, which
+ * never reads its packed buffers back): nodes (nodes8 x 80 bytes, the bc_node8 layout of
+ * csrc/bc_layout.h), records (prims x 36 bytes, nine floats) and record ids (prims x uint32,
+ * primitive index | 0x80000000 for spheres), counts from bc_scene_stats; NULL skips an array. */
+int bc_scene_arrays(bc_ctx* ctx, void* nodes, void* records, uint32_t* ids);
 
 /* Render description: the RenderSettings fields the path reads (renderer.go:1027-1062), the
  * camera basis renderOpenCL passes as kernel args 11-14 (gpu_opencl_windows.go:580-612) in
@@ -138,7 +151,7 @@ int bc_batch_upload(bc_ctx* ctx, int n, const bc_scene_view* scenes);
 int bc_batch_render_device(bc_ctx* ctx, int n, const bc_render_desc* descs, float* const* d_rgba,
                            uint64_t* rays_out, uint64_t* shadow_out);
 
-/* Post stack on the device (SURVEY "next" row: guide + post stack): restates, in fp32, the part of
+/* Post stack on the device (compact guides + denoise/AA/tone map): restates, in fp32, the part of
  * RenderState.Start after the render (renderer.go:2842-2895) with HBAO, temporal reuse, bloom,
  * debug views and upscaling off: atrousDenoise with the compact guides (renderer.go:2228-2287,
  * guide decode gpu_opencl_windows.go:552-562, 686-716), applyFXAA (:2557-2586),
