@@ -304,10 +304,17 @@ int bc_scene_refit(bc_ctx* c, const bc_scene_view* v, uint64_t geom) {
         }
     }
     (void)hipSetDevice(c->device);
-    // moved primitive records in leaf order
+    // moved primitive records (and their materials) in leaf order
     std::vector<bc_prim> recs(c->h_ids.size());
-    for (size_t r = 0; r < recs.size(); r++) recs[r] = bcx::prim_record(*v, (int)(c->h_ids[r] & ~BC_SPHERE_BIT));
-    if (!upload_vec(c, c->d_prims, c->cap_prims, recs, "upload primitive records")) return BC_ERR_DEVICE;
+    std::vector<int32_t> pmat(c->h_ids.size());
+    for (size_t r = 0; r < recs.size(); r++) {
+        const int id = (int)(c->h_ids[r] & ~BC_SPHERE_BIT);
+        recs[r] = bcx::prim_record(*v, id);
+        pmat[r] = v->prim_mat[id];
+    }
+    if (!upload_vec(c, c->d_prims, c->cap_prims, recs, "upload primitive records") ||
+        !upload_vec(c, c->d_pmat, c->cap_pmat, pmat, "upload primitive materials"))
+        return BC_ERR_DEVICE;
     void* nb = c->d_nbox;
     if (!bc_ensure(c, nb, c->cap_nbox, (size_t)c->nnode8 * 32, "refit boxes")) return BC_ERR_NOMEM;
     c->d_nbox = (float4*)nb;

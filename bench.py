@@ -96,12 +96,22 @@ def bench_render(bc, dev, args, threads):
         od = orc.desc_from(d)
         rows = 16
         t0 = time.perf_counter()
-        o.render(od, precision=64, y0=120, rows=rows, threads=threads)
+        cpu_img = o.render(od, precision=64, y0=120, rows=rows, threads=threads)[0]
         csec = time.perf_counter() - t0
         cpu_rate = w * rows * spp / csec
         out["cpu_baseline"] = {"path_samples_per_s": round(cpu_rate, 1), "cores": threads, "kind": "port",
                                "sample": f"rows 120-{120 + rows} of the same frame, fp64 trace restatement"}
         out["gpu_over_cpu"] = round(samples / sec / cpu_rate, 1)
+        # matched image error: both 16-spp bands against an independent-seed fp64 render at 4096 spp
+        dref = bc.render_desc(box, w, h, 4096, bounces, rr_depth=3, sampler=1, mis=True, power=True, seed=0x51ED)
+        ref = o.render(orc.desc_from(dref), precision=64, y0=120, rows=rows, threads=threads)[0][..., :3]
+        gpu_img = dev.render(d, y0=120, rows=rows)[0]
+        scale = float(ref.mean())
+        e_gpu = float(np.sqrt(np.mean((gpu_img[..., :3] - ref) ** 2))) / scale
+        e_cpu = float(np.sqrt(np.mean((cpu_img[..., :3] - ref) ** 2))) / scale
+        out["matched_rmse"] = {"reference": f"fp64 oracle, 4096 spp, independent seed, rows 120-{120 + rows}",
+                               "rel_rmse_gpu": round(e_gpu, 5), "rel_rmse_cpu": round(e_cpu, 5),
+                               "cpu_over_gpu": round(e_cpu / e_gpu, 4)}
     return out
 
 
@@ -242,6 +252,31 @@ def bench_batch(bc, dev, args, threads):
     return out
 
 
+def bench_refit(bc, dev, args):
+    """Geometry-only update of the traversal soup: half of the triangles move; a full flatten +
+    upload (bc_scene_upload) against the device refit (bc_scene_refit), both timed around the C
+    call with the host view already packed."""
+    lib = bc.lib()
+    s = bc.HostScene.workload(bc.SOUP, param=args.tris)
+    view = s.view()
+    topo, geom = s.revisions()
+    t0 = time.perf_counter()
+    dev.upload_view(view, topo, geom)
+    full = time.perf_counter() - t0
+    s.translate(0, args.tris // 2, (0.25, -0.1, 0.05))
+    s.refit()
+    view = s.view()
+    geom = s.revisions()[1]
+    times = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        dev._check(lib.bc_scene_refit(dev.ctx, view, geom))
+        times.append(time.perf_counter() - t0)
+    return {"workload": f"{args.tris}-triangle soup, {args.tris // 2} triangles translated",
+            "full_upload_ms": round(full * 1e3, 2), "refit_ms": round(min(times) * 1e3, 2),
+            "full_over_refit": round(full / min(times), 2)}
+
+
 def bench_adjoint(bc, dev):
     """Differentiable-render config: closed box retessellated to ~200k triangles with 64 materials,
     512x512, 32 spp, 6 segments; time of one forward and one path-replay backward pass."""
@@ -276,7 +311,7 @@ def bench_adjoint(bc, dev):
 PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_soup1M_16Mrays.json")
 
 
-def roofline_entry(kernel, achieved, alg_bytes):
+def roofline_entry(kernel, achieved, alg_bytes, kernel_ms=None):
     """Roofline of the dominant kernel.  traffic: HBM-side bytes per launch of that kernel from the
     committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE passes, scaled by a 1 GiB copy),
     with the VALU-busy fraction of the same runs as the secondary roofline."""
@@ -289,6 +324,9 @@ def roofline_entry(kernel, achieved, alg_bytes):
         if k:
             out["traffic"] = k["traffic_bytes"]
             out["traffic_source"] = os.path.relpath(PROFILE_SUMMARY, ROOT)
+            if kernel_ms:  # the measured HBM-side bytes over this run's launch time
+                out["traffic_gbs"] = round(k["traffic_bytes"] / (kernel_ms * 1e-3) / 1e9, 1)
+                out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
             out["valu_busy"] = k.get("valu_busy")
             out["l2_hit"] = k.get("l2_hit")
     except (OSError, ValueError, KeyError):
@@ -317,7 +355,7 @@ def main():
     ap.add_argument("--terrain-cells", type=int, default=2047)
     ap.add_argument("--terrain-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-scene batch measurement")
-    ap.add_argument("--only", choices=("render", "terrain", "batch", "adjoint"),
+    ap.add_argument("--only", choices=("render", "terrain", "batch", "adjoint", "refit"),
                     help="run this render leg alone (for profiling) and print its result")
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--batch-spp", type=int, default=8, help="samples per pixel of the batch leg (Ultra: 256)")
@@ -336,7 +374,8 @@ def main():
     if args.only:
         dev = bc.Device(0)
         legs = {"render": lambda: bench_render(bc, dev, args, threads), "terrain": lambda: bench_terrain(bc, dev, args, threads),
-                "batch": lambda: bench_batch(bc, dev, args, threads), "adjoint": lambda: bench_adjoint(bc, dev)}
+                "batch": lambda: bench_batch(bc, dev, args, threads), "adjoint": lambda: bench_adjoint(bc, dev),
+                "refit": lambda: bench_refit(bc, dev, args)}
         out = legs[args.only]()
         dev.close()
         print(json.dumps({"leg": args.only, args.only: out}))
@@ -453,6 +492,7 @@ def main():
     adjoint = None if args.no_render else bench_adjoint(bc, dev)
     terrain = None if (args.no_render or args.no_terrain) else bench_terrain(bc, dev, args, threads)
     batch = None if (args.no_render or args.no_batch) else bench_batch(bc, dev, args, threads)
+    refit = None if args.no_render else bench_refit(bc, dev, args)
 
     names = {("coherent", False): "coherent_closest", ("coherent", True): "coherent_any",
              ("incoherent", False): "incoherent_closest", ("incoherent", True): "incoherent_any"}
@@ -478,12 +518,13 @@ def main():
             "bvh8_depth": int(st["depth8"]),
             "scene_bytes": int(st["bytes"]),
         },
-        "roofline": roofline_entry(f"k_trace ({names[dominant]})", achieved, alg[dominant]),
+        "roofline": roofline_entry(f"k_trace ({names[dominant]})", achieved, alg[dominant], kernel_ms[dominant]),
         "cpu_baseline": cpu,
         "render": render,
         "adjoint": adjoint,
         "terrain": terrain,
         "batch": batch,
+        "refit": refit,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
                                "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4)}
                     for k in launches},

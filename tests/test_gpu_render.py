@@ -123,6 +123,26 @@ def test_same_seed_fp64_and_converged_mean(dev):
     assert abs(img2[..., :3].mean() / ref64[..., :3].mean() - 1) < 0.01
 
 
+def test_converged_4096spp_rmse(dev):
+    """Converged images at 4096 spp against fp64 CPU renders with independent seeds: the GPU
+    image is as close to an fp64 render as a second fp64 render is (no bias beyond noise)."""
+    print("case: closed box 64x64 4096 spp vs fp64", flush=True)
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s, nee_triangles=True)
+    o = orc.Scene.from_host(s, nee_triangles=True)
+    kw = dict(rr_depth=3, sampler=1, mis=True, power=True)
+    gpu, _, _ = dev.render(bc.render_desc(s, 64, 64, 4096, 4, seed=11, **kw))
+    ref_a = o.render(orc.desc_from(bc.render_desc(s, 64, 64, 4096, 4, seed=22, **kw)), precision=64)[0]
+    ref_b = o.render(orc.desc_from(bc.render_desc(s, 64, 64, 4096, 4, seed=33, **kw)), precision=64)[0]
+    scale = ref_a[..., :3].mean()
+    e_gpu = np.sqrt(np.mean((gpu[..., :3] - ref_a[..., :3]) ** 2)) / scale
+    e_cpu = np.sqrt(np.mean((ref_b[..., :3] - ref_a[..., :3]) ** 2)) / scale
+    assert e_gpu < 0.05
+    # two independent 64x64 fp64 renders differ by a relative RMSE with a spread of about 3%
+    assert 0.85 < e_gpu / e_cpu < 1.18, f"gpu rmse {e_gpu:.5f} vs fp64 pair {e_cpu:.5f}"
+    assert abs(gpu[..., :3].mean() / scale - 1) < 0.003
+
+
 @pytest.mark.parametrize("pool,iters,segment,msort", [(4096, 4, 0, 0), (50000, 8, 1024, 0), (3000, 6, 512, 1),
                                                       (20000, 4, 256, 1), (0, 0, 0, 0)])
 def test_persistent_schedule_equals_plain(dev, pool, iters, segment, msort):
