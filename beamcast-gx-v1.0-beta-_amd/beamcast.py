@@ -110,6 +110,9 @@ def lib():
             "bch_add_triangle": (None, [vp, POINTER(c_double), POINTER(c_double), POINTER(c_double), c_int]),
             "bch_build": (None, [vp, c_int, c_int]),
             "bch_refit": (None, [vp]),
+            "bch_build_device": (c_int, [vp, vp, c_int, c_int]),
+            "bch_export_bounds": (None, [vp, vp, vp]),
+            "bc_build_bvh": (c_int, [vp, c_int32, vp, vp, c_int32, c_int32, c_int32, vp, vp, vp, vp]),
             "bch_counts": (None, [vp, POINTER(c_int32)]),
             "bch_revisions": (None, [vp, POINTER(c_uint64)]),
             "bch_view": (POINTER(SceneView), [vp, c_int]),
@@ -185,6 +188,18 @@ class HostScene:
 
     def refit(self):
         lib().bch_refit(self.h)
+
+    def build_device(self, device, bins=32, leaf=4):
+        """Scene.Build with the tree made on the GPU (bc_build_bvh); same Nodes and Order."""
+        device._check(lib().bch_build_device(self.h, device.ctx, bins, leaf))
+
+    def bounds(self):
+        """(Primitive.Box [n, 6], Primitive.Centroid [n, 3])."""
+        n = self.counts()[0]
+        box = np.zeros((n, 6))
+        cen = np.zeros((n, 3))
+        lib().bch_export_bounds(self.h, box.ctypes.data_as(POINTER(c_double)), cen.ctypes.data_as(POINTER(c_double)))
+        return box, cen
 
     def translate(self, start, count, d):
         d = np.asarray(d, dtype=np.float64)
@@ -359,6 +374,20 @@ class Device:
         self._check(lib().bc_render(self.ctx, ctypes.byref(desc), y0, rows, _ptr(rgba), _ptr(g) if g is not None else None,
                                     ctypes.byref(rays), _ptr(cflag) if cflag is not None else None))
         return rgba, g, int(rays.value)
+
+    def build_bvh(self, prim_box, prim_centroid, bins=16, leaf=4):
+        """bc_build_bvh: the Go SAH tree built on the GPU -> (node_box [m, 6], node_link [m, 4], order [n])."""
+        box = np.ascontiguousarray(prim_box, np.float64)
+        cen = np.ascontiguousarray(prim_centroid, np.float64)
+        n = box.shape[0]
+        cap = max(1, 2 * n - 1)
+        nbox = np.zeros((cap, 6))
+        link = np.zeros((cap, 4), np.int32)
+        order = np.zeros(n, np.int32)
+        nn = c_int32()
+        self._check(lib().bc_build_bvh(self.ctx, n, _ptr(box), _ptr(cen), bins, leaf, cap, _ptr(nbox), _ptr(link),
+                                       _ptr(order), ctypes.byref(nn)))
+        return nbox[: nn.value], link[: nn.value], order
 
     def update_materials(self, mat_kind, mat_data):
         mk = np.ascontiguousarray(mat_kind, np.int32)

@@ -88,6 +88,44 @@ void Scene::Build() {  // renderer.go:409-449 (SAH mode)
     if (!Order.empty()) buildNodeSAH(0, (int)Order.size());
 }
 
+int Scene::BuildDevice(bc_ctx* ctx, std::string& err) {  // Build (renderer.go:409-449), tree from the GPU
+    if (BVHBins < 4) BVHBins = 16;
+    if (BVHBins > 32) BVHBins = 32;
+    if (BVHLeafSize < 1) BVHLeafSize = 4;
+    if (BVHLeafSize > 16) BVHLeafSize = 16;
+    const size_t n = Primitives.size();
+    if (n == 0) {
+        Nodes.clear();
+        Order.clear();
+        GPUTopologyRevision++;
+        GPUGeometryRevision++;
+        return BC_OK;
+    }
+    std::vector<double> box(6 * n), cen(3 * n), nbox(12 * n);
+    std::vector<int32_t> link(8 * n), order(n);
+    for (size_t i = 0; i < n; i++) {
+        const Primitive& p = Primitives[i];
+        const double b[6] = {p.Box.Min.x, p.Box.Min.y, p.Box.Min.z, p.Box.Max.x, p.Box.Max.y, p.Box.Max.z};
+        std::copy(b, b + 6, box.begin() + 6 * (ptrdiff_t)i);
+        cen[3 * i] = p.Centroid.x; cen[3 * i + 1] = p.Centroid.y; cen[3 * i + 2] = p.Centroid.z;
+    }
+    int32_t nn = 0;
+    const int rc = bc_build_bvh(ctx, (int32_t)n, box.data(), cen.data(), BVHBins, BVHLeafSize, (int32_t)(2 * n), nbox.data(),
+                                link.data(), order.data(), &nn);
+    if (rc != BC_OK) { err = bc_last_error(ctx); return rc; }
+    GPUTopologyRevision++;
+    GPUGeometryRevision++;
+    Nodes.assign((size_t)nn, BVHNode{});
+    for (size_t i = 0; i < (size_t)nn; i++) {
+        BVHNode& d = Nodes[i];
+        const double* b = &nbox[6 * i];
+        d.Box = AABB{V(b[0], b[1], b[2]), V(b[3], b[4], b[5])};
+        d.Left = link[4 * i]; d.Right = link[4 * i + 1]; d.Start = link[4 * i + 2]; d.Count = link[4 * i + 3];
+    }
+    Order.assign(order.begin(), order.end());
+    return BC_OK;
+}
+
 // buildNodeSAH (renderer.go:526-633): one axis (largest centroid extent), binned sweep,
 // leaf when count <= leaf, degenerate extent, no valid split, or the 0.98 rule; in-place
 // two-pointer partition; children appended depth first (left subtree first).

@@ -63,6 +63,9 @@ public:
     void AddSphere(Vec3 c, double r, int mat);
     void AddTriangle(Vec3 a, Vec3 b, Vec3 c, int mat);
     void Build();
+    // Build with the SAH tree made on the GPU (bc_build_bvh): the same Nodes and Order as
+    // Build(); returns the status of the call (the scene is unchanged on an error).
+    int BuildDevice(bc_ctx* ctx, std::string& err);
     void Refit();
     // Flat views for the C ABI; valid until the scene is mutated.
     const bc_scene_view& View(bool nee_triangles);

@@ -141,6 +141,21 @@ void bch_build(bch_scene* s, int bins, int leaf) {
     s->s.BVHLeafSize = leaf;
     s->s.Build();
 }
+int bch_build_device(bch_scene* s, bc_ctx* ctx, int bins, int leaf) {
+    s->s.BVHBins = bins;
+    s->s.BVHLeafSize = leaf;
+    std::string err;
+    return s->s.BuildDevice(ctx, err);
+}
+void bch_export_bounds(const bch_scene* s, double* box6, double* centroid3) {
+    for (size_t i = 0; i < s->s.Primitives.size(); i++) {
+        const bcx::Primitive& p = s->s.Primitives[i];
+        const double b[6] = {p.Box.Min.x, p.Box.Min.y, p.Box.Min.z, p.Box.Max.x, p.Box.Max.y, p.Box.Max.z};
+        std::memcpy(box6 + 6 * i, b, sizeof b);
+        const double c[3] = {p.Centroid.x, p.Centroid.y, p.Centroid.z};
+        std::memcpy(centroid3 + 3 * i, c, sizeof c);
+    }
+}
 void bch_refit(bch_scene* s) { s->s.Refit(); }
 void bch_counts(const bch_scene* s, int32_t* o) {
     o[0] = (int32_t)s->s.Primitives.size();

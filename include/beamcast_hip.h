@@ -66,6 +66,18 @@ int bc_scene_upload(bc_ctx* ctx, const bc_scene_view* scene, uint64_t topo_rev, 
  * BVH: new Color/Roughness/IOR/Emission for the uploaded scene (same material count and kinds
  * of the lights).  The differentiable render updates its parameters through this call. */
 int bc_update_materials(bc_ctx* ctx, int32_t nmat, const int32_t* mat_kind, const double* mat_data);
+/* Device SAH build of the binary BVH (opt-in replacement of the host build): the Nodes and
+ * Order of Scene.Build in SAH mode (renderer.go:409-449, buildNodeSAH :526-633), node for node
+ * and index for index equal to the host build (same single-axis binned sweep, 0.98 leaf rule,
+ * two-pointer partition order and depth-first node numbering).  Inputs: nprim primitive boxes
+ * (Primitive.Box: min xyz, max xyz) and centroids (Primitive.Centroid), BVHBins and
+ * BVHLeafSize (clamped as Build does: bins to [4, 32] with < 4 meaning 16, leaf to [1, 16]).
+ * Outputs: node_box (node_cap x 6: Box min, max), node_link (node_cap x 4: Left, Right, Start,
+ * Count), order (nprim), *nnode; node_cap = 2 * nprim - 1 always suffices.  Non-finite input is
+ * rejected (BC_ERR_ARG).  The median mode and the light distribution stay with the caller. */
+int bc_build_bvh(bc_ctx* ctx, int32_t nprim, const double* prim_box, const double* prim_centroid, int32_t bins,
+                 int32_t leaf_size, int32_t node_cap, double* node_box, int32_t* node_link, int32_t* order,
+                 int32_t* nnode);
 /* Geometry-only update of the resident single scene: primitives moved, topology unchanged (same
  * primitive count, kinds and Go BVH leaf membership).  Replaces updateGeometryLocked
  * (gpu_opencl_windows.go:447-468), which re-packs and re-uploads every node after Scene.Refit

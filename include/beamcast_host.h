@@ -28,6 +28,11 @@ void bch_add_sphere(bch_scene* s, const double* center, double radius, int mat);
 void bch_add_triangle(bch_scene* s, const double* a, const double* b, const double* c, int mat);
 /* Scene.Build with BVHMode SAH (renderer.go:409-449); bins/leaf clamped as there */
 void bch_build(bch_scene* s, int bins, int leaf);
+/* Scene.Build with the tree made on the GPU (bc_build_bvh): the same Nodes and Order as
+ * bch_build; returns the bc_build_bvh status (message via bc_last_error(ctx)) */
+int bch_build_device(bch_scene* s, bc_ctx* ctx, int bins, int leaf);
+/* Primitive.Box (min xyz, max xyz) and Primitive.Centroid of every primitive: nprim x 6, x 3 */
+void bch_export_bounds(const bch_scene* s, double* box6, double* centroid3);
 /* Scene.Refit (renderer.go:451-473) */
 void bch_refit(bch_scene* s);
 /* counts: nprim, nnode, nmat; revisions: topology, geometry */
