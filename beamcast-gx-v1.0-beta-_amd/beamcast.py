@@ -111,6 +111,16 @@ def lib():
             "bch_build": (None, [vp, c_int, c_int]),
             "bch_refit": (None, [vp]),
             "bch_build_device": (c_int, [vp, vp, c_int, c_int]),
+            "bch_load_obj": (vp, [ctypes.c_char_p, vp, vp, c_int, c_int]),
+            "bch_write_obj": (c_int, [vp, ctypes.c_char_p]),
+            "bch_ingest_obj": (vp, [vp, ctypes.c_char_p, c_int, c_int, c_int, vp, vp, c_int]),
+            "bch_pages_write": (c_int, [ctypes.c_char_p, vp, c_int64, ctypes.c_uint32]),
+            "bch_pages_open": (vp, [ctypes.c_char_p, vp, c_int]),
+            "bch_pages_close": (None, [vp]),
+            "bch_pages_count": (c_int64, [vp]),
+            "bch_pages_info": (c_int, [vp, c_int64, vp, vp]),
+            "bch_pages_load": (c_int, [vp, c_int64, vp]),
+            "bch_pages_append": (c_int, [vp, vp, c_int64, c_int64]),
             "bch_export_bounds": (None, [vp, vp, vp]),
             "bc_build_bvh": (c_int, [vp, c_int32, vp, vp, c_int32, c_int32, c_int32, vp, vp, vp, vp]),
             "bch_counts": (None, [vp, POINTER(c_int32)]),
@@ -157,6 +167,39 @@ class HostScene:
         if not self.h:
             raise ValueError("invalid scene")
         self.camera = camera
+
+    @classmethod
+    def load_obj(cls, path, threads=0):
+        """LoadOBJ (parallel parse): (scene, camera[10]); not built.  Raises ValueError with
+        LoadOBJ's message."""
+        cam = np.zeros(10)
+        err = ctypes.create_string_buffer(512)
+        h = lib().bch_load_obj(os.fsencode(path), cam.ctypes.data_as(POINTER(c_double)), err, 512, threads)
+        if not h:
+            raise ValueError(err.value.decode())
+        return cls(h, cam), cam
+
+    @classmethod
+    def ingest_obj(cls, device, path, bins=32, leaf=4, threads=0):
+        """OBJ straight to a resident scene on `device` (parallel parse, device SAH build,
+        upload): (scene, camera[10])."""
+        cam = np.zeros(10)
+        err = ctypes.create_string_buffer(512)
+        h = lib().bch_ingest_obj(device.ctx, os.fsencode(path), bins, leaf, threads,
+                                 cam.ctypes.data_as(POINTER(c_double)), err, 512)
+        if not h:
+            raise ValueError(err.value.decode())
+        return cls(h, cam), cam
+
+    def write_obj(self, path):
+        """Writes the triangles as OBJ text (three vertices and one face per triangle, %.17g)."""
+        if lib().bch_write_obj(self.h, os.fsencode(path)) != 0:
+            raise OSError(f"cannot write {path}")
+
+    def append_pages(self, pages, first=0, count=None):
+        count = pages.count() - first if count is None else count
+        if lib().bch_pages_append(self.h, pages.h, first, count) != 0:
+            raise ValueError("cannot append pages")
 
     @classmethod
     def workload(cls, kind, seed=0x5EED, param=0, bins=32, leaf=4):
@@ -238,6 +281,51 @@ class HostScene:
         md = np.zeros((nm, 8))
         lib().bch_export_materials(self.h, _ptr(mk), _ptr(md))
         return kind, mat, geom, mk, md
+
+
+PACKED_TRIANGLE = np.dtype([("v0", "<f4", 4), ("e1", "<f4", 4), ("e2", "<f4", 4), ("normal", "<f4", 4),
+                            ("material", "<u4"), ("pad", "<u4", 3)])
+
+
+def pages_write(path, tris, per_page=65536):
+    """PagedTriangleStore::write of PACKED_TRIANGLE records."""
+    tris = np.ascontiguousarray(tris, PACKED_TRIANGLE)
+    rc = lib().bch_pages_write(os.fsencode(path), _ptr(tris), len(tris), per_page)
+    if rc != 0:
+        raise OSError(f"page store write failed ({rc})")
+
+
+class Pages:
+    """PagedTriangleStore (BGXPAGE1) opened for reading."""
+
+    def __init__(self, path):
+        err = ctypes.create_string_buffer(256)
+        self.h = lib().bch_pages_open(os.fsencode(path), err, 256)
+        if not self.h:
+            raise ValueError(err.value.decode())
+
+    def close(self):
+        if self.h:
+            lib().bch_pages_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def count(self):
+        return int(lib().bch_pages_count(self.h))
+
+    def info(self, page):
+        off, cnt = ctypes.c_uint64(), ctypes.c_uint32()
+        if lib().bch_pages_info(self.h, page, ctypes.byref(off), ctypes.byref(cnt)) != 0:
+            raise IndexError(page)
+        return int(off.value), int(cnt.value)
+
+    def load(self, page):
+        out = np.zeros(self.info(page)[1], PACKED_TRIANGLE)
+        if lib().bch_pages_load(self.h, page, _ptr(out)) != 0:
+            raise ValueError("truncated Beamcast GX page payload")
+        return out
 
 
 def workload_rays(set_id, width, any_hit=False, first=0, count=None, seed=0x5EED):

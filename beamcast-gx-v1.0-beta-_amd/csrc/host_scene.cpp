@@ -21,8 +21,8 @@ AABB emptyAABB() {
     return AABB{V(inf, inf, inf), V(-inf, -inf, -inf)};
 }
 AABB expandPoint(AABB b, Vec3 p) {  // renderer.go:117-125
-    b.Min = V(std::min(b.Min.x, p.x), std::min(b.Min.y, p.y), std::min(b.Min.z, p.z));
-    b.Max = V(std::max(b.Max.x, p.x), std::max(b.Max.y, p.y), std::max(b.Max.z, p.z));
+    b.Min = V(GoMin(b.Min.x, p.x), GoMin(b.Min.y, p.y), GoMin(b.Min.z, p.z));
+    b.Max = V(GoMax(b.Max.x, p.x), GoMax(b.Max.y, p.y), GoMax(b.Max.z, p.z));
     return b;
 }
 AABB expandBox(AABB b, const AABB& c) { return expandPoint(expandPoint(b, c.Min), c.Max); }

@@ -6,6 +6,7 @@
 #include <atomic>
 #include <cstdint>
 #include <functional>
+#include <cmath>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,17 @@ inline Vec3 operator*(Vec3 a, double s) { return V(a.x * s, a.y * s, a.z * s); }
 inline double Dot(Vec3 a, Vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline Vec3 Cross(Vec3 a, Vec3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 Vec3 Unit(Vec3 a);
+// math.Min / math.Max: NaN wins, -0 is below +0
+inline double GoMin(double a, double b) {
+    if (a != a || b != b) return a + b;
+    if (a == 0 && b == 0) return std::signbit(a) ? a : b;
+    return a < b ? a : b;
+}
+inline double GoMax(double a, double b) {
+    if (a != a || b != b) return a + b;
+    if (a == 0 && b == 0) return std::signbit(a) ? b : a;
+    return a > b ? a : b;
+}
 
 enum MaterialKind { MatDiffuse = 0, MatMetal = 1, MatGlass = 2, MatEmissive = 3 };  // renderer.go:86-91
 enum PrimKind { PrimSphere = 0, PrimTriangle = 1 };                               // renderer.go:101-104
@@ -112,3 +124,8 @@ int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, s
               int64_t& rays, std::string& deviceName, std::string& err);
 
 }  // namespace bcx
+
+// The opaque scene handle of the C entry points (beamcast_host.h).
+struct bch_scene {
+    bcx::Scene s;
+};

@@ -8,10 +8,6 @@
 #include "../../include/beamcast_host.h"
 #include "host_scene.hpp"
 
-struct bch_scene {
-    bcx::Scene s;
-};
-
 namespace {
 
 using bcx::V;

@@ -252,6 +252,69 @@ def bench_batch(bc, dev, args, threads):
     return out
 
 
+def bench_build(bc, dev, args):
+    """Scene.Build (SAH, 32 bins, leaf 4) on the host against the same tree built on the device
+    (bc_build_bvh): the 1M-triangle soup and the 8.4M-triangle terrain.  The device time is the
+    whole call: upload of the primitive bounds, the build, download of nodes and order."""
+    out = {}
+    for name, kind, param in (("soup", bc.SOUP, args.tris), ("terrain", bc.TERRAIN, args.terrain_cells)):
+        s = bc.HostScene.workload(kind, param=param, bins=32, leaf=4)
+        t0 = time.perf_counter()
+        s.build(32, 4)
+        host = time.perf_counter() - t0
+        box, cen = s.bounds()
+        dev.build_bvh(box, cen, 32, 4)  # first call: allocations
+        t0 = time.perf_counter()
+        nbox, _, _ = dev.build_bvh(box, cen, 32, 4)
+        gpu = time.perf_counter() - t0
+        n = s.counts()
+        if len(nbox) != n[1]:
+            raise RuntimeError(f"device build of {name}: {len(nbox)} nodes, host {n[1]}")
+        out[name] = {"primitives": n[0], "nodes": n[1], "host_build_s": round(host, 3), "device_build_s": round(gpu, 4),
+                     "host_over_device": round(host / gpu, 1)}
+        log(f"[bench] build {name}: host {host:.2f}s, device {gpu * 1e3:.1f} ms ({n[0]} primitives)")
+    return out
+
+
+def bench_ingest(bc, dev, args):
+    """OBJ file -> resident BVH8 on the GPU for the 1M-triangle soup (204 MB of OBJ text): the
+    reference's order of work (serial LoadOBJ parse, host Scene.Build, flatten + upload) against
+    bch_ingest_obj (parallel parse, device SAH build, flatten + upload)."""
+    import tempfile
+
+    src = bc.HostScene.workload(bc.SOUP, param=args.tris, bins=32, leaf=4)
+    threads = min(16, host_cores())
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "soup.obj")
+        src.write_obj(path)
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        s, _ = bc.HostScene.load_obj(path, threads=1)
+        t1 = time.perf_counter()
+        s.build(32, 4)
+        t2 = time.perf_counter()
+        dev.upload(s)
+        t3 = time.perf_counter()
+        del s
+        t4 = time.perf_counter()
+        s2, _ = bc.HostScene.load_obj(path, threads=threads)
+        t5 = time.perf_counter()
+        s2.build_device(dev, 32, 4)
+        t6 = time.perf_counter()
+        dev.upload(s2)
+        t7 = time.perf_counter()
+    serial = t3 - t0
+    fast = t7 - t4
+    log(f"[bench] ingest: serial parse {t1 - t0:.2f}s + host build {t2 - t1:.2f}s + upload {t3 - t2:.2f}s; "
+        f"parallel parse {t5 - t4:.2f}s + device build {t6 - t5:.2f}s + upload {t7 - t6:.2f}s")
+    return {"workload": f"{args.tris}-triangle soup as OBJ text ({size / 1e6:.0f} MB) -> resident BVH8",
+            "serial_host": {"parse_s": round(t1 - t0, 3), "build_s": round(t2 - t1, 3), "upload_s": round(t3 - t2, 3),
+                            "total_s": round(serial, 3)},
+            "ingest": {"threads": threads, "parse_s": round(t5 - t4, 3), "build_s": round(t6 - t5, 3),
+                       "upload_s": round(t7 - t6, 3), "total_s": round(fast, 3)},
+            "speedup": round(serial / fast, 2)}
+
+
 def bench_refit(bc, dev, args):
     """Geometry-only update of the traversal soup: half of the triangles move; a full flatten +
     upload (bc_scene_upload) against the device refit (bc_scene_refit), both timed around the C
@@ -355,7 +418,7 @@ def main():
     ap.add_argument("--terrain-cells", type=int, default=2047)
     ap.add_argument("--terrain-spp", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true", help="skip the multi-scene batch measurement")
-    ap.add_argument("--only", choices=("render", "terrain", "batch", "adjoint", "refit"),
+    ap.add_argument("--only", choices=("render", "terrain", "batch", "adjoint", "refit", "build", "ingest"),
                     help="run this render leg alone (for profiling) and print its result")
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--batch-spp", type=int, default=8, help="samples per pixel of the batch leg (Ultra: 256)")
@@ -375,7 +438,8 @@ def main():
         dev = bc.Device(0)
         legs = {"render": lambda: bench_render(bc, dev, args, threads), "terrain": lambda: bench_terrain(bc, dev, args, threads),
                 "batch": lambda: bench_batch(bc, dev, args, threads), "adjoint": lambda: bench_adjoint(bc, dev),
-                "refit": lambda: bench_refit(bc, dev, args)}
+                "refit": lambda: bench_refit(bc, dev, args), "build": lambda: bench_build(bc, dev, args),
+                "ingest": lambda: bench_ingest(bc, dev, args)}
         out = legs[args.only]()
         dev.close()
         print(json.dumps({"leg": args.only, args.only: out}))
@@ -493,6 +557,8 @@ def main():
     terrain = None if (args.no_render or args.no_terrain) else bench_terrain(bc, dev, args, threads)
     batch = None if (args.no_render or args.no_batch) else bench_batch(bc, dev, args, threads)
     refit = None if args.no_render else bench_refit(bc, dev, args)
+    build = None if (args.no_render or args.no_terrain) else bench_build(bc, dev, args)
+    ingest = None if args.no_render else bench_ingest(bc, dev, args)
 
     names = {("coherent", False): "coherent_closest", ("coherent", True): "coherent_any",
              ("incoherent", False): "incoherent_closest", ("incoherent", True): "incoherent_any"}
@@ -525,6 +591,8 @@ def main():
         "terrain": terrain,
         "batch": batch,
         "refit": refit,
+        "build": build,
+        "ingest": ingest,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
                                "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4)}
                     for k in launches},

@@ -46,6 +46,42 @@ void bch_export_materials(const bch_scene* s, int32_t* kind, double* data8);
 /* translate primitives [start, start+count) by d (translatePrimitive, renderer.go:282-303) */
 void bch_translate(bch_scene* s, int start, int count, const double* d);
 
+/* LoadOBJ (renderer.go:2996-3069), parsed in parallel on `threads` host threads (0 = all):
+ * the same primitives in the same order, one diffuse material (.72, .76, .82), and camera_out
+ * (may be NULL) = pos.xyz, target.xyz, up.xyz, fov as LoadOBJ frames it.  The scene is not built
+ * (bch_build or bch_build_device).  NULL on error, with LoadOBJ's message in err. */
+bch_scene* bch_load_obj(const char* path, double* camera_out, char* err, int err_cap, int threads);
+
+/* OBJ straight to a resident device scene: bch_load_obj (parallel parse), the SAH tree built on
+ * the device (bch_build_device with bins/leaf), then bc_scene_upload on ctx.  Returns the scene
+ * (the Go Scene mirror, for the CPU paths) or NULL with the message in err. */
+bch_scene* bch_ingest_obj(bc_ctx* ctx, const char* path, int bins, int leaf, int threads, double* camera_out,
+                          char* err, int err_cap);
+/* Writes the triangles of s as OBJ text (three "v" lines with %.17g coordinates and one "f" line
+ * per triangle, every other face with relative indices): an ingestion input that loads back to
+ * the same primitives.  0 ok, -1 bad arguments, -2 I/O error. */
+int bch_write_obj(const bch_scene* s, const char* path);
+
+/* Beamcast GX triangle page store (BGXPAGE1, PagedTriangleStore, gx_stream.cpp:9-55): records
+ * laid out as hx::PackedTriangle (four 16-byte float vectors and the material, 80 bytes). */
+typedef struct bch_packed_triangle {
+    float v0[4], e1[4], e2[4], normal[4];
+    uint32_t material, pad[3];
+} bch_packed_triangle;
+typedef struct bch_pages bch_pages;
+/* PagedTriangleStore::write: 0 ok, -1 bad arguments, -2 I/O error */
+int bch_pages_write(const char* path, const bch_packed_triangle* tris, int64_t n, uint32_t triangles_per_page);
+/* PagedTriangleStore constructor: NULL with the message in err on a missing or invalid file */
+bch_pages* bch_pages_open(const char* path, char* err, int err_cap);
+void bch_pages_close(bch_pages* p);
+int64_t bch_pages_count(const bch_pages* p);
+int bch_pages_info(const bch_pages* p, int64_t page, uint64_t* file_offset, uint32_t* triangle_count);
+/* load_page into out (triangle_count records): 0 ok, -1 bad page, -2/-3 I/O or truncated */
+int bch_pages_load(const bch_pages* p, int64_t page, bch_packed_triangle* out);
+/* append_page_to_scene for pages [first, first + npages): AddTriangle(v0, v0 + e1, v0 + e2,
+ * material) with the sums in float as there */
+int bch_pages_append(bch_scene* s, const bch_pages* p, int64_t first, int64_t npages);
+
 /* Camera.basis (renderer.go:947-960): out12 = origin, lower, horiz, vert */
 void bch_camera_basis(const double* pos, const double* target, const double* up, double fov,
                       double aspect, double* out12);
