@@ -93,6 +93,7 @@ def lib():
             "bc_update_materials": (c_int, [vp, c_int32, vp, vp]),
             "bc_scene_refit": (c_int, [vp, vp, c_uint64]),
             "bc_scene_arrays": (c_int, [vp, vp, vp, vp]),
+            "bc_set_flatten": (c_int, [vp, c_int]),
             "bc_trace": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_trace_device": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_alloc": (vp, [vp, c_uint64]),
@@ -383,6 +384,10 @@ class Device:
         topology.  Call scene.refit() (Scene.Refit) first so the host BVH matches."""
         geom = geom_revision if geom_revision is not None else scene.revisions()[1]
         self._check(lib().bc_scene_refit(self.ctx, scene.view(nee_triangles), geom))
+
+    def set_flatten(self, mode):
+        """BVH8 flattening of later uploads: 0 automatic, 1 host, 2 device (same bytes)."""
+        self._check(lib().bc_set_flatten(self.ctx, mode))
 
     def upload_view(self, view, topo, geom):
         self._check(lib().bc_scene_upload(self.ctx, view, topo, geom))

@@ -15,7 +15,11 @@
 #include "../../include/beamcast_hip.h"
 #include "api_internal.hpp"
 #include "bvh8_build.hpp"
+#include "bvh8_flatten.hpp"
 #include "kernels.hpp"
+
+// Scenes of at least this many primitives are flattened on the device by default.
+constexpr int kDeviceFlattenMin = 1 << 16;
 
 namespace {
 thread_local std::string g_create_error;
@@ -174,13 +178,23 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     if (!c || !v) return BC_ERR_ARG;
     if (!check_view(c, *v)) return BC_ERR_SCENE;
     hipSetDevice(c->device);
-    bcx::Bvh8 b;
     std::string err;
-    if (!bcx::build_bvh8(*v, b, err)) { c->err = err; return BC_ERR_SCENE; }
-    const int stack = bcd::stack_for_depth(b.depth);
-    if (stack < 0) { c->err = "BVH8 depth " + std::to_string(b.depth) + " exceeds the traversal stack (33 levels)"; return BC_ERR_SCENE; }
+    if (!bcx::validate_view(*v, err)) { c->err = err; return BC_ERR_SCENE; }
+    const bool on_device = c->flatten == 2 || (c->flatten == 0 && v->nprim >= kDeviceFlattenMin);
+    bcx::Bvh8 b;
+    if (!on_device && !bcx::build_bvh8(*v, b, err)) { c->err = err; return BC_ERR_SCENE; }
     c->has_scene = false;
     c->batch.clear();
+    bcd::FlattenResult fr;
+    if (on_device) {
+        const int rc = bcd::flatten_device(c, *v, fr);
+        if (rc != BC_OK) return rc;
+    } else {
+        fr.nnode8 = (int64_t)b.nodes.size();
+        fr.depth = b.depth;
+    }
+    const int stack = bcd::stack_for_depth(fr.depth);
+    if (stack < 0) { c->err = "BVH8 depth " + std::to_string(fr.depth) + " exceeds the traversal stack (33 levels)"; return BC_ERR_SCENE; }
     c->h_lights.clear();
     c->h_light_area.clear();
     collect_lights(*v, 0, 0, c->h_lights, c->h_light_area);
@@ -191,16 +205,18 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
         const int rc = bc_update_materials(c, v->nmat, v->mat_kind, v->mat_data);
         if (rc != BC_OK) return rc;
     }
-    if (!upload_vec(c, c->d_nodes, c->cap_nodes, b.nodes, "upload nodes") ||
-        !upload_vec(c, c->d_prims, c->cap_prims, b.prims, "upload prims") ||
-        !upload_vec(c, c->d_ids, c->cap_ids, b.ids, "upload prim ids") ||
-        !upload_vec(c, c->d_pmat, c->cap_pmat, b.mats, "upload prim materials") ||
-        false)
-        return BC_ERR_DEVICE;
-    // node lists by level for device refits (deepest level first)
-    {
+    if (on_device) {
+        c->level_off = std::move(fr.level_off);
+        c->h_ids = std::move(fr.ids);
+    } else {
+        if (!upload_vec(c, c->d_nodes, c->cap_nodes, b.nodes, "upload nodes") ||
+            !upload_vec(c, c->d_prims, c->cap_prims, b.prims, "upload prims") ||
+            !upload_vec(c, c->d_ids, c->cap_ids, b.ids, "upload prim ids") ||
+            !upload_vec(c, c->d_pmat, c->cap_pmat, b.mats, "upload prim materials"))
+            return BC_ERR_DEVICE;
+        // node lists by level for device refits (deepest level first)
         std::vector<std::vector<uint32_t>> lv((size_t)b.depth + 1);
-        for (size_t i = 0; i < b.node_depth.size(); i++) lv[b.node_depth[i]].push_back((uint32_t)i);
+        for (size_t i = 0; i < b.node_depth.size(); i++) lv[(size_t)b.node_depth[i]].push_back((uint32_t)i);
         std::vector<uint32_t> flat;
         c->level_off.assign(1, 0u);
         for (int d = b.depth; d >= 1; d--) {
@@ -208,16 +224,22 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
             c->level_off.push_back((uint32_t)flat.size());
         }
         if (!upload_vec(c, c->d_levels, c->cap_levels, flat, "upload node levels")) return BC_ERR_DEVICE;
-        c->h_ids = b.ids;
+        c->h_ids = std::move(b.ids);
     }
     if (!ok(c, hipStreamSynchronize(c->stream), "scene upload")) return BC_ERR_DEVICE;
-    c->nnode8 = (int64_t)b.nodes.size();
-    c->depth8 = b.depth;
+    c->nnode8 = fr.nnode8;
+    c->depth8 = fr.depth;
     c->stack = stack;
     c->topo = topo;
     c->geom = geom;
-    c->scene_bytes = (int64_t)(b.nodes.size() * sizeof(bc_node8) + b.prims.size() * (sizeof(bc_prim) + 8));
+    c->scene_bytes = (int64_t)((size_t)fr.nnode8 * sizeof(bc_node8) + (size_t)v->nprim * (sizeof(bc_prim) + 8));
     c->has_scene = true;
+    return BC_OK;
+}
+
+int bc_set_flatten(bc_ctx* c, int mode) {
+    if (!c || mode < 0 || mode > 2) return BC_ERR_ARG;
+    c->flatten = mode;
     return BC_OK;
 }
 

@@ -51,6 +51,7 @@ struct bc_ctx {
     uint32_t* d_levels = nullptr;
     float4* d_nbox = nullptr;
     size_t cap_levels = 0, cap_nbox = 0;
+    int flatten = 0;       // BVH8 flattening: 0 auto (device for large scenes), 1 host, 2 device
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;

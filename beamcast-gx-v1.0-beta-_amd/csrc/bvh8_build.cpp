@@ -9,7 +9,8 @@
 // one wide node -- a leaf of at most kLeafMax primitives, a wide internal node, or the budget
 // split between the two binary children -- with cost = area x (kNodeCost per node visited,
 // kPrimCost per primitive tested).  Leaves hold at most kLeafMax primitives so that eight of
-// them fit the 32-primitive run of a node; larger binary leaves become several leaf slots.
+// them fit the 32-primitive run of a node; larger binary leaves become several leaf slots, and
+// a leaf that needs more than four slots may instead become a wide node of its own.
 // Every primitive appears in exactly one leaf; which leaf only changes culling, never a hit
 // result (equal distances resolve to the smaller primitive index).
 #include "bvh8_build.hpp"
@@ -78,28 +79,12 @@ Box prim_box(const bc_scene_view& v, int p) {
     return b;
 }
 
-Child make_child(const bc_scene_view& v, int k) {
-    Child c;
-    const int32_t* l = v.node_link + 4 * (size_t)k;
-    c.box = node_box(v, k);
-    c.node2 = k;
-    if (l[3] > 0) { c.leaf = true; c.start = l[2]; c.count = l[3]; }
-    else { c.leaf = false; c.start = 0; c.count = 0; }
-    return c;
-}
-
 void write_prim(const bc_scene_view& v, int id, Bvh8& out) {
     out.prims.push_back(prim_record(v, id));
     out.ids.push_back((uint32_t)id | (v.prim_kind[id] == 0 ? BC_SPHERE_BIT : 0u));
     out.mats.push_back(v.prim_mat[id]);
 }
 
-#ifndef BC_LEAF_MAX
-#define BC_LEAF_MAX 2
-#endif
-#ifndef BC_PRIM_COST
-#define BC_PRIM_COST 0.3
-#endif
 constexpr int kLeafMax = BC_LEAF_MAX;       // primitives per leaf slot (8 slots x kLeafMax <= 32)
 constexpr double kNodeCost = 1.0;           // relative cost of visiting one wide node
 constexpr double kPrimCost = BC_PRIM_COST;  // relative cost of one primitive test
@@ -135,8 +120,11 @@ void expand(const bc_scene_view& v, const std::vector<Plan>& plan, int n, int sl
                 out.push_back(c);
             }
         }
-    } else if (h == 2) {
-        out.push_back(make_child(v, n));
+    } else if (h == 2) {  // a wide node of its own (an internal binary node, or a large leaf)
+        Child c;
+        c.leaf = false; c.node2 = n; c.start = 0; c.count = 0;
+        c.box = node_box(v, n);
+        out.push_back(c);
     } else {
         const int k = h - 2;
         expand(v, plan, l[0], k, out);
@@ -157,9 +145,11 @@ void make_plans(const bc_scene_view& v, std::vector<Plan>& plan) {
             P.count = l[3];
             const int need = (l[3] + kLeafMax - 1) / kLeafMax;
             const float c = (float)(A * kPrimCost * l[3]);
+            // a leaf needing more than half a node may also be a wide node of its own (one slot)
+            const float c_wide = need > 4 ? c + (float)(A * kNodeCost) : inf;
             for (int i = 1; i <= 8; i++) {
-                P.cost[i - 1] = i >= need ? c : inf;
-                P.how[i - 1] = 1;
+                P.cost[i - 1] = i >= need ? c : c_wide;
+                P.how[i - 1] = i >= need ? 1 : 2;
             }
             continue;
         }
@@ -210,13 +200,7 @@ bc_prim prim_record(const bc_scene_view& v, int id) {
     return r;
 }
 
-bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
-    out.nodes.clear();
-    out.prims.clear();
-    out.ids.clear();
-    out.mats.clear();
-    out.node_depth.clear();
-    out.depth = 0;
+bool validate_view(const bc_scene_view& v, std::string& err) {
     if (v.nnode <= 0 || v.nprim <= 0) { err = "scene has no BVH nodes or primitives"; return false; }
     // validateGPUSceneEncoding analogue (gpu_opencl_windows.go:470-482)
     if (v.nnode >= (1 << 27) || v.nprim >= (1 << 27)) { err = "BVH supports fewer than 2^27 nodes/primitives"; return false; }
@@ -228,6 +212,17 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
         }
         if (l[3] > 0 && (l[2] < 0 || (int64_t)l[2] + l[3] > v.nprim)) { err = "BVH leaf range out of bounds"; return false; }
     }
+    return true;
+}
+
+bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
+    out.nodes.clear();
+    out.prims.clear();
+    out.ids.clear();
+    out.mats.clear();
+    out.node_depth.clear();
+    out.depth = 0;
+    if (!validate_view(v, err)) return false;
     out.prims.reserve((size_t)v.nprim);
     out.ids.reserve((size_t)v.nprim);
     out.mats.reserve((size_t)v.nprim);
@@ -303,7 +298,7 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
             pad[a] = std::max((mag + ext) * 0x1p-20, 1e-20);
             float o = f_down(nb.mn[a] - pad[a]);
             double ext2 = (nb.mx[a] + pad[a]) - (double)o;
-            int e = (int)std::ceil(std::log2(ext2 / 255.0));
+            int e = ceil_log2(ext2 / 255.0);
             while (std::ldexp(255.0, e) < ext2) e++;
             if (e < -126) e = -126;
             if (e > 127) { err = "scene extent too large for quantised BVH8"; return false; }

@@ -95,7 +95,9 @@ __global__ __launch_bounds__(256) void k_refit_level(bc_node8* __restrict__ node
         pad[a] = fmax((mag + ext) * 0x1p-20, 1e-20);
         const float o = f_down(nb.mn[a] - pad[a]);
         const double ext2 = (nb.mx[a] + pad[a]) - (double)o;
-        int e = (int)ceil(log2(ext2 / 255.0));
+        int k2;
+        const double m2 = frexp(ext2 / 255.0, &k2);  // smallest e with 2^e >= ext2 / 255, exactly
+        int e = m2 == 0.5 ? k2 - 1 : k2;
         while (ldexp(255.0, e) < ext2) e++;
         if (e < -126) e = -126;
         if (e > 127) { atomicOr(err, 4); e = 127; }

@@ -86,6 +86,10 @@ int bc_build_bvh(bc_ctx* ctx, int32_t nprim, const double* prim_box, const doubl
  * bc_scene_upload.  Lights and their distribution are rebuilt from the new geometry.  Traversal
  * stays exact (the hit set of a ray does not depend on the box tightness); geom_rev is recorded. */
 int bc_scene_refit(bc_ctx* ctx, const bc_scene_view* scene, uint64_t geom_rev);
+/* Where bc_scene_upload flattens the binary tree into BVH8 nodes (no counterpart in the
+ * reference): 0 automatic (the device for scenes of 65,536 primitives or more), 1 the host, 2 the
+ * device.  Both produce the same bytes; the device path uploads the view's arrays instead. */
+int bc_set_flatten(bc_ctx* ctx, int mode);
 /* Statistics of the uploaded scene: nodes8, depth8, prims, bytes resident. */
 int bc_scene_stats(const bc_ctx* ctx, int64_t* out4);
 /* Copies the resident BVH8 to host memory for inspection (no counterpart in the reference, which
