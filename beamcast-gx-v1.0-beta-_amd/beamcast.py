@@ -94,6 +94,7 @@ def lib():
             "bc_scene_refit": (c_int, [vp, vp, c_uint64]),
             "bc_scene_arrays": (c_int, [vp, vp, vp, vp]),
             "bc_set_flatten": (c_int, [vp, c_int]),
+            "bc_set_trace_refill": (c_int, [vp, c_int, c_int]),
             "bc_trace": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_trace_device": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_alloc": (vp, [vp, c_uint64]),
@@ -384,6 +385,10 @@ class Device:
         topology.  Call scene.refit() (Scene.Refit) first so the host BVH matches."""
         geom = geom_revision if geom_revision is not None else scene.revisions()[1]
         self._check(lib().bc_scene_refit(self.ctx, scene.view(nee_triangles), geom))
+
+    def set_trace_refill(self, closest, any_hit):
+        """Ray-query schedule: refill a wave's finished lanes once that many are idle (0: off)."""
+        self._check(lib().bc_set_trace_refill(self.ctx, closest, any_hit))
 
     def set_flatten(self, mode):
         """BVH8 flattening of later uploads: 0 automatic, 1 host, 2 device (same bytes)."""

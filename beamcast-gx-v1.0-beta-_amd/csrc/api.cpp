@@ -160,7 +160,7 @@ void bc_destroy(bc_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox};
+                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox, c->d_stage};
     for (void* q : ptrs)
         if (q) hipFree(q);
     bc_render_release(c);
@@ -234,6 +234,13 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     c->geom = geom;
     c->scene_bytes = (int64_t)((size_t)fr.nnode8 * sizeof(bc_node8) + (size_t)v->nprim * (sizeof(bc_prim) + 8));
     c->has_scene = true;
+    return BC_OK;
+}
+
+int bc_set_trace_refill(bc_ctx* c, int closest_lanes, int any_lanes) {
+    if (!c || closest_lanes < 0 || closest_lanes > 64 || any_lanes < 0 || any_lanes > 64) return BC_ERR_ARG;
+    c->trace_refill[0] = closest_lanes;
+    c->trace_refill[1] = any_lanes;
     return BC_OK;
 }
 
@@ -326,16 +333,20 @@ int bc_scene_refit(bc_ctx* c, const bc_scene_view* v, uint64_t geom) {
         }
     }
     (void)hipSetDevice(c->device);
-    // moved primitive records (and their materials) in leaf order
-    std::vector<bc_prim> recs(c->h_ids.size());
-    std::vector<int32_t> pmat(c->h_ids.size());
-    for (size_t r = 0; r < recs.size(); r++) {
-        const int id = (int)(c->h_ids[r] & ~BC_SPHERE_BIT);
-        recs[r] = bcx::prim_record(*v, id);
-        pmat[r] = v->prim_mat[id];
-    }
-    if (!upload_vec(c, c->d_prims, c->cap_prims, recs, "upload primitive records") ||
-        !upload_vec(c, c->d_pmat, c->cap_pmat, pmat, "upload primitive materials"))
+    // the view's geometry, kinds and materials to the device; the moved records (and their
+    // materials) are rewritten there in leaf order
+    const size_t n = (size_t)v->nprim;
+    void* stg = c->d_stage;
+    if (!bc_ensure(c, stg, c->cap_stage, 88 * n, "refit staging")) return BC_ERR_NOMEM;
+    c->d_stage = stg;
+    double* g = (double*)stg;
+    int32_t* kd = (int32_t*)(g + 10 * n);
+    int32_t* mt = kd + n;
+    if (!ok(c, hipMemcpyAsync(g, v->prim_geom, 80 * n, hipMemcpyHostToDevice, c->stream), "refit upload") ||
+        !ok(c, hipMemcpyAsync(kd, v->prim_kind, 4 * n, hipMemcpyHostToDevice, c->stream), "refit upload") ||
+        !ok(c, hipMemcpyAsync(mt, v->prim_mat, 4 * n, hipMemcpyHostToDevice, c->stream), "refit upload") ||
+        !ok(c, bcd::launch_refit_records(g, kd, mt, c->d_ids, (uint32_t)n, c->d_prims, c->d_pmat, c->d_err, c->stream),
+            "refit records"))
         return BC_ERR_DEVICE;
     void* nb = c->d_nbox;
     if (!bc_ensure(c, nb, c->cap_nbox, (size_t)c->nnode8 * 32, "refit boxes")) return BC_ERR_NOMEM;
@@ -467,7 +478,7 @@ int bc_trace_device(bc_ctx* c, const void* d_rays, int64_t n, int any_hit, void*
     if (!c || n < 0 || (n > 0 && (!d_rays || !d_out))) return BC_ERR_ARG;
     if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
     hipError_t e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, c->d_ids, (const bc_ray*)d_rays, n, d_out,
-                                     c->d_err, c->stream);
+                                     c->trace_refill[any_hit != 0 ? 1 : 0], c->d_err, c->stream);
     return ok(c, e, "trace launch") ? BC_OK : BC_ERR_DEVICE;
 }
 

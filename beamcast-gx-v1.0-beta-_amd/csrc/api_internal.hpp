@@ -51,7 +51,12 @@ struct bc_ctx {
     uint32_t* d_levels = nullptr;
     float4* d_nbox = nullptr;
     size_t cap_levels = 0, cap_nbox = 0;
+    void* d_stage = nullptr;  // refit staging: the view's geometry, kinds and materials
+    size_t cap_stage = 0;
     int flatten = 0;       // BVH8 flattening: 0 auto (device for large scenes), 1 host, 2 device
+    // ray queries (closest, any hit): refill a wave's lanes once this many are idle (0: one ray
+    // per thread).  Any-hit rays end at very different depths, closest-hit rays much less so.
+    int trace_refill[2] = {0, 8};
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;

@@ -15,9 +15,11 @@ int stack_for_depth(int depth);
 // Ray queries: any_hit writes int32 occlusion flags, else bc_hit records.  *err is set
 // non-zero by the kernel if a traversal stack would overflow.
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids,
-                        const bc_ray* rays,
-                        int64_t n, void* out, int* err, hipStream_t st);
+                        const bc_ray* rays, int64_t n, void* out, int refill, int* err, hipStream_t st);
 
+// Refit: records (and materials) of the moved primitives in leaf order from the view's arrays.
+hipError_t launch_refit_records(const double* geom, const int32_t* kind, const int32_t* mat, const uint32_t* ids, uint32_t n,
+                                bc_prim* prims, int32_t* pmat, int* err, hipStream_t st);
 // One level of the device BVH8 refit: recomputes the boxes of the `n` nodes in `list`
 // (whose children are already refit) and stores each node's union box in nbox[2 i .. 2 i + 1].
 hipError_t launch_refit_level(bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, float4* nbox,

@@ -123,6 +123,38 @@ __global__ __launch_bounds__(256) void k_refit_level(bc_node8* __restrict__ node
     nbox[2 * (size_t)idx + 1] = make_float4(f_up(nb.mx[0]), f_up(nb.mx[1]), f_up(nb.mx[2]), 0.0f);
 }
 
+// Records of the moved primitives in leaf order (prim_record of the host flattening) and their
+// materials; a primitive whose kind differs from the resident record's sets error bit 16.
+__global__ __launch_bounds__(256) void k_refit_records(const double* __restrict__ geom, const int32_t* __restrict__ kind,
+                                                       const int32_t* __restrict__ mat, const uint32_t* __restrict__ ids,
+                                                       uint32_t n, bc_prim* __restrict__ prims, int32_t* __restrict__ pmat,
+                                                       int* __restrict__ err) {
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t id = ids[r] & ~BC_SPHERE_BIT;
+    const bool sphere = (ids[r] & BC_SPHERE_BIT) != 0;
+    if ((kind[id] == 0) != sphere) { atomicOr(err, 16); return; }
+    const double* g = geom + 10 * (size_t)id;
+    bc_prim p;
+    for (int q = 0; q < 9; q++) p.v[q] = 0.f;
+    if (sphere) {
+        const double rad = g[3];
+        p.v[0] = (float)g[0]; p.v[1] = (float)g[1]; p.v[2] = (float)g[2];
+        p.v[3] = (float)rad; p.v[4] = (float)(rad * rad); p.v[5] = (float)(1.0 / rad);
+    } else {
+        for (int q = 0; q < 9; q++) p.v[q] = (float)g[q];
+    }
+    prims[r] = p;
+    pmat[r] = mat[id];
+}
+
+hipError_t launch_refit_records(const double* geom, const int32_t* kind, const int32_t* mat, const uint32_t* ids, uint32_t n,
+                                bc_prim* prims, int32_t* pmat, int* err, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_refit_records, dim3((n + 255u) / 256u), dim3(256), 0, st, geom, kind, mat, ids, n, prims, pmat, err);
+    return hipGetLastError();
+}
+
 hipError_t launch_refit_level(bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, float4* nbox,
                               const uint32_t* list, uint32_t n, int* err, hipStream_t st) {
     if (n == 0) return hipSuccess;

@@ -172,109 +172,147 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
     pg_mask = pm;
 }
 
-// Traverses the BVH8.  ANY=true stops at the first primitive hit in [tmin, tmax].
-// stk points at this lane's column of the LDS stack (entry i at stk[i * stride]).
-// Primitive records fetched together in a leaf (see the leaf loop of traverse).
+// Primitive records fetched together in a leaf (see the leaf loop of trav_step).
 #ifndef BC_PRIM_BATCH
 #define BC_PRIM_BATCH 2
 #endif
 constexpr int kPrimBatch = BC_PRIM_BATCH;
 
+// Traversal state of one lane: the ray, its reciprocal direction and octant, the best hit so
+// far, the node group being expanded and the primitive group being tested, and the depth of its
+// LDS stack.  trav_step advances it by one node visit, the primitive group that visit produced,
+// and one push or pop, so a wave can run it until some lanes finish and then give those lanes
+// new rays (see k_trace) while the others continue.
+struct TravState {
+    Ray r;
+    float ix, iy, iz, tmax;
+    uint32_t oct;
+    HitRec best;
+    uint32_t ng_base, ng_bits, pg_base, pg_mask, node;
+    int sp;
+    bool have_node;
+};
+
 // root: index of the scene's root node (0 for a single scene; scenes packed in one arena
 // each have their own).
+__device__ __forceinline__ void trav_init(TravState& st, const Ray& r, float tmax, uint32_t root) {
+    st.r = r;
+    st.ix = safe_rcp(r.dx);
+    st.iy = safe_rcp(r.dy);
+    st.iz = safe_rcp(r.dz);
+    st.oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
+    st.best.t = tmax;
+    st.best.id = -1;
+    st.best.rec = 0xffffffffu;
+    st.ng_base = st.ng_bits = st.pg_base = st.pg_mask = 0;
+    st.node = root;
+    st.sp = 0;
+    st.have_node = true;
+}
+
+// One step; returns true when the traversal of this lane's ray is complete (st.best final).
+// ANY=true stops at the first primitive hit in [tmin, tmax].  stk points at this lane's column
+// of the LDS stack (entry i at stk[i * stride]).
+template <bool ANY, int STACK>
+__device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restrict__ nodes, const bc_prim* __restrict__ prims,
+                                          const uint32_t* __restrict__ ids, uint2* stk, int stride, int* __restrict__ err) {
+    const Ray& r = st.r;
+    if (st.have_node) {
+        visit_node(nodes, st.node, r, st.ix, st.iy, st.iz, st.oct, st.best.t, st.ng_base, st.ng_bits, st.pg_base, st.pg_mask);
+        st.have_node = false;
+    }
+    const bool sph_node = (st.pg_base & BC_NODE_HAS_SPHERES) != 0;
+    const uint32_t pbase = st.pg_base & ~BC_NODE_HAS_SPHERES;
+    // primitives kPrimBatch at a time: their records are loaded before any is tested, so the
+    // memory round trips overlap (the loop is latency-bound); tests stay in mask order
+    while (st.pg_mask) {
+        uint32_t rr[kPrimBatch];
+        int cnt = 0;
+#pragma unroll
+        for (int u = 0; u < kPrimBatch; u++) {
+            const bool any = st.pg_mask != 0u;
+            rr[u] = pbase + (any ? (uint32_t)__builtin_ctz(st.pg_mask) : 0u);
+            if (!any) rr[u] = rr[0];
+            cnt += any ? 1 : 0;
+            st.pg_mask &= st.pg_mask - 1u;
+        }
+        float g[kPrimBatch][9];
+#pragma unroll
+        for (int u = 0; u < kPrimBatch; u++)
+#pragma unroll
+            for (int q = 0; q < 9; q++) g[u][q] = prims[rr[u]].v[q];
+        bool sp[kPrimBatch];
+#pragma unroll
+        for (int u = 0; u < kPrimBatch; u++) sp[u] = sph_node && (ids[rr[u]] & BC_SPHERE_BIT);
+#pragma unroll
+        for (int u = 0; u < kPrimBatch; u++) {
+            if (u >= cnt) break;
+            const uint32_t ri = rr[u];
+            float t;
+            const bool h = sp[u] ? sphere_test(g[u], r, r.tmin, st.best.t, t) : tri_test(g[u], r, r.tmin, st.best.t, t);
+            if (h) {
+                if (ANY) {
+                    st.best.t = t;
+                    st.best.id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
+                    st.best.rec = ri;
+                    return true;
+                }
+                // the test admits t <= best.t; only an exact tie needs the primitive indices
+                // (smaller wins), so the id table is read in that rare case only
+                if (st.best.rec == 0xffffffffu || t < st.best.t ||
+                    (ids[ri] & ~BC_SPHERE_BIT) < (ids[st.best.rec] & ~BC_SPHERE_BIT)) {
+                    st.best.t = t;
+                    st.best.rec = ri;
+                }
+            }
+        }
+    }
+    if (st.ng_bits & 0xffu) {
+        const uint32_t k = (uint32_t)__builtin_ctz(st.ng_bits & 0xffu);
+        st.ng_bits &= ~(1u << k);
+        const uint32_t slot = k ^ st.oct;
+        const uint32_t imask = (st.ng_bits >> 8) & 0xffu;
+        st.node = st.ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
+        if (st.ng_bits & 0xffu) {
+            if (st.sp >= STACK) {  // host guarantees depth <= STACK; report instead of overflowing
+                atomicOr(err, 1);
+                return true;
+            }
+            stk[st.sp * stride] = make_uint2(st.ng_base, st.ng_bits);
+            st.sp++;
+        }
+        st.have_node = true;
+        return false;
+    }
+    if (st.sp > 0) {
+        st.sp--;
+        const uint2 e = stk[st.sp * stride];
+        st.ng_base = e.x;
+        st.ng_bits = e.y;
+        // closest hit: a group whose nearest child starts beyond the current hit is done
+        if (!ANY && __uint_as_float(e.y & 0xffff0000u) > st.best.t) st.ng_bits = 0;
+        return false;
+    }
+    return true;
+}
+
+// The closest hit resolves its primitive index once, at the end.
+__device__ __forceinline__ HitRec trav_result(const TravState& st, const uint32_t* __restrict__ ids, bool any) {
+    HitRec h = st.best;
+    if (!any && h.rec != 0xffffffffu) h.id = (int32_t)(ids[h.rec] & ~BC_SPHERE_BIT);
+    return h;
+}
+
+// Traverses the BVH8 for one ray to completion.
 template <bool ANY, int STACK>
 __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, const bc_prim* __restrict__ prims,
                                            const uint32_t* __restrict__ ids, uint32_t root, const Ray& r, float tmax,
                                            uint2* stk, int stride, int* __restrict__ err) {
-    const float ix = safe_rcp(r.dx), iy = safe_rcp(r.dy), iz = safe_rcp(r.dz);
-    const uint32_t oct = (r.dx < 0.0f ? 1u : 0u) | (r.dy < 0.0f ? 2u : 0u) | (r.dz < 0.0f ? 4u : 0u);
-    HitRec best;
-    best.t = tmax;
-    best.id = -1;
-    best.rec = 0xffffffffu;
-    uint32_t ng_base = 0, ng_bits = 0, pg_base = 0, pg_mask = 0;
-    uint32_t node = root;
-    bool have_node = true;
-    int sp = 0;
-    for (;;) {
-        if (have_node) {
-            visit_node(nodes, node, r, ix, iy, iz, oct, best.t, ng_base, ng_bits, pg_base, pg_mask);
-            have_node = false;
-        }
-        const bool sph_node = (pg_base & BC_NODE_HAS_SPHERES) != 0;
-        const uint32_t pbase = pg_base & ~BC_NODE_HAS_SPHERES;
-        // primitives kPrimBatch at a time: their records are loaded before any is tested, so
-        // the memory round trips overlap (the loop is latency-bound); tests stay in mask order
-        while (pg_mask) {
-            uint32_t rr[kPrimBatch];
-            int cnt = 0;
-#pragma unroll
-            for (int u = 0; u < kPrimBatch; u++) {
-                const bool any = pg_mask != 0u;
-                rr[u] = pbase + (any ? (uint32_t)__builtin_ctz(pg_mask) : 0u);
-                if (!any) rr[u] = rr[0];
-                cnt += any ? 1 : 0;
-                pg_mask &= pg_mask - 1u;
-            }
-            float g[kPrimBatch][9];
-#pragma unroll
-            for (int u = 0; u < kPrimBatch; u++)
-#pragma unroll
-                for (int q = 0; q < 9; q++) g[u][q] = prims[rr[u]].v[q];
-            bool sp[kPrimBatch];
-#pragma unroll
-            for (int u = 0; u < kPrimBatch; u++) sp[u] = sph_node && (ids[rr[u]] & BC_SPHERE_BIT);
-#pragma unroll
-            for (int u = 0; u < kPrimBatch; u++) {
-                if (u >= cnt) break;
-                const uint32_t ri = rr[u];
-                float t;
-                const bool h = sp[u] ? sphere_test(g[u], r, r.tmin, best.t, t) : tri_test(g[u], r, r.tmin, best.t, t);
-                if (h) {
-                    if (ANY) {
-                        best.t = t;
-                        best.id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
-                        best.rec = ri;
-                        return best;
-                    }
-                    // the test admits t <= best.t; only an exact tie needs the primitive
-                    // indices (smaller wins), so the id table is read in that rare case only
-                    if (best.rec == 0xffffffffu || t < best.t ||
-                        (ids[ri] & ~BC_SPHERE_BIT) < (ids[best.rec] & ~BC_SPHERE_BIT)) {
-                        best.t = t;
-                        best.rec = ri;
-                    }
-                }
-            }
-        }
-        if (ng_bits & 0xffu) {
-            const uint32_t k = (uint32_t)__builtin_ctz(ng_bits & 0xffu);
-            ng_bits &= ~(1u << k);
-            const uint32_t slot = k ^ oct;
-            const uint32_t imask = (ng_bits >> 8) & 0xffu;
-            node = ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
-            if (ng_bits & 0xffu) {
-                if (sp >= STACK) {  // host guarantees depth <= STACK; report instead of overflowing
-                    atomicOr(err, 1);
-                    return best;
-                }
-                stk[sp * stride] = make_uint2(ng_base, ng_bits);
-                sp++;
-            }
-            have_node = true;
-        } else if (sp > 0) {
-            sp--;
-            const uint2 e = stk[sp * stride];
-            ng_base = e.x;
-            ng_bits = e.y;
-            // closest hit: a group whose nearest child starts beyond the current hit is done
-            if (!ANY && __uint_as_float(e.y & 0xffff0000u) > best.t) ng_bits = 0;
-        } else {
-            break;
-        }
+    TravState st;
+    trav_init(st, r, tmax, root);
+    while (!trav_step<ANY, STACK>(st, nodes, prims, ids, stk, stride, err)) {
     }
-    if (!ANY && best.rec != 0xffffffffu) best.id = (int32_t)(ids[best.rec] & ~BC_SPHERE_BIT);
-    return best;
+    return trav_result(st, ids, ANY);
 }
 
 

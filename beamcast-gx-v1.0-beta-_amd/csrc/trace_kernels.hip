@@ -32,9 +32,89 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const bc_node8* __restric
     }
 }
 
+__device__ __forceinline__ void load_ray(const bc_ray* __restrict__ rays, int64_t i, Ray& r, float& tmax) {
+    const float4* rp = reinterpret_cast<const float4*>(rays + i);
+    const float4 a = rp[0], b = rp[1];
+    r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
+    r.dx = b.x; r.dy = b.y; r.dz = b.z;
+    tmax = b.w;
+}
+
+__device__ __forceinline__ void store_hit(void* __restrict__ out, int64_t i, const HitRec& h, bool any) {
+    if (any) {
+        reinterpret_cast<int32_t*>(out)[i] = h.rec != 0xffffffffu ? 1 : 0;
+    } else {
+        bc_hit o;
+        o.t = h.id >= 0 ? h.t : 0.0f;
+        o.prim = h.id;
+        reinterpret_cast<bc_hit*>(out)[i] = o;
+    }
+}
+
+// Lane refill: each wave owns a run of kRefillRays consecutive rays.  Its lanes start on the
+// first 64; whenever at least `refill` lanes have finished, the wave hands them the next rays
+// of its run (ballot + prefix count, no atomics), so a wave keeps most lanes busy instead of
+// waiting for its slowest ray.  Per-ray results are those of k_trace.
+constexpr int kRefillRays = 512;
+
+template <bool ANY, int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_refill(const bc_node8* __restrict__ nodes,
+                                                              const bc_prim* __restrict__ prims,
+                                                              const uint32_t* __restrict__ ids,
+                                                              const bc_ray* __restrict__ rays, int64_t n,
+                                                              void* __restrict__ out, int refill, int* __restrict__ err) {
+    __shared__ uint2 stk[STACK * kTraceBlock];
+    const int lane = (int)(threadIdx.x & 63u);
+    const int64_t wave = (int64_t)xcd_chunk(blockIdx.x, gridDim.x, 16u) * (kTraceBlock / 64) + (threadIdx.x >> 6);
+    const int64_t first = wave * kRefillRays;
+    const int64_t end = min(n, first + (int64_t)kRefillRays);
+    if (first >= end) return;
+    int64_t next = first + 64;  // wave-uniform: the next unassigned ray of the run
+    int64_t idx = first + lane;
+    bool active = idx < end;
+    TravState st;
+    if (active) {
+        Ray r;
+        float tmax;
+        load_ray(rays, idx, r, tmax);
+        trav_init(st, r, tmax, 0u);
+    }
+    uint2* my = stk + threadIdx.x;
+    for (;;) {
+        if (active && trav_step<ANY, STACK>(st, nodes, prims, ids, my, kTraceBlock, err)) {
+            store_hit(out, idx, trav_result(st, ids, ANY), ANY);
+            active = false;
+        }
+        const uint64_t idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (nidle == 64 && next >= end) break;
+        if (nidle >= refill && next < end) {
+            if (!active) {
+                const int64_t k = next + __popcll(idle & ((1ull << lane) - 1ull));
+                if (k < end) {
+                    idx = k;
+                    Ray r;
+                    float tmax;
+                    load_ray(rays, idx, r, tmax);
+                    trav_init(st, r, tmax, 0u);
+                    active = true;
+                }
+            }
+            next += nidle;
+        }
+    }
+}
+
 template <bool ANY, int STACK>
 static hipError_t launch_trace_t(const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays, int64_t n,
-                                 void* out, int* err, hipStream_t st) {
+                                 void* out, int refill, int* err, hipStream_t st) {
+    if (refill > 0) {
+        const int64_t per_block = (int64_t)kRefillRays * (kTraceBlock / 64);
+        const int64_t blocks = (n + per_block - 1) / per_block;
+        hipLaunchKernelGGL((k_trace_refill<ANY, STACK>), dim3((unsigned)blocks), dim3(kTraceBlock), 0, st, nodes, prims, ids,
+                           rays, n, out, refill, err);
+        return hipGetLastError();
+    }
     const int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
     hipLaunchKernelGGL((k_trace<ANY, STACK>), dim3((unsigned)blocks), dim3(kTraceBlock), 0, st, nodes, prims, ids, rays,
                        n, out, err);
@@ -43,21 +123,22 @@ static hipError_t launch_trace_t(const bc_node8* nodes, const bc_prim* prims, co
 
 template <bool ANY>
 static hipError_t launch_trace_any(int stack, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
-                                   int64_t n, void* out, int* err, hipStream_t st) {
+                                   int64_t n, void* out, int refill, int* err, hipStream_t st) {
     switch (stack) {
-    case 8: return launch_trace_t<ANY, 8>(nodes, prims, ids, rays, n, out, err, st);
-    case 16: return launch_trace_t<ANY, 16>(nodes, prims, ids, rays, n, out, err, st);
-    case 24: return launch_trace_t<ANY, 24>(nodes, prims, ids, rays, n, out, err, st);
-    case 32: return launch_trace_t<ANY, 32>(nodes, prims, ids, rays, n, out, err, st);
+    case 8: return launch_trace_t<ANY, 8>(nodes, prims, ids, rays, n, out, refill, err, st);
+    case 16: return launch_trace_t<ANY, 16>(nodes, prims, ids, rays, n, out, refill, err, st);
+    case 24: return launch_trace_t<ANY, 24>(nodes, prims, ids, rays, n, out, refill, err, st);
+    case 32: return launch_trace_t<ANY, 32>(nodes, prims, ids, rays, n, out, refill, err, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
-                        int64_t n, void* out, int* err, hipStream_t st) {
+                        int64_t n, void* out, int refill, int* err, hipStream_t st) {
     if (n <= 0) return hipSuccess;
-    return any_hit ? launch_trace_any<true>(stack, nodes, prims, ids, rays, n, out, err, st)
-                   : launch_trace_any<false>(stack, nodes, prims, ids, rays, n, out, err, st);
+    if (refill > 64) refill = 64;
+    return any_hit ? launch_trace_any<true>(stack, nodes, prims, ids, rays, n, out, refill, err, st)
+                   : launch_trace_any<false>(stack, nodes, prims, ids, rays, n, out, refill, err, st);
 }
 
 int stack_for_depth(int depth) {

@@ -86,6 +86,11 @@ int bc_build_bvh(bc_ctx* ctx, int32_t nprim, const double* prim_box, const doubl
  * bc_scene_upload.  Lights and their distribution are rebuilt from the new geometry.  Traversal
  * stays exact (the hit set of a ray does not depend on the box tightness); geom_rev is recorded. */
 int bc_scene_refit(bc_ctx* ctx, const bc_scene_view* scene, uint64_t geom_rev);
+/* Ray-query schedule of bc_trace / bc_trace_device (no counterpart in the reference), for
+ * closest-hit and any-hit queries: with lanes > 0 each wave owns a run of rays and gives new rays
+ * to its finished lanes once `lanes` of its 64 are idle; 0 runs one ray per thread to completion.
+ * Defaults 0 (closest) and 8 (any hit).  Results do not depend on it. */
+int bc_set_trace_refill(bc_ctx* ctx, int closest_lanes, int any_lanes);
 /* Where bc_scene_upload flattens the binary tree into BVH8 nodes (no counterpart in the
  * reference): 0 automatic (the device for scenes of 65,536 primitives or more), 1 the host, 2 the
  * device.  Both produce the same bytes; the device path uploads the view's arrays instead. */

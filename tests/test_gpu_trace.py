@@ -155,3 +155,25 @@ def test_scene_encoding_errors(dev):
         dev.upload_view(vp, 99, 99)
     link[:] = saved
     dev.upload_view(vp, 100, 100)
+
+
+@pytest.mark.parametrize("n", [1, 63, 1000, 70_001])
+def test_refill_schedules(dev, n):
+    """Lane refill (a wave hands finished lanes new rays of its run) returns the results of one
+    ray per thread, for closest and any hit, for batch sizes that leave partial waves and runs."""
+    print(f"case: refill schedules, {n} rays", flush=True)
+    s = bc.HostScene.workload(bc.SOUP, param=100_000)
+    dev.upload(s)
+    rays = np.concatenate([bc.workload_rays(0, 512, count=512 * 512)[::7], bc.workload_rays(1, 512, count=70_000)])
+    rays = rays[np.random.default_rng(n).permutation(len(rays))[:n]]
+    ra = rays.copy()
+    ra[:, 7] = np.random.default_rng(1).random(n) * 0.5
+    o = orc.Scene.from_host(s)
+    prim, t = o.hit32(rays)
+    occ = o.any32(ra)
+    for closest, anyh in ((0, 0), (1, 1), (8, 8), (33, 64), (64, 4)):
+        dev.set_trace_refill(closest, anyh)
+        got = dev.trace(rays)
+        assert np.array_equal(got["prim"], prim) and np.array_equal(got["t"], t), (closest, anyh)
+        assert np.array_equal(dev.trace(ra, any_hit=True), occ), (closest, anyh)
+    dev.set_trace_refill(0, 8)
