@@ -59,6 +59,7 @@ class RenderDesc(ctypes.Structure):
         ("origin", c_double * 3), ("lower", c_double * 3), ("horiz", c_double * 3), ("vert", c_double * 3),
         ("mode", c_int32), ("samples_per_pass", c_int32), ("reserved", c_int32 * 6),
         ("restir_candidates", c_int32), ("sun_dir", c_float * 3), ("sun_irradiance", c_float * 3),
+        ("restir_spatial", c_int32), ("restir_radius", c_int32),
     ]
 
 
@@ -565,7 +566,8 @@ class Device:
 
 def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampler=1, mis=True, power=True,
                 firefly_clamp=0.0, seed=0x9E3779B9, sky_mode=0, sky_color=(0.3, 0.3, 0.3), mode=0,
-                samples_per_pass=0, pool=0, iters=0, segment=0, xcd_group=0, material_sort=0, restir=0, sun_dir=(0.0, 1.0, 0.0), sun=(0.0, 0.0, 0.0)):
+                samples_per_pass=0, pool=0, iters=0, segment=0, xcd_group=0, material_sort=0, restir=0, sun_dir=(0.0, 1.0, 0.0), sun=(0.0, 0.0, 0.0),
+                spatial=0, radius=0):
     """RenderDesc for a workload scene (camera from the workload, Camera.basis in fp64)."""
     cam = scene.camera
     basis = camera_basis(cam[0:3], cam[3:6], cam[6:9], cam[9], width / height)
@@ -580,6 +582,7 @@ def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampl
     d.reserved[0], d.reserved[1], d.reserved[2], d.reserved[3] = pool, iters, segment, xcd_group
     d.reserved[4] = material_sort
     d.restir_candidates = restir
+    d.restir_spatial, d.restir_radius = spatial, radius
     n = np.asarray(sun_dir, np.float64)
     n = n / np.linalg.norm(n)
     for k in range(3):

@@ -101,6 +101,9 @@ struct bc_hit {
 #define BC_DIM_RIS0 0x100000u
 #define BC_RIS_DIMS 32u
 #define BC_RIS_MAX 64
+/* spatial reuse: neighbour i draws dims BC_DIM_SPATIAL0 + 4 i + o, o = 0 x offset, 1 y offset
+ * (integer hashes), 2 reservoir update (i = 0: the pixel's own reservoir) */
+#define BC_DIM_SPATIAL0 0x200000u
 
 BC_HD static inline uint32_t bc_mix32(uint32_t x) {
     // "lowbias32" integer hash (C. Wellons, hash-prospector; public domain)

@@ -61,6 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, f
             const uint32_t sl = j / (uint32_t)P.npix, pl = j - sl * (uint32_t)P.npix;
             init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
             B.q[0][j] = j;
+            if (P.spatial > 0) B.rv[0][j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);  // no reservoir yet
             if (P.s0 == 0 && sl == 0) {
                 float* a = acc + 4 * (size_t)pl;
                 a[0] = 0.0f; a[1] = 0.0f; a[2] = 0.0f; a[3] = 0.0f;
@@ -132,6 +133,115 @@ __global__ __launch_bounds__(kBlock) void k_shade(SceneBufs S, PathBufs B, Rende
             B.qc[cur ^ 1][seg] = bc;
             B.sqc[seg] = bs;
         }
+    }
+}
+
+// ------------------------------------------------------------------------------ spatial reuse
+// After the primary-vertex shading of a pass: every path with a reservoir combines it with up to
+// P.spatial neighbours' (same sample index, pixel offsets of at most P.radius within the rendered
+// rows, normals within 25 degrees and hit distances within 10%), picks one sample with the
+// streaming rule of the reservoir, and queues its shadow ray with contribution F(y) W.  The
+// segment's shadow queue (empty after the primary shading in this mode) is written here.
+__global__ __launch_bounds__(kBlock) void k_spatial(SceneBufs S, PathBufs B, RenderParams P, uint32_t nseg) {
+    __shared__ uint32_t lds[2 * 2 * kWaves];
+    uint32_t parity = 0;
+    const uint32_t n = (uint32_t)P.ns * (uint32_t)P.npix;
+    const int R = P.radius, span = 2 * P.radius + 1;
+    const int rows = P.npix / P.width;
+    for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
+        const uint32_t cnt = seg_slots(seg, B.seg, n), base = seg * B.seg;
+        uint32_t bs = 0;
+        for (uint32_t r0 = 0; r0 < cnt; r0 += kBlock) {
+            const uint32_t k = r0 + threadIdx.x;
+            bool shadow = false;
+            uint32_t j = 0;
+            float4 so = make_float4(0, 0, 0, 0), sdv = so, sc = so;
+            if (k < cnt) {
+                j = base + k;
+                const float4 a0 = B.rv[0][j];
+                if (a0.w > 0.0f) {
+                    const float4 a1 = B.rv[1][j], a2 = B.rv[2][j];
+                    const float3 p = xyz(a0), N = xyz(a1), f = xyz(a2);
+                    const uint32_t key = B.meta[j].z;
+                    const uint32_t sl = j / (uint32_t)P.npix, pl = j - sl * (uint32_t)P.npix;
+                    const int px = (int)(pl % (uint32_t)P.width), py = (int)(pl / (uint32_t)P.width);
+                    float wsum = 0.0f, ph_s = 0.0f;
+                    int Mtot = 0, comp_s = -1;
+                    float3 F_s = f3(0.0f, 0.0f, 0.0f), y_s = F_s;
+                    for (int i = 0; i <= P.spatial; i++) {
+                        uint32_t jn = j;
+                        if (i > 0) {
+                            const uint32_t hx = bc_rng_u32(key, BC_DIM_SPATIAL0 + 4u * (uint32_t)i);
+                            const uint32_t hy = bc_rng_u32(key, BC_DIM_SPATIAL0 + 4u * (uint32_t)i + 1u);
+                            const int ox = (int)(hx % (uint32_t)span) - R, oy = (int)(hy % (uint32_t)span) - R;
+                            const int nx = px + ox, ny = py + oy;
+                            if ((ox == 0 && oy == 0) || nx < 0 || nx >= P.width || ny < 0 || ny >= rows) continue;
+                            jn = sl * (uint32_t)P.npix + (uint32_t)(ny * P.width + nx);
+                            const float4 b0 = B.rv[0][jn];
+                            if (!(b0.w > 0.0f)) continue;
+                            const float3 Nn = xyz(B.rv[1][jn]);
+                            if (dot(N, Nn) < (float)0.9063 || fabsf(b0.w - a0.w) > (float)0.1 * a0.w) continue;
+                        }
+                        const uint32_t code = __float_as_uint(B.rv[1][jn].w);
+                        const float4 c2 = B.rv[2][jn], c3 = B.rv[3][jn];
+                        const int Mi = __float_as_int(c3.w);
+                        Mtot += Mi;
+                        const int comp = (int)(code & 3u) - 1;
+                        if (comp < 0) continue;
+                        const int li = (int)(code >> 2);
+                        const float3 y = xyz(c3);
+                        const float3 F = spatial_eval(S, P, p, N, f, comp, li, y, xyz(B.rv[4][jn]));
+                        const float ph = lum(F);
+                        const float w = ph * c2.w * (float)Mi;
+                        if (w > 0.0f) {
+                            wsum = wsum + w;
+                            if (bc_rng(key, BC_DIM_SPATIAL0 + 4u * (uint32_t)i + 2u) * wsum < w) {
+                                comp_s = comp;
+                                F_s = F;
+                                ph_s = ph;
+                                y_s = y;
+                            }
+                        }
+                    }
+                    if (comp_s >= 0) {
+                        const float W = wsum / ((float)Mtot * ph_s);
+                        const float3 c = mul(F_s, W);  // the primary vertex has unit throughput
+                        const float3 from = add(p, mul(N, (float)2e-4));
+                        if (comp_s == 1) {  // visible(), renderer.go:1333-1341
+                            const float3 dd = sub(y_s, from);
+                            const float dd2 = dot(dd, dd);
+                            if (dd2 < (float)1e-12) {
+                                const float4 l = B.L[j];
+                                B.L[j] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+                            } else {
+                                const float dist = sqrtf(dd2);
+                                const float3 dir = mul(dd, 1.0f / dist);
+                                so = make_float4(from.x, from.y, from.z, dist - (float)1e-4);
+                                sdv = make_float4(dir.x, dir.y, dir.z, 0.0f);
+                                sc = make_float4(c.x, c.y, c.z, 0.0f);
+                                shadow = true;
+                            }
+                        } else {
+                            so = make_float4(from.x, from.y, from.z, (float)1e30);
+                            sdv = make_float4(y_s.x, y_s.y, y_s.z, 0.0f);
+                            sc = make_float4(c.x, c.y, c.z, 0.0f);
+                            shadow = true;
+                        }
+                    }
+                }
+            }
+            const bool pr[2] = {shadow, false};
+            uint32_t rk[2], tot[2];
+            block_rank<2>(pr, rk, tot, lds, parity);
+            if (shadow) {
+                B.sq[base + bs + rk[0]] = j;
+                B.so[j] = so;
+                B.sd[j] = sdv;
+                B.sc[j] = sc;
+            }
+            bs += tot[0];
+        }
+        if (threadIdx.x == 0) B.sqc[seg] = bs;
     }
 }
 
@@ -606,8 +716,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, 
 
 // ------------------------------------------------------------------------------ host side
 struct RenderState {
-    void* buf[25] = {};
-    size_t bufcap[25] = {};
+    void* buf[30] = {};
+    size_t bufcap[30] = {};
     float4* sg = nullptr;
     // differentiable render: per-sample radiance of the last bc_forward, its shape
     void* keep = nullptr;
@@ -646,7 +756,7 @@ size_t segments_for(size_t paths, uint32_t ss) { return (paths + ss - 1) / ss; }
 
 // Allocates (or keeps) path state for `paths` slots in whole segments of `ss` slots.
 bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t window_samples, uint32_t ss,
-                  int njobs = 1) {
+                  int njobs = 1, bool reservoirs = false) {
     if (!c->rs) c->rs = new RenderState();
     RenderState& R = *c->rs;
     if (!R.h_cnt) {
@@ -670,6 +780,10 @@ bool ensure_state(bc_ctx* c, size_t paths, size_t npix, bool guides, size_t wind
     };
     for (const Want& x : w)
         if (!bc_ensure(c, R.buf[x.slot], R.bufcap[x.slot], std::max<size_t>(x.bytes, 64), "render buffers")) return false;
+    for (int k = 0; k < 5; k++) {  // primary-vertex reservoirs of spatial reuse
+        if (reservoirs && !bc_ensure(c, R.buf[25 + k], R.bufcap[25 + k], slots * 16, "reservoir buffers")) return false;
+        R.B.rv[k] = reservoirs ? (float4*)R.buf[25 + k] : nullptr;
+    }
     R.nseg = (uint32_t)nseg;
     R.B.seg = ss;
     R.B.o = (float4*)R.buf[0];
@@ -759,6 +873,8 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
     P.sun_dir[1] = sl > 0 ? (float)(sy / sl) : 0.0f;
     P.sun_dir[2] = sl > 0 ? (float)(sz / sl) : 0.0f;
     for (int k = 0; k < 3; k++) P.sun[k] = d->sun_irradiance[k];
+    P.spatial = d->restir_candidates > 0 ? d->restir_spatial : 0;
+    P.radius = d->restir_radius > 0 ? d->restir_radius : 16;
     return P;
 }
 
@@ -767,6 +883,8 @@ void launch_bounce(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::R
                    uint32_t* guides, int grid, uint32_t nseg, hipStream_t st) {
     hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg, P.root);
     hipLaunchKernelGGL(bcd::k_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, nseg);
+    if (bounce == 0 && P.spatial > 0 && P.restir > 0)
+        hipLaunchKernelGGL(bcd::k_spatial, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, nseg);
     hipLaunchKernelGGL((bcd::k_shadow<STACK, false>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg, P.root,
                        (const bcd::RenderParams*)nullptr);
 }
@@ -824,7 +942,8 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     int spp_pass = d->samples_per_pass > 0 ? d->samples_per_pass : (int)std::max<size_t>(1, max_paths / npix);
     spp_pass = std::min(spp_pass, d->spp);
     const size_t paths = npix * (size_t)spp_pass;
-    if (!ensure_state(c, paths, npix, d_guides != nullptr, 1, plain_segment(c, paths))) return BC_ERR_NOMEM;
+    const bool reservoirs = d->restir_candidates > 0 && d->restir_spatial > 0;
+    if (!ensure_state(c, paths, npix, d_guides != nullptr, 1, plain_segment(c, paths), 1, reservoirs)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     R.B.xg = xcd_group(d);
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
@@ -1096,6 +1215,14 @@ bool valid_desc(bc_ctx* c, const bc_render_desc* d, int y0, int rows) {
         y0 < 0 || rows <= 0 || y0 + rows > d->height || d->sampler < 0 || d->sampler > 2 ||
         (int64_t)d->width * d->height > ((int64_t)1 << 28) || d->mode < 0 || d->mode > 1) {
         c->err = "invalid render description or row range";
+        return false;
+    }
+    if (d->restir_spatial < 0 || d->restir_spatial > 16 || d->restir_radius < 0 || d->restir_radius > 64) {
+        c->err = "restir_spatial must be in [0, 16] and restir_radius in [0, 64]";
+        return false;
+    }
+    if (d->restir_spatial > 0 && (d->restir_candidates == 0 || d->mode != 0)) {
+        c->err = "spatial reuse (restir_spatial > 0) needs restir_candidates > 0 and the plain schedule (mode 0)";
         return false;
     }
     if (d->restir_candidates < 0 || d->restir_candidates > BC_RIS_MAX) {

@@ -26,6 +26,7 @@ struct RenderParams {
     uint32_t root, lbase;      // scene root node and first light of the scene (packed scenes)
     int32_t restir;            // candidates of the resampled direct lighting (0 = one-sample NEE)
     float sun_dir[3], sun[3];  // directional light (restir only)
+    int32_t spatial, radius;   // spatial reuse at primary vertices: neighbours, pixel radius (0 = off)
 };
 
 // Path slots are grouped in segments of B.seg consecutive slots (a multiple of the workgroup
@@ -67,6 +68,7 @@ struct PathBufs {
     uint32_t* dqc;   // per-segment terminated-path count (persistent schedule)
     uint32_t* snext; // per-segment next sample ordinal (persistent schedule)
     uint32_t* flag;  // [0] != 0: some segment had work (persistent drain test)
+    float4* rv[5];   // spatial reuse: primary-vertex reservoirs (p, t | N, code | f, W | y, M | light normal)
     unsigned long long* tally;   // path segments per pixel-kernel block [0, kMaxGrid), then shadow rays per segment
     unsigned long long* totals;  // [0] path segments, [1] shadow rays
 };
@@ -278,9 +280,19 @@ __device__ __forceinline__ float3 hemi_dir(float3 N, float x, float y, float z) 
 // W = sum(w) / (M target(y)).  Returns false when nothing survives; otherwise the unshadowed
 // contribution F(y) W (before throughput) and the visibility ray: towards the point y when
 // `point`, else along the direction y.
-__device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParams& P, float3 p, float3 N,
-                                           float3 color, uint32_t key, int bounce, float3& contrib, float3& y_out,
-                                           bool& point_out) {
+// The reservoir of resampled direct lighting after its candidates: the survivor, its target and
+// weight sum (ris_sample), from which ris_direct forms F(y) W and spatial reuse starts.
+struct Reservoir {
+    int comp;         // -1 none, 0 sun, 1 emitter point, 2 sky direction
+    int li;           // emitter: light index in the scene's table
+    float3 y, lnrm;   // point or direction; emitter normal at the point
+    float3 F;         // unshadowed contribution at the vertex (solid-angle integrand)
+    float ph, wsum, cosL, dist2;
+    int M;
+};
+
+__device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParams& P, float3 p, float3 N, float3 color,
+                                           uint32_t key, int bounce, Reservoir& rs) {
     const int M = P.restir < BC_RIS_MAX ? P.restir : BC_RIS_MAX;
     const float3 sd = f3(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
     const float3 E = f3(P.sun[0], P.sun[1], P.sun[2]);
@@ -289,9 +301,10 @@ __device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParam
     const int nc = 1 + (sun_on ? 1 : 0) + (area_on ? 1 : 0);
     const float pc = 1.0f / (float)nc;
     const float3 f = mul(color, INVPI_F);
-    float wsum = 0.0f, ph_s = 0.0f;
-    float3 F_s = f3(0.0f, 0.0f, 0.0f), y_s = F_s;
-    bool have = false, point_s = false;
+    float wsum = 0.0f, ph_s = 0.0f, cosL_s = 0.0f, dist2_s = 0.0f;
+    float3 F_s = f3(0.0f, 0.0f, 0.0f), y_s = F_s, ln_s = F_s;
+    bool have = false;
+    int comp_s = -1, li_s = -1;
     for (int k = 0; k < M; k++) {
         const uint32_t base = BC_DIM_RIS0 + ((uint32_t)bounce * 64u + (uint32_t)k) * BC_RIS_DIMS;
         int ci = (int)(bc_rng(key, base) * (float)nc);
@@ -299,9 +312,9 @@ __device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParam
         int comp;  // 0 sun, 1 emitters, 2 sky
         if (sun_on) comp = ci == 0 ? 0 : ((area_on && ci == 1) ? 1 : 2);
         else comp = (area_on && ci == 0) ? 1 : 2;
-        float w = 0.0f, ph = 0.0f;
-        float3 F = f3(0.0f, 0.0f, 0.0f), y = F;
-        bool point = false;
+        float w = 0.0f, ph = 0.0f, cosL_c = 0.0f, dist2_c = 0.0f;
+        float3 F = f3(0.0f, 0.0f, 0.0f), y = F, ln_c = F;
+        int li_c = -1;
         if (comp == 0) {
             const float cs = dot(N, sd);
             if (cs > 0.0f) {
@@ -366,7 +379,10 @@ __device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParam
                         ph = lum(F);
                         w = ph / (pc * pin);
                         y = qpt;
-                        point = true;
+                        ln_c = lnrm;
+                        li_c = li;
+                        cosL_c = cosL;
+                        dist2_c = dist2;
                     }
                 }
             }
@@ -399,16 +415,66 @@ __device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParam
                 F_s = F;
                 ph_s = ph;
                 y_s = y;
-                point_s = point;
+                comp_s = comp;
+                li_s = li_c;
+                ln_s = ln_c;
+                cosL_s = cosL_c;
+                dist2_s = dist2_c;
             }
         }
     }
-    if (!have) return false;
-    const float W = wsum / ((float)M * ph_s);
-    contrib = mul(F_s, W);
-    y_out = y_s;
-    point_out = point_s;
+    rs.comp = have ? comp_s : -1;
+    rs.li = li_s;
+    rs.y = y_s;
+    rs.lnrm = ln_s;
+    rs.F = F_s;
+    rs.ph = ph_s;
+    rs.wsum = wsum;
+    rs.cosL = cosL_s;
+    rs.dist2 = dist2_s;
+    rs.M = M;
+}
+
+__device__ __forceinline__ bool ris_direct(const SceneBufs& S, const RenderParams& P, float3 p, float3 N,
+                                           float3 color, uint32_t key, int bounce, float3& contrib, float3& y_out,
+                                           bool& point_out) {
+    Reservoir rs;
+    ris_sample(S, P, p, N, color, key, bounce, rs);
+    if (rs.comp < 0) return false;
+    const float W = rs.wsum / ((float)rs.M * rs.ph);
+    contrib = mul(rs.F, W);
+    y_out = rs.y;
+    point_out = rs.comp == 1;
     return true;
+}
+
+// Spatial reuse of primary-vertex reservoirs (ReSTIR DI, Bitterli et al., SIGGRAPH 2020, the
+// biased combination with 1/M normalisation).  Samples live in the measure of their kind:
+// emitter points by area, sky directions by solid angle, the sun as a discrete direction; a
+// reservoir's W is stored in that measure (an emitter sample's solid-angle W times dist^2 /
+// cosL).  spatial_eval is the integrand of a sample at receiver (p, N, f) in the same measure.
+__device__ __forceinline__ float3 spatial_eval(const SceneBufs& S, const RenderParams& P, float3 p, float3 N, float3 f,
+                                               int comp, int li, float3 y, float3 lnrm) {
+    const float3 zero = f3(0.0f, 0.0f, 0.0f);
+    if (comp == 0) {
+        const float3 sd = f3(P.sun_dir[0], P.sun_dir[1], P.sun_dir[2]);
+        const float cs = dot(N, sd);
+        return cs > 0.0f ? mul(had(f, f3(P.sun[0], P.sun[1], P.sun[2])), cs) : zero;
+    }
+    if (comp == 2) {
+        const float cv = rmax(0.0f, dot(N, y));
+        return cv > 0.0f ? mul(had(f, sky_radiance(P, y)), cv) : zero;
+    }
+    const bc_light& Lr = S.lights[P.lbase + li];
+    const float3 to = sub(y, p);
+    const float dist2 = dot(to, to);
+    if (dist2 < (float)1e-8) return zero;
+    const float3 ld = unitv(to);
+    const float nd = rmax(0.0f, dot(N, ld));
+    const float cl = dot(mul(lnrm, -1.0f), ld);
+    const float cosL = Lr.kind == 0 ? rmax(0.0f, cl) : fabsf(cl);
+    if (nd <= 0.0f || cosL <= 0.0f) return zero;
+    return mul(had(f, f3(Lr.le[0], Lr.le[1], Lr.le[2])), (nd * cosL) / dist2);
 }
 
 // Adjoint state of the path-replay backward pass: per-pixel d loss / d image, the replayed
@@ -511,7 +577,23 @@ __device__ __forceinline__ bool shade_path(const SceneBufs& S, const PathBufs& B
             alive = false;
         } else if (M.kind == 0) {  // diffuse: NEE then cosine sampling
             if (ADJ) adj_color(A, mat, g, Lt, L, color);
-            if (!ADJ && P.restir > 0) {
+            if (!ADJ && P.restir > 0 && P.spatial > 0 && bounce == 0) {
+                // primary vertex: keep the reservoir; k_spatial combines it with its neighbours'
+                Reservoir rs;
+                ris_sample(S, P, p, N, color, key, bounce, rs);
+                float W = 0.0f;
+                if (rs.comp >= 0) {
+                    W = rs.wsum / ((float)rs.M * rs.ph);
+                    if (rs.comp == 1) W = W * (rs.dist2 / rs.cosL);
+                }
+                const float3 f = mul(color, INVPI_F);
+                const uint32_t code = (uint32_t)(rs.comp + 1) | ((uint32_t)(rs.li < 0 ? 0 : rs.li) << 2);
+                B.rv[0][j] = make_float4(p.x, p.y, p.z, t);
+                B.rv[1][j] = make_float4(N.x, N.y, N.z, __uint_as_float(code));
+                B.rv[2][j] = make_float4(f.x, f.y, f.z, W);
+                B.rv[3][j] = make_float4(rs.y.x, rs.y.y, rs.y.z, __int_as_float(rs.M));
+                B.rv[4][j] = make_float4(rs.lnrm.x, rs.lnrm.y, rs.lnrm.z, 0.0f);
+            } else if (!ADJ && P.restir > 0) {
                 float3 dl, y;
                 bool point;
                 if (ris_direct(S, P, p, N, color, key, bounce, dl, y, point)) {

@@ -131,6 +131,12 @@ typedef struct bc_render_desc {
     int32_t restir_candidates;
     float sun_dir[3];           /* direction towards the sun */
     float sun_irradiance[3];    /* rgb; 0 = no sun (needs restir_candidates > 0) */
+    /* ReSTIR DI spatial reuse at primary vertices (mode 0, single scene, needs restir_candidates
+     * > 0): each pixel combines its reservoir with up to restir_spatial (<= 16) neighbours'
+     * reservoirs of the same sample index, picked within restir_radius (1..64, 0 = 16) pixels
+     * among the rendered rows, before its shadow ray.  0 = off. */
+    int32_t restir_spatial;
+    int32_t restir_radius;
 } bc_render_desc;
 
 /* Replaces renderOpenCL's chunk loop (gpu_opencl_windows.go:565-723): renders rows

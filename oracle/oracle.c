@@ -1013,6 +1013,9 @@ void orc_camera_basis(const double* pos, const double* target, const double* up,
 #define DIM_RIS0 0x100000u
 #define RIS_DIMS 32u
 #define RIS_MAX 64
+/* spatial reuse: neighbour i draws dims DIM_SPATIAL0 + 4 i + o, o = 0 x offset, 1 y offset
+ * (integer hashes), 2 reservoir update (i = 0: the pixel's own reservoir) */
+#define DIM_SPATIAL0 0x200000u
 
 /* sun direction of a render description, normalised in double precision and rounded to float */
 static inline void orc_sun_dir(const orc_render_desc* d, float out[3]) {

@@ -113,6 +113,8 @@ typedef struct orc_render_desc {
     int32_t restir_candidates;  /* >0: resampled direct lighting over {sun, sky, emitters} */
     float   sun_dir[3];         /* unit direction towards the sun */
     float   sun_irradiance[3];
+    int32_t restir_spatial;     /* >0: spatial reuse at primary vertices over that many neighbours */
+    int32_t restir_radius;      /* pixel radius of the neighbours (0 = 16) */
 } orc_render_desc;
 
 /* Renders rows [y0, y0+rows).  rgba: rows*W*4 (rgb mean radiance, w = path segments),

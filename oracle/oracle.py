@@ -23,6 +23,7 @@ class RenderDesc(ctypes.Structure):
         ("firefly_clamp", c_float), ("seed", c_uint32), ("sky_color", c_float * 3),
         ("origin", c_double * 3), ("lower", c_double * 3), ("horiz", c_double * 3), ("vert", c_double * 3),
         ("restir_candidates", c_int32), ("sun_dir", c_float * 3), ("sun_irradiance", c_float * 3),
+        ("restir_spatial", c_int32), ("restir_radius", c_int32),
     ]
 
 

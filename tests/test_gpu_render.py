@@ -204,3 +204,44 @@ def test_restir_argument_errors(dev):
         dev.render(bc.render_desc(s, 16, 16, 1, 2, restir=65))       # at most 64 candidates
     with pytest.raises(bc.BeamcastError):
         dev.forward(bc.render_desc(s, 16, 16, 1, 2, restir=8))       # adjoint: one-sample NEE only
+
+
+@pytest.mark.parametrize("kind,param,sun,nee,k,radius", [(bc.CLOSED_BOX, 0, False, True, 8, 8),
+                                                          (bc.TERRAIN, 32, True, False, 4, 16),
+                                                          (bc.SHOWCASE, 0, True, False, 16, 3)])
+def test_spatial_reuse_parity(dev, kind, param, sun, nee, k, radius):
+    """ReSTIR DI spatial reuse at primary vertices matches the oracle per pixel: the same
+    neighbours, similarity tests, measure conversions and streaming choice in fp32."""
+    print(f"case: spatial reuse scene {kind} k {k} radius {radius}", flush=True)
+    s = bc.HostScene.workload(kind, param=param, bins=16, leaf=4)
+    extra = SUN if sun else {}
+    d = bc.render_desc(s, 48, 32, 4, 5, rr_depth=3, sampler=1, sky_mode=1, sky_color=(0.3, 0.3, 0.3), restir=8,
+                       spatial=k, radius=radius, **extra)
+    img, _, _ = parity(dev, s, d, nee=nee, bins=16)
+    assert img[..., :3].mean() > 0.02
+
+
+def test_spatial_reuse_row_chunks(dev):
+    """Neighbours come from the rendered rows: a chunk render equals the oracle's render of the
+    same rows."""
+    print("case: spatial reuse in a row chunk", flush=True)
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s, nee_triangles=True)
+    d = bc.render_desc(s, 40, 40, 3, 3, rr_depth=3, sampler=1, restir=8, spatial=8, radius=6)
+    img, _, rays = dev.render(d, y0=13, rows=9)
+    ref, _, rrays, _ = orc.Scene.from_host(s, nee_triangles=True).render(orc.desc_from(d), precision=32, y0=13, rows=9)
+    assert rays == rrays
+    err = np.abs(img[..., :3] - ref[..., :3])
+    assert not (err > 1e-4 * np.abs(ref[..., :3]) + 1e-6).any()
+
+
+def test_spatial_reuse_errors(dev):
+    print("case: spatial reuse argument errors", flush=True)
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    dev.upload(s, nee_triangles=True)
+    with pytest.raises(bc.BeamcastError, match="spatial"):
+        dev.render(bc.render_desc(s, 16, 16, 1, 2, spatial=4))                   # needs restir > 0
+    with pytest.raises(bc.BeamcastError, match="spatial"):
+        dev.render(bc.render_desc(s, 16, 16, 1, 2, restir=8, spatial=4, mode=1))  # plain schedule only
+    with pytest.raises(bc.BeamcastError):
+        dev.render(bc.render_desc(s, 16, 16, 1, 2, restir=8, spatial=17))

@@ -365,6 +365,50 @@ def test_ris_matches_one_sample_nee():
     assert np.all(np.abs(means[1] / means[0] - 1) < 0.02)
 
 
+def test_spatial_reuse_without_neighbours_is_ris():
+    """Spatial reuse whose neighbours all fall outside the rendered rows (a one-pixel-wide image
+    rendered row by row) keeps the pixel's own reservoir: the same image as resampled direct
+    lighting alone, although the emitter samples go through the area measure and back."""
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    o = orc.Scene.from_host(s, nee_triangles=True)
+    kw = dict(rr_depth=3, sampler=1, restir=16, seed=7)
+    for y in range(8):
+        a = o.render(orc.desc_from(bc.render_desc(s, 1, 8, 8, 3, **kw)), precision=64, y0=y, rows=1)[0]
+        b = o.render(orc.desc_from(bc.render_desc(s, 1, 8, 8, 3, spatial=4, radius=2, **kw)), precision=64, y0=y,
+                     rows=1)[0]
+        assert np.allclose(a, b, rtol=1e-9, atol=1e-12)
+
+
+def test_spatial_reuse_on_sunlit_plane():
+    """On a diffuse plane under a constant sky and a sun every pixel has the same expected
+    radiance albedo (Ls + E (n.s) / pi): reusing neighbours' reservoirs keeps the mean and cuts
+    the per-pixel spread."""
+    a, ls, e = 0.5, float(np.float32(0.3)), 3.0
+    sd = np.array([0.3, 0.8, 0.2]) / np.linalg.norm([0.3, 0.8, 0.2])
+    s = sunlit_plane_scene(a)
+    o = orc.Scene.from_host(s)
+    want = np.float32(a) * (ls + e * sd[1] / np.pi)
+    spread = {}
+    for sp in (0, 16):
+        d = orc.desc_from(bc.render_desc(s, 32, 32, 1, 1, sky_mode=1, sky_color=(ls, ls, ls), restir=2, sun_dir=sd,
+                                         sun=(e, e, e), spatial=sp, radius=8, seed=11))
+        img = o.render(d, precision=64)[0][..., :3]
+        assert abs(img.mean() / want - 1) < 0.02
+        spread[sp] = img.std() / img.mean()
+    assert spread[16] < 0.6 * spread[0]
+
+
+def test_spatial_reuse_mean_closed_box():
+    """The reuse (biased, 1/M normalisation) stays close to the unbiased estimate in the mean."""
+    s = bc.HostScene.workload(bc.CLOSED_BOX)
+    o = orc.Scene.from_host(s, nee_triangles=True)
+    means = []
+    for sp in (0, 8):
+        d = orc.desc_from(bc.render_desc(s, 40, 40, 64, 4, rr_depth=3, sampler=1, restir=8, seed=5, spatial=sp, radius=8))
+        means.append(o.render(d, precision=64)[0][..., :3].reshape(-1, 3).mean(axis=0))
+    assert np.all(np.abs(means[1] / means[0] - 1) < 0.03)
+
+
 # ---------------------------------------------------------------- post stack (oracle_post.c)
 
 def rgba_bytes(word):
