@@ -148,6 +148,11 @@ def bench_terrain(bc, dev, args, threads):
     post_ms = dev.timer_stop() / 5
     dev.free(d_px)
     dev.free(d_g)
+    # ReSTIR DI spatial reuse at the primary vertices (plain schedule): 4 neighbours within 16 px
+    dsp = bc.render_desc(s, w, h, spp, bounces, spatial=4, radius=16, **kw)
+    t0 = time.perf_counter()
+    ssegs, _ = dev.render_device(dsp, 0, h, d_rgba)
+    ssec = time.perf_counter() - t0
     dp = bc.render_desc(s, w, h, spp, bounces, mode=1, **kw)
     t0 = time.perf_counter()
     dev.render_device(dp, 0, h, d_rgba)  # autotunes and captures once
@@ -169,7 +174,9 @@ def bench_terrain(bc, dev, args, threads):
                           "over_plain": round(sec / psec, 3), "tuning": dev.tuning(),
                           "autotune_and_first_frame_s": round(tune, 2)},
            "post": {"ms_per_frame": round(post_ms, 3), "stages": "a-trous denoise 3 passes, FXAA 2, TXAA 1, toRGBA",
-                    "readback_bytes_per_pixel": {"with_device_post": 4, "without": 28}}}
+                    "readback_bytes_per_pixel": {"with_device_post": 4, "without": 28}},
+           "spatial_reuse": {"ms_per_frame": round(ssec * 1e3, 2), "neighbours": 4, "radius_px": 16,
+                             "over_plain_time": round(ssec / sec, 3)}}
     if not args.no_cpu:
         import oracle as orc
 
