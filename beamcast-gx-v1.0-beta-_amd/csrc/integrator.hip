@@ -28,6 +28,12 @@
 
 namespace bcd {
 
+// Resident waves per SIMD that a traversal kernel's register allocation is held to (7: at most
+// 72 VGPRs, 6: at most 80).  Traversal is latency-bound, so more resident waves hide more of the
+// node and primitive fetch latency; the persistent extension kernel, which also regenerates
+// paths in place, spills a few registers outside its traversal loop at 6 and is still faster.
+#define BC_WAVES(N) __attribute__((amdgpu_waves_per_eu(N, N)))
+
 constexpr uint32_t kIdle = 0xffffffffu;
 
 // Slots of segment `seg` (of `ss` slots each) among the first `total` slots.
@@ -76,9 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, f
 }
 
 // ------------------------------------------------------------------------------ extend
-template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg, uint32_t root) {
-    __shared__ uint2 stk[STACK * kBlock];
+__global__ __launch_bounds__(kBlock) BC_WAVES(7) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg, uint32_t root) {
+    extern __shared__ uint2 stk[];  // S.stack entries per lane
     const uint32_t* q = B.q[cur];
     for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.qc[cur][seg], base = seg * B.seg;
@@ -89,8 +94,8 @@ __global__ __launch_bounds__(kBlock) void k_extend(SceneBufs S, PathBufs B, int 
             r.ox = o.x; r.oy = o.y; r.oz = o.z;
             r.dx = d.x; r.dy = d.y; r.dz = d.z;
             r.tmin = 1e-4f;
-            const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, root, r, 1e30f, stk + threadIdx.x, kBlock,
-                                                    S.err);
+            const HitRec h = traverse<false>(S.nodes, S.prims, S.ids, root, r, 1e30f, stk + threadIdx.x, kBlock, S.stack,
+                                             S.err);
             B.hit[j] = make_float2(h.t, __uint_as_float(h.rec));
             B.nseg[j] += 1u;
         }
@@ -248,10 +253,10 @@ __global__ __launch_bounds__(kBlock) void k_spatial(SceneBufs S, PathBufs B, Ren
 // ------------------------------------------------------------------------------ shadow
 // Occluded(p + N*2e-4 -> q, 1e-4, dist - 1e-4) of visible() (renderer.go:1333-1341, 851-895).
 // MULTI: packed scenes, the root of each path's scene from its scene index.
-template <int STACK, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint32_t nseg, uint32_t root,
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) BC_WAVES(7) void k_shadow(SceneBufs S, PathBufs B, uint32_t nseg, uint32_t root,
                                                    const RenderParams* __restrict__ Ps) {
-    __shared__ uint2 stk[STACK * kBlock];
+    extern __shared__ uint2 stk[];  // S.stack entries per lane
     for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
         const uint32_t n = B.sqc[seg], base = seg * B.seg;
         if (threadIdx.x == 0) B.tally[kMaxGrid + seg] += n;
@@ -263,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_shadow(SceneBufs S, PathBufs B, uint
             r.dx = d.x; r.dy = d.y; r.dz = d.z;
             r.tmin = (float)1e-4;
             const uint32_t rt = MULTI ? Ps[B.sid[j]].root : root;
-            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, rt, r, o.w, stk + threadIdx.x, kBlock, S.err);
+            const HitRec h = traverse<true>(S.nodes, S.prims, S.ids, rt, r, o.w, stk + threadIdx.x, kBlock, S.stack, S.err);
             if (h.rec == 0xffffffffu) {
                 const float4 c = B.sc[j];
                 const float4 l = B.L[j];
@@ -403,11 +408,11 @@ __device__ __forceinline__ uint32_t scene_of(const uint32_t* __restrict__ gpre, 
 
 // MULTI: the window's samples are those of several packed scenes (Pd[s], s < nscene); each
 // restarted path records its scene and is traced from that scene's root.
-template <int STACK, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) BC_WAVES(6) void k_p_extend(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
                                                      int cur, uint32_t nseg, const uint32_t* __restrict__ gpre,
                                                      uint32_t nscene) {
-    __shared__ uint2 stk[STACK * kBlock];
+    extern __shared__ uint2 stk[];  // S.stack entries per lane
     const RenderParams P = *Pd;
     const uint32_t W = MULTI ? gpre[nscene] : (uint32_t)P.ns * (uint32_t)P.npix;
     uint32_t* q = B.q[cur];
@@ -459,8 +464,8 @@ __global__ __launch_bounds__(kBlock) void k_p_extend(SceneBufs S, PathBufs B, co
                 r.dx = d.x; r.dy = d.y; r.dz = d.z;
                 r.tmin = 1e-4f;
                 const uint32_t rt = MULTI ? Pd[B.sid[j]].root : P.root;
-                const HitRec h = traverse<false, STACK>(S.nodes, S.prims, S.ids, rt, r, 1e30f, stk + threadIdx.x, kBlock,
-                                                        S.err);
+                const HitRec h = traverse<false>(S.nodes, S.prims, S.ids, rt, r, 1e30f, stk + threadIdx.x, kBlock, S.stack,
+                                                 S.err);
                 B.hit[j] = make_float2(h.t, __uint_as_float(h.rec));
                 B.nseg[j] += 1u;
             }
@@ -679,10 +684,9 @@ __global__ __launch_bounds__(kBlock) void k_shade_adj(SceneBufs S, PathBufs B, R
     adj_lds_flush(lds, ng, grad, P.nmat, 0);
 }
 
-template <int STACK>
-__global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, RenderParams P, Adj A,
+__global__ __launch_bounds__(kBlock) BC_WAVES(6) void k_shadow_adj(SceneBufs S, PathBufs B, RenderParams P, Adj A,
                                                        float* __restrict__ grad, uint32_t nseg) {
-    __shared__ uint2 stk[STACK * kBlock];
+    extern __shared__ uint2 stk[];  // S.stack entries per lane
     __shared__ float lds[kAdjMaxMat * 3];
     const int ng = P.nmat * 3;
     adj_lds_init(lds, ng);
@@ -696,8 +700,8 @@ __global__ __launch_bounds__(kBlock) void k_shadow_adj(SceneBufs S, PathBufs B, 
             r.ox = o.x; r.oy = o.y; r.oz = o.z;
             r.dx = d.x; r.dy = d.y; r.dz = d.z;
             r.tmin = (float)1e-4;
-            const HitRec h = traverse<true, STACK>(S.nodes, S.prims, S.ids, P.root, r, o.w, stk + threadIdx.x, kBlock,
-                                                   S.err);
+            const HitRec h = traverse<true>(S.nodes, S.prims, S.ids, P.root, r, o.w, stk + threadIdx.x, kBlock, S.stack,
+                                            S.err);
             if (h.rec == 0xffffffffu) {
                 const float4 c = B.sc[j];
                 const float4 l = B.L[j];
@@ -878,31 +882,33 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
     return P;
 }
 
-template <int STACK>
+// Dynamic LDS of the traversal kernels: S.stack stack entries per lane.
+size_t stack_lds(const bcd::SceneBufs& S) { return (size_t)S.stack * bcd::kBlock * sizeof(uint2); }
+
 void launch_bounce(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur, int bounce,
                    uint32_t* guides, int grid, uint32_t nseg, hipStream_t st) {
-    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg, P.root);
+    hipLaunchKernelGGL(bcd::k_extend, dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, cur, nseg, P.root);
     hipLaunchKernelGGL(bcd::k_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, nseg);
     if (bounce == 0 && P.spatial > 0 && P.restir > 0)
         hipLaunchKernelGGL(bcd::k_spatial, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, nseg);
-    hipLaunchKernelGGL((bcd::k_shadow<STACK, false>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg, P.root,
+    hipLaunchKernelGGL((bcd::k_shadow<false>), dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, nseg, P.root,
                        (const bcd::RenderParams*)nullptr);
 }
 
 // One iteration of the persistent schedule; MULTI: the jobs are packed scenes of a batch.
-template <int STACK, bool MULTI>
+template <bool MULTI>
 void launch_iteration(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd,
                       const uint32_t* gpre, uint32_t njobs, int cur, uint32_t* guides, int grid, uint32_t nseg,
                       bool last, bool sort, hipStream_t st) {
     // the drain test reads the flag written by the last iteration of a replay only
     if (last) hipLaunchKernelGGL(bcd::k_p_clear_flag, dim3(1), dim3(64), 0, st, B);
-    hipLaunchKernelGGL((bcd::k_p_extend<STACK, MULTI>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, nseg,
+    hipLaunchKernelGGL((bcd::k_p_extend<MULTI>), dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, Pd, cur, nseg,
                        gpre, njobs);
     if (sort)
         hipLaunchKernelGGL((bcd::k_p_shade<MULTI, true>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
     else
         hipLaunchKernelGGL((bcd::k_p_shade<MULTI, false>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
-    hipLaunchKernelGGL((bcd::k_shadow<STACK, MULTI>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, nseg, 0u, Pd);
+    hipLaunchKernelGGL((bcd::k_shadow<MULTI>), dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, nseg, 0u, Pd);
 }
 
 bool clear_tally(bc_ctx* c) {
@@ -946,7 +952,8 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     if (!ensure_state(c, paths, npix, d_guides != nullptr, 1, plain_segment(c, paths), 1, reservoirs)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     R.B.xg = xcd_group(d);
-    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+                     c->stack};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr, scene);
     const uint32_t nseg = R.nseg;
     const int grid = seg_grid(c, nseg);
@@ -960,12 +967,7 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
         hipLaunchKernelGGL(bcd::k_raygen, dim3(grid), dim3(bcd::kBlock), 0, st, P, R.B, (float*)R.acc, guides, nseg);
         for (int b = 0; b < d->bounces; b++) {
             const int cur = b & 1;
-            switch (c->stack) {
-            case 8: launch_bounce<8>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
-            case 16: launch_bounce<16>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
-            case 24: launch_bounce<24>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
-            default: launch_bounce<32>(S, R.B, P, cur, b, guides, grid, nseg, st); break;
-            }
+            launch_bounce(S, R.B, P, cur, b, guides, grid, nseg, st);
         }
         hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, R.B, R.acc, keep);
         if (!bc_check(c, hipGetLastError(), "render launch")) return BC_ERR_DEVICE;
@@ -981,13 +983,12 @@ struct Job {
     uint32_t* d_guides;
 };
 
-template <int STACK>
 void capture_iterations(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams* Pd,
                         const uint32_t* gpre, uint32_t njobs, bool multi, uint32_t* guides, int grid, uint32_t nseg,
                         int iters, bool sort, hipStream_t st) {
     for (int it = 0; it < iters; it++) {
-        if (multi) launch_iteration<STACK, true>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, sort, st);
-        else launch_iteration<STACK, false>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, sort, st);
+        if (multi) launch_iteration<true>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, sort, st);
+        else launch_iteration<false>(S, B, Pd, gpre, njobs, it & 1, guides, grid, nseg, it == iters - 1, sort, st);
     }
 }
 
@@ -1017,7 +1018,8 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
     if (!ensure_state(c, slots, npix_all, guides_on, total_first, ss, (int)njobs)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     R.B.xg = xcd_group(jobs[0].d);
-    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+                     c->stack};
     std::vector<bcd::RenderParams> P(njobs);
     for (uint32_t k = 0; k < njobs; k++)
         P[k] = make_params(c, jobs[k].d, jobs[k].y0, jobs[k].rows, guides_on, jobs[k].scene);
@@ -1037,12 +1039,7 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
         if (R.exec) { (void)hipGraphExecDestroy(R.exec); R.exec = nullptr; }
         hipGraph_t graph = nullptr;
         if (!bc_check(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal), "begin capture")) return BC_ERR_DEVICE;
-        switch (c->stack) {
-        case 8: capture_iterations<8>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
-        case 16: capture_iterations<16>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
-        case 24: capture_iterations<24>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
-        default: capture_iterations<32>(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st); break;
-        }
+        capture_iterations(S, R.B, R.Pd, R.gpre, njobs, multi, guides, grid, nseg, iters, sort, st);
         if (!bc_check(c, hipStreamEndCapture(st, &graph), "end capture")) return BC_ERR_DEVICE;
         hipError_t e = hipGraphInstantiate(&R.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -1351,14 +1348,13 @@ extern "C" int bc_render(bc_ctx* c, const bc_render_desc* d, int y0, int rows, f
 }
 
 namespace {
-template <int STACK>
 void launch_bounce_adj(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur,
                        int bounce, uint32_t* guides, const bcd::Adj& A, float* grad, int grid, int grid_adj,
                        uint32_t nseg, hipStream_t st) {
-    hipLaunchKernelGGL((bcd::k_extend<STACK>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, cur, nseg, P.root);
+    hipLaunchKernelGGL(bcd::k_extend, dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, cur, nseg, P.root);
     hipLaunchKernelGGL(bcd::k_shade_adj, dim3(grid_adj), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, A, grad,
                        nseg);
-    hipLaunchKernelGGL((bcd::k_shadow_adj<STACK>), dim3(grid_adj), dim3(bcd::kBlock), 0, st, S, B, P, A, grad, nseg);
+    hipLaunchKernelGGL(bcd::k_shadow_adj, dim3(grid_adj), dim3(bcd::kBlock), stack_lds(S), st, S, B, P, A, grad, nseg);
 }
 bool same_shape(const bc_render_desc& a, const bc_render_desc& b) {
     return a.width == b.width && a.height == b.height && a.spp == b.spp && a.bounces == b.bounces &&
@@ -1427,7 +1423,8 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
     if (!bc_check(c, hipMemcpyAsync(d_dl, dl, npix * 12, hipMemcpyHostToDevice, st), "upload d loss") ||
         !bc_check(c, hipMemsetAsync(d_grad, 0, ngc * 4, st), "clear gradient"))
         return BC_ERR_DEVICE;
-    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err};
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+                     c->stack};
     bcd::RenderParams P = make_params(c, d, y0, rows, false);
     bcd::Adj A{};
     A.dl = d_dl;
@@ -1445,12 +1442,7 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
                            R.B.res, nseg);
         for (int b = 0; b < d->bounces; b++) {
             const int cur = b & 1;
-            switch (c->stack) {
-            case 8: launch_bounce_adj<8>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
-            case 16: launch_bounce_adj<16>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
-            case 24: launch_bounce_adj<24>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
-            default: launch_bounce_adj<32>(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st); break;
-            }
+            launch_bounce_adj(S, R.B, P, cur, b, R.guide_tmp, A, d_grad, grid, grid_adj, nseg, st);
         }
         if (!bc_check(c, hipGetLastError(), "adjoint launch")) return BC_ERR_DEVICE;
     }

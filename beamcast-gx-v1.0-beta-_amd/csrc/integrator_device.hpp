@@ -83,6 +83,7 @@ struct SceneBufs {
     const float* pdf;
     const int32_t* light_of;
     int* err;
+    int stack;  // traversal stack entries per lane (BVH8 depth - 1); kernels take that much LDS
 };
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }

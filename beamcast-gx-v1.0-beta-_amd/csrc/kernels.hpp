@@ -8,12 +8,16 @@
 namespace bcd {
 
 constexpr int kTraceBlock = 256;
+// Largest traversal stack (entries per lane): 32 x 256 lanes x 8 bytes = 64 KiB of LDS per block.
+constexpr int kMaxStack = 32;
 
-// Smallest compiled LDS stack depth that holds a BVH8 of the given depth, or -1.
+// LDS stack entries per lane that a BVH8 of the given depth needs (depth - 1, at least 1), or
+// -1 beyond kMaxStack.
 int stack_for_depth(int depth);
 
-// Ray queries: any_hit writes int32 occlusion flags, else bc_hit records.  *err is set
-// non-zero by the kernel if a traversal stack would overflow.
+// Ray queries: any_hit writes int32 occlusion flags, else bc_hit records.  stack: LDS stack
+// entries per lane (stack_for_depth).  *err is set non-zero by the kernel if a traversal stack
+// would overflow.
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids,
                         const bc_ray* rays, int64_t n, void* out, int refill, int* err, hipStream_t st);
 

@@ -9,8 +9,9 @@
 //   * a prim group  : (prim_base, 32-bit mask of primitives in hit leaves)
 // Groups that still hold children are pushed to a per-lane stack in LDS laid out
 // [depth][lane] so that a wave's pushes and pops are conflict-free 8-byte accesses.
-// A group consumes one stack entry per tree level, so the stack depth needed equals
-// the BVH8 depth, which the host checks before launch.
+// A group consumes one stack entry per tree level above the deepest, so a BVH8 of depth D
+// needs D - 1 entries per lane; kernels size their (dynamic) LDS stack to exactly that, so
+// shallow trees leave room for more resident waves.
 //
 // Exactness: only culling uses approximate arithmetic (quantised, outward-padded boxes,
 // hardware reciprocal).  Primitive tests are evaluated with the operation order of
@@ -212,10 +213,11 @@ __device__ __forceinline__ void trav_init(TravState& st, const Ray& r, float tma
 
 // One step; returns true when the traversal of this lane's ray is complete (st.best final).
 // ANY=true stops at the first primitive hit in [tmin, tmax].  stk points at this lane's column
-// of the LDS stack (entry i at stk[i * stride]).
-template <bool ANY, int STACK>
+// of the LDS stack (entry i at stk[i * stride], i < cap).
+template <bool ANY>
 __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restrict__ nodes, const bc_prim* __restrict__ prims,
-                                          const uint32_t* __restrict__ ids, uint2* stk, int stride, int* __restrict__ err) {
+                                          const uint32_t* __restrict__ ids, uint2* stk, int stride, int cap,
+                                          int* __restrict__ err) {
     const Ray& r = st.r;
     if (st.have_node) {
         visit_node(nodes, st.node, r, st.ix, st.iy, st.iz, st.oct, st.best.t, st.ng_base, st.ng_bits, st.pg_base, st.pg_mask);
@@ -274,7 +276,7 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
         const uint32_t imask = (st.ng_bits >> 8) & 0xffu;
         st.node = st.ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
         if (st.ng_bits & 0xffu) {
-            if (st.sp >= STACK) {  // host guarantees depth <= STACK; report instead of overflowing
+            if (st.sp >= cap) {  // the host sizes cap to the tree depth; report instead of overflowing
                 atomicOr(err, 1);
                 return true;
             }
@@ -304,13 +306,13 @@ __device__ __forceinline__ HitRec trav_result(const TravState& st, const uint32_
 }
 
 // Traverses the BVH8 for one ray to completion.
-template <bool ANY, int STACK>
+template <bool ANY>
 __device__ __forceinline__ HitRec traverse(const bc_node8* __restrict__ nodes, const bc_prim* __restrict__ prims,
                                            const uint32_t* __restrict__ ids, uint32_t root, const Ray& r, float tmax,
-                                           uint2* stk, int stride, int* __restrict__ err) {
+                                           uint2* stk, int stride, int cap, int* __restrict__ err) {
     TravState st;
     trav_init(st, r, tmax, root);
-    while (!trav_step<ANY, STACK>(st, nodes, prims, ids, stk, stride, err)) {
+    while (!trav_step<ANY>(st, nodes, prims, ids, stk, stride, cap, err)) {
     }
     return trav_result(st, ids, ANY);
 }

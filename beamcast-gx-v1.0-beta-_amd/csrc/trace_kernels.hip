@@ -7,13 +7,19 @@
 
 namespace bcd {
 
-template <bool ANY, int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_trace(const bc_node8* __restrict__ nodes,
-                                                       const bc_prim* __restrict__ prims,
-                                                       const uint32_t* __restrict__ ids,
-                                                       const bc_ray* __restrict__ rays, int64_t n,
-                                                       void* __restrict__ out, int* __restrict__ err) {
-    __shared__ uint2 stk[STACK * kTraceBlock];
+// Resident waves per SIMD the register allocation is held to (64 VGPRs for 8 waves, 72 for 7).
+// The any-hit refill loop fits 64 without spilling; the other loops fit 72 and would spill at 64.
+#define BC_TRACE_WAVES(N) __attribute__((amdgpu_waves_per_eu(N, N)))
+
+// The LDS traversal stack is dynamic: cap entries per lane, cap * kTraceBlock uint2 per block.
+template <bool ANY>
+__global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(7) void k_trace(const bc_node8* __restrict__ nodes,
+                                                                           const bc_prim* __restrict__ prims,
+                                                                           const uint32_t* __restrict__ ids,
+                                                                           const bc_ray* __restrict__ rays, int64_t n,
+                                                                           void* __restrict__ out, int cap,
+                                                                           int* __restrict__ err) {
+    extern __shared__ uint2 stk[];
     const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x, 64u) * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
@@ -21,7 +27,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(const bc_node8* __restric
     Ray r;
     r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tmin = a.w;
     r.dx = b.x; r.dy = b.y; r.dz = b.z;
-    const HitRec h = traverse<ANY, STACK>(nodes, prims, ids, 0u, r, b.w, stk + threadIdx.x, kTraceBlock, err);
+    const HitRec h = traverse<ANY>(nodes, prims, ids, 0u, r, b.w, stk + threadIdx.x, kTraceBlock, cap, err);
     if (ANY) {
         reinterpret_cast<int32_t*>(out)[i] = h.rec != 0xffffffffu ? 1 : 0;
     } else {
@@ -57,45 +63,48 @@ __device__ __forceinline__ void store_hit(void* __restrict__ out, int64_t i, con
 // waiting for its slowest ray.  Per-ray results are those of k_trace.
 constexpr int kRefillRays = 512;
 
-template <bool ANY, int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_refill(const bc_node8* __restrict__ nodes,
-                                                              const bc_prim* __restrict__ prims,
-                                                              const uint32_t* __restrict__ ids,
-                                                              const bc_ray* __restrict__ rays, int64_t n,
-                                                              void* __restrict__ out, int refill, int* __restrict__ err) {
-    __shared__ uint2 stk[STACK * kTraceBlock];
+template <bool ANY>
+__global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(ANY ? 8 : 7) void k_trace_refill(const bc_node8* __restrict__ nodes,
+                                                                                  const bc_prim* __restrict__ prims,
+                                                                                  const uint32_t* __restrict__ ids,
+                                                                                  const bc_ray* __restrict__ rays, int64_t n,
+                                                                                  void* __restrict__ out, int refill, int cap,
+                                                                                  int* __restrict__ err) {
+    extern __shared__ uint2 stk[];
     const int lane = (int)(threadIdx.x & 63u);
-    const int64_t wave = (int64_t)xcd_chunk(blockIdx.x, gridDim.x, 16u) * (kTraceBlock / 64) + (threadIdx.x >> 6);
-    const int64_t first = wave * kRefillRays;
-    const int64_t end = min(n, first + (int64_t)kRefillRays);
-    if (first >= end) return;
-    int64_t next = first + 64;  // wave-uniform: the next unassigned ray of the run
-    int64_t idx = first + lane;
-    bool active = idx < end;
+    // the run's bounds are wave-uniform (scalar registers); lanes hold 32-bit offsets into it
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int64_t first = ((int64_t)xcd_chunk(blockIdx.x, gridDim.x, 16u) * (kTraceBlock / 64) + wib) * kRefillRays;
+    if (first >= n) return;
+    const int cnt = (int)min(n - first, (int64_t)kRefillRays);
+    const bc_ray* run = rays + first;
+    int next = 64;  // wave-uniform: the next unassigned ray of the run
+    int off = lane;
+    bool active = off < cnt;
     TravState st;
     if (active) {
         Ray r;
         float tmax;
-        load_ray(rays, idx, r, tmax);
+        load_ray(run, off, r, tmax);
         trav_init(st, r, tmax, 0u);
     }
     uint2* my = stk + threadIdx.x;
     for (;;) {
-        if (active && trav_step<ANY, STACK>(st, nodes, prims, ids, my, kTraceBlock, err)) {
-            store_hit(out, idx, trav_result(st, ids, ANY), ANY);
+        if (active && trav_step<ANY>(st, nodes, prims, ids, my, kTraceBlock, cap, err)) {
+            store_hit(out, first + off, trav_result(st, ids, ANY), ANY);
             active = false;
         }
         const uint64_t idle = __ballot(!active);
         const int nidle = __popcll(idle);
-        if (nidle == 64 && next >= end) break;
-        if (nidle >= refill && next < end) {
+        if (nidle == 64 && next >= cnt) break;
+        if (nidle >= refill && next < cnt) {
             if (!active) {
-                const int64_t k = next + __popcll(idle & ((1ull << lane) - 1ull));
-                if (k < end) {
-                    idx = k;
+                const int k = next + __popcll(idle & ((1ull << lane) - 1ull));
+                if (k < cnt) {
+                    off = k;
                     Ray r;
                     float tmax;
-                    load_ray(rays, idx, r, tmax);
+                    load_ray(run, off, r, tmax);
                     trav_init(st, r, tmax, 0u);
                     active = true;
                 }
@@ -105,49 +114,36 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_refill(const bc_node8* __
     }
 }
 
-template <bool ANY, int STACK>
-static hipError_t launch_trace_t(const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays, int64_t n,
-                                 void* out, int refill, int* err, hipStream_t st) {
+template <bool ANY>
+static hipError_t launch_trace_t(int cap, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
+                                 int64_t n, void* out, int refill, int* err, hipStream_t st) {
+    const size_t lds = (size_t)cap * kTraceBlock * sizeof(uint2);
     if (refill > 0) {
         const int64_t per_block = (int64_t)kRefillRays * (kTraceBlock / 64);
         const int64_t blocks = (n + per_block - 1) / per_block;
-        hipLaunchKernelGGL((k_trace_refill<ANY, STACK>), dim3((unsigned)blocks), dim3(kTraceBlock), 0, st, nodes, prims, ids,
-                           rays, n, out, refill, err);
+        hipLaunchKernelGGL((k_trace_refill<ANY>), dim3((unsigned)blocks), dim3(kTraceBlock), lds, st, nodes, prims, ids, rays,
+                           n, out, refill, cap, err);
         return hipGetLastError();
     }
     const int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
-    hipLaunchKernelGGL((k_trace<ANY, STACK>), dim3((unsigned)blocks), dim3(kTraceBlock), 0, st, nodes, prims, ids, rays,
-                       n, out, err);
+    hipLaunchKernelGGL((k_trace<ANY>), dim3((unsigned)blocks), dim3(kTraceBlock), lds, st, nodes, prims, ids, rays, n, out,
+                       cap, err);
     return hipGetLastError();
-}
-
-template <bool ANY>
-static hipError_t launch_trace_any(int stack, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
-                                   int64_t n, void* out, int refill, int* err, hipStream_t st) {
-    switch (stack) {
-    case 8: return launch_trace_t<ANY, 8>(nodes, prims, ids, rays, n, out, refill, err, st);
-    case 16: return launch_trace_t<ANY, 16>(nodes, prims, ids, rays, n, out, refill, err, st);
-    case 24: return launch_trace_t<ANY, 24>(nodes, prims, ids, rays, n, out, refill, err, st);
-    case 32: return launch_trace_t<ANY, 32>(nodes, prims, ids, rays, n, out, refill, err, st);
-    default: return hipErrorInvalidValue;
-    }
 }
 
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
                         int64_t n, void* out, int refill, int* err, hipStream_t st) {
     if (n <= 0) return hipSuccess;
+    if (stack < 1 || stack > kMaxStack) return hipErrorInvalidValue;
     if (refill > 64) refill = 64;
-    return any_hit ? launch_trace_any<true>(stack, nodes, prims, ids, rays, n, out, refill, err, st)
-                   : launch_trace_any<false>(stack, nodes, prims, ids, rays, n, out, refill, err, st);
+    return any_hit ? launch_trace_t<true>(stack, nodes, prims, ids, rays, n, out, refill, err, st)
+                   : launch_trace_t<false>(stack, nodes, prims, ids, rays, n, out, refill, err, st);
 }
 
 int stack_for_depth(int depth) {
     const int need = depth - 1;
-    if (need <= 8) return 8;
-    if (need <= 16) return 16;
-    if (need <= 24) return 24;
-    if (need <= 32) return 32;
-    return -1;
+    if (need > kMaxStack) return -1;
+    return need < 1 ? 1 : need;
 }
 
 
