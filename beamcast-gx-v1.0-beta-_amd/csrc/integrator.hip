@@ -33,6 +33,12 @@ namespace bcd {
 // node and primitive fetch latency; the persistent extension kernel, which also regenerates
 // paths in place, spills a few registers outside its traversal loop at 6 and is still faster.
 #define BC_WAVES(N) __attribute__((amdgpu_waves_per_eu(N, N)))
+#ifndef BC_EXTEND_WAVES
+#define BC_EXTEND_WAVES 7
+#endif
+#ifndef BC_P_EXTEND_WAVES
+#define BC_P_EXTEND_WAVES 6
+#endif
 
 constexpr uint32_t kIdle = 0xffffffffu;
 
@@ -82,7 +88,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(RenderParams P, PathBufs B, f
 }
 
 // ------------------------------------------------------------------------------ extend
-__global__ __launch_bounds__(kBlock) BC_WAVES(7) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg, uint32_t root) {
+__global__ __launch_bounds__(kBlock) BC_WAVES(BC_EXTEND_WAVES) void k_extend(SceneBufs S, PathBufs B, int cur, uint32_t nseg, uint32_t root) {
     extern __shared__ uint2 stk[];  // S.stack entries per lane
     const uint32_t* q = B.q[cur];
     for (uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg); seg < nseg; seg += gridDim.x) {
@@ -409,7 +415,7 @@ __device__ __forceinline__ uint32_t scene_of(const uint32_t* __restrict__ gpre, 
 // MULTI: the window's samples are those of several packed scenes (Pd[s], s < nscene); each
 // restarted path records its scene and is traced from that scene's root.
 template <bool MULTI>
-__global__ __launch_bounds__(kBlock) BC_WAVES(6) void k_p_extend(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
+__global__ __launch_bounds__(kBlock) BC_WAVES(BC_P_EXTEND_WAVES) void k_p_extend(SceneBufs S, PathBufs B, const RenderParams* __restrict__ Pd,
                                                      int cur, uint32_t nseg, const uint32_t* __restrict__ gpre,
                                                      uint32_t nscene) {
     extern __shared__ uint2 stk[];  // S.stack entries per lane

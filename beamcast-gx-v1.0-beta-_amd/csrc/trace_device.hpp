@@ -182,8 +182,8 @@ constexpr int kPrimBatch = BC_PRIM_BATCH;
 // Traversal state of one lane: the ray, its reciprocal direction and octant, the best hit so
 // far, the node group being expanded and the primitive group being tested, and the depth of its
 // LDS stack.  trav_step advances it by one node visit, the primitive group that visit produced,
-// and one push or pop, so a wave can run it until some lanes finish and then give those lanes
-// new rays (see k_trace) while the others continue.
+// and the choice of the next node (a push, or pops), so a wave can run it until some lanes
+// finish and then give those lanes new rays (see k_trace_refill) while the others continue.
 struct TravState {
     Ray r;
     float ix, iy, iz, tmax;
@@ -269,33 +269,40 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
             }
         }
     }
-    if (st.ng_bits & 0xffu) {
-        const uint32_t k = (uint32_t)__builtin_ctz(st.ng_bits & 0xffu);
-        st.ng_bits &= ~(1u << k);
-        const uint32_t slot = k ^ st.oct;
-        const uint32_t imask = (st.ng_bits >> 8) & 0xffu;
-        st.node = st.ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
+    // next node: the nearest remaining child of the current group, else of the nearest pending
+    // group on the stack.  Any hit pops and descends in the same step, so that every step of
+    // every lane is one node visit and a wave's lanes fetch their nodes in the same iterations.
+    // Closest hit returns after a pop (its next step descends or pops again): descending in
+    // the same step measured slower for it (more registers live across the step).
+    for (;;) {
         if (st.ng_bits & 0xffu) {
-            if (st.sp >= cap) {  // the host sizes cap to the tree depth; report instead of overflowing
-                atomicOr(err, 1);
-                return true;
+            const uint32_t k = (uint32_t)__builtin_ctz(st.ng_bits & 0xffu);
+            st.ng_bits &= ~(1u << k);
+            const uint32_t slot = k ^ st.oct;
+            const uint32_t imask = (st.ng_bits >> 8) & 0xffu;
+            st.node = st.ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
+            if (st.ng_bits & 0xffu) {
+                if (st.sp >= cap) {  // the host sizes cap to the tree depth; report instead of overflowing
+                    atomicOr(err, 1);
+                    return true;
+                }
+                stk[st.sp * stride] = make_uint2(st.ng_base, st.ng_bits);
+                st.sp++;
             }
-            stk[st.sp * stride] = make_uint2(st.ng_base, st.ng_bits);
-            st.sp++;
+            st.have_node = true;
+            return false;
         }
-        st.have_node = true;
-        return false;
-    }
-    if (st.sp > 0) {
+        if (st.sp == 0) return true;
         st.sp--;
         const uint2 e = stk[st.sp * stride];
         st.ng_base = e.x;
         st.ng_bits = e.y;
-        // closest hit: a group whose nearest child starts beyond the current hit is done
-        if (!ANY && __uint_as_float(e.y & 0xffff0000u) > st.best.t) st.ng_bits = 0;
-        return false;
+        if (!ANY) {
+            // closest hit: a group whose nearest child starts beyond the current hit is done
+            if (__uint_as_float(e.y & 0xffff0000u) > st.best.t) st.ng_bits = 0;
+            return false;
+        }
     }
-    return true;
 }
 
 // The closest hit resolves its primitive index once, at the end.

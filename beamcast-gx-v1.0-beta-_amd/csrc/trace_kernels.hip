@@ -10,10 +10,13 @@ namespace bcd {
 // Resident waves per SIMD the register allocation is held to (64 VGPRs for 8 waves, 72 for 7).
 // The any-hit refill loop fits 64 without spilling; the other loops fit 72 and would spill at 64.
 #define BC_TRACE_WAVES(N) __attribute__((amdgpu_waves_per_eu(N, N)))
+#ifndef BC_CLOSEST_WAVES
+#define BC_CLOSEST_WAVES 7
+#endif
 
 // The LDS traversal stack is dynamic: cap entries per lane, cap * kTraceBlock uint2 per block.
 template <bool ANY>
-__global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(7) void k_trace(const bc_node8* __restrict__ nodes,
+__global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(BC_CLOSEST_WAVES) void k_trace(const bc_node8* __restrict__ nodes,
                                                                            const bc_prim* __restrict__ prims,
                                                                            const uint32_t* __restrict__ ids,
                                                                            const bc_ray* __restrict__ rays, int64_t n,
@@ -64,7 +67,7 @@ __device__ __forceinline__ void store_hit(void* __restrict__ out, int64_t i, con
 constexpr int kRefillRays = 512;
 
 template <bool ANY>
-__global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(ANY ? 8 : 7) void k_trace_refill(const bc_node8* __restrict__ nodes,
+__global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_WAVES) void k_trace_refill(const bc_node8* __restrict__ nodes,
                                                                                   const bc_prim* __restrict__ prims,
                                                                                   const uint32_t* __restrict__ ids,
                                                                                   const bc_ray* __restrict__ rays, int64_t n,
