@@ -7,7 +7,7 @@
 
 namespace bcd {
 
-constexpr int kTraceBlock = 256;
+constexpr int kTraceBlock = 64;  // threads per workgroup of the one-ray-per-thread query kernel
 // Largest traversal stack (entries per lane): 32 x 256 lanes x 8 bytes = 64 KiB of LDS per block.
 constexpr int kMaxStack = 32;
 

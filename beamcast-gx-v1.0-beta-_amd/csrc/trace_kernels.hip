@@ -10,11 +10,15 @@ namespace bcd {
 // Resident waves per SIMD the register allocation is held to (64 VGPRs for 8 waves, 72 for 7).
 // The any-hit refill loop fits 64 without spilling; the other loops fit 72 and would spill at 64.
 #define BC_TRACE_WAVES(N) __attribute__((amdgpu_waves_per_eu(N, N)))
+// Consecutive workgroups of k_trace that one XCD takes per turn (see xcd_chunk): 4096 rays.
+constexpr uint32_t kTraceXcdRun = 64;
 #ifndef BC_CLOSEST_WAVES
 #define BC_CLOSEST_WAVES 7
 #endif
 
-// The LDS traversal stack is dynamic: cap entries per lane, cap * kTraceBlock uint2 per block.
+// The LDS traversal stack is dynamic: cap entries per lane, cap * (threads per block) uint2 per
+// block.  One ray per thread, one wave per workgroup (kTraceBlock): a finished wave frees its
+// slot at once instead of waiting for the other waves of a larger workgroup.
 template <bool ANY>
 __global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(BC_CLOSEST_WAVES) void k_trace(const bc_node8* __restrict__ nodes,
                                                                            const bc_prim* __restrict__ prims,
@@ -23,7 +27,7 @@ __global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(BC_CLOSEST_WAVES) void 
                                                                            void* __restrict__ out, int cap,
                                                                            int* __restrict__ err) {
     extern __shared__ uint2 stk[];
-    const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x, 64u) * kTraceBlock + threadIdx.x;
+    const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x, kTraceXcdRun) * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const float4* rp = reinterpret_cast<const float4*>(rays + i);
     const float4 a = rp[0], b = rp[1];
@@ -65,9 +69,10 @@ __device__ __forceinline__ void store_hit(void* __restrict__ out, int64_t i, con
 // of its run (ballot + prefix count, no atomics), so a wave keeps most lanes busy instead of
 // waiting for its slowest ray.  Per-ray results are those of k_trace.
 constexpr int kRefillRays = 512;
+constexpr int kRefillBlock = 256;  // four waves, each with its own run
 
 template <bool ANY>
-__global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_WAVES) void k_trace_refill(const bc_node8* __restrict__ nodes,
+__global__ __launch_bounds__(kRefillBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_WAVES) void k_trace_refill(const bc_node8* __restrict__ nodes,
                                                                                   const bc_prim* __restrict__ prims,
                                                                                   const uint32_t* __restrict__ ids,
                                                                                   const bc_ray* __restrict__ rays, int64_t n,
@@ -77,7 +82,7 @@ __global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_WA
     const int lane = (int)(threadIdx.x & 63u);
     // the run's bounds are wave-uniform (scalar registers); lanes hold 32-bit offsets into it
     const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int64_t first = ((int64_t)xcd_chunk(blockIdx.x, gridDim.x, 16u) * (kTraceBlock / 64) + wib) * kRefillRays;
+    const int64_t first = ((int64_t)xcd_chunk(blockIdx.x, gridDim.x, 16u) * (kRefillBlock / 64) + wib) * kRefillRays;
     if (first >= n) return;
     const int cnt = (int)min(n - first, (int64_t)kRefillRays);
     const bc_ray* run = rays + first;
@@ -93,7 +98,7 @@ __global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_WA
     }
     uint2* my = stk + threadIdx.x;
     for (;;) {
-        if (active && trav_step<ANY>(st, nodes, prims, ids, my, kTraceBlock, cap, err)) {
+        if (active && trav_step<ANY>(st, nodes, prims, ids, my, kRefillBlock, cap, err)) {
             store_hit(out, first + off, trav_result(st, ids, ANY), ANY);
             active = false;
         }
@@ -120,14 +125,15 @@ __global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_WA
 template <bool ANY>
 static hipError_t launch_trace_t(int cap, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
                                  int64_t n, void* out, int refill, int* err, hipStream_t st) {
-    const size_t lds = (size_t)cap * kTraceBlock * sizeof(uint2);
     if (refill > 0) {
-        const int64_t per_block = (int64_t)kRefillRays * (kTraceBlock / 64);
+        const size_t lds = (size_t)cap * kRefillBlock * sizeof(uint2);
+        const int64_t per_block = (int64_t)kRefillRays * (kRefillBlock / 64);
         const int64_t blocks = (n + per_block - 1) / per_block;
-        hipLaunchKernelGGL((k_trace_refill<ANY>), dim3((unsigned)blocks), dim3(kTraceBlock), lds, st, nodes, prims, ids, rays,
+        hipLaunchKernelGGL((k_trace_refill<ANY>), dim3((unsigned)blocks), dim3(kRefillBlock), lds, st, nodes, prims, ids, rays,
                            n, out, refill, cap, err);
         return hipGetLastError();
     }
+    const size_t lds = (size_t)cap * kTraceBlock * sizeof(uint2);
     const int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
     hipLaunchKernelGGL((k_trace<ANY>), dim3((unsigned)blocks), dim3(kTraceBlock), lds, st, nodes, prims, ids, rays, n, out,
                        cap, err);
