@@ -96,6 +96,8 @@ def lib():
             "bc_scene_arrays": (c_int, [vp, vp, vp, vp]),
             "bc_set_flatten": (c_int, [vp, c_int]),
             "bc_set_trace_refill": (c_int, [vp, c_int, c_int]),
+            "bc_set_trace_order": (c_int, [vp, c_int, c_int]),
+            "bc_trace_order_last": (c_int, [vp, vp]),
             "bc_trace": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_trace_device": (c_int, [vp, vp, i64, c_int, vp]),
             "bc_alloc": (vp, [vp, c_uint64]),
@@ -390,6 +392,17 @@ class Device:
     def set_trace_refill(self, closest, any_hit):
         """Ray-query schedule: refill a wave's finished lanes once that many are idle (0: off)."""
         self._check(lib().bc_set_trace_refill(self.ctx, closest, any_hit))
+
+    def set_trace_order(self, mode, cell_bits=4):
+        """Ray binning before queries of 65,536 rays or more: 0 off (default), 1 always, 2 when the
+        batch is incoherent; cell_bits 2..4 = log2 of the origin cells per axis."""
+        self._check(lib().bc_set_trace_order(self.ctx, mode, cell_bits))
+
+    def trace_order_last(self):
+        """(binned, differing sampled pairs of 4096 or -1) for the last ray query."""
+        out = np.zeros(2, np.int32)
+        self._check(lib().bc_trace_order_last(self.ctx, _ptr(out)))
+        return int(out[0]), int(out[1])
 
     def set_flatten(self, mode):
         """BVH8 flattening of later uploads: 0 automatic, 1 host, 2 device (same bytes)."""

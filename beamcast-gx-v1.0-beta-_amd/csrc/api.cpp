@@ -20,6 +20,8 @@
 
 // Scenes of at least this many primitives are flattened on the device by default.
 constexpr int kDeviceFlattenMin = 1 << 16;
+// Ray batches of at least this many rays may be binned before a query (bc_set_trace_order).
+constexpr int64_t kOrderMinRays = 1 << 16;
 
 namespace {
 thread_local std::string g_create_error;
@@ -160,7 +162,7 @@ void bc_destroy(bc_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox, c->d_stage};
+                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox, c->d_stage, c->d_order, c->d_sorted};
     for (void* q : ptrs)
         if (q) hipFree(q);
     bc_render_release(c);
@@ -241,6 +243,24 @@ int bc_set_trace_refill(bc_ctx* c, int closest_lanes, int any_lanes) {
     if (!c || closest_lanes < 0 || closest_lanes > 64 || any_lanes < 0 || any_lanes > 64) return BC_ERR_ARG;
     c->trace_refill[0] = closest_lanes;
     c->trace_refill[1] = any_lanes;
+    return BC_OK;
+}
+
+int bc_set_trace_order(bc_ctx* c, int mode, int cell_bits) {
+    if (!c || mode < 0 || mode > 2 || cell_bits < bcd::kOrderMinBits || cell_bits > bcd::kOrderMaxBits) return BC_ERR_ARG;
+    c->order_mode = mode;
+    c->order_bits = cell_bits;
+    return BC_OK;
+}
+
+int bc_trace_order_last(bc_ctx* c, int* out2) {
+    if (!c || !out2) return BC_ERR_ARG;
+    out2[0] = 0;
+    out2[1] = -1;
+    if (!c->last_order_flag) return BC_OK;
+    if (!ok(c, hipMemcpyAsync(out2, c->last_order_flag, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream), "read order flag") ||
+        !ok(c, hipStreamSynchronize(c->stream), "read order flag"))
+        return BC_ERR_DEVICE;
     return BC_OK;
 }
 
@@ -477,8 +497,36 @@ int bc_scene_arrays(bc_ctx* c, void* nodes, void* prims, uint32_t* ids) {
 int bc_trace_device(bc_ctx* c, const void* d_rays, int64_t n, int any_hit, void* d_out) {
     if (!c || n < 0 || (n > 0 && (!d_rays || !d_out))) return BC_ERR_ARG;
     if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
-    hipError_t e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, c->d_ids, (const bc_ray*)d_rays, n, d_out,
-                                     c->trace_refill[any_hit != 0 ? 1 : 0], c->d_err, c->stream);
+    const bc_ray* rays = (const bc_ray*)d_rays;
+    const bc_ray* sorted = nullptr;
+    const uint32_t* perm = nullptr;
+    const int* oflag = nullptr;
+    c->last_order_flag = nullptr;
+    if (c->order_mode != 0 && n >= kOrderMinRays && n <= bcd::kOrderMaxRays) {
+        // one contiguous chunk of at least 16,384 rays per workgroup, at most one workgroup per CU
+        const int G = (int)std::min<int64_t>(std::min(c->num_cus, bcd::kOrderMaxGroups), (n + 16383) / 16384);
+        const size_t nb = bcd::order_bins(c->order_bits);
+        const size_t words = (size_t)G * nb + 2 * nb + (size_t)n + 2;
+        if (!bc_ensure(c, c->d_order, c->cap_order, words * sizeof(uint32_t), "hipMalloc(ray order)") ||
+            !bc_ensure(c, c->d_sorted, c->cap_sorted, (size_t)n * sizeof(bc_ray), "hipMalloc(sorted rays)"))
+            return BC_ERR_NOMEM;
+        uint32_t* w = (uint32_t*)c->d_order;
+        bcd::OrderScratch s;
+        s.cnt = w;
+        s.total = s.cnt + (size_t)G * nb;
+        s.bstart = s.total + nb;
+        s.perm = s.bstart + nb;
+        s.flag = (int*)(s.perm + n);
+        s.sorted = (bc_ray*)c->d_sorted;
+        if (!ok(c, bcd::launch_order(c->order_bits, c->order_mode, rays, n, c->d_nodes, G, s, c->stream), "ray order launch"))
+            return BC_ERR_DEVICE;
+        sorted = s.sorted;
+        perm = s.perm;
+        oflag = s.flag;
+        c->last_order_flag = s.flag;
+    }
+    hipError_t e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, c->d_ids, rays, n, d_out,
+                                     c->trace_refill[any_hit != 0 ? 1 : 0], c->d_err, c->stream, sorted, perm, oflag);
     return ok(c, e, "trace launch") ? BC_OK : BC_ERR_DEVICE;
 }
 

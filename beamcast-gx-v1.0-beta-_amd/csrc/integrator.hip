@@ -284,6 +284,73 @@ __global__ __launch_bounds__(kBlock) BC_WAVES(7) void k_shadow(SceneBufs S, Path
     }
 }
 
+// Shadow rays with lane refill.  Wave w of a workgroup takes the w-th quarter of the shadow
+// queue of each segment the workgroup owns (segments seg, seg + grid, ...), as one run: whenever
+// at least `refill` of its lanes are idle, it hands them the next rays of the run (ballot +
+// prefix count), moving on to the next segment when the current quarter is used up, so lanes do
+// not wait for the wave's slowest ray and the waves of a workgroup never wait for each other.
+// Each path has at most one shadow ray in the queue, so its radiance is updated by one lane, and
+// results equal k_shadow's.
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) BC_WAVES(8) void k_shadow_refill(SceneBufs S, PathBufs B, uint32_t nseg, uint32_t root,
+                                                                      const RenderParams* __restrict__ Ps) {
+    extern __shared__ uint2 stk[];  // S.stack entries per lane
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int refill = S.shadow_refill;
+    uint2* my = stk + threadIdx.x;
+    // wave-uniform cursor: segment, this wave's range [pos, end) of its queue
+    uint32_t seg = xcd_chunk(blockIdx.x, gridDim.x, B.xg), pos = 0, end = 0, base = 0;
+    bool started = false;
+    TravState st;
+    bool active = false;
+    uint32_t j = 0;
+    for (;;) {
+        if (active && trav_step<true>(st, S.nodes, S.prims, S.ids, my, kBlock, S.stack, S.err)) {
+            if (st.best.rec == 0xffffffffu) {
+                const float4 c = B.sc[j];
+                const float4 l = B.L[j];
+                B.L[j] = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.0f);
+            }
+            active = false;
+        }
+        while (pos >= end && seg < nseg) {
+            if (started) {
+                seg += gridDim.x;
+                if (seg >= nseg) break;
+            }
+            started = true;
+            const uint32_t n = B.sqc[seg];
+            if (threadIdx.x == 0) B.tally[kMaxGrid + seg] += n;
+            base = seg * B.seg;
+            pos = n * wib / kWaves;
+            end = n * (wib + 1u) / kWaves;
+        }
+        const uint64_t idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (pos >= end) {
+            if (nidle == 64) break;
+            continue;
+        }
+        if (nidle >= refill) {
+            if (!active) {
+                const uint32_t k = pos + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (k < end) {
+                    j = B.sq[base + k];
+                    const float4 o = B.so[j], d = B.sd[j];
+                    Ray r;
+                    r.ox = o.x; r.oy = o.y; r.oz = o.z;
+                    r.dx = d.x; r.dy = d.y; r.dz = d.z;
+                    r.tmin = (float)1e-4;
+                    trav_init(st, r, o.w, MULTI ? Ps[B.sid[j]].root : root);
+                    active = true;
+                }
+            }
+            pos = min(end, pos + (uint32_t)nidle);
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------ accumulate
 // Per-pixel sum in sample order with clampFirefly (renderer.go:1917-1926, 1951-1953).
 __device__ __forceinline__ float4 clamp_sample(const RenderParams& P, float4 l) {
@@ -891,14 +958,23 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
 // Dynamic LDS of the traversal kernels: S.stack stack entries per lane.
 size_t stack_lds(const bcd::SceneBufs& S) { return (size_t)S.stack * bcd::kBlock * sizeof(uint2); }
 
+// Shadow queue of every segment: lane refill unless S.shadow_refill is 0.
+template <bool MULTI>
+void launch_shadow(const bcd::SceneBufs& S, const bcd::PathBufs& B, int grid, uint32_t nseg, uint32_t root,
+                   const bcd::RenderParams* Pd, hipStream_t st) {
+    if (S.shadow_refill > 0)
+        hipLaunchKernelGGL((bcd::k_shadow_refill<MULTI>), dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, nseg, root, Pd);
+    else
+        hipLaunchKernelGGL((bcd::k_shadow<MULTI>), dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, nseg, root, Pd);
+}
+
 void launch_bounce(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd::RenderParams& P, int cur, int bounce,
                    uint32_t* guides, int grid, uint32_t nseg, hipStream_t st) {
     hipLaunchKernelGGL(bcd::k_extend, dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, cur, nseg, P.root);
     hipLaunchKernelGGL(bcd::k_shade, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, cur, bounce, guides, nseg);
     if (bounce == 0 && P.spatial > 0 && P.restir > 0)
         hipLaunchKernelGGL(bcd::k_spatial, dim3(grid), dim3(bcd::kBlock), 0, st, S, B, P, nseg);
-    hipLaunchKernelGGL((bcd::k_shadow<false>), dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, nseg, P.root,
-                       (const bcd::RenderParams*)nullptr);
+    launch_shadow<false>(S, B, grid, nseg, P.root, nullptr, st);
 }
 
 // One iteration of the persistent schedule; MULTI: the jobs are packed scenes of a batch.
@@ -914,7 +990,7 @@ void launch_iteration(const bcd::SceneBufs& S, const bcd::PathBufs& B, const bcd
         hipLaunchKernelGGL((bcd::k_p_shade<MULTI, true>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
     else
         hipLaunchKernelGGL((bcd::k_p_shade<MULTI, false>), dim3(grid), dim3(bcd::kBlock), 0, st, S, B, Pd, cur, guides, nseg);
-    hipLaunchKernelGGL((bcd::k_shadow<MULTI>), dim3(grid), dim3(bcd::kBlock), stack_lds(S), st, S, B, nseg, 0u, Pd);
+    launch_shadow<MULTI>(S, B, grid, nseg, 0u, Pd, st);
 }
 
 bool clear_tally(bc_ctx* c) {
@@ -959,7 +1035,7 @@ int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_
     RenderState& R = *c->rs;
     R.B.xg = xcd_group(d);
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                     c->stack};
+                     c->stack, c->trace_refill[1]};
     bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr, scene);
     const uint32_t nseg = R.nseg;
     const int grid = seg_grid(c, nseg);
@@ -1025,7 +1101,7 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
     RenderState& R = *c->rs;
     R.B.xg = xcd_group(jobs[0].d);
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                     c->stack};
+                     c->stack, c->trace_refill[1]};
     std::vector<bcd::RenderParams> P(njobs);
     for (uint32_t k = 0; k < njobs; k++)
         P[k] = make_params(c, jobs[k].d, jobs[k].y0, jobs[k].rows, guides_on, jobs[k].scene);
@@ -1430,7 +1506,7 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
         !bc_check(c, hipMemsetAsync(d_grad, 0, ngc * 4, st), "clear gradient"))
         return BC_ERR_DEVICE;
     bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                     c->stack};
+                     c->stack, c->trace_refill[1]};
     bcd::RenderParams P = make_params(c, d, y0, rows, false);
     bcd::Adj A{};
     A.dl = d_dl;

@@ -84,6 +84,7 @@ struct SceneBufs {
     const int32_t* light_of;
     int* err;
     int stack;  // traversal stack entries per lane (BVH8 depth - 1); kernels take that much LDS
+    int shadow_refill;  // shadow rays: refill a wave's lanes once this many are idle (0: one ray per thread)
 };
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }

@@ -15,11 +15,35 @@ constexpr int kMaxStack = 32;
 // -1 beyond kMaxStack.
 int stack_for_depth(int depth);
 
+// Spatial binning of a ray batch (order.hip): bins = (Morton cell of the origin on a
+// 2^bits-per-axis grid over the root box) x (direction octant).  cnt holds G x bins counts,
+// total and bstart one entry per bin, sorted and perm one entry per ray; flag[0] is the decision
+// (sort or not), flag[1] the probe's count of sampled adjacent pairs in different bins.
+struct OrderScratch {
+    uint32_t* cnt;
+    uint32_t* total;
+    uint32_t* bstart;
+    bc_ray* sorted;
+    uint32_t* perm;
+    int* flag;
+};
+constexpr int kOrderMinBits = 2;
+constexpr int kOrderMaxBits = 4;
+constexpr int kOrderMaxGroups = 1024;
+constexpr int64_t kOrderMaxRays = (int64_t)1 << 31;
+size_t order_bins(int bits);
+// mode 1: always sort; 2: sort when at least half of the sampled adjacent pairs differ in bin.
+// G workgroups count and scatter one contiguous chunk each.
+hipError_t launch_order(int bits, int mode, const bc_ray* rays, int64_t n, const bc_node8* nodes, int G,
+                        const OrderScratch& s, hipStream_t st);
+
 // Ray queries: any_hit writes int32 occlusion flags, else bc_hit records.  stack: LDS stack
 // entries per lane (stack_for_depth).  *err is set non-zero by the kernel if a traversal stack
-// would overflow.
+// would overflow.  With oflag non-null and oflag[0] set (by launch_order), the kernels read
+// the rays from `sorted` and write the result of sorted ray j to out[perm[j]].
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids,
-                        const bc_ray* rays, int64_t n, void* out, int refill, int* err, hipStream_t st);
+                        const bc_ray* rays, int64_t n, void* out, int refill, int* err, hipStream_t st,
+                        const bc_ray* sorted = nullptr, const uint32_t* perm = nullptr, const int* oflag = nullptr);
 
 // Refit: records (and materials) of the moved primitives in leaf order from the view's arrays.
 hipError_t launch_refit_records(const double* geom, const int32_t* kind, const int32_t* mat, const uint32_t* ids, uint32_t n,

@@ -430,6 +430,14 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--batch-spp", type=int, default=8, help="samples per pixel of the batch leg (Ultra: 256)")
     ap.add_argument("--batch-no-warmup", action="store_true", help="time the plain batch render without a warm-up")
+    ap.add_argument("--any-refill", type=int, default=8,
+                    help="any-hit queries and integrator shadow rays: refill a wave's lanes once this many are idle "
+                         "(0: one ray per thread)")
+    ap.add_argument("--closest-refill", type=int, default=0,
+                    help="closest-hit queries: refill a wave's lanes once this many are idle (0: one ray per thread)")
+    ap.add_argument("--trace-order", type=int, default=0, choices=(0, 1, 2),
+                    help="ray binning before queries: 0 off, 1 always, 2 when the batch is incoherent")
+    ap.add_argument("--order-bits", type=int, default=4, choices=(2, 3, 4), help="log2 of origin cells per axis")
     ap.add_argument("--copy-check", action="store_true",
                     help="also run one 1 GiB device-to-device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -443,6 +451,7 @@ def main():
     threads = min(16, host_cores())
     if args.only:
         dev = bc.Device(0)
+        dev.set_trace_refill(args.closest_refill, args.any_refill)
         legs = {"render": lambda: bench_render(bc, dev, args, threads), "terrain": lambda: bench_terrain(bc, dev, args, threads),
                 "batch": lambda: bench_batch(bc, dev, args, threads), "adjoint": lambda: bench_adjoint(bc, dev),
                 "refit": lambda: bench_refit(bc, dev, args), "build": lambda: bench_build(bc, dev, args),
@@ -455,6 +464,8 @@ def main():
     scene = bc.HostScene.workload(bc.SOUP, seed=0x5EED, param=args.tris, bins=32, leaf=4)
     log(f"[bench] host SAH build of {args.tris} triangles: {time.time() - t0:.2f}s")
     dev = bc.Device(0)
+    dev.set_trace_order(args.trace_order, args.order_bits)
+    dev.set_trace_refill(args.closest_refill, args.any_refill)
     t0 = time.time()
     dev.upload(scene)
     st = dev.stats()
@@ -500,13 +511,14 @@ def main():
     log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} Mrays/s")
 
     # per-kernel durations with stream-ordered HIP events, back-to-back launches
-    kernel_ms = {}
+    kernel_ms, binned = {}, {}
     reps = max(3, args.steps // 2)
     for k in launches:
         dev.timer_start()
         for _ in range(reps):
             dev.trace_device(d_rays[k], n, k[1], d_out[k])
         kernel_ms[k] = dev.timer_stop() / reps
+        binned[k] = dev.trace_order_last()[0]
     # results of the last launch, for a sanity summary
     hits = {}
     for k in launches:
@@ -601,7 +613,8 @@ def main():
         "build": build,
         "ingest": ingest,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
-                               "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4)}
+                               "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4),
+                               "binned": binned[k]}
                     for k in launches},
     }
     for k in launches:

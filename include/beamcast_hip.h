@@ -91,6 +91,19 @@ int bc_scene_refit(bc_ctx* ctx, const bc_scene_view* scene, uint64_t geom_rev);
  * to its finished lanes once `lanes` of its 64 are idle; 0 runs one ray per thread to completion.
  * Defaults 0 (closest) and 8 (any hit).  Results do not depend on it. */
 int bc_set_trace_refill(bc_ctx* ctx, int closest_lanes, int any_lanes);
+/* Ray binning of bc_trace / bc_trace_device (no counterpart in the reference): batches of 65,536
+ * rays or more can be counting-sorted on the device by (Morton cell of the origin on a grid of
+ * 2^cell_bits cells per axis over the scene box, direction octant) before traversal, so that a
+ * wave's rays start in one cell and share their upper nodes; results are written back to each
+ * ray's own index.  mode 0 off (default), 1 always, 2 when at least half of 4,096 sampled
+ * adjacent ray pairs of the batch fall into different bins.  cell_bits 2..4 (default 4).  Results
+ * do not depend on it; it keeps 36 bytes per ray of scratch. */
+int bc_set_trace_order(bc_ctx* ctx, int mode, int cell_bits);
+/* Decision of the last bc_trace / bc_trace_device call (synchronises the stream): out2[0] = 1 if
+ * its rays were binned, out2[1] = sampled adjacent pairs (of 4,096) in different bins, or -1 when
+ * the call ran no binning pass (mode 0 or fewer than 65,536 rays).  No counterpart in the
+ * reference. */
+int bc_trace_order_last(bc_ctx* ctx, int* out2);
 /* Where bc_scene_upload flattens the binary tree into BVH8 nodes (no counterpart in the
  * reference): 0 automatic (the device for scenes of 65,536 primitives or more), 1 the host, 2 the
  * device.  Both produce the same bytes; the device path uploads the view's arrays instead. */

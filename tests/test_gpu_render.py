@@ -170,6 +170,26 @@ def test_persistent_schedule_equals_plain(dev, pool, iters, segment, msort):
 SUN = dict(sun_dir=(0.3, 0.8, 0.2), sun=(3.0, 3.0, 3.0))
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_shadow_refill_equals_one_ray_per_thread(dev, mode):
+    """Shadow rays with lane refill (a wave hands its idle lanes the next shadow rays of its runs,
+    across segments) give the image of one shadow ray per thread, bit for bit, in both schedules."""
+    print(f"case: shadow refill, schedule {mode}", flush=True)
+    s = bc.HostScene.workload(bc.TERRAIN, param=40, bins=16, leaf=4)
+    dev.upload(s)
+    kw = dict(rr_depth=3, sampler=1, sky_mode=1, sky_color=(0.3, 0.3, 0.3), restir=8, mode=mode, **SUN)
+    if mode:
+        kw.update(pool=6000, iters=4)
+    imgs = []
+    for lanes in (0, 8, 64, 1):
+        dev.set_trace_refill(0, lanes)
+        img, _, rays = dev.render(bc.render_desc(s, 72, 40, 5, 6, **kw))
+        imgs.append((img, rays))
+    dev.set_trace_refill(0, 8)
+    for img, rays in imgs[1:]:
+        assert rays == imgs[0][1] and np.array_equal(img, imgs[0][0])
+
+
 @pytest.mark.parametrize("kind,param,sun,nee", [(bc.CLOSED_BOX, 0, False, True), (bc.TERRAIN, 48, True, False),
                                                 (bc.SHOWCASE, 0, True, False)])
 def test_restir_parity(dev, kind, param, sun, nee):

@@ -177,3 +177,69 @@ def test_refill_schedules(dev, n):
         assert np.array_equal(got["prim"], prim) and np.array_equal(got["t"], t), (closest, anyh)
         assert np.array_equal(dev.trace(ra, any_hit=True), occ), (closest, anyh)
     dev.set_trace_refill(0, 8)
+
+
+@pytest.mark.parametrize("bits", [2, 3, 4])
+def test_ray_order(dev, bits):
+    """Binned queries (the batch counting-sorted by origin cell and direction octant, each result
+    written back to its ray's index) return the results of the unbinned batch, for closest and
+    any hit, with and without lane refill; origins outside the scene box and NaN origins fall into
+    border bins."""
+    print(f"case: ray order, cell bits {bits}", flush=True)
+    s = bc.HostScene.workload(bc.SOUP, param=100_000)
+    dev.upload(s)
+    n = 65536 + 4099
+    rng = np.random.default_rng(bits)
+    rays = bc.workload_rays(1, 512, count=n)
+    rays[:64, :3] = rng.uniform(-4.0, 5.0, (64, 3)).astype(np.float32)
+    rays[64:72, :3] = np.array([0.0, 0.0, 0.0], np.float32)  # a corner of the scene box
+    rays[72:80, 0] = np.nan
+    ra = rays.copy()
+    ra[:, 7] = rng.random(n).astype(np.float32) * 0.5
+    ok = slice(80, None)  # the oracle comparison leaves the NaN rows out
+    o = orc.Scene.from_host(s)
+    prim, t = o.hit32(rays[ok])
+    occ = o.any32(ra[ok])
+    base = None
+    for mode in (0, 1, 2):
+        dev.set_trace_order(mode, bits)
+        for closest, anyh in ((0, 8), (0, 0), (8, 8)):
+            dev.set_trace_refill(closest, anyh)
+            got = dev.trace(rays)
+            binned = dev.trace_order_last()
+            assert binned[0] == (1 if mode else 0), (mode, binned)
+            g_any = dev.trace(ra, any_hit=True)
+            assert np.array_equal(got["prim"][ok], prim) and np.array_equal(got["t"][ok], t), (mode, closest)
+            assert np.array_equal(g_any[ok], occ), (mode, anyh)
+            if base is None:
+                base = (got.copy(), g_any.copy())
+            assert np.array_equal(got["prim"], base[0]["prim"]) and np.array_equal(got["t"].view(np.int32),
+                                                                                    base[0]["t"].view(np.int32))
+            assert np.array_equal(g_any, base[1])
+    dev.set_trace_refill(0, 8)
+    dev.set_trace_order(0, 4)
+
+
+def test_ray_order_decision(dev):
+    """The automatic mode bins an incoherent batch and leaves a coherent one (pinhole primaries)
+    in the caller's order; batches below 65,536 rays run no binning pass."""
+    print("case: ray order decision", flush=True)
+    s = bc.HostScene.workload(bc.SOUP, param=100_000)
+    dev.upload(s)
+    dev.set_trace_order(2, 4)
+    dev.trace(bc.workload_rays(1, 512, count=1 << 17))
+    b, diff = dev.trace_order_last()
+    assert b == 1 and diff > 3000
+    dev.trace(bc.workload_rays(0, 512, count=1 << 17))
+    b, diff = dev.trace_order_last()
+    assert b == 0 and 0 <= diff < 200
+    dev.trace(bc.workload_rays(1, 512, count=1000))
+    assert dev.trace_order_last() == (0, -1)
+    dev.set_trace_order(0, 4)
+    dev.trace(bc.workload_rays(1, 512, count=1 << 17))
+    assert dev.trace_order_last() == (0, -1)
+    with pytest.raises(bc.BeamcastError):
+        dev.set_trace_order(3, 4)
+    with pytest.raises(bc.BeamcastError):
+        dev.set_trace_order(1, 5)
+    dev.set_trace_order(0, 4)
