@@ -9,7 +9,8 @@
 //
 // Build with: go build -tags beamcast_hip (CGO_LDFLAGS=-L<dir> -lbeamcast_hip, and the
 // include/ directory of this repository on CGO_CFLAGS).  The stub must then carry the
-// matching build tag "!windows && !beamcast_hip".
+// matching build tag "!windows && !beamcast_hip".  diff_hip_linux.go (same tag) adds the
+// differentiable render on the same context.
 package main
 
 /*
@@ -101,12 +102,7 @@ func (rt *hipRuntime) ensureSceneLocked(scene *Scene) error {
 	}
 	mkind := unsafe.Slice((*C.int32_t)(rt.cAlloc(4*nm+4)), nm)
 	mdata := unsafe.Slice((*C.double)(rt.cAlloc(8*8*nm+8)), 8*nm)
-	for i, m := range scene.Materials {
-		mkind[i] = C.int32_t(m.Kind)
-		d := mdata[8*i : 8*i+8]
-		d[0], d[1], d[2], d[3], d[4] = C.double(m.Color.X), C.double(m.Color.Y), C.double(m.Color.Z), C.double(m.Roughness), C.double(m.IOR)
-		d[5], d[6], d[7] = C.double(m.Emission.X), C.double(m.Emission.Y), C.double(m.Emission.Z)
-	}
+	fillMaterials(mkind, mdata, scene.Materials)
 	view := (*C.bc_scene_view)(rt.cAlloc(int(unsafe.Sizeof(C.bc_scene_view{}))))
 	*view = C.bc_scene_view{
 		nnode: C.int32_t(nn), node_box: &box[0], node_link: &link[0],
@@ -127,28 +123,37 @@ func (rt *hipRuntime) ensureSceneLocked(scene *Scene) error {
 	return nil
 }
 
-func renderOpenCL(scene *Scene, cam Camera, settings RenderSettings, cancel *atomic.Bool,
-	onChunk func(y0, rows int, data []Vec3, progress float64)) ([]Vec3, []Guide, int64, string, error) {
-	if scene == nil || len(scene.Primitives) == 0 || len(scene.Nodes) == 0 {
-		return nil, nil, 0, "", errors.New("HIP backend requires a non-empty built scene")
+// fillMaterials packs Material records as bc_scene_view's mat_kind / mat_data (8 doubles each:
+// Color, Roughness, IOR, Emission), the material part of uploadFullSceneLocked
+// (gpu_opencl_windows.go:384-396).
+func fillMaterials(mkind []C.int32_t, mdata []C.double, mats []Material) {
+	for i, m := range mats {
+		mkind[i] = C.int32_t(m.Kind)
+		d := mdata[8*i : 8*i+8]
+		d[0], d[1], d[2], d[3], d[4] = C.double(m.Color.X), C.double(m.Color.Y), C.double(m.Color.Z), C.double(m.Roughness), C.double(m.IOR)
+		d[5], d[6], d[7] = C.double(m.Emission.X), C.double(m.Emission.Y), C.double(m.Emission.Z)
 	}
-	rt := &persistentHIP
+}
+
+// lockedContext takes the runtime lock and creates the context on first use; the caller unlocks.
+func (rt *hipRuntime) lockedContext() error {
 	rt.mu.Lock()
-	defer rt.mu.Unlock()
 	if rt.ctx == nil {
 		rt.ctx = C.bc_create(0)
 		if rt.ctx == nil {
-			return nil, nil, 0, "", hipErr(nil)
+			rt.mu.Unlock()
+			return hipErr(nil)
 		}
 	}
-	device := C.GoString(C.bc_device_name(rt.ctx))
-	if err := rt.ensureSceneLocked(scene); err != nil {
-		return nil, nil, 0, device, err
-	}
-	w, h := internalSize(settings)
-	needGuides := settings.Denoise || settings.HBAO > 0 || settings.HBAOPlus > 0 || temporalWeight(settings) > 0 || settings.DebugView != DebugBeauty
-	origin, lower, horiz, vert := cam.basis(float64(w) / float64(h))
-	var d C.bc_render_desc
+	return nil
+}
+
+// hipRenderDesc fills the render description of one frame from the camera and settings, as
+// renderOpenCL's kernel arguments are filled (gpu_opencl_windows.go:580-611), and returns the
+// internal image size and the camera basis.
+func hipRenderDesc(cam Camera, settings RenderSettings) (d C.bc_render_desc, w, h int, origin, lower, horiz, vert Vec3) {
+	w, h = internalSize(settings)
+	origin, lower, horiz, vert = cam.basis(float64(w) / float64(h))
 	d.width, d.height = C.int32_t(w), C.int32_t(h)
 	d.spp, d.bounces, d.rr_depth = C.int32_t(effectiveSPP(&settings)), C.int32_t(settings.Bounces), C.int32_t(settings.RRDepth)
 	switch settings.Sampler {
@@ -169,6 +174,25 @@ func renderOpenCL(scene *Scene, cam Camera, settings RenderSettings, cancel *ato
 		arr := [4]*[3]C.double{&d.origin, &d.lower, &d.horiz, &d.vert}[k]
 		arr[0], arr[1], arr[2] = C.double(v.X), C.double(v.Y), C.double(v.Z)
 	}
+	return
+}
+
+func renderOpenCL(scene *Scene, cam Camera, settings RenderSettings, cancel *atomic.Bool,
+	onChunk func(y0, rows int, data []Vec3, progress float64)) ([]Vec3, []Guide, int64, string, error) {
+	if scene == nil || len(scene.Primitives) == 0 || len(scene.Nodes) == 0 {
+		return nil, nil, 0, "", errors.New("HIP backend requires a non-empty built scene")
+	}
+	rt := &persistentHIP
+	if err := rt.lockedContext(); err != nil {
+		return nil, nil, 0, "", err
+	}
+	defer rt.mu.Unlock()
+	device := C.GoString(C.bc_device_name(rt.ctx))
+	if err := rt.ensureSceneLocked(scene); err != nil {
+		return nil, nil, 0, device, err
+	}
+	d, w, h, origin, lower, horiz, vert := hipRenderDesc(cam, settings)
+	needGuides := settings.Denoise || settings.HBAO > 0 || settings.HBAOPlus > 0 || temporalWeight(settings) > 0 || settings.DebugView != DebugBeauty
 	chunkRows := 64
 	if !settings.ProgressiveUpdates {
 		chunkRows = h
