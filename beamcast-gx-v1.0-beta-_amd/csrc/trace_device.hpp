@@ -130,7 +130,7 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
     const uint32_t ny0 = ny ? hiy0 : loy0, ny1 = ny ? hiy1 : loy1, fy0 = ny ? loy0 : hiy0, fy1 = ny ? loy1 : hiy1;
     const uint32_t nz0 = nz ? hiz0 : loz0, nz1 = nz ? hiz1 : loz1, fz0 = nz ? loz0 : hiz0, fz1 = nz ? loz1 : hiz1;
     const uint32_t meta0 = w1.z, meta1 = w1.w;
-    uint32_t hits = 0;
+    uint32_t hits = 0, ord = 0;
     float tnmin = 3.0e38f;  // smallest entry distance among hit internal children
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -146,14 +146,13 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
         const uint32_t meta = ((hiw ? meta1 : meta0) >> sh) & 0xffu;
         const bool h = (tn <= tf) && (meta != BC_META_EMPTY);
+        const bool hin = h && ((imask >> s) & 1u);
         hits |= (h ? 1u : 0u) << s;
-        if (h && ((imask >> s) & 1u)) tnmin = fminf(tnmin, tn);
+        // internal children go straight to near-to-far order: bit k is slot k ^ oct
+        ord |= (hin ? 1u : 0u) << ((uint32_t)s ^ oct);
+        // the minimum of canonical operands needs no NaN quieting step
+        tnmin = fminf(tnmin, hin ? tn : 3.0e38f);
     }
-    // internal children: reorder hit bits so that bit k is slot k ^ oct
-    const uint32_t ih = hits & imask;
-    uint32_t ord = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) ord |= ((ih >> ((uint32_t)k ^ oct)) & 1u) << k;
     ng_base = w1.x;
     // bits 16-31: upper half of the smallest entry distance.  Truncating a non-negative
     // float rounds towards zero, so it is a lower bound usable to cull the group after a
