@@ -307,9 +307,12 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
     float3 F_s = f3(0.0f, 0.0f, 0.0f), y_s = F_s, ln_s = F_s;
     bool have = false;
     int comp_s = -1, li_s = -1;
+    // The candidates draw from a key of their own per vertex, so the dimension half of every hash
+    // depends only on (k, offset): the same for all lanes of a wave, whatever their bounce.
+    const uint32_t rk = bc_rng_u32(key, BC_DIM_RIS0 + (uint32_t)bounce);
     for (int k = 0; k < M; k++) {
-        const uint32_t base = BC_DIM_RIS0 + ((uint32_t)bounce * 64u + (uint32_t)k) * BC_RIS_DIMS;
-        int ci = (int)(bc_rng(key, base) * (float)nc);
+        const uint32_t base = (uint32_t)k * BC_RIS_DIMS;
+        int ci = (int)(bc_rng(rk, base) * (float)nc);
         if (ci >= nc) ci = nc - 1;
         int comp;  // 0 sun, 1 emitters, 2 sky
         if (sun_on) comp = ci == 0 ? 0 : ((area_on && ci == 1) ? 1 : 2);
@@ -326,7 +329,7 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
                 y = sd;
             }
         } else if (comp == 1) {
-            const float u = bc_rng(key, base + 1u);
+            const float u = bc_rng(rk, base + 1u);
             int li;
             float sel;
             if (!P.power) {
@@ -349,8 +352,8 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
             if (sph) {
                 float3 ln = f3(0.0f, 0.0f, 1.0f);
                 for (int t = 0; t < BC_NTRY_SPH; t++) {
-                    const float a = bc_rng(key, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
-                    const float b = bc_rng(key, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                    const float a = bc_rng(rk, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                    const float b = bc_rng(rk, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
                     const float ss = a * a + b * b;
                     if (ss >= 1.0f || ss <= (float)1e-18) continue;
                     const float ff = 2.0f * sqrtf(1.0f - ss);
@@ -360,7 +363,7 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
                 qpt = add(f3(Lr.g0[0], Lr.g0[1], Lr.g0[2]), mul(ln, Lr.g0[3]));
                 lnrm = ln;
             } else {
-                const float u1 = bc_rng(key, base + 2u), u2 = bc_rng(key, base + 3u);
+                const float u1 = bc_rng(rk, base + 2u), u2 = bc_rng(rk, base + 3u);
                 const float su = sqrtf(u1);
                 const float b1 = su * (1.0f - u2), b2 = su * u2;
                 const float3 e1 = f3(Lr.g1[0], Lr.g1[1], Lr.g1[2]), e2 = f3(Lr.g2[0], Lr.g2[1], Lr.g2[2]);
@@ -392,27 +395,29 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
             float x = 0.0f, yy = 0.0f, ss = 0.0f;
             bool ok = false;
             for (int t = 0; t < BC_NTRY_COS && !ok; t++) {
-                x = bc_rng(key, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
-                yy = bc_rng(key, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                x = bc_rng(rk, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+                yy = bc_rng(rk, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
                 ss = x * x + yy * yy;
                 ok = ss < 1.0f;
             }
             if (ok) {
+                // cos(theta) of the sample is its z about N; the world direction is needed only
+                // for a sky that varies with direction, and for the survivor (built after the loop)
                 const float z = sqrtf(rmax(0.0f, 1.0f - ss));
-                const float3 dir = hemi_dir(N, x, yy, z);
-                const float cv = rmax(0.0f, dot(N, dir));
+                const float cv = z;
                 if (cv > 0.0f) {
                     const float pin = cv / PI_F;
-                    F = mul(had(f, sky_radiance(P, dir)), cv);
+                    const float3 Ls = P.sky_mode == 1 ? f3(P.sky[0], P.sky[1], P.sky[2]) : sky_radiance(P, hemi_dir(N, x, yy, z));
+                    F = mul(had(f, Ls), cv);
                     ph = lum(F);
                     w = ph / (pc * pin);
-                    y = dir;
+                    y = f3(x, yy, z);
                 }
             }
         }
         if (w > 0.0f) {
             wsum = wsum + w;
-            if (bc_rng(key, base + 20u) * wsum < w) {
+            if (bc_rng(rk, base + 20u) * wsum < w) {
                 have = true;
                 F_s = F;
                 ph_s = ph;
@@ -427,7 +432,7 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
     }
     rs.comp = have ? comp_s : -1;
     rs.li = li_s;
-    rs.y = y_s;
+    rs.y = comp_s == 2 ? hemi_dir(N, y_s.x, y_s.y, y_s.z) : y_s;  // sky: local sample -> direction
     rs.lnrm = ln_s;
     rs.F = F_s;
     rs.ph = ph_s;
