@@ -114,6 +114,9 @@ def lib():
             "bch_add_sphere": (None, [vp, POINTER(c_double), c_double, c_int]),
             "bch_add_triangle": (None, [vp, POINTER(c_double), POINTER(c_double), POINTER(c_double), c_int]),
             "bch_build": (None, [vp, c_int, c_int]),
+            "bch_build_median": (None, [vp, c_int]),
+            "bch_quality": (c_int, [ctypes.c_char_p, ctypes.c_char_p, vp, POINTER(c_double), ctypes.c_char_p, c_int]),
+            "bch_effective_spp": (c_int, [ctypes.c_char_p, c_int, c_int, c_int]),
             "bch_refit": (None, [vp]),
             "bch_build_device": (c_int, [vp, vp, c_int, c_int]),
             "bch_load_obj": (vp, [ctypes.c_char_p, vp, vp, c_int, c_int]),
@@ -154,6 +157,38 @@ def device_available():
     buf = ctypes.create_string_buffer(256)
     ok = lib().bc_device_available(buf, 256)
     return bool(ok), buf.value.decode()
+
+
+QUALITY_FIELDS = ("Width", "Height", "SPP", "Bounces", "Denoise", "MIS", "PowerLightSampling", "AdaptiveSampling",
+                  "ProgressiveUpdates", "Sampler", "MSAA", "TSAA", "TAA", "TXAA", "FXAA", "HBAO", "HBAOPlus", "RRDepth",
+                  "Backend", "BVHBins")
+SAMPLERS = ("Random", "Halton", "R2")
+BACKENDS = ("Auto", "CPU", "OpenCL GPU")
+
+
+def quality(name="", backend=None):
+    """Quality(name) of the host mirror, or UltraRealismSettings(backend) when backend is given:
+    a dict of the RenderSettings fields the presets set (Sampler and Backend as names)."""
+    out = np.zeros(20, np.int32)
+    ff = c_double()
+    label = ctypes.create_string_buffer(64)
+    rc = lib().bch_quality(name.encode(), backend.encode() if backend is not None else None, _ptr(out), ctypes.byref(ff),
+                           label, 64)
+    if rc != 0:
+        raise ValueError(f"unknown backend {backend!r}")
+    q = {k: int(v) for k, v in zip(QUALITY_FIELDS, out)}
+    for k in ("Denoise", "MIS", "PowerLightSampling", "AdaptiveSampling", "ProgressiveUpdates"):
+        q[k] = bool(q[k])
+    q["Sampler"] = SAMPLERS[q["Sampler"]]
+    q["Backend"] = BACKENDS[q["Backend"]]
+    q["FireflyClamp"] = ff.value
+    q["Quality"] = label.value.decode()
+    return q
+
+
+def effective_spp(name, spp, msaa, tsaa):
+    """effectiveSPP of Quality(name) with SPP, MSAA and TSAA replaced."""
+    return int(lib().bch_effective_spp(name.encode(), spp, msaa, tsaa))
 
 
 def camera_basis(pos, target, up, fov, aspect):
@@ -233,6 +268,10 @@ class HostScene:
 
     def build(self, bins=32, leaf=4):
         lib().bch_build(self.h, bins, leaf)
+
+    def build_median(self, leaf=4):
+        """Scene.Build with BVHMode "Median"."""
+        lib().bch_build_median(self.h, leaf)
 
     def refit(self):
         lib().bch_refit(self.h)

@@ -3,6 +3,7 @@
 #include "host_scene.hpp"
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -74,9 +75,10 @@ void Scene::AddTriangle(Vec3 a, Vec3 b, Vec3 c, int mat) {  // renderer.go:257-2
     Primitives.push_back(p);
 }
 
-void Scene::Build() {  // renderer.go:409-449 (SAH mode)
+void Scene::Build() {  // renderer.go:409-449
     GPUTopologyRevision++;
     GPUGeometryRevision++;
+    if (BVHMode.empty()) BVHMode = "SAH";
     if (BVHBins < 4) BVHBins = 16;
     if (BVHBins > 32) BVHBins = 32;
     if (BVHLeafSize < 1) BVHLeafSize = 4;
@@ -85,7 +87,42 @@ void Scene::Build() {  // renderer.go:409-449 (SAH mode)
     Nodes.reserve(Primitives.size() * 2);
     Order.resize(Primitives.size());
     for (size_t i = 0; i < Order.size(); i++) Order[i] = (int)i;
-    if (!Order.empty()) buildNodeSAH(0, (int)Order.size());
+    if (Order.empty()) return;
+    std::string mode = BVHMode;
+    for (char& ch : mode) ch = (char)std::tolower((unsigned char)ch);
+    if (mode == "median") buildNodeMedian(0, (int)Order.size());
+    else buildNodeSAH(0, (int)Order.size());
+}
+
+// buildNodeMedian (renderer.go:506-524): leaf at count <= BVHLeafSize, else sort the range by the
+// centroid on the largest centroid-extent axis and split at the middle.  Go sorts with the
+// unstable sort.Slice, so the order among equal centroids is not pinned; std::sort is unstable
+// too, and any order of equals gives a valid tree of the same shape.
+int Scene::buildNodeMedian(int start, int end) {
+    const int idx = (int)Nodes.size();
+    Nodes.emplace_back();
+    AABB box = emptyAABB(), cb = emptyAABB();
+    for (int i = start; i < end; i++) {
+        const Primitive& p = Primitives[(size_t)Order[(size_t)i]];
+        box = expandBox(box, p.Box);
+        cb = expandPoint(cb, p.Centroid);
+    }
+    const int count = end - start;
+    if (count <= BVHLeafSize) {
+        BVHNode& n = Nodes[(size_t)idx];
+        n.Box = box; n.Left = -1; n.Right = -1; n.Start = start; n.Count = count;
+        return idx;
+    }
+    const int axis = largestAxis(cb.Max - cb.Min);
+    std::sort(Order.begin() + start, Order.begin() + end, [&](int a, int b) {
+        return axisOf(Primitives[(size_t)a].Centroid, axis) < axisOf(Primitives[(size_t)b].Centroid, axis);
+    });
+    const int mid = (start + end) / 2;
+    const int left = buildNodeMedian(start, mid);
+    const int right = buildNodeMedian(mid, end);
+    BVHNode& n = Nodes[(size_t)idx];
+    n.Box = box; n.Left = left; n.Right = right; n.Start = 0; n.Count = 0;
+    return idx;
 }
 
 int Scene::BuildDevice(bc_ctx* ctx, std::string& err) {  // Build (renderer.go:409-449), tree from the GPU
@@ -302,24 +339,24 @@ RenderSettings Quality(const std::string& name) {  // renderer.go:1064-1117 (fie
     RenderSettings b;
     b.Backend = "Auto"; b.Sampler = "Halton"; b.MIS = true; b.PowerLightSampling = true;
     b.FireflyClamp = 24; b.RRDepth = 4; b.PublishHz = 30; b.ProgressiveUpdates = true;
-    b.AdaptiveSampling = true; b.TAA = 1; b.MSAA = 1; b.BVHBins = 16; b.BVHLeafSize = 4;
+    b.AdaptiveSampling = true; b.TAA = 1; b.MSAA = 1; b.FXAA = 1; b.BVHBins = 16; b.BVHLeafSize = 4;
     auto set = [&](int w, int h, int spp, int bounces, bool denoise, const std::string& q) {
         b.Width = w; b.Height = h; b.SPP = spp; b.Bounces = bounces; b.Denoise = denoise; b.Quality = q;
     };
     if (name == "Draft") {
-        set(640, 360, 2, 4, false, name); b.TAA = 0; b.FireflyClamp = 12;
+        set(640, 360, 2, 4, false, name); b.FXAA = 1; b.TAA = 0; b.FireflyClamp = 12;
     } else if (name == "Balanced") {
-        set(1280, 720, 16, 8, true, name); b.TAA = 2; b.HBAO = 1;
+        set(1280, 720, 16, 8, true, name); b.FXAA = 2; b.TAA = 2; b.HBAO = 1;
     } else if (name == "High") {
-        set(1600, 900, 48, 10, true, name); b.TAA = 2; b.TSAA = 1; b.HBAOPlus = 2; b.MSAA = 2; b.FireflyClamp = 32;
+        set(1600, 900, 48, 10, true, name); b.FXAA = 2; b.TAA = 2; b.TSAA = 1; b.HBAOPlus = 2; b.MSAA = 2; b.FireflyClamp = 32;
     } else if (name == "Ultra") {
-        set(1920, 1080, 96, 12, true, name); b.TAA = 3; b.TSAA = 2; b.TXAA = 1; b.HBAOPlus = 3; b.MSAA = 2; b.FireflyClamp = 48;
+        set(1920, 1080, 96, 12, true, name); b.FXAA = 3; b.TAA = 3; b.TSAA = 2; b.TXAA = 1; b.HBAOPlus = 3; b.MSAA = 2; b.FireflyClamp = 48;
     } else if (name == "Ultra Realism") {
-        set(2560, 1440, 256, 16, true, name); b.TAA = 1; b.TSAA = 0; b.TXAA = 0; b.MSAA = 1;
+        set(2560, 1440, 256, 16, true, name); b.FXAA = 0; b.TAA = 1; b.TSAA = 0; b.TXAA = 0; b.MSAA = 1;
         b.HBAO = 0; b.HBAOPlus = 0; b.FireflyClamp = 0; b.RRDepth = 6; b.ProgressiveUpdates = false;
         b.BVHBins = 32; b.BVHLeafSize = 4;
     } else {
-        set(960, 540, 8, 6, true, "Preview");
+        set(960, 540, 8, 6, true, "Preview"); b.FXAA = 1; b.TAA = 1;
     }
     return b;
 }

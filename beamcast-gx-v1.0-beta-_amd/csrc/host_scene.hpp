@@ -68,6 +68,7 @@ public:
     std::vector<BVHNode> Nodes;
     std::vector<int> Order;
     std::string Name;
+    std::string BVHMode = "SAH";  // "SAH" or "Median" (case-insensitive), as Scene.BVHMode
     int BVHBins = 0, BVHLeafSize = 0;
     uint64_t GPUTopologyRevision = 0, GPUGeometryRevision = 0;
 
@@ -84,6 +85,7 @@ public:
 
 private:
     int buildNodeSAH(int start, int end);
+    int buildNodeMedian(int start, int end);
     std::vector<double> node_box_, prim_geom_, mat_data_;
     std::vector<int32_t> node_link_, order_, prim_kind_, prim_mat_, mat_kind_;
     bc_scene_view view_{};
@@ -100,7 +102,7 @@ struct RenderSettings {
     int Width = 0, Height = 0, SPP = 1, Bounces = 4;
     bool Denoise = false;
     std::string Quality, Backend = "Auto", Sampler = "Halton", Upscale = "Off";
-    int MSAA = 1, TSAA = 0, TAA = 0, TXAA = 0, HBAO = 0, HBAOPlus = 0, DebugView = 0;
+    int MSAA = 1, TSAA = 0, TAA = 0, TXAA = 0, FXAA = 1, HBAO = 0, HBAOPlus = 0, DebugView = 0;
     bool MIS = true, PowerLightSampling = true, AdaptiveSampling = true, ProgressiveUpdates = true;
     double FireflyClamp = 24;
     int RRDepth = 4, PublishHz = 30;

@@ -3,7 +3,9 @@
 // (seed, item index, draw index), so any sub-range of a workload can be regenerated.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <string>
 
 #include "../../include/beamcast_host.h"
 #include "host_scene.hpp"
@@ -137,6 +139,12 @@ void bch_build(bch_scene* s, int bins, int leaf) {
     s->s.BVHLeafSize = leaf;
     s->s.Build();
 }
+void bch_build_median(bch_scene* s, int leaf) {
+    s->s.BVHMode = "Median";
+    s->s.BVHLeafSize = leaf;
+    s->s.Build();
+    s->s.BVHMode = "SAH";
+}
 int bch_build_device(bch_scene* s, bc_ctx* ctx, int bins, int leaf) {
     s->s.BVHBins = bins;
     s->s.BVHLeafSize = leaf;
@@ -208,6 +216,34 @@ void bch_translate(bch_scene* s, int start, int count, const double* d) {  // tr
                     std::max(p.A.z, std::max(p.B.z, p.C.z)));
         p.Box = bcx::AABB{mn - V(1e-6, 1e-6, 1e-6), mx + V(1e-6, 1e-6, 1e-6)};
     }
+}
+
+int bch_quality(const char* name, const char* backend, int32_t* o, double* firefly, char* label, int cap) {
+    bcx::RenderSettings q;
+    if (backend) {
+        const std::string b = backend;
+        if (b != "Auto" && b != "CPU" && b != "OpenCL GPU") return -1;
+        q = bcx::UltraRealismSettings(b);
+    } else {
+        q = bcx::Quality(name ? name : "");
+    }
+    const int sampler = q.Sampler == "Halton" ? 1 : (q.Sampler == "R2" ? 2 : 0);
+    const int be = q.Backend == "CPU" ? 1 : (q.Backend == "OpenCL GPU" ? 2 : 0);
+    const int32_t v[20] = {q.Width, q.Height, q.SPP, q.Bounces, q.Denoise, q.MIS, q.PowerLightSampling, q.AdaptiveSampling,
+                           q.ProgressiveUpdates, sampler, q.MSAA, q.TSAA, q.TAA, q.TXAA, q.FXAA, q.HBAO, q.HBAOPlus,
+                           q.RRDepth, be, q.BVHBins};
+    std::memcpy(o, v, sizeof v);
+    if (firefly) *firefly = q.FireflyClamp;
+    if (label && cap > 0) std::snprintf(label, (size_t)cap, "%s", q.Quality.c_str());
+    return 0;
+}
+
+int bch_effective_spp(const char* name, int spp, int msaa, int tsaa) {
+    bcx::RenderSettings q = bcx::Quality(name ? name : "");
+    q.SPP = spp;
+    q.MSAA = msaa;
+    q.TSAA = tsaa;
+    return bcx::effectiveSPP(q);
 }
 
 void bch_camera_basis(const double* pos, const double* target, const double* up, double fov, double aspect, double* o) {

@@ -28,6 +28,10 @@ void bch_add_sphere(bch_scene* s, const double* center, double radius, int mat);
 void bch_add_triangle(bch_scene* s, const double* a, const double* b, const double* c, int mat);
 /* Scene.Build with BVHMode SAH (renderer.go:409-449); bins/leaf clamped as there */
 void bch_build(bch_scene* s, int bins, int leaf);
+/* Scene.Build with BVHMode "Median" (renderer.go:409-449, buildNodeMedian :506-524); leaf clamped
+ * to [1, 16] as there.  The order among equal centroids is not pinned (Go's sort.Slice is
+ * unstable). */
+void bch_build_median(bch_scene* s, int leaf);
 /* Scene.Build with the tree made on the GPU (bc_build_bvh): the same Nodes and Order as
  * bch_build; returns the bc_build_bvh status (message via bc_last_error(ctx)) */
 int bch_build_device(bch_scene* s, bc_ctx* ctx, int bins, int leaf);
@@ -81,6 +85,16 @@ int bch_pages_load(const bch_pages* p, int64_t page, bch_packed_triangle* out);
 /* append_page_to_scene for pages [first, first + npages): AddTriangle(v0, v0 + e1, v0 + e2,
  * material) with the sums in float as there */
 int bch_pages_append(bch_scene* s, const bch_pages* p, int64_t first, int64_t npages);
+
+/* Quality(name) (renderer.go:1064-1117), or UltraRealismSettings(backend) (:1119-1135) when backend
+ * is not NULL (then name is ignored): out20 = Width, Height, SPP, Bounces, Denoise, MIS,
+ * PowerLightSampling, AdaptiveSampling, ProgressiveUpdates, Sampler (0 Random, 1 Halton, 2 R2),
+ * MSAA, TSAA, TAA, TXAA, FXAA, HBAO, HBAOPlus, RRDepth, Backend (0 Auto, 1 CPU, 2 OpenCL GPU),
+ * BVHBins; *firefly = FireflyClamp; label (cap bytes, may be NULL) = the Quality label.  Returns 0,
+ * or -1 for an unknown backend name ("Auto", "CPU", "OpenCL GPU"). */
+int bch_quality(const char* name, const char* backend, int32_t* out20, double* firefly, char* label, int cap);
+/* effectiveSPP (renderer.go:2395-2417) of Quality(name) with its SPP, MSAA and TSAA replaced */
+int bch_effective_spp(const char* name, int spp, int msaa, int tsaa);
 
 /* Camera.basis (renderer.go:947-960): out12 = origin, lower, horiz, vert */
 void bch_camera_basis(const double* pos, const double* target, const double* up, double fov,

@@ -1028,6 +1028,29 @@ static inline void orc_sun_dir(const orc_render_desc* d, float out[3]) {
 static inline double rnd64(uint32_t key, uint32_t dim) { return (double)(orc_rng_u32(key, dim) >> 8) * (1.0 / 16777216.0); }
 static inline float rnd32(uint32_t key, uint32_t dim) { return (float)(orc_rng_u32(key, dim) >> 8) * (1.0f / 16777216.0f); }
 
+/* Camera jitter of sample smp at pixel (x0, y) of a W-wide image (renderer.go:1844-1915,
+ * 1943-1950): Halton (bases 2, 3) or R2 with the pixel's Cranley-Patterson rotation, or two
+ * counter-based uniforms for the random sampler; both in [0, 1). */
+void orc_camera_jitter(int sampler, uint32_t seed, int W, int x0, int y, int smp, double* jx, double* jy) {
+    double rotX = 0, rotY = 0;
+    orc_pixel_rotation(x0, y, &rotX, &rotY);
+    if (sampler == 1) {
+        double hx = orc_radical_inverse((uint64_t)smp + 1, 2), hy = orc_radical_inverse((uint64_t)smp + 1, 3);
+        *jx = hx + rotX; *jx = *jx - floor(*jx);
+        *jy = hy + rotY; *jy = *jy - floor(*jy);
+    } else if (sampler == 2) {
+        const double g = 1.324717957244746, a1 = 1.0 / g, a2 = 1.0 / (g * g);
+        double sx = 0.5 + (double)(smp + 1) * a1, sy = 0.5 + (double)(smp + 1) * a2;
+        sx = sx - floor(sx); sy = sy - floor(sy);
+        *jx = sx + rotX; *jx = *jx - floor(*jx);
+        *jy = sy + rotY; *jy = *jy - floor(*jy);
+    } else {
+        uint32_t key = orc_path_key(seed, (uint32_t)y * (uint32_t)W + (uint32_t)x0, (uint32_t)smp);
+        *jx = rnd64(key, DIM_CAM_X);
+        *jy = rnd64(key, DIM_CAM_Y);
+    }
+}
+
 void orc_cos_hemisphere64(const double* nn, uint32_t key, uint32_t dim0, double* out) {
     /* randCosHemisphere, renderer.go:1290-1311, with the integrator's bounded retries */
     v3 n = V(nn[0], nn[1], nn[2]);

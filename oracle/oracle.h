@@ -89,6 +89,9 @@ double orc_time_hit64(const orc_scene* s, int64_t n, const float* rays, int any_
 /* Sampling primitives */
 double   orc_radical_inverse(uint64_t n, uint64_t base);
 void     orc_pixel_rotation(int x, int y, double* rx, double* ry);
+/* camera jitter of sample smp at pixel (x, y) of a W-wide image: sampler 1 Halton, 2 R2 (both
+ * with the pixel's Cranley-Patterson rotation), else counter-based uniforms */
+void     orc_camera_jitter(int sampler, uint32_t seed, int W, int x, int y, int smp, double* jx, double* jy);
 double   orc_power_heuristic(double a, double b);
 uint32_t orc_path_key(uint32_t seed, uint32_t pixel, uint32_t sample);
 uint32_t orc_rng_u32(uint32_t key, uint32_t dim);

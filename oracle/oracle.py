@@ -58,6 +58,7 @@ def lib():
             "orc_time_hit64": (c_double, [vp, c_int64, vp, c_int, c_int, vp]),
             "orc_radical_inverse": (c_double, [ctypes.c_uint64, ctypes.c_uint64]),
             "orc_pixel_rotation": (None, [c_int, c_int, POINTER(c_double), POINTER(c_double)]),
+            "orc_camera_jitter": (None, [c_int, c_uint32, c_int, c_int, c_int, c_int, POINTER(c_double), POINTER(c_double)]),
             "orc_power_heuristic": (c_double, [c_double, c_double]),
             "orc_path_key": (c_uint32, [c_uint32, c_uint32, c_uint32]),
             "orc_rng_u32": (c_uint32, [c_uint32, c_uint32]),
@@ -217,6 +218,13 @@ def radical_inverse(n, base):
 def pixel_rotation(x, y):
     a, b = c_double(), c_double()
     lib().orc_pixel_rotation(x, y, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def camera_jitter(sampler, x, y, smp, seed=0x9E3779B9, width=1 << 16):
+    """Camera jitter (jx, jy) of sample smp at pixel (x, y): sampler 1 Halton, 2 R2, else random."""
+    a, b = c_double(), c_double()
+    lib().orc_camera_jitter(sampler, seed, width, x, y, smp, ctypes.byref(a), ctypes.byref(b))
     return a.value, b.value
 
 

@@ -86,3 +86,29 @@ def test_post_argument_errors(dev, frame):
     dev.postprocess_device(bc.post_desc(fxaa=3, txaa=1), 2, 2, d_img, None, d_px)
     dev.free(d_img)
     dev.free(d_px)
+
+
+def test_post_determinism(dev, frame):
+    """TestParallelPostDeterminism (renderer_test.go:720-745): the post stack gives the same bits
+    however its work is scheduled.  The reference image pattern (x*13 + y*7 mod 97) with the
+    rendered guides goes through a-trous denoise and FXAA twice on the device, interleaved with
+    other launches; both results are bit-identical and match the oracle."""
+    print("case: post determinism", flush=True)
+    _, _, img, g = frame
+    h, w = img.shape[:2]
+    y, x = np.mgrid[0:h, 0:w]
+    v = ((x * 13 + y * 7) % 97) / 97.0
+    pat = np.stack([0.1 + 2 * v, 0.2 + v, 0.3 + 0.5 * v, img[..., 3]], -1).astype(np.float32)
+    post = bc.post_desc(denoise=2, fxaa=2, hdr=2)
+    outs = []
+    for _ in range(2):
+        d_img, d_g = dev.to_device(pat), dev.to_device(g)
+        d_px, d_lin = dev.alloc(w * h * 4), dev.alloc(w * h * 16)
+        dev.postprocess_device(post, w, h, d_img, d_g, d_px, d_lin)
+        outs.append((dev.download(d_px, np.empty((h, w), np.uint32)), dev.download(d_lin, np.empty((h, w, 4), np.float32))))
+        for p in (d_img, d_g, d_px, d_lin):
+            dev.free(p)
+        dev.trace(bc.workload_rays(1, 64, count=4096))  # other work between the two runs
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1].view(np.int32), outs[1][1].view(np.int32))
+    lin_ref, px_ref = orc.post(pat, g, denoise=2, fxaa=2, hdr=2)
+    compare(outs[0][1], outs[0][0], lin_ref, px_ref)

@@ -92,3 +92,71 @@ def test_workload_rays_subrange_and_shape():
     assert np.median(coh[:, 7]) > 1.5  # rays entering the unit cube get tmax beyond its entry plane
     inc = bc.workload_rays(1, 64, any_hit=True, count=4096)
     assert inc[:, 7].min() >= 0.05 and inc[:, 7].max() <= 1.0
+
+
+def test_sampling_levels():
+    """TestSamplingLevels (renderer_test.go:198-212): effectiveSPP multiplies SPP by the MSAA
+    factor (2, 4 or 8, else 1) and the TSAA frames (TSAA 1-3 -> 2-4), clamped to [1, 4096]."""
+    assert bc.effective_spp("Preview", 2, 4, 2) == 24
+    assert bc.effective_spp("Preview", 200, 8, 3) == 4096
+    assert bc.effective_spp("Preview", 5, 3, 0) == 5
+    assert bc.effective_spp("Preview", 5, 2, 1) == 20
+    assert bc.effective_spp("Preview", 0, 4, 3) == 1
+
+
+def test_ultra_realism_preset():
+    """TestUltraRealismPreset (renderer_test.go:846-864): the CPU preset keeps the full transport
+    and no biased effects; the OpenCL-GPU one is the megakernel-compatible subset."""
+    cpu = bc.quality(backend="CPU")
+    assert cpu["Quality"] == "Ultra Realism" and (cpu["Width"], cpu["Height"]) == (2560, 1440)
+    assert cpu["SPP"] >= 256 and cpu["Bounces"] >= 16 and cpu["MIS"] and cpu["PowerLightSampling"]
+    assert cpu["Sampler"] == "Halton"
+    assert cpu["HBAO"] == 0 and cpu["HBAOPlus"] == 0 and cpu["FXAA"] == 0 and cpu["FireflyClamp"] == 0
+    gpu = bc.quality(backend="OpenCL GPU")
+    assert gpu["Backend"] == "OpenCL GPU" and gpu["Sampler"] == "Random"
+    assert not gpu["MIS"] and not gpu["PowerLightSampling"] and not gpu["AdaptiveSampling"]
+    assert gpu["SPP"] >= 256 and gpu["Bounces"] >= 16 and gpu["Denoise"]
+    with pytest.raises(ValueError):
+        bc.quality(backend="Vulkan")
+    # the other presets as Quality (renderer.go:1064-1117) sets them
+    want = {"Draft": (640, 360, 2, 4, 1, 0), "Balanced": (1280, 720, 16, 8, 2, 2), "High": (1600, 900, 48, 10, 2, 2),
+            "Ultra": (1920, 1080, 96, 12, 3, 3), "anything else": (960, 540, 8, 6, 1, 1)}
+    for name, (w, h, spp, b, fxaa, taa) in want.items():
+        q = bc.quality(name)
+        assert (q["Width"], q["Height"], q["SPP"], q["Bounces"], q["FXAA"], q["TAA"]) == (w, h, spp, b, fxaa, taa), name
+    assert bc.quality("anything else")["Quality"] == "Preview"
+
+
+def test_median_build_invariants():
+    """BVHMode "Median" (buildNodeMedian, renderer.go:506-524): a pre-order tree whose every node
+    splits its range at the middle after sorting on the largest centroid axis, leaves of at most
+    BVHLeafSize primitives, boxes that contain their children, every primitive in one leaf."""
+    s = bc.HostScene.workload(bc.STRESS, bins=16)
+    s.build_median(leaf=4)
+    box, link, order = s.nodes()
+    n = len(order)
+    assert sorted(order.tolist()) == list(range(n))
+    kind, _, geom, _, _ = s.export()
+    cen = np.where((kind == bc.TRIANGLE)[:, None], (geom[:, 0:3] + geom[:, 3:6] + geom[:, 6:9]) / 3.0, geom[:, 0:3])
+    seen = np.zeros(n, np.int32)
+
+    def walk(i, start, end):
+        l, r, st, c = link[i]
+        if l < 0:
+            assert st == start and c == end - start and 0 < c <= 4
+            seen[start:end] += 1
+            return
+        assert l == i + 1  # pre-order: the left child follows its parent
+        mid = (start + end) // 2
+        for ch in (l, r):
+            assert np.all(box[ch, :3] >= box[i, :3]) and np.all(box[ch, 3:] <= box[i, 3:])
+        c = cen[order[start:end]]
+        ext = c.max(axis=0) - c.min(axis=0)
+        axis = 1 if ext[1] > ext[0] and ext[1] >= ext[2] else (2 if ext[2] > ext[0] and ext[2] > ext[1] else 0)
+        assert cen[order[start:mid], axis].max() <= cen[order[mid:end], axis].min()
+        walk(l, start, mid)
+        walk(r, mid, end)
+
+    walk(0, 0, n)
+    assert np.all(seen == 1)
+    assert len(link) <= 2 * n - 1

@@ -492,3 +492,16 @@ def test_fxaa_and_txaa_properties():
     assert np.isclose(lin[3, 4, 0], a * 0.5 + b * 0.5, rtol=1e-12)
     assert np.isclose(lin[2, 4, 0], b * 0.5 + (a + 3 * b) / 4 * 0.5, rtol=1e-12)
     assert np.allclose(lin[0], b) and np.allclose(lin[:, 0], b)
+
+
+@pytest.mark.parametrize("sampler", [1, 2])
+def test_prepared_sample_sequence_bounds(sampler):
+    """TestPreparedSampleSequenceBounds (renderer_test.go:610-627): the Halton and R2 camera
+    sequences, rotated by the pixel's Cranley-Patterson offset (pixel 17, 29), stay in [0, 1)."""
+    for smp in range(64):
+        jx, jy = orc.camera_jitter(sampler, 17, 29, smp)
+        assert 0.0 <= jx < 1.0 and 0.0 <= jy < 1.0
+    # the Halton pair of sample 0 is (1/2, 1/3) plus the rotation, modulo 1
+    rx, ry = orc.pixel_rotation(17, 29)
+    jx, jy = orc.camera_jitter(1, 17, 29, 0)
+    assert jx == pytest.approx((0.5 + rx) % 1.0, abs=1e-15) and jy == pytest.approx((1 / 3 + ry) % 1.0, abs=1e-15)
