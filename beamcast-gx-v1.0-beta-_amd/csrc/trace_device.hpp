@@ -144,8 +144,9 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
         const float tfz = __builtin_fmaf(u8f(hiw ? fz1 : fz0, sh), adjz, offz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        const uint32_t meta = ((hiw ? meta1 : meta0) >> sh) & 0xffu;
-        const bool h = (tn <= tf) && (meta != BC_META_EMPTY);
+        // an empty slot's box is inverted (255 / 0); rounding can still let it pass here, so
+        // emptiness is checked where it matters, for leaf hits below (internal slots never are)
+        const bool h = tn <= tf;
         const bool hin = h && ((imask >> s) & 1u);
         hits |= (h ? 1u : 0u) << s;
         // internal children go straight to near-to-far order: bit k is slot k ^ oct
@@ -165,6 +166,7 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
         const uint32_t s = (uint32_t)__builtin_ctz(lh);
         lh &= lh - 1u;
         const uint32_t meta = ((s >= 4 ? meta1 : meta0) >> ((s & 3u) * 8u)) & 0xffu;
+        if (meta == BC_META_EMPTY) continue;
         const uint32_t cnt = (meta >> 5) + 1u, offs = meta & 31u;
         pm |= (uint32_t)(((1ull << cnt) - 1ull) << offs);
     }
