@@ -115,6 +115,7 @@ def lib():
             "bch_add_triangle": (None, [vp, POINTER(c_double), POINTER(c_double), POINTER(c_double), c_int]),
             "bch_build": (None, [vp, c_int, c_int]),
             "bch_build_median": (None, [vp, c_int]),
+            "bch_flatten_check": (c_int, [vp, POINTER(c_int64)]),
             "bch_quality": (c_int, [ctypes.c_char_p, ctypes.c_char_p, vp, POINTER(c_double), ctypes.c_char_p, c_int]),
             "bch_effective_spp": (c_int, [ctypes.c_char_p, c_int, c_int, c_int]),
             "bch_refit": (None, [vp]),
@@ -268,6 +269,14 @@ class HostScene:
 
     def build(self, bins=32, leaf=4):
         lib().bch_build(self.h, bins, leaf)
+
+    def flatten_check(self):
+        """Host BVH8 flattening without a device: (wide nodes, depth, leaf slots, every primitive in
+        exactly one leaf slot)."""
+        o = (c_int64 * 4)()
+        if lib().bch_flatten_check(self.h, o) != 0:
+            raise ValueError("scene view cannot be encoded")
+        return int(o[0]), int(o[1]), int(o[2]), bool(o[3])
 
     def build_median(self, leaf=4):
         """Scene.Build with BVHMode "Median"."""

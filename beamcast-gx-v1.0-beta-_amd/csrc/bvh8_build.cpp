@@ -8,9 +8,10 @@
 // 1..8 it keeps the cheapest way to represent the subtree as at most that many children of
 // one wide node -- a leaf of at most kLeafMax primitives, a wide internal node, or the budget
 // split between the two binary children -- with cost = area x (kNodeCost per node visited,
-// kPrimCost per primitive tested).  Leaves hold at most kLeafMax primitives so that eight of
-// them fit the 32-primitive run of a node; larger binary leaves become several leaf slots, and
-// a leaf that needs more than four slots may instead become a wide node of its own.
+// kPrimCost per primitive tested).  Leaf slots hold kLeafMax primitives (more only for binary
+// leaves that would otherwise need more than eight slots, bc_leaf_run), so a node's leaves fit
+// its 32-primitive run; larger binary leaves become several leaf slots, and a leaf that needs
+// more than four slots may instead become a wide node of its own.
 // Every primitive appears in exactly one leaf; which leaf only changes culling, never a hit
 // result (equal distances resolve to the smaller primitive index).
 #include "bvh8_build.hpp"
@@ -112,9 +113,10 @@ void expand(const bc_scene_view& v, const std::vector<Plan>& plan, int n, int sl
             c.box = node_box(v, n);
             out.push_back(c);
         } else {
-            for (int s = 0; s < P.count; s += kLeafMax) {
+            const int run = bc_leaf_run(P.count);
+            for (int s = 0; s < P.count; s += run) {
                 Child c;
-                c.leaf = true; c.node2 = -1; c.start = P.start + s; c.count = std::min(kLeafMax, P.count - s);
+                c.leaf = true; c.node2 = -1; c.start = P.start + s; c.count = std::min(run, P.count - s);
                 c.box.empty();
                 for (int k = 0; k < c.count; k++) c.box.grow(prim_box(v, v.order[c.start + k]));
                 out.push_back(c);
@@ -140,10 +142,11 @@ void make_plans(const bc_scene_view& v, std::vector<Plan>& plan) {
         Plan& P = plan[(size_t)n];
         const int32_t* l = v.node_link + 4 * (size_t)n;
         const double A = node_box(v, n).area();
-        if (l[3] > 0) {  // binary leaf: ceil(count / kLeafMax) leaf slots
+        if (l[3] > 0) {  // binary leaf: ceil(count / run) leaf slots, at most eight
             P.start = l[2];
             P.count = l[3];
-            const int need = (l[3] + kLeafMax - 1) / kLeafMax;
+            const int run = bc_leaf_run(l[3]);
+            const int need = (l[3] + run - 1) / run;
             const float c = (float)(A * kPrimCost * l[3]);
             // a leaf needing more than half a node may also be a wide node of its own (one slot)
             const float c_wide = need > 4 ? c + (float)(A * kNodeCost) : inf;

@@ -126,7 +126,8 @@ __global__ void k_plans(Inputs in, const int* __restrict__ level, int n, Plan* _
     if (l[3] > 0) {
         P.start = l[2];
         P.count = l[3];
-        const int need = (l[3] + kLeafMax - 1) / kLeafMax;
+        const int run = bc_leaf_run(l[3]);
+        const int need = (l[3] + run - 1) / run;
         const float c = (float)(A * kPrimCost * l[3]);
         const float c_wide = need > 4 ? c + (float)(A * kNodeCost) : inf;
         for (int s = 1; s <= 8; s++) {
@@ -205,10 +206,11 @@ __device__ int expand(const Inputs& in, const Plan* __restrict__ plan, int n0, i
                 c.node2 = -1; c.start = P.start; c.count = P.count;
                 c.box = node_box(in, n);
             } else {
-                for (int s = 0; s < P.count; s += kLeafMax) {
+                const int run = bc_leaf_run(P.count);
+                for (int s = 0; s < P.count; s += run) {
                     if (nout >= 8) { atomicOr(err, 2); return nout; }
                     Child& c = out[nout++];
-                    c.node2 = -1; c.start = P.start + s; c.count = min(kLeafMax, P.count - s);
+                    c.node2 = -1; c.start = P.start + s; c.count = min(run, P.count - s);
                     box_empty(c.box);
                     for (int k = 0; k < c.count; k++) box_grow(c.box, prim_box(in, in.order[c.start + k]));
                 }

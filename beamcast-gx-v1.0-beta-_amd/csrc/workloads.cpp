@@ -6,8 +6,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/beamcast_host.h"
+#include "bvh8_build.hpp"
 #include "host_scene.hpp"
 
 namespace {
@@ -139,6 +141,33 @@ void bch_build(bch_scene* s, int bins, int leaf) {
     s->s.BVHLeafSize = leaf;
     s->s.Build();
 }
+int bch_flatten_check(bch_scene* s, int64_t* out4) {
+    const bc_scene_view& v = s->s.View(false);
+    bcx::Bvh8 b;
+    std::string err;
+    if (!bcx::build_bvh8(v, b, err)) return -1;
+    std::vector<int> seen((size_t)v.nprim, 0);
+    int64_t slots = 0;
+    bool ok = true;
+    for (const bc_node8& n : b.nodes)
+        for (int k = 0; k < 8; k++) {
+            if (((n.imask >> k) & 1u) || n.meta[k] == BC_META_EMPTY) continue;
+            const uint32_t cnt = (n.meta[k] >> 5) + 1u, off = n.meta[k] & 31u;
+            if (off + cnt > 32u) { ok = false; continue; }
+            for (uint32_t j = 0; j < cnt; j++) {
+                const uint32_t id = b.ids[(n.prim_base & ~BC_NODE_HAS_SPHERES) + off + j] & ~BC_SPHERE_BIT;
+                if (id < (uint32_t)v.nprim) seen[id]++; else ok = false;
+            }
+            slots++;
+        }
+    for (int x : seen) ok = ok && x == 1;
+    out4[0] = (int64_t)b.nodes.size();
+    out4[1] = b.depth;
+    out4[2] = slots;
+    out4[3] = ok ? 1 : 0;
+    return 0;
+}
+
 void bch_build_median(bch_scene* s, int leaf) {
     s->s.BVHMode = "Median";
     s->s.BVHLeafSize = leaf;

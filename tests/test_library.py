@@ -160,3 +160,14 @@ def test_median_build_invariants():
     walk(0, 0, n)
     assert np.all(seen == 1)
     assert len(link) <= 2 * n - 1
+
+
+@pytest.mark.parametrize("kind,param", [(bc.CLOSED_BOX, 0), (bc.SOUP, 20000), (bc.STRESS, 0), (bc.CORNELL, 0),
+                                        (bc.TERRAIN, 40), (bc.BOX_RETESS, 3000)])
+@pytest.mark.parametrize("leaf", [1, 4, 8, 16])
+def test_bvh8_flatten_covers_every_primitive(kind, param, leaf):
+    """The BVH8 collapse puts every primitive in exactly one leaf slot for Go leaves of up to 16
+    primitives (BVHLeafSize 16), with a node's leaf runs inside its 32-primitive window."""
+    s = bc.HostScene.workload(kind, param=param, bins=16, leaf=leaf)
+    nodes, depth, slots, ok = s.flatten_check()
+    assert ok and nodes >= 1 and depth >= 1 and slots <= s.counts()[0]
