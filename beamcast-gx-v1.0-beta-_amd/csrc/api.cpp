@@ -47,7 +47,7 @@ bool upload_vec(bc_ctx* c, T*& dst, size_t& cap_bytes, const std::vector<T>& v, 
         dst = (T*)p;
         cap_bytes = bytes;
     }
-    if (!v.empty() && !ok(c, hipMemcpyAsync(dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), what))
+    if (!v.empty() && !bc_h2d(c, dst, v.data(), v.size() * sizeof(T), c->stream, what))
         return false;
     return true;
 }
@@ -167,6 +167,7 @@ void bc_destroy(bc_ctx* c) {
         if (q) hipFree(q);
     bc_render_release(c);
     bc_post_release(c);
+    bc_pinned_release(c);
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -362,9 +363,9 @@ int bc_scene_refit(bc_ctx* c, const bc_scene_view* v, uint64_t geom) {
     double* g = (double*)stg;
     int32_t* kd = (int32_t*)(g + 10 * n);
     int32_t* mt = kd + n;
-    if (!ok(c, hipMemcpyAsync(g, v->prim_geom, 80 * n, hipMemcpyHostToDevice, c->stream), "refit upload") ||
-        !ok(c, hipMemcpyAsync(kd, v->prim_kind, 4 * n, hipMemcpyHostToDevice, c->stream), "refit upload") ||
-        !ok(c, hipMemcpyAsync(mt, v->prim_mat, 4 * n, hipMemcpyHostToDevice, c->stream), "refit upload") ||
+    if (!bc_h2d(c, g, v->prim_geom, 80 * n, c->stream, "refit upload") ||
+        !bc_h2d(c, kd, v->prim_kind, 4 * n, c->stream, "refit upload") ||
+        !bc_h2d(c, mt, v->prim_mat, 4 * n, c->stream, "refit upload") ||
         !ok(c, bcd::launch_refit_records(g, kd, mt, c->d_ids, (uint32_t)n, c->d_prims, c->d_pmat, c->d_err, c->stream),
             "refit records"))
         return BC_ERR_DEVICE;
@@ -538,7 +539,7 @@ int bc_trace(bc_ctx* c, const float* rays, int64_t n, int any_hit, void* out) {
     if (!bc_ensure(c, c->tmp_in, c->cap_in, in_b, "hipMalloc(rays)") ||
         !bc_ensure(c, c->tmp_out, c->cap_out, out_b, "hipMalloc(hits)"))
         return BC_ERR_NOMEM;
-    if (!ok(c, hipMemcpyAsync(c->tmp_in, rays, in_b, hipMemcpyHostToDevice, c->stream), "copy rays")) return BC_ERR_DEVICE;
+    if (!bc_h2d(c, c->tmp_in, rays, in_b, c->stream, "copy rays")) return BC_ERR_DEVICE;
     int rc = bc_trace_device(c, c->tmp_in, n, any_hit, c->tmp_out);
     if (rc != BC_OK) return rc;
     if (!ok(c, hipMemcpyAsync(out, c->tmp_out, out_b, hipMemcpyDeviceToHost, c->stream), "copy hits") ||
@@ -560,7 +561,7 @@ int bc_free(bc_ctx* c, void* p) {
 }
 int bc_upload(bc_ctx* c, void* d, const void* s, uint64_t bytes) {
     if (!c) return BC_ERR_ARG;
-    if (!ok(c, hipMemcpyAsync(d, s, bytes, hipMemcpyHostToDevice, c->stream), "bc_upload") ||
+    if (!bc_h2d(c, d, s, bytes, c->stream, "bc_upload") ||
         !ok(c, hipStreamSynchronize(c->stream), "bc_upload"))
         return BC_ERR_DEVICE;
     return BC_OK;

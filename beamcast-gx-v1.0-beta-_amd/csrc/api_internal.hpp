@@ -60,12 +60,19 @@ struct bc_ctx {
     // ray binning before queries (order.hip): mode 0 off, 1 always, 2 when a sample of the
     // batch is incoherent; bits = log2 of the origin cells per axis.  Off by default: on the 1M
     // soup it lifts the incoherent L2 hit rate from 0.47 to 0.95 but saves only about what its
-    // passes cost (the traversal is bound by load latency and lane divergence, not L2 misses).
+    // passes cost (closest hit is bound by VALU issue and any hit by load latency, not by L2
+    // misses).
     int order_mode = 0, order_bits = 4;
     void* d_order = nullptr;  // bin counts, totals, starts, permutation, decision flag
     void* d_sorted = nullptr; // the sorted rays
     size_t cap_order = 0, cap_sorted = 0;
     int* last_order_flag = nullptr;  // decision of the last query, if it ran the binning passes
+    // two pinned buffers for large host-to-device copies (upload.cpp), allocated on first use
+    static constexpr size_t kPinChunk = size_t(16) << 20;
+    void* h_pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    bool pin_busy[2] = {false, false};
+    int pin_next = 0;
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;
@@ -82,6 +89,11 @@ struct bc_ctx {
 
 bool bc_ensure(bc_ctx* c, void*& p, size_t& cap, size_t bytes, const char* what);
 bool bc_check(bc_ctx* c, hipError_t e, const char* what);
+// Host-to-device copy of a pageable buffer on stream st; large ones are staged through the
+// context's pinned buffers by host threads, overlapping the DMA (upload.cpp).  The source may be
+// reused as soon as it returns.
+bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what);
+void bc_pinned_release(bc_ctx* c);
 // Material table and per-group light distributions from the material data (api.cpp).
 int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data);
 int bc_check_kernel_error(bc_ctx* c);

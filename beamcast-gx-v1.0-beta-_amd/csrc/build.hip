@@ -703,8 +703,8 @@ extern "C" int bc_build_bvh(bc_ctx* c, int32_t n, const double* prim_box, const 
         !obox.alloc(c, 48 * 2 * N, "node boxes") || !olink.alloc(c, 16 * 2 * N, "node links"))
         return BC_ERR_NOMEM;
     int* lst = lists.as<int>();
-    if (!bc_check(c, hipMemcpyAsync(pbox.p, prim_box, 48 * N, hipMemcpyHostToDevice, st), "upload boxes") ||
-        !bc_check(c, hipMemcpyAsync(pcen.p, prim_centroid, 24 * N, hipMemcpyHostToDevice, st), "upload centroids") ||
+    if (!bc_h2d(c, pbox.p, prim_box, 48 * N, st, "upload boxes") ||
+        !bc_h2d(c, pcen.p, prim_centroid, 24 * N, st, "upload centroids") ||
         !bc_check(c, hipMemsetAsync(c->d_err, 0, sizeof(int), st), "clear error"))
         return BC_ERR_DEVICE;
     hipLaunchKernelGGL(k_iota, dim3(grid_of(N, kThreads)), dim3(kThreads), 0, st, ord.as<int>(), n);

@@ -461,12 +461,12 @@ int flatten_device(bc_ctx* c, const bc_scene_view& v, FlattenResult& res) {
         !mat.alloc(c, 4 * np, "flatten inputs") || !geom.alloc(c, 80 * np, "flatten inputs") ||
         !plan.alloc(c, sizeof(Plan) * nn, "plans") || !lv.alloc(c, 4 * nn, "binary levels") || !cnt.alloc(c, 16, "counters"))
         return BC_ERR_NOMEM;
-    if (!bc_check(c, hipMemcpyAsync(nbox.p, v.node_box, 48 * nn, hipMemcpyHostToDevice, st), "upload nodes") ||
-        !bc_check(c, hipMemcpyAsync(link.p, v.node_link, 16 * nn, hipMemcpyHostToDevice, st), "upload links") ||
-        !bc_check(c, hipMemcpyAsync(order.p, v.order, 4 * np, hipMemcpyHostToDevice, st), "upload order") ||
-        !bc_check(c, hipMemcpyAsync(kind.p, v.prim_kind, 4 * np, hipMemcpyHostToDevice, st), "upload kinds") ||
-        !bc_check(c, hipMemcpyAsync(mat.p, v.prim_mat, 4 * np, hipMemcpyHostToDevice, st), "upload materials") ||
-        !bc_check(c, hipMemcpyAsync(geom.p, v.prim_geom, 80 * np, hipMemcpyHostToDevice, st), "upload geometry") ||
+    if (!bc_h2d(c, nbox.p, v.node_box, 48 * nn, st, "upload nodes") ||
+        !bc_h2d(c, link.p, v.node_link, 16 * nn, st, "upload links") ||
+        !bc_h2d(c, order.p, v.order, 4 * np, st, "upload order") ||
+        !bc_h2d(c, kind.p, v.prim_kind, 4 * np, st, "upload kinds") ||
+        !bc_h2d(c, mat.p, v.prim_mat, 4 * np, st, "upload materials") ||
+        !bc_h2d(c, geom.p, v.prim_geom, 80 * np, st, "upload geometry") ||
         !bc_check(c, hipMemsetAsync(c->d_err, 0, sizeof(int), st), "clear error"))
         return BC_ERR_DEVICE;
     const Inputs in{nbox.as<double>(), link.as<int32_t>(), order.as<int32_t>(), kind.as<int32_t>(), mat.as<int32_t>(),

@@ -1,0 +1,96 @@
+// upload.cpp -- host-to-device copies of large pageable buffers (scene views, refit geometry,
+// build inputs).
+//
+// hipMemcpyAsync from pageable memory stages the data through the runtime's own buffers with
+// one host thread.  Here a large source is cut into chunks; host threads copy a chunk into one
+// of two pinned buffers owned by the context while the DMA engine reads the other, so the host
+// copy and the PCIe transfer overlap.  An event per buffer keeps a buffer from being rewritten
+// before the transfer that reads it has finished.  Small copies go straight to hipMemcpyAsync.
+#include <algorithm>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "api_internal.hpp"
+
+namespace {
+
+constexpr size_t kDirectMax = size_t(4) << 20;  // up to this, one plain hipMemcpyAsync
+constexpr unsigned kMaxThreads = 8;
+
+void par_copy(char* dst, const char* src, size_t n, unsigned nth) {
+    if (nth <= 1 || n < (size_t(1) << 20)) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = ((n + nth - 1) / nth + 4095) & ~size_t(4095);
+    std::vector<std::thread> pool;
+    for (size_t a = per; a < n; a += per) {
+        const size_t b = std::min(n, a + per);
+        pool.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+    }
+    std::memcpy(dst, src, std::min(n, per));
+    for (auto& t : pool) t.join();
+}
+
+bool ensure_pinned(bc_ctx* c) {
+    if (c->h_pin[0]) return true;
+    void* p[2] = {nullptr, nullptr};
+    hipEvent_t e[2] = {nullptr, nullptr};
+    bool good = true;
+    for (int i = 0; i < 2 && good; i++)
+        good = bc_check(c, hipHostMalloc(&p[i], bc_ctx::kPinChunk, hipHostMallocDefault), "pinned upload buffer") &&
+               bc_check(c, hipEventCreateWithFlags(&e[i], hipEventDisableTiming), "upload event");
+    if (!good) {
+        for (int i = 0; i < 2; i++) {
+            if (p[i]) (void)hipHostFree(p[i]);
+            if (e[i]) (void)hipEventDestroy(e[i]);
+        }
+        return false;
+    }
+    for (int i = 0; i < 2; i++) {
+        c->h_pin[i] = p[i];
+        c->pin_ev[i] = e[i];
+        c->pin_busy[i] = false;
+    }
+    return true;
+}
+
+}  // namespace
+
+bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what) {
+    if (bytes == 0) return true;
+    if (bytes <= kDirectMax) return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what);
+    if (!ensure_pinned(c)) return false;
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nth = std::max(1u, std::min(kMaxThreads, hw));
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    int b = c->pin_next;
+    for (size_t off = 0; off < bytes; off += bc_ctx::kPinChunk) {
+        const size_t n = std::min(bc_ctx::kPinChunk, bytes - off);
+        if (c->pin_busy[b] && !bc_check(c, hipEventSynchronize(c->pin_ev[b]), what)) return false;
+        c->pin_busy[b] = false;
+        par_copy(static_cast<char*>(c->h_pin[b]), s + off, n, nth);
+        if (!bc_check(c, hipMemcpyAsync(d + off, c->h_pin[b], n, hipMemcpyHostToDevice, st), what) ||
+            !bc_check(c, hipEventRecord(c->pin_ev[b], st), what))
+            return false;
+        c->pin_busy[b] = true;
+        b ^= 1;
+    }
+    c->pin_next = b;
+    return true;
+}
+
+void bc_pinned_release(bc_ctx* c) {
+    for (int i = 0; i < 2; i++) {
+        if (c->pin_ev[i]) {
+            if (c->pin_busy[i]) (void)hipEventSynchronize(c->pin_ev[i]);
+            (void)hipEventDestroy(c->pin_ev[i]);
+        }
+        if (c->h_pin[i]) (void)hipHostFree(c->h_pin[i]);
+        c->h_pin[i] = nullptr;
+        c->pin_ev[i] = nullptr;
+        c->pin_busy[i] = false;
+    }
+}
