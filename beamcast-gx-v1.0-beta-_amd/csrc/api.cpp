@@ -542,9 +542,7 @@ int bc_trace(bc_ctx* c, const float* rays, int64_t n, int any_hit, void* out) {
     if (!bc_h2d(c, c->tmp_in, rays, in_b, c->stream, "copy rays")) return BC_ERR_DEVICE;
     int rc = bc_trace_device(c, c->tmp_in, n, any_hit, c->tmp_out);
     if (rc != BC_OK) return rc;
-    if (!ok(c, hipMemcpyAsync(out, c->tmp_out, out_b, hipMemcpyDeviceToHost, c->stream), "copy hits") ||
-        !ok(c, hipStreamSynchronize(c->stream), "trace"))
-        return BC_ERR_DEVICE;
+    if (!bc_d2h(c, out, c->tmp_out, out_b, c->stream, "copy hits")) return BC_ERR_DEVICE;
     return bc_check_kernel_error(c);
 }
 
@@ -568,9 +566,7 @@ int bc_upload(bc_ctx* c, void* d, const void* s, uint64_t bytes) {
 }
 int bc_download(bc_ctx* c, void* d, const void* s, uint64_t bytes) {
     if (!c) return BC_ERR_ARG;
-    if (!ok(c, hipMemcpyAsync(d, s, bytes, hipMemcpyDeviceToHost, c->stream), "bc_download") ||
-        !ok(c, hipStreamSynchronize(c->stream), "bc_download"))
-        return BC_ERR_DEVICE;
+    if (!bc_d2h(c, d, s, bytes, c->stream, "bc_download")) return BC_ERR_DEVICE;
     return bc_check_kernel_error(c);
 }
 int bc_copy_device(bc_ctx* c, void* d_dst, const void* d_src, uint64_t bytes) {

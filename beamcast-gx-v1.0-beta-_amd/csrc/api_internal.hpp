@@ -93,6 +93,9 @@ bool bc_check(bc_ctx* c, hipError_t e, const char* what);
 // context's pinned buffers by host threads, overlapping the DMA (upload.cpp).  The source may be
 // reused as soon as it returns.
 bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what);
+// Device-to-host copy on stream st into pageable memory, staged the same way; returns once dst
+// holds the data.
+bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what);
 void bc_pinned_release(bc_ctx* c);
 // Material table and per-group light distributions from the material data (api.cpp).
 int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data);

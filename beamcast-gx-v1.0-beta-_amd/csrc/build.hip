@@ -857,10 +857,9 @@ extern "C" int bc_build_bvh(bc_ctx* c, int32_t n, const double* prim_box, const 
     const int total = root.size;
     *nnode_out = total;
     if (total > node_cap) { c->err = "bc_build_bvh: node_cap too small (" + std::to_string(total) + " nodes)"; return BC_ERR_ARG; }
-    if (!bc_check(c, hipMemcpyAsync(node_box, obox.p, 48 * (size_t)total, hipMemcpyDeviceToHost, st), "read node boxes") ||
-        !bc_check(c, hipMemcpyAsync(node_link, olink.p, 16 * (size_t)total, hipMemcpyDeviceToHost, st), "read node links") ||
-        !bc_check(c, hipMemcpyAsync(order, ord.p, 4 * N, hipMemcpyDeviceToHost, st), "read order") ||
-        !bc_check(c, hipStreamSynchronize(st), "build readback"))
+    if (!bc_d2h(c, node_box, obox.p, 48 * (size_t)total, st, "read node boxes") ||
+        !bc_d2h(c, node_link, olink.p, 16 * (size_t)total, st, "read node links") ||
+        !bc_d2h(c, order, ord.p, 4 * N, st, "read order"))
         return BC_ERR_DEVICE;
     return BC_OK;
 }

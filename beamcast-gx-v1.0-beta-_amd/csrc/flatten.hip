@@ -568,8 +568,7 @@ int flatten_device(bc_ctx* c, const bc_scene_view& v, FlattenResult& res) {
         return BC_ERR_SCENE;
     }
     res.ids.resize(np);
-    if (!bc_check(c, hipMemcpyAsync(res.ids.data(), c->d_ids, 4 * np, hipMemcpyDeviceToHost, st), "read ids") ||
-        !bc_check(c, hipStreamSynchronize(st), "flatten"))
+    if (!bc_d2h(c, res.ids.data(), c->d_ids, 4 * np, st, "read ids"))
         return BC_ERR_DEVICE;
     res.nnode8 = total;
     res.depth = nlevels;

@@ -1422,8 +1422,8 @@ extern "C" int bc_render(bc_ctx* c, const bc_render_desc* d, int y0, int rows, f
     uint32_t* dg = guides ? (uint32_t*)((char*)c->out_buf + npix * 16) : nullptr;
     int rc = bc_render_device(c, d, y0, rows, (float*)c->out_buf, dg, rays_out, nullptr);
     if (rc == BC_OK) {
-        if (!bc_check(c, hipMemcpy(rgba, c->out_buf, npix * 16, hipMemcpyDeviceToHost), "copy image") ||
-            (guides && !bc_check(c, hipMemcpy(guides, dg, npix * 12, hipMemcpyDeviceToHost), "copy guides")))
+        if (!bc_d2h(c, rgba, c->out_buf, npix * 16, c->stream, "copy image") ||
+            (guides && !bc_d2h(c, guides, dg, npix * 12, c->stream, "copy guides")))
             rc = BC_ERR_DEVICE;
     }
     return rc;
@@ -1470,7 +1470,7 @@ extern "C" int bc_forward(bc_ctx* c, const bc_render_desc* d, int y0, int rows, 
     if (!bc_ensure(c, c->out_buf, c->out_cap, npix * 28 + 64, "render output")) return BC_ERR_NOMEM;
     int rc = render_plain(c, d, y0, rows, (float*)c->out_buf, nullptr, rays_out, nullptr, (float4*)R.keep);
     if (rc != BC_OK) return rc;
-    if (!bc_check(c, hipMemcpy(rgba, c->out_buf, npix * 16, hipMemcpyDeviceToHost), "copy image")) return BC_ERR_DEVICE;
+    if (!bc_d2h(c, rgba, c->out_buf, npix * 16, c->stream, "copy image")) return BC_ERR_DEVICE;
     R.keep_desc = *d;
     R.keep_y0 = y0;
     R.keep_rows = rows;

@@ -1,11 +1,12 @@
-// upload.cpp -- host-to-device copies of large pageable buffers (scene views, refit geometry,
-// build inputs).
+// upload.cpp -- copies between the device and large pageable host buffers (scene views, refit
+// geometry, build inputs and outputs, ray batches, rendered frames).
 //
-// hipMemcpyAsync from pageable memory stages the data through the runtime's own buffers with
-// one host thread.  Here a large source is cut into chunks; host threads copy a chunk into one
-// of two pinned buffers owned by the context while the DMA engine reads the other, so the host
-// copy and the PCIe transfer overlap.  An event per buffer keeps a buffer from being rewritten
-// before the transfer that reads it has finished.  Small copies go straight to hipMemcpyAsync.
+// hipMemcpyAsync to or from pageable memory stages the data through the runtime's own buffers
+// with one host thread.  Here a large copy is cut into chunks; host threads copy a chunk into
+// (or out of) one of two pinned buffers owned by the context while the DMA engine moves the
+// other, so the host copy and the PCIe transfer overlap.  An event per buffer keeps a buffer from
+// being reused before the transfer that reads or fills it has finished.  Small copies go
+// straight to hipMemcpyAsync.
 #include <algorithm>
 #include <cstring>
 #include <thread>
@@ -79,6 +80,37 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
         b ^= 1;
     }
     c->pin_next = b;
+    return true;
+}
+
+bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what) {
+    if (bytes == 0) return true;
+    if (bytes <= kDirectMax)
+        return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), what) &&
+               bc_check(c, hipStreamSynchronize(st), what);
+    if (!ensure_pinned(c)) return false;
+    for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
+        if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
+        c->pin_busy[i] = false;
+    }
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nth = std::max(1u, std::min(kMaxThreads, hw));
+    const char* s = static_cast<const char*>(src);
+    char* d = static_cast<char*>(dst);
+    const size_t K = bc_ctx::kPinChunk, nchunk = (bytes + K - 1) / K;
+    // chunk k lands in buffer k & 1; chunk k + 1 is queued before chunk k is copied out, into
+    // the buffer whose chunk (k - 1) the host has already copied out
+    auto queue = [&](size_t k) {
+        const size_t n = std::min(K, bytes - k * K);
+        return bc_check(c, hipMemcpyAsync(c->h_pin[k & 1], s + k * K, n, hipMemcpyDeviceToHost, st), what) &&
+               bc_check(c, hipEventRecord(c->pin_ev[k & 1], st), what);
+    };
+    if (!queue(0)) return false;
+    for (size_t k = 0; k < nchunk; k++) {
+        if (k + 1 < nchunk && !queue(k + 1)) return false;
+        if (!bc_check(c, hipEventSynchronize(c->pin_ev[k & 1]), what)) return false;
+        par_copy(d + k * K, static_cast<const char*>(c->h_pin[k & 1]), std::min(K, bytes - k * K), nth);
+    }
     return true;
 }
 

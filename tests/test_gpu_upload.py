@@ -1,10 +1,11 @@
-"""Host-to-device copies through the C ABI (bc_upload / bc_download).
+"""Copies between host and device through the C ABI (bc_upload / bc_download).
 
-Copies above 4 MB are staged through two 16 MB pinned buffers filled by host threads while the
-other one's DMA runs; scene uploads, refit, the device build and host-pointer ray queries all go
-through the same routine.  Bar: every byte identical after a round trip, at sizes on both sides
-of the direct-copy limit, ragged tails past whole chunks, and back-to-back uploads that reuse
-both pinned buffers before anything is read back.
+Copies above 4 MB, in either direction, are staged through two 16 MB pinned buffers that host
+threads fill or drain while the other one's DMA runs; scene uploads, refit, the device build,
+host-pointer ray queries and frame read-backs all go through the same routines.  Bar: every
+byte identical after a round trip, at sizes on both sides of the direct-copy limit, ragged
+tails past whole chunks, and back-to-back uploads that reuse both pinned buffers before
+anything is read back.
 """
 import numpy as np
 import pytest
