@@ -1105,3 +1105,67 @@ int64_t orc_render32(const orc_scene* s, const orc_render_desc* d, int y0, int r
                      uint32_t* guides, int64_t* shadow_rays, int nthreads) {
     return orc_render_32(s, d, y0, rows, rgba, guides, shadow_rays, nthreads);
 }
+
+/* ------------------------------------------------------------------ single-primitive entry points
+ * For known-answer tests of the primitive tests and the guide packing on their own (test
+ * infrastructure, like the rest of this file). */
+
+/* triHit (renderer.go:684-711) in fp64 on A, E1, E2 (9 doubles); ray6 = o.xyz, d.xyz */
+int orc_tri_hit64(const double* g9, const double* ray6, double tmin, double tmax, double* t_out) {
+    prim_t p;
+    memset(&p, 0, sizeof p);
+    p.kind = ORC_TRIANGLE;
+    p.a = V(g9[0], g9[1], g9[2]);
+    p.e1 = V(g9[3], g9[4], g9[5]);
+    p.e2 = V(g9[6], g9[7], g9[8]);
+    p.n = unit(cross(p.e1, p.e2));
+    hit64 h;
+    int r = tri_hit(&p, V(ray6[0], ray6[1], ray6[2]), V(ray6[3], ray6[4], ray6[5]), tmin, tmax, &h);
+    if (t_out) *t_out = r ? h.t : 0;
+    return r;
+}
+/* sphereHitAccel (renderer.go:650-678) in fp64; c4 = center.xyz, radius */
+int orc_sphere_hit64(const double* c4, const double* ray6, double tmin, double tmax, double* t_out) {
+    prim_t p;
+    memset(&p, 0, sizeof p);
+    p.kind = ORC_SPHERE;
+    p.center = V(c4[0], c4[1], c4[2]);
+    p.radius = c4[3];
+    rayacc ar = prepare(V(ray6[0], ray6[1], ray6[2]), V(ray6[3], ray6[4], ray6[5]));
+    hit64 h;
+    int r = sphere_hit(&p, &ar, tmin, tmax, &h);
+    if (t_out) *t_out = r ? h.t : 0;
+    return r;
+}
+/* the fp32 triangle test of the GPU traversal on A, E1, E2 (9 floats) */
+int orc_tri_test32(const float* g9, const float* o, const float* d, float tmin, float tmax, float* t_out) {
+    prim32_t p;
+    memset(&p, 0, sizeof p);
+    p.kind = ORC_TRIANGLE;
+    for (int k = 0; k < 3; k++) { p.a[k] = g9[k]; p.e1[k] = g9[3 + k]; p.e2[k] = g9[6 + k]; }
+    float t = 0;
+    int r = tri_test32(&p, o, d, tmin, tmax, &t);
+    if (t_out) *t_out = r ? t : 0.0f;
+    return r;
+}
+/* the fp32 sphere test of the GPU traversal; c4 = center.xyz, radius (r^2 rounded from fp64 as
+ * the record packer does) */
+int orc_sphere_test32(const double* c4, const float* o, const float* d, float tmin, float tmax, float* t_out) {
+    prim32_t p;
+    memset(&p, 0, sizeof p);
+    p.kind = ORC_SPHERE;
+    for (int k = 0; k < 3; k++) p.c[k] = (float)c4[k];
+    p.r = (float)c4[3];
+    p.r2 = (float)(c4[3] * c4[3]);
+    p.invr = (float)(1.0 / c4[3]);
+    float t = 0;
+    int r = sphere_test32(&p, o, d, tmin, tmax, &t);
+    if (t_out) *t_out = r ? t : 0.0f;
+    return r;
+}
+/* packGuideNormalMat / packJitter (gpu_opencl_windows.go:787-797) as the integrator writes them */
+uint32_t orc_pack_guide32(const float* n3, int mat) {
+    vec_32 n = {n3[0], n3[1], n3[2]};
+    return pack_guide_32(n, mat);
+}
+uint32_t orc_pack_jitter(float ju, float jv) { return pack_jitter_32(ju, jv); }

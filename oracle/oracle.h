@@ -137,6 +137,14 @@ int orc_post(int w, int h, const float* rgba, const uint32_t* guides, int denois
              int txaa_level, double exposure, double gamma, int hdr, double* linear_out, uint32_t* pixels_out);
 uint32_t orc_to_rgba(const double* c, double exposure, double gamma, int hdr);
 void orc_decode_oct_normal(uint32_t qx, uint32_t qy, double* n);
+/* single primitives (known-answer tests): triHit / sphereHitAccel in fp64, the GPU's fp32 tests */
+int orc_tri_hit64(const double* g9, const double* ray6, double tmin, double tmax, double* t_out);
+int orc_sphere_hit64(const double* c4, const double* ray6, double tmin, double tmax, double* t_out);
+int orc_tri_test32(const float* g9, const float* o, const float* d, float tmin, float tmax, float* t_out);
+int orc_sphere_test32(const double* c4, const float* o, const float* d, float tmin, float tmax, float* t_out);
+/* packGuideNormalMat / packJitter (gpu_opencl_windows.go:787-797) */
+uint32_t orc_pack_guide32(const float* n3, int mat);
+uint32_t orc_pack_jitter(float ju, float jv);
 
 #ifdef __cplusplus
 }

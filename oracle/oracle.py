@@ -70,6 +70,12 @@ def lib():
             "orc_post": (c_int, [c_int, c_int, vp, vp, c_int, c_int, c_int, c_double, c_double, c_int, vp, vp]),
             "orc_to_rgba": (c_uint32, [vp, c_double, c_double, c_int]),
             "orc_decode_oct_normal": (None, [c_uint32, c_uint32, vp]),
+            "orc_tri_hit64": (c_int, [vp, vp, c_double, c_double, POINTER(c_double)]),
+            "orc_sphere_hit64": (c_int, [vp, vp, c_double, c_double, POINTER(c_double)]),
+            "orc_tri_test32": (c_int, [vp, vp, vp, c_float, c_float, POINTER(c_float)]),
+            "orc_sphere_test32": (c_int, [vp, vp, vp, c_float, c_float, POINTER(c_float)]),
+            "orc_pack_guide32": (c_uint32, [vp, c_int]),
+            "orc_pack_jitter": (c_uint32, [c_float, c_float]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -260,3 +266,48 @@ def decode_oct_normal(qx, qy):
     n = np.zeros(3)
     lib().orc_decode_oct_normal(qx, qy, _p(n))
     return n
+
+
+def tri_hit64(a, e1, e2, o, d, tmin=1e-4, tmax=1e30):
+    """triHit (renderer.go:684-711) in fp64: (hit, t)."""
+    g = np.asarray(list(a) + list(e1) + list(e2), np.float64)
+    r = np.asarray(list(o) + list(d), np.float64)
+    t = c_double()
+    return bool(lib().orc_tri_hit64(_p(g), _p(r), tmin, tmax, ctypes.byref(t))), t.value
+
+
+def sphere_hit64(center, radius, o, d, tmin=1e-4, tmax=1e30):
+    """sphereHitAccel (renderer.go:650-678) in fp64: (hit, t)."""
+    c = np.asarray(list(center) + [radius], np.float64)
+    r = np.asarray(list(o) + list(d), np.float64)
+    t = c_double()
+    return bool(lib().orc_sphere_hit64(_p(c), _p(r), tmin, tmax, ctypes.byref(t))), t.value
+
+
+def tri_test32(a, e1, e2, o, d, tmin=1e-4, tmax=1e30):
+    """The GPU's fp32 triangle test: (hit, t)."""
+    g = np.asarray(list(a) + list(e1) + list(e2), np.float32)
+    of = np.asarray(o, np.float32)
+    df = np.asarray(d, np.float32)
+    t = c_float()
+    return bool(lib().orc_tri_test32(_p(g), _p(of), _p(df), tmin, tmax, ctypes.byref(t))), t.value
+
+
+def sphere_test32(center, radius, o, d, tmin=1e-4, tmax=1e30):
+    """The GPU's fp32 sphere test: (hit, t)."""
+    c = np.asarray(list(center) + [radius], np.float64)
+    of = np.asarray(o, np.float32)
+    df = np.asarray(d, np.float32)
+    t = c_float()
+    return bool(lib().orc_sphere_test32(_p(c), _p(of), _p(df), tmin, tmax, ctypes.byref(t))), t.value
+
+
+def pack_guide(n, mat):
+    """packGuideNormalMat (gpu_opencl_windows.go:787-791) as the integrator writes it."""
+    v = np.asarray(n, np.float32)
+    return int(lib().orc_pack_guide32(_p(v), int(mat)))
+
+
+def pack_jitter(ju, jv):
+    """packJitter (gpu_opencl_windows.go:792-796)."""
+    return int(lib().orc_pack_jitter(float(ju), float(jv)))
