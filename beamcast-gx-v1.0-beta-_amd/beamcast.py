@@ -84,6 +84,7 @@ def lib():
             "bc_scene_stats": (c_int, [vp, POINTER(c_int64)]),
             "bc_render": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), vp]),
             "bc_render_device": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
+            "bc_render_noalloc": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_render_tuning": (c_int, [vp, POINTER(c_int32)]),
             "bc_batch_upload": (c_int, [vp, c_int, vp]),
             "bc_postprocess_device": (c_int, [vp, POINTER(PostDesc), c_int, c_int, vp, vp, vp, vp]),
@@ -581,6 +582,13 @@ class Device:
         rays, shadow = c_uint64(), c_uint64()
         self._check(lib().bc_render_device(self.ctx, ctypes.byref(desc), y0, rows, d_rgba, d_guides,
                                            ctypes.byref(rays), ctypes.byref(shadow)))
+        return int(rays.value), int(shadow.value)
+
+    def render_noalloc(self, desc: RenderDesc, y0, rows, d_rgba, d_guides=None):
+        """bc_render_noalloc: the plain schedule with its state in the library's fixed scratch array."""
+        rays, shadow = c_uint64(), c_uint64()
+        self._check(lib().bc_render_noalloc(self.ctx, ctypes.byref(desc), y0, rows, d_rgba, d_guides,
+                                            ctypes.byref(rays), ctypes.byref(shadow)))
         return int(rays.value), int(shadow.value)
 
     def postprocess_device(self, post, width, height, d_rgba, d_guides, d_pixels, d_linear=None):

@@ -154,6 +154,7 @@ bc_ctx* bc_create(int device_ordinal) {
         bc_destroy(c);
         return nullptr;
     }
+    bc_scratch_init(c);
     return c;
 }
 

@@ -85,6 +85,10 @@ struct bc_ctx {
     RenderState* rs = nullptr;
     // post stack
     PostState* ps = nullptr;
+    // device address of this device's fixed scratch array (bc_render_noalloc), resolved at
+    // bc_create; null with the reason in scratch_err if that failed
+    void* scratch = nullptr;
+    std::string scratch_err;
 };
 
 bool bc_ensure(bc_ctx* c, void*& p, size_t& cap, size_t bytes, const char* what);
@@ -101,4 +105,6 @@ void bc_pinned_release(bc_ctx* c);
 int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data);
 int bc_check_kernel_error(bc_ctx* c);
 void bc_render_release(bc_ctx* c);
+// Resolves the fixed scratch array of bc_render_noalloc on c's device (integrator.hip).
+void bc_scratch_init(bc_ctx* c);
 void bc_post_release(bc_ctx* c);

@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "api_internal.hpp"
@@ -1396,6 +1398,140 @@ extern "C" int bc_batch_render_device(bc_ctx* c, int n, const bc_render_desc* de
     if (rays_out) *rays_out = rays;
     if (shadow_out) *shadow_out = shadow;
     return BC_OK;
+}
+
+// ------------------------------------------------------------------------------ no-allocation render
+// bc_render_noalloc runs the plain schedule with its path state, pixel sums, queue lengths and
+// tallies in one fixed device array, `scratch`, instead of buffers the library allocates.  Each GPU
+// holds its own copy of the array from the time the library's kernels are loaded on it until the
+// process ends.  Host code never passes the array's name to a kernel (on the host the name is a
+// stand-in, not a device address); it uses the address hipGetSymbolAddress returns for the current
+// device, resolved once per device by bc_scratch_init and kept in every context on that device.
+namespace {
+constexpr size_t kNaSlots = (size_t)1 << 18;   // path slots of one pass
+constexpr size_t kNaPixels = (size_t)1 << 18;  // pixels (rows x width) of one call
+constexpr size_t kNaMaxSeg = kNaSlots / (size_t)bcd::kBlock;  // plain segments are >= kBlock slots
+constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+// Byte offsets of the arrays carved from the scratch array.
+constexpr size_t kNaVec16 = 9;  // o d T L pp meta so sd sc: 16 bytes per slot each
+constexpr size_t kNaOffHit = kNaVec16 * kNaSlots * 16;
+constexpr size_t kNaOffU32 = kNaOffHit + kNaSlots * 8;   // nseg q0 q1 sq: 4 bytes per slot each
+constexpr size_t kNaOffAcc = kNaOffU32 + 4 * kNaSlots * 4;
+constexpr size_t kNaOffSeg = kNaOffAcc + kNaPixels * 16;  // qc0 qc1 sqc: 4 bytes per segment each
+constexpr size_t kNaOffTally = na_align(kNaOffSeg + 3 * kNaMaxSeg * 4);
+constexpr size_t kNaTallyWords = (size_t)bcd::kMaxGrid + kNaMaxSeg;
+constexpr size_t kNaOffTotals = na_align(kNaOffTally + kNaTallyWords * 8);
+constexpr size_t kNaBytes = na_align(kNaOffTotals + 64);
+static_assert(kNaBytes < ((size_t)49 << 20), "scratch of the no-allocation render is stated as under 49 MiB");
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+void* g_na_ptr[kNaMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+namespace bcd {
+__global__ __launch_bounds__(kBlock) void k_na_clear(unsigned long long* __restrict__ w, uint32_t n) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) w[i] = 0ull;
+}
+}  // namespace bcd
+
+void bc_scratch_init(bc_ctx* c) {
+    if (!c) return;
+    if (c->device < 0 || c->device >= kNaMaxDevices) {
+        c->scratch_err = "device ordinal beyond the scratch table";
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[c->device]) {
+        // bc_create made c->device current; the address is that device's copy of the array
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            c->scratch_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return;
+        }
+        g_na_ptr[c->device] = p;
+    }
+    c->scratch = g_na_ptr[c->device];
+}
+
+extern "C" int bc_render_noalloc(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba,
+                                 uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_out) {
+    if (!c || !d) return BC_ERR_ARG;
+    if (!c->scratch) {
+        c->err = "no scratch array on this device: " + c->scratch_err;
+        return BC_ERR_DEVICE;
+    }
+    if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
+    if (!d_rgba) { c->err = "invalid render description or row range"; return BC_ERR_ARG; }
+    if (!valid_desc(c, d, y0, rows)) return BC_ERR_ARG;
+    if (d->mode != 0 || d->restir_spatial != 0) {
+        c->err = "the no-allocation render runs the plain schedule (mode 0) without spatial reuse";
+        return BC_ERR_ARG;
+    }
+    const size_t npix = (size_t)rows * (size_t)d->width;
+    if (npix > kNaPixels) {
+        c->err = "the no-allocation render takes at most 262144 pixels (rows x width) per call";
+        return BC_ERR_ARG;
+    }
+    (void)hipSetDevice(c->device);
+    int spp_pass = (int)std::max<size_t>(1, kNaSlots / npix);
+    if (d->samples_per_pass > 0) spp_pass = std::min(spp_pass, d->samples_per_pass);
+    spp_pass = std::min(spp_pass, d->spp);
+    const size_t paths = npix * (size_t)spp_pass;
+    const uint32_t ss = plain_segment(c, paths);
+    const uint32_t nseg = (uint32_t)segments_for(paths, ss);
+    char* base = static_cast<char*>(c->scratch);
+    bcd::PathBufs B{};
+    B.seg = ss;
+    B.xg = xcd_group(d);
+    float4** v16[kNaVec16 - 1] = {&B.o, &B.d, &B.T, &B.L, &B.pp, &B.so, &B.sd, &B.sc};
+    for (size_t k = 0; k < kNaVec16 - 1; k++) *v16[k] = (float4*)(base + k * kNaSlots * 16);
+    B.meta = (uint4*)(base + (kNaVec16 - 1) * kNaSlots * 16);
+    B.hit = (float2*)(base + kNaOffHit);
+    uint32_t* u32 = (uint32_t*)(base + kNaOffU32);
+    B.nseg = u32;
+    B.q[0] = u32 + kNaSlots;
+    B.q[1] = u32 + 2 * kNaSlots;
+    B.sq = u32 + 3 * kNaSlots;
+    float4* acc = (float4*)(base + kNaOffAcc);
+    uint32_t* segw = (uint32_t*)(base + kNaOffSeg);
+    B.qc[0] = segw;
+    B.qc[1] = segw + kNaMaxSeg;
+    B.sqc = segw + 2 * kNaMaxSeg;
+    B.tally = (unsigned long long*)(base + kNaOffTally);
+    B.totals = (unsigned long long*)(base + kNaOffTotals);
+    B.flag = (uint32_t*)(base + kNaOffTotals + 32);
+    bcd::SceneBufs S{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
+                     c->stack, c->trace_refill[1]};
+    bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr);
+    const int grid = seg_grid(c, nseg);
+    const int grid_pix = pix_grid(c, npix);
+    hipStream_t st = c->stream;
+    // the array keeps the previous call's contents: the tallies start at zero by a fill kernel
+    // (the pixel sums are zeroed by the first pass's raygen, queue lengths written before use)
+    const uint32_t ntally = (uint32_t)((size_t)bcd::kMaxGrid + nseg);
+    hipLaunchKernelGGL(bcd::k_na_clear, dim3((ntally + bcd::kBlock - 1) / bcd::kBlock), dim3(bcd::kBlock), 0, st, B.tally,
+                       ntally);
+    for (int s0 = 0; s0 < d->spp; s0 += spp_pass) {
+        P.s0 = s0;
+        P.ns = std::min(spp_pass, d->spp - s0);
+        hipLaunchKernelGGL(bcd::k_raygen, dim3(grid), dim3(bcd::kBlock), 0, st, P, B, (float*)acc, d_guides, nseg);
+        for (int b = 0; b < d->bounces; b++) launch_bounce(S, B, P, b & 1, b, d_guides, grid, nseg, st);
+        hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, B, acc, (float4*)nullptr);
+        if (!bc_check(c, hipGetLastError(), "render launch")) return BC_ERR_DEVICE;
+    }
+    hipLaunchKernelGGL(bcd::k_finalize, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, (const float4*)acc, (float4*)d_rgba);
+    hipLaunchKernelGGL(bcd::k_tally, dim3(1), dim3(bcd::kTallyBlock), 0, st, B, nseg);
+    unsigned long long tot[2] = {0, 0};
+    if (!bc_check(c, hipGetLastError(), "finalize launch") ||
+        !bc_check(c, hipMemcpyAsync(tot, B.totals, 16, hipMemcpyDeviceToHost, st), "read totals") ||
+        !bc_check(c, hipStreamSynchronize(st), "render"))
+        return BC_ERR_DEVICE;
+    if (rays_out) *rays_out = tot[0];
+    if (shadow_out) *shadow_out = tot[1];
+    return bc_check_kernel_error(c);
 }
 
 extern "C" int bc_render_tuning(const bc_ctx* c, int32_t* out8) {

@@ -168,6 +168,21 @@ int bc_render(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* 
 int bc_render_device(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* d_rgba,
                      uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_rays_out);
 
+/* No counterpart in the reference: bc_render_device for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Renders rows [y0, y0+rows) with the plain schedule (desc->mode must be 0, restir_spatial 0)
+ * into the caller's d_rgba / d_guides (device memory, bc_render_device's layout) with results
+ * bit-identical to bc_render_device.  It calls no hipMalloc, hipFree or other allocating routine:
+ * its path state, pixel sums, queue lengths and tallies live in one fixed device array of under
+ * 49 MiB that each GPU holds from the library's first use until the process ends.  Limits:
+ * rows * width <= 262144 pixels per call (a row chunk of a larger frame, as renderOpenCL renders
+ * them); samples are rendered in passes of at most 262144 paths.  Larger calls return
+ * BC_ERR_ARG.  Every call on one GPU shares that GPU's array and leaves its contents behind (the
+ * next call clears what must start at zero): calls that overlap on one GPU, from several
+ * contexts or threads, race and must be serialised by the caller. */
+int bc_render_noalloc(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* d_rgba,
+                      uint32_t* d_guides, uint64_t* rays_out, uint64_t* shadow_out);
+
 /* Differentiable render (path-replay backpropagation; no counterpart in the reference).
  * bc_forward renders like bc_render (plain schedule, host rgba) and keeps each sample's path
  * radiance; bc_backward replays every sample from its key and accumulates (+=) the gradient of
