@@ -576,7 +576,7 @@ class Device:
         o = (c_int32 * 8)()
         self._check(lib().bc_render_tuning(self.ctx, o))
         return {"pool": o[0], "iters_per_graph": o[1], "stack": o[2], "windows": o[3], "segment": o[4],
-                "material_sort": o[5]}
+                "material_sort": o[5], "plain": o[6]}
 
     def render_device(self, desc: RenderDesc, y0, rows, d_rgba, d_guides=None):
         rays, shadow = c_uint64(), c_uint64()
@@ -636,7 +636,7 @@ class Device:
 def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampler=1, mis=True, power=True,
                 firefly_clamp=0.0, seed=0x9E3779B9, sky_mode=0, sky_color=(0.3, 0.3, 0.3), mode=0,
                 samples_per_pass=0, pool=0, iters=0, segment=0, xcd_group=0, material_sort=0, restir=0, sun_dir=(0.0, 1.0, 0.0), sun=(0.0, 0.0, 0.0),
-                spatial=0, radius=0):
+                spatial=0, radius=0, window=0):
     """RenderDesc for a workload scene (camera from the workload, Camera.basis in fp64)."""
     cam = scene.camera
     basis = camera_basis(cam[0:3], cam[3:6], cam[6:9], cam[9], width / height)
@@ -649,7 +649,7 @@ def render_desc(scene: HostScene, width, height, spp, bounces, rr_depth=4, sampl
         d.origin[k], d.lower[k], d.horiz[k], d.vert[k] = basis[0][k], basis[1][k], basis[2][k], basis[3][k]
     d.mode, d.samples_per_pass = mode, samples_per_pass
     d.reserved[0], d.reserved[1], d.reserved[2], d.reserved[3] = pool, iters, segment, xcd_group
-    d.reserved[4] = material_sort
+    d.reserved[4], d.reserved[5] = material_sort, window
     d.restir_candidates = restir
     d.restir_spatial, d.restir_radius = spatial, radius
     n = np.asarray(sun_dir, np.float64)

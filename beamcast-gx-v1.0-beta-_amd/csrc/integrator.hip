@@ -819,10 +819,10 @@ struct RenderState {
     struct Tune {
         int w, rows, spp, bounces;
         uint64_t topo, geom, shape;  // shape: hash of a batch's frame sizes (0: one scene)
-        int pool, iters, seg, sort;
+        int pool, iters, seg, sort, plain;
     };
     std::vector<Tune> tunes;
-    int last_pool = 0, last_iters = 0, last_windows = 0, last_seg = 0, last_sort = 0;
+    int last_pool = 0, last_iters = 0, last_windows = 0, last_seg = 0, last_sort = 0, last_plain = 0;
 };
 
 namespace {
@@ -1093,7 +1093,9 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
         max_spp = std::max(max_spp, jb.d->spp);
     }
     const size_t cap = multi ? kBatchCap : kResultCap;
-    const int window_ns = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(1, cap / npix_all));
+    int window_ns = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(1, cap / npix_all));
+    // reserved[5] > 0 caps the samples per window below what the result buffer holds
+    if (jobs[0].d->reserved[5] > 0) window_ns = std::min(window_ns, jobs[0].d->reserved[5]);
     size_t total_first = 0;
     for (const Job& jb : jobs)
         total_first += (size_t)jb.rows * (size_t)jb.d->width * (size_t)std::min(window_ns, jb.d->spp);
@@ -1184,6 +1186,7 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
     R.last_seg = (int)ss;
     R.last_sort = sort ? 1 : 0;
     R.last_windows = windows;
+    R.last_plain = 0;
     return finish_tally(c, rays_out, shadow_out);
 }
 
@@ -1197,13 +1200,29 @@ uint64_t jobs_shape(const std::vector<Job>& jobs) {
     return h;
 }
 
+// Renders the jobs with the plain schedule: each scene (or the single scene's rows) in turn.
+int render_jobs_plain(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* rays_out, uint64_t* shadow_out) {
+    uint64_t rays = 0, shadow = 0;
+    for (const Job& jb : jobs) {
+        uint64_t r = 0, sh = 0;
+        const int rc = render_plain(c, jb.d, jb.y0, jb.rows, jb.d_rgba, jb.d_guides, &r, &sh, nullptr, jb.scene);
+        if (rc != BC_OK) return rc;
+        rays += r;
+        shadow += sh;
+    }
+    if (rays_out) *rays_out = rays;
+    if (shadow_out) *shadow_out = shadow;
+    return BC_OK;
+}
+
 // Autotuner: times short renders of the same jobs, with enough samples to refill the pool about
 // three times (at least 1 spp), and keeps the fastest per sample for this image shape, scene and
 // transport depth.  Pool size and segment size are swept at 8 iterations per graph, then the
-// graph length for the best pair, then material-sorted shading for the best triple.  Each trial
-// is timed after one untimed run (which captures the graph) unless the trial itself holds more
-// than 2^24 path samples.
-bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg, bool& sort) {
+// graph length for the best pair, then material-sorted shading for the best triple, and last the
+// plain schedule on the same samples (on a small scene its passes already fill the chip and the
+// persistent bookkeeping only costs).  Each trial is timed after one untimed run (which captures
+// the graph) unless the trial itself holds more than 2^24 path samples.
+bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg, bool& sort, bool& plain) {
     RenderState& R = *c->rs;
     const bc_render_desc* d = jobs[0].d;
     const uint64_t shape = jobs_shape(jobs);
@@ -1214,6 +1233,7 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
             iters = t.iters;
             seg = (uint32_t)t.seg;
             sort = t.sort != 0;
+            plain = t.plain != 0;
             return true;
         }
     size_t npix = 0;
@@ -1231,6 +1251,8 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
     iters = 8;
     seg = bcd::kSegPersist;
     sort = false;
+    plain = false;
+    // p: pool slots (also the trial's sample count); it < 0 times the plain schedule
     auto time_trial = [&](int p, int it, uint32_t ss, bool so, float& per_sample) -> bool {
         const int tspp = (int)std::min<size_t>((size_t)max_spp, std::max<size_t>(1, (3 * (size_t)p + npix - 1) / npix));
         for (size_t k = 0; k < jobs.size(); k++) {
@@ -1238,12 +1260,13 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
             trial[k].spp = std::min(trial[k].spp, tspp);
             tj[k].d = &trial[k];
         }
-        if (npix * (size_t)tspp <= ((size_t)1 << 24) &&
-            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss, so) != BC_OK)
-            return false;
+        auto run = [&]() {
+            return it < 0 ? render_jobs_plain(c, tj, nullptr, nullptr)
+                          : render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss, so);
+        };
+        if (npix * (size_t)tspp <= ((size_t)1 << 24) && run() != BC_OK) return false;
         float ms = 0;
-        if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") ||
-            render_persistent_jobs(c, tj, nullptr, nullptr, p, it, ss, so) != BC_OK ||
+        if (!bc_check(c, hipEventRecord(c->ev0, c->stream), "tune") || run() != BC_OK ||
             !bc_check(c, hipEventRecord(c->ev1, c->stream), "tune") ||
             !bc_check(c, hipEventSynchronize(c->ev1), "tune") ||
             !bc_check(c, hipEventElapsedTime(&ms, c->ev0, c->ev1), "tune"))
@@ -1269,26 +1292,57 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
         if (!time_trial(pool, iters, seg, true, ms)) return false;
         if (ms < best) { best = ms; sort = true; }
     }
+    {
+        float ms;
+        if (!time_trial(pool, -1, seg, false, ms)) return false;
+        if (ms < best) { best = ms; plain = true; }
+    }
     R.tunes.push_back({d->width, jobs[0].rows, d->spp, d->bounces, c->topo, c->geom, shape, pool, iters, (int)seg,
-                       sort ? 1 : 0});
+                       sort ? 1 : 0, plain ? 1 : 0});
     return true;
 }
 
-// Schedule shape of a mode-1 render from the first description: explicit or autotuned.
-int persistent_shape(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg, bool& sort) {
+// Schedule shape of a mode-1 render from the first description: explicit or autotuned (which may
+// choose the plain schedule).
+int persistent_shape(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, uint32_t& seg, bool& sort,
+                     bool& plain) {
     const bc_render_desc* d = jobs[0].d;
     pool = d->reserved[0];
     iters = d->reserved[1];
     seg = d->reserved[2] > 0 ? (uint32_t)d->reserved[2] : bcd::kSegPersist;
     sort = d->reserved[4] != 0;
+    plain = false;
+    if (d->reserved[5] < 0) {
+        c->err = "samples per window (reserved[5]) must not be negative";
+        return BC_ERR_ARG;
+    }
     if (seg % bcd::kBlock != 0 || seg > bcd::kSegMax) {
         c->err = "segment size (reserved[2]) must be a multiple of 256 up to 4096";
         return BC_ERR_ARG;
     }
     if (pool <= 0 || iters <= 0) {
-        if (!autotune(c, jobs, pool, iters, seg, sort)) return BC_ERR_DEVICE;
+        if (!autotune(c, jobs, pool, iters, seg, sort, plain)) return BC_ERR_DEVICE;
     }
     return BC_OK;
+}
+
+// Runs the jobs of a mode-1 call with the schedule persistent_shape chose.
+int render_mode1(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* rays_out, uint64_t* shadow_out) {
+    int pool, iters;
+    uint32_t seg;
+    bool sort, plain;
+    int rc = persistent_shape(c, jobs, pool, iters, seg, sort, plain);
+    if (rc != BC_OK) return rc;
+    if (!plain) return render_persistent_jobs(c, jobs, rays_out, shadow_out, pool, iters, seg, sort);
+    rc = render_jobs_plain(c, jobs, rays_out, shadow_out);
+    RenderState& R = *c->rs;
+    R.last_pool = pool;
+    R.last_iters = iters;
+    R.last_seg = (int)seg;
+    R.last_sort = sort ? 1 : 0;
+    R.last_windows = 0;
+    R.last_plain = 1;
+    return rc;
 }
 
 bool valid_desc(bc_ctx* c, const bc_render_desc* d, int y0, int rows) {
@@ -1342,12 +1396,7 @@ extern "C" int bc_render_device(bc_ctx* c, const bc_render_desc* d, int y0, int 
     if (d->mode == 0 || d->bounces == 0) return render_plain(c, d, y0, rows, d_rgba, d_guides, rays_out, shadow_out);
     if (!ensure_state(c, 1, (size_t)rows * (size_t)d->width, d_guides != nullptr, 1, bcd::kSegPersist)) return BC_ERR_NOMEM;
     const std::vector<Job> jobs = {Job{d, y0, rows, -1, d_rgba, d_guides}};
-    int pool, iters;
-    uint32_t seg;
-    bool sort;
-    const int rc = persistent_shape(c, jobs, pool, iters, seg, sort);
-    if (rc != BC_OK) return rc;
-    return render_persistent_jobs(c, jobs, rays_out, shadow_out, pool, iters, seg, sort);
+    return render_mode1(c, jobs, rays_out, shadow_out);
 }
 
 extern "C" int bc_batch_render_device(bc_ctx* c, int n, const bc_render_desc* descs, float* const* d_rgba,
@@ -1387,12 +1436,7 @@ extern "C" int bc_batch_render_device(bc_ctx* c, int n, const bc_render_desc* de
             return BC_ERR_ARG;
         }
         if (!ensure_state(c, 1, 1, false, 1, bcd::kSegPersist, n)) return BC_ERR_NOMEM;
-        int pool, iters;
-        uint32_t seg;
-        bool sort;
-        int rc = persistent_shape(c, jobs, pool, iters, seg, sort);
-        if (rc != BC_OK) return rc;
-        rc = render_persistent_jobs(c, jobs, &rays, &shadow, pool, iters, seg, sort);
+        const int rc = render_mode1(c, jobs, &rays, &shadow);
         if (rc != BC_OK) return rc;
     }
     if (rays_out) *rays_out = rays;
@@ -1542,7 +1586,8 @@ extern "C" int bc_render_tuning(const bc_ctx* c, int32_t* out8) {
     out8[3] = c->rs ? c->rs->last_windows : 0;
     out8[4] = c->rs ? c->rs->last_seg : 0;
     out8[5] = c->rs ? c->rs->last_sort : 0;
-    for (int k = 6; k < 8; k++) out8[k] = 0;
+    out8[6] = c->rs ? c->rs->last_plain : 0;
+    out8[7] = 0;
     return BC_OK;
 }
 

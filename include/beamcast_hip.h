@@ -135,9 +135,12 @@ typedef struct bc_render_desc {
     int32_t mode;               /* 0 plain wavefront, 1 persistent + graph */
     int32_t samples_per_pass;   /* 0 = automatic */
     int32_t reserved[6];        /* mode 1: [0] pool slots, [1] iterations per graph (0 for either =
-                                   autotune), [2] slots per segment (0 = 256; multiple of 256);
+                                   autotune, which may pick the plain schedule), [2] slots per
+                                   segment (0 = 256; multiple of 256);
                                    any mode: [3] path segments per XCD turn (0 = 1);
-                                   mode 1: [4] 1 = shade each segment's paths sorted by material */
+                                   mode 1: [4] 1 = shade each segment's paths sorted by material,
+                                   [5] at most this many samples per sample window (0 = as many
+                                   as the per-sample result buffer holds) */
     /* Direct lighting by resampled importance sampling (ReSTIR DI initial candidates; an
      * extension, the Go integrator has one-sample NEE): restir_candidates > 0 draws that many
      * candidates per diffuse vertex from {sun, sky, emitters} and keeps one in a reservoir. */
@@ -231,8 +234,9 @@ int bc_render_post(bc_ctx* ctx, const bc_render_desc* desc, const bc_post_desc* 
                    float* linear, uint64_t* rays_out);
 
 /* Schedule last used by a mode-1 render, 8 ints: pool slots, iterations per graph, traversal
- * stack depth, sample windows, slots per segment, material-sorted shading (0/1), 0, 0.  No counterpart in the reference
- * (its kernel has one fixed shape). */
+ * stack depth, sample windows, slots per segment, material-sorted shading (0/1), 1 if the
+ * autotuner chose the plain schedule (the other fields then give the best persistent shape it
+ * found), 0.  No counterpart in the reference (its kernel has one fixed shape). */
 int bc_render_tuning(const bc_ctx* ctx, int32_t* out8);
 
 /* Ray queries against the uploaded scene (the traversal of gpu_opencl_windows.go:758-777

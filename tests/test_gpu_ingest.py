@@ -2,13 +2,15 @@
 build, upload).
 
 Bar: the ingested scene's tree equals the host build of the same primitives node for node, and
-its traversal equals that of the source scene uploaded from the host build (ids and distances
-bit-exact); a page store appended and built on the device traces like the host-built scene.
+its traversal equals both that of the source scene uploaded from the host build and the oracle's
+fp32 closest hit over the ingested primitives (ids and distances bit-exact); a page store appended
+and built on the device traces like the host-built scene and like the oracle.
 """
 import numpy as np
 import pytest
 
 import beamcast as bc
+import oracle as orc
 
 pytestmark = pytest.mark.gpu
 
@@ -37,6 +39,8 @@ def test_obj_to_device(dev, tmp_path):
     got = dev.trace(rays)
     assert np.array_equal(want["prim"], got["prim"]) and np.array_equal(want["t"], got["t"])
     assert (got["prim"] >= 0).mean() > 0.3
+    prim, t = orc.Scene.from_host(got_scene).hit32(rays)
+    assert np.array_equal(got["prim"], prim) and np.array_equal(got["t"], t)
     a, b = src.nodes(), got_scene.nodes()
     assert all(np.array_equal(x, y) for x, y in zip(a, b))
 
@@ -64,6 +68,9 @@ def test_pages_to_device(dev, tmp_path):
     dev.upload(gpu)
     b = dev.trace(rays)
     assert np.array_equal(a["prim"], b["prim"]) and np.array_equal(a["t"], b["t"])
+    prim, t = orc.Scene.from_host(gpu).hit32(rays)
+    assert np.array_equal(b["prim"], prim) and np.array_equal(b["t"], t)
+    assert (prim >= 0).mean() > 0.05
 
 
 def test_ingest_errors(dev, tmp_path):

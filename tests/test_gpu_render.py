@@ -143,11 +143,34 @@ def test_converged_4096spp_rmse(dev):
     assert abs(gpu[..., :3].mean() / scale - 1) < 0.003
 
 
+_SHOWCASE_REF = {}
+
+
+def showcase_oracle(s, d):
+    """The oracle's fp32 render of the persistent-schedule case (computed once per module)."""
+    if "ref" not in _SHOWCASE_REF:
+        ref, rg, rrays, _ = orc.Scene.from_host(s, bins=16, leaf=4).render(orc.desc_from(d), precision=32,
+                                                                            guides=True)
+        _SHOWCASE_REF["ref"] = (ref, rg, rrays)
+    return _SHOWCASE_REF["ref"]
+
+
+def assert_oracle(img, rays, ref, rrays, g=None, rg=None):
+    assert rays == rrays, f"path segments {rays} vs {rrays}"
+    assert np.array_equal(img[..., 3], ref[..., 3])
+    err = np.abs(img[..., :3] - ref[..., :3])
+    bad = err > 1e-4 * np.abs(ref[..., :3]) + 1e-6
+    assert not bad.any(), f"{bad.sum()} channel values outside 1e-4 rel; max err {err.max()}"
+    if g is not None:
+        assert np.array_equal(g, rg)
+
+
 @pytest.mark.parametrize("pool,iters,segment,msort", [(4096, 4, 0, 0), (50000, 8, 1024, 0), (3000, 6, 512, 1),
                                                       (20000, 4, 256, 1), (0, 0, 0, 0)])
 def test_persistent_schedule_equals_plain(dev, pool, iters, segment, msort):
     """In-place regeneration, hipGraph replay and the autotuner (pool=0) change the schedule but
-    not the image: per-sample results are reduced in sample order, so rgba is bit-identical."""
+    not the image: per-sample results are reduced in sample order, so rgba is bit-identical to
+    the plain schedule's, and within 1e-4 of the oracle's fp32 render of the same samples."""
     print(f"case: persistent pool={pool} iters={iters} segment={segment} material sort={msort}", flush=True)
     s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
     dev.upload(s)
@@ -158,13 +181,44 @@ def test_persistent_schedule_equals_plain(dev, pool, iters, segment, msort):
     pers, gq, rq = dev.render(dp, guides=True)
     assert rp == rq
     assert np.array_equal(plain, pers) and np.array_equal(gp, gq)
+    ref, rg, rrays = showcase_oracle(s, d)
+    assert_oracle(pers, rq, ref, rrays, gq, rg)
     t = dev.tuning()
     assert t["pool"] > 0 and t["iters_per_graph"] >= 2 and t["pool"] % t["segment"] == 0
     if pool:
-        assert t["segment"] == (segment or 256)
+        assert t["segment"] == (segment or 256) and t["plain"] == 0
     # a second call reuses the captured graph (and the tuned shape)
     again, _, _ = dev.render(dp, guides=True)
     assert np.array_equal(again, pers)
+
+
+@pytest.mark.parametrize("window,pool,iters", [(2, 4096, 4), (1, 3000, 6), (4, 0, 0)])
+def test_persistent_sample_windows(dev, window, pool, iters):
+    """Samples split into windows (reserved[5] caps the samples per window, as the per-sample
+    result buffer does for large frames): the per-window parameter rewrite, the accumulator reset
+    of the first window and the per-window reductions give the plain image bit for bit and the
+    oracle's within 1e-4."""
+    print(f"case: persistent windows of {window} samples, pool {pool}", flush=True)
+    s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
+    dev.upload(s)
+    d = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20)
+    plain, gp, rp = dev.render(d, guides=True)
+    dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=pool, iters=iters,
+                        window=window)
+    pers, gq, rq = dev.render(dp, guides=True)
+    t = dev.tuning()
+    if not t["plain"]:
+        assert t["windows"] == -(-6 // window), t
+    assert rp == rq and np.array_equal(plain, pers) and np.array_equal(gp, gq)
+    ref, rg, rrays = showcase_oracle(s, d)
+    assert_oracle(pers, rq, ref, rrays, gq, rg)
+    # a row chunk with windows: rows 7..29 against the oracle's render of the same rows
+    pc, _, rc = dev.render(dp, y0=7, rows=22)
+    refc, _, rrc, _ = orc.Scene.from_host(s, bins=16, leaf=4).render(orc.desc_from(d), precision=32, y0=7, rows=22)
+    assert_oracle(pc, rc, refc, rrc)
+    with pytest.raises(bc.BeamcastError, match="reserved"):
+        bad = bc.render_desc(s, 16, 16, 2, 2, mode=1, pool=4096, iters=4, window=-1)
+        dev.render(bad)
 
 
 SUN = dict(sun_dir=(0.3, 0.8, 0.2), sun=(3.0, 3.0, 3.0))
