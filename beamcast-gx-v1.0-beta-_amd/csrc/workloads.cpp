@@ -397,7 +397,7 @@ bch_scene* bch_workload_scene(int kind, uint64_t seed, int64_t param, int bins, 
     case 8: {  // scene `param` of the multi-scene batch: boxes, spheres and 2-16 quad lights
         s.Name = "batch scene";
         const uint64_t i = (uint64_t)std::max<int64_t>(0, param);
-        // triangle budget log-uniform in [1e4, 2e6]
+        // primitive budget log-uniform in [1e4, 2e6]
         const double tris = std::exp(std::log(1e4) + u01(seed, i, 0) * std::log(200.0));
         std::vector<int> pal;
         for (int k = 0; k < 6; k++)
@@ -415,8 +415,12 @@ bch_scene* bch_workload_scene(int kind, uint64_t seed, int64_t param, int bins, 
         pal.push_back(s.AddMaterial(glass));
         const int floor_m = mat_diffuse(s, V(.6, .6, .6));
         add_quad(s, V(-30, 0, -30), V(-30, 0, 30), V(30, 0, 30), V(30, 0, -30), floor_m);
-        // instanced boxes: 12 triangles each, about 90% of the budget, over a 20 x 20 lot
-        const int64_t nbox = std::max<int64_t>(1, (int64_t)(tris * .9 / 12));
+        // analytic spheres (0.5% of the budget) and 2-16 emissive quads overhead; instanced boxes
+        // of 12 triangles fill the rest of the budget, so the scene holds between the budget and
+        // 11 primitives more
+        const int64_t nsph = std::max<int64_t>(4, (int64_t)(tris / 200));
+        const int nl = 2 + (int)(u01(seed, i, 14) * 15);
+        const int64_t nbox = std::max<int64_t>(1, (int64_t)std::ceil((tris - 2.0 - (double)nsph - 2.0 * nl) / 12.0));
         const double lot = 10;
         for (int64_t b = 0; b < nbox; b++) {
             const uint64_t it = i * 0x100000000ull + (uint64_t)b;

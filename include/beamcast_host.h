@@ -113,8 +113,9 @@ void bch_camera_basis(const double* pos, const double* target, const double* up,
  *   5  Stress Grid (renderer.go:2980-2995)
  *   6  heightfield terrain: param x param grid cells, fBm heights, 64 materials
  *   7  closed box retessellated to ~param triangles with 64 materials
- *   8  scene `param` of the multi-scene batch: log-uniform [1e4, 2e6] triangles of instanced
- *      boxes, analytic spheres, 2-16 emissive quads, diffuse/metal/glass palette
+ *   8  scene `param` of the multi-scene batch: a log-uniform [1e4, 2e6] primitive budget filled
+ *      by instanced boxes (12 triangles each), analytic spheres and 2-16 emissive quads (between
+ *      the budget and 11 primitives more), diffuse/metal/glass palette
  * The scene is built (SAH, bins, leaf) before it is returned. */
 bch_scene* bch_workload_scene(int kind, uint64_t seed, int64_t param, int bins, int leaf, double* camera_out);
 /* Rays for the traversal workload (n x 8 floats: o.xyz, tmin, d.xyz, tmax):
