@@ -50,6 +50,23 @@ def post_desc(denoise=0, fxaa=0, txaa=0, hdr=2, exposure=0.0, gamma=2.2):
     return p
 
 
+class Settings(ctypes.Structure):
+    """bch_settings: the RenderSettings fields of the GPU render path (renderer.go:1027-1062)."""
+    _fields_ = [(n, c_int32) for n in ("width", "height", "spp", "bounces", "rr_depth", "sampler", "mis",
+                                       "power_lights", "msaa", "tsaa", "taa", "txaa", "hbao", "hbao_plus",
+                                       "debug_view", "denoise", "progressive", "publish_hz", "upscale")]
+    _fields_ += [("firefly_clamp", c_double)]
+
+
+class Guide(ctypes.Structure):
+    """bch_guide: Guide (renderer.go:726-731) as renderOpenCL decodes it."""
+    _fields_ = [("p", c_double * 3), ("n", c_double * 3), ("albedo", c_double * 3), ("depth", c_double),
+                ("valid", c_int32), ("pad", c_int32)]
+
+
+CHUNK_FN = ctypes.CFUNCTYPE(None, c_void_p, c_int32, c_int32, POINTER(c_double), c_double)
+
+
 class RenderDesc(ctypes.Structure):
     _fields_ = [
         ("width", c_int32), ("height", c_int32), ("spp", c_int32), ("bounces", c_int32),
@@ -140,6 +157,12 @@ def lib():
             "bch_export_materials": (None, [vp, vp, vp]),
             "bch_translate": (None, [vp, c_int, c_int, POINTER(c_double)]),
             "bch_camera_basis": (None, [POINTER(c_double)] * 3 + [c_double, c_double, POINTER(c_double)]),
+            "bch_settings_quality": (c_int, [ctypes.c_char_p, POINTER(Settings)]),
+            "bch_render_desc_for": (c_int, [POINTER(c_double), POINTER(Settings), POINTER(RenderDesc)]),
+            "bch_render_hip": (c_int, [vp, POINTER(c_double), POINTER(Settings), POINTER(c_int32), CHUNK_FN, vp,
+                                       POINTER(c_double), POINTER(Guide), POINTER(ctypes.c_int64), POINTER(c_int32),
+                                       ctypes.c_char_p, c_int, ctypes.c_char_p, c_int]),
+            "bch_render_hip_release": (None, []),
             "bch_workload_scene": (vp, [c_int, c_uint64, c_int64, c_int, c_int, POINTER(c_double)]),
             "bch_workload_rays": (c_int, [c_int, c_uint64, c_int64, c_int, c_int64, c_int64, vp]),
         }
@@ -191,6 +214,61 @@ def quality(name="", backend=None):
 def effective_spp(name, spp, msaa, tsaa):
     """effectiveSPP of Quality(name) with SPP, MSAA and TSAA replaced."""
     return int(lib().bch_effective_spp(name.encode(), spp, msaa, tsaa))
+
+
+def settings_quality(name):
+    """Quality(name) (renderer.go:1064-1117) as a Settings structure."""
+    st = Settings()
+    if lib().bch_settings_quality(name.encode(), ctypes.byref(st)) != 0:
+        raise ValueError("bch_settings_quality")
+    return st
+
+
+def render_desc_for(camera, settings):
+    """The RenderDesc renderOpenCL's host loop renders a frame with."""
+    cam = np.ascontiguousarray(camera, np.float64)
+    d = RenderDesc()
+    if lib().bch_render_desc_for(cam.ctypes.data_as(POINTER(c_double)), ctypes.byref(settings), ctypes.byref(d)) != 0:
+        raise ValueError("bch_render_desc_for")
+    return d
+
+
+def render_hip(scene, camera, settings, cancel=None, on_chunk=None, guides=False):
+    """renderOpenCL over the HIP backend (bch_render_hip, the process-wide runtime).
+
+    cancel: a ctypes c_int32 polled before every chunk; on_chunk(y0, rows, rgb[rows, W, 3], progress).
+    Returns dict(linear[H, W, 3], guides (structured array or None), rays, scene_action, device)."""
+    d = render_desc_for(camera, settings)
+    w, h = d.width, d.height
+    lin = np.zeros((h, w, 3))
+    gd = (Guide * (w * h))() if guides else None
+    rays = ctypes.c_int64()
+    action = c_int32()
+    name = ctypes.create_string_buffer(256)
+    err = ctypes.create_string_buffer(512)
+
+    def cb(_user, y0, rows, data, progress):
+        if on_chunk is not None:
+            arr = np.ctypeslib.as_array(data, shape=(rows * w * 3,)).reshape(rows, w, 3).copy()
+            on_chunk(int(y0), int(rows), arr, float(progress))
+
+    fn = CHUNK_FN(cb)
+    cam = np.ascontiguousarray(camera, np.float64)
+    rc = lib().bch_render_hip(scene.h, cam.ctypes.data_as(POINTER(c_double)), ctypes.byref(settings),
+                              ctypes.byref(cancel) if cancel is not None else None, fn, None,
+                              lin.ctypes.data_as(POINTER(c_double)), gd, ctypes.byref(rays), ctypes.byref(action),
+                              name, 256, err, 512)
+    if rc != 0:
+        raise BeamcastError(f"status {rc}: {err.value.decode()}")
+    g = None
+    if guides:
+        g = np.ctypeslib.as_array(gd).copy()
+    return {"linear": lin, "guides": g, "rays": int(rays.value), "scene_action": int(action.value),
+            "device": name.value.decode()}
+
+
+def render_hip_release():
+    lib().bch_render_hip_release()
 
 
 def camera_basis(pos, target, up, fov, aspect):

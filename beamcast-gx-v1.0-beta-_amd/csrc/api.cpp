@@ -26,6 +26,15 @@ constexpr int64_t kOrderMinRays = 1 << 16;
 namespace {
 thread_local std::string g_create_error;
 
+// renderOpenCL's deviceName: the marketing name, or when the runtime reports none (some drivers
+// leave hipDeviceProp_t::name empty) one made from the architecture and CU count.
+std::string device_label(const hipDeviceProp_t& p) {
+    const std::string arch = p.gcnArchName;
+    if (p.name[0] != '\0') return std::string(p.name) + " (" + arch + ")";
+    return "AMD GPU " + arch.substr(0, arch.find(':')) + " (" + std::to_string(p.multiProcessorCount) + " CUs, " +
+           arch + ")";
+}
+
 bool ok(bc_ctx* c, hipError_t e, const char* what) {
     if (e == hipSuccess) return true;
     if (c) c->err = std::string(what) + ": " + hipGetErrorString(e);
@@ -115,7 +124,7 @@ int bc_device_available(char* name, int cap) {
     } else {
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, 0) == hipSuccess && std::strstr(p.gcnArchName, "gfx950")) {
-            msg = std::string(p.name) + " (" + p.gcnArchName + ")";
+            msg = device_label(p);
             found = 1;
         } else {
             msg = "device 0 is not gfx950";
@@ -144,7 +153,7 @@ bc_ctx* bc_create(int device_ordinal) {
     }
     bc_ctx* c = new bc_ctx();
     c->device = device_ordinal;
-    c->name = std::string(p.name) + " (" + p.gcnArchName + ")";
+    c->name = device_label(p);
     c->num_cus = p.multiProcessorCount;
     if (!ok(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate") ||
         !ok(c, hipEventCreate(&c->ev0), "hipEventCreate") || !ok(c, hipEventCreate(&c->ev1), "hipEventCreate") ||

@@ -1,6 +1,7 @@
 // host_scene.cpp -- C++ mirror of the Go scene build and GPU render entry.
 // Each function names the renderer.go / gpu_opencl_windows.go code whose behaviour it keeps.
 #include "host_scene.hpp"
+#include "../../include/beamcast_host.h"
 
 #include <algorithm>
 #include <cctype>
@@ -419,10 +420,40 @@ Vec3 materialPreviewColor(const Material& m) {  // renderer.go:1467-1475
 
 // renderOpenCL (gpu_opencl_windows.go:565-723) over the C ABI: same chunking, output
 // decoding, guide decoding and error/cancel behaviour.
-int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, std::atomic<bool>* cancel,
+bc_render_desc renderDescFor(const Camera& cam, const RenderSettings& settings) {
+    int w = 0, h = 0;
+    internalSize(settings, w, h);
+    Vec3 origin, lower, horiz, vert;
+    cam.basis((double)w / (double)h, origin, lower, horiz, vert);
+    bc_render_desc d{};
+    d.width = w; d.height = h; d.spp = effectiveSPP(settings); d.bounces = settings.Bounces;
+    d.rr_depth = settings.RRDepth;
+    d.sampler = settings.Sampler == "Halton" ? 1 : (settings.Sampler == "R2" ? 2 : 0);
+    d.mis = settings.MIS; d.power_lights = settings.PowerLightSampling;
+    d.firefly_clamp = (float)settings.FireflyClamp;
+    d.seed = 0x9e3779b9u;  // the reference kernel's seed argument (gpu_opencl_windows.go:606)
+    const Vec3* src[4] = {&origin, &lower, &horiz, &vert};
+    double* dst[4] = {d.origin, d.lower, d.horiz, d.vert};
+    for (int k = 0; k < 4; k++) { dst[k][0] = src[k]->x; dst[k][1] = src[k]->y; dst[k][2] = src[k]->z; }
+    return d;
+}
+
+void renderHIPForget(const Scene* scene) {
+    std::lock_guard<std::mutex> lock(g_rt.mu);
+    if (!scene) {
+        if (g_rt.ctx) bc_destroy(g_rt.ctx);
+        g_rt.ctx = nullptr;
+        g_rt.scene = nullptr;
+    } else if (g_rt.scene == scene) {
+        g_rt.scene = nullptr;
+    }
+}
+
+int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, const volatile int32_t* cancel,
               const ChunkFn& onChunk, std::vector<Vec3>& linear, std::vector<Guide>& guides, int64_t& rays,
-              std::string& deviceName, std::string& err) {
+              std::string& deviceName, std::string& err, int* sceneAction) {
     rays = 0;
+    if (sceneAction) *sceneAction = 0;
     if (scene.Primitives.empty() || scene.Nodes.empty()) {
         err = "HIP backend requires a non-empty built scene";
         return BC_ERR_SCENE;
@@ -439,32 +470,27 @@ int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, s
     bool refitted = false;
     if (g_rt.scene == &scene && g_rt.topo == scene.GPUTopologyRevision && g_rt.geom != scene.GPUGeometryRevision) {
         refitted = bc_scene_refit(ctx, &scene.View(false), scene.GPUGeometryRevision) == BC_OK;
-        if (refitted) g_rt.geom = scene.GPUGeometryRevision;
+        if (refitted) {
+            g_rt.geom = scene.GPUGeometryRevision;
+            if (sceneAction) *sceneAction = 1;
+        }
     }
     if (!refitted &&
         (g_rt.scene != &scene || g_rt.topo != scene.GPUTopologyRevision || g_rt.geom != scene.GPUGeometryRevision)) {
+        g_rt.scene = nullptr;  // whatever was resident is replaced, even if this upload fails
         int rc = bc_scene_upload(ctx, &scene.View(false), scene.GPUTopologyRevision, scene.GPUGeometryRevision);
         if (rc != BC_OK) { err = bc_last_error(ctx); return rc; }
         g_rt.scene = &scene;
         g_rt.topo = scene.GPUTopologyRevision;
         g_rt.geom = scene.GPUGeometryRevision;
+        if (sceneAction) *sceneAction = 2;
     }
-    int w = 0, h = 0;
-    internalSize(settings, w, h);
+    const bc_render_desc d = renderDescFor(cam, settings);
+    const int w = d.width, h = d.height;
     const bool needGuides = settings.Denoise || settings.HBAO > 0 || settings.HBAOPlus > 0 ||
                             std::max(settings.TAA, std::max(settings.TSAA, settings.TXAA)) > 0 || settings.DebugView != 0;
-    Vec3 origin, lower, horiz, vert;
-    cam.basis((double)w / (double)h, origin, lower, horiz, vert);
-    bc_render_desc d{};
-    d.width = w; d.height = h; d.spp = effectiveSPP(settings); d.bounces = settings.Bounces;
-    d.rr_depth = settings.RRDepth;
-    d.sampler = settings.Sampler == "Halton" ? 1 : (settings.Sampler == "R2" ? 2 : 0);
-    d.mis = settings.MIS; d.power_lights = settings.PowerLightSampling;
-    d.firefly_clamp = (float)settings.FireflyClamp;
-    d.seed = 0x9e3779b9u;  // the reference kernel's seed argument (gpu_opencl_windows.go:606)
-    const Vec3* src[4] = {&origin, &lower, &horiz, &vert};
-    double* dst[4] = {d.origin, d.lower, d.horiz, d.vert};
-    for (int k = 0; k < 4; k++) { dst[k][0] = src[k]->x; dst[k][1] = src[k]->y; dst[k][2] = src[k]->z; }
+    const Vec3 origin = V(d.origin[0], d.origin[1], d.origin[2]), lower = V(d.lower[0], d.lower[1], d.lower[2]);
+    const Vec3 horiz = V(d.horiz[0], d.horiz[1], d.horiz[2]), vert = V(d.vert[0], d.vert[1], d.vert[2]);
     int chunkRows = 64;
     if (!settings.ProgressiveUpdates) chunkRows = h;
     else if (settings.PublishHz <= 15) chunkRows = 128;
@@ -474,7 +500,7 @@ int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, s
     std::vector<float> raw((size_t)w * chunkRows * 4);
     std::vector<uint32_t> rawGuide(needGuides ? (size_t)w * chunkRows * 3 : 0);
     for (int y0 = 0; y0 < h; y0 += chunkRows) {
-        if (cancel && cancel->load()) { err = "render cancelled"; return BC_ERR_CANCELLED; }
+        if (cancel && *cancel) { err = "render cancelled"; return BC_ERR_CANCELLED; }
         int rows = std::min(chunkRows, h - y0);
         uint64_t chunkRays = 0;
         int rc = bc_render(ctx, &d, y0, rows, raw.data(), needGuides ? rawGuide.data() : nullptr, &chunkRays, nullptr);
@@ -513,3 +539,120 @@ int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, s
 }
 
 }  // namespace bcx
+
+// ------------------------------------------------------------------------------ C entry points
+namespace {
+using bcx::V;
+using bcx::Vec3;
+bcx::RenderSettings settings_from(const bch_settings& c) {
+    bcx::RenderSettings s;
+    s.Width = c.width; s.Height = c.height; s.SPP = c.spp; s.Bounces = c.bounces; s.RRDepth = c.rr_depth;
+    s.Sampler = c.sampler == 1 ? "Halton" : (c.sampler == 2 ? "R2" : "Random");
+    s.MIS = c.mis != 0; s.PowerLightSampling = c.power_lights != 0;
+    s.MSAA = c.msaa; s.TSAA = c.tsaa; s.TAA = c.taa; s.TXAA = c.txaa; s.HBAO = c.hbao; s.HBAOPlus = c.hbao_plus;
+    s.DebugView = c.debug_view; s.Denoise = c.denoise != 0;
+    s.ProgressiveUpdates = c.progressive != 0; s.PublishHz = c.publish_hz;
+    static const char* const kUpscale[4] = {"Off", "Ultra Quality", "Balanced", "Performance"};
+    s.Upscale = kUpscale[c.upscale >= 0 && c.upscale < 4 ? c.upscale : 0];
+    s.FireflyClamp = c.firefly_clamp;
+    return s;
+}
+bcx::Camera camera_from(const double* c10) {
+    bcx::Camera cam;
+    cam.Position = V(c10[0], c10[1], c10[2]);
+    cam.Target = V(c10[3], c10[4], c10[5]);
+    cam.Up = V(c10[6], c10[7], c10[8]);
+    cam.FOV = c10[9];
+    return cam;
+}
+void put(char* dst, int cap, const std::string& s) {
+    if (dst && cap > 0) std::snprintf(dst, (size_t)cap, "%s", s.c_str());
+}
+}  // namespace
+
+extern "C" {
+
+int bch_settings_quality(const char* name, bch_settings* o) {
+    if (!o) return BC_ERR_ARG;
+    const bcx::RenderSettings q = bcx::Quality(name ? name : "");
+    std::memset(o, 0, sizeof *o);
+    o->width = q.Width; o->height = q.Height; o->spp = q.SPP; o->bounces = q.Bounces; o->rr_depth = q.RRDepth;
+    o->sampler = q.Sampler == "Halton" ? 1 : (q.Sampler == "R2" ? 2 : 0);
+    o->mis = q.MIS; o->power_lights = q.PowerLightSampling;
+    o->msaa = q.MSAA; o->tsaa = q.TSAA; o->taa = q.TAA; o->txaa = q.TXAA; o->hbao = q.HBAO; o->hbao_plus = q.HBAOPlus;
+    o->debug_view = q.DebugView; o->denoise = q.Denoise;
+    o->progressive = q.ProgressiveUpdates; o->publish_hz = q.PublishHz;
+    o->upscale = q.Upscale == "Ultra Quality" ? 1 : (q.Upscale == "Balanced" ? 2 : (q.Upscale == "Performance" ? 3 : 0));
+    o->firefly_clamp = q.FireflyClamp;
+    return BC_OK;
+}
+
+int bch_render_desc_for(const double* camera10, const bch_settings* st, bc_render_desc* out) {
+    if (!camera10 || !st || !out) return BC_ERR_ARG;
+    *out = bcx::renderDescFor(camera_from(camera10), settings_from(*st));
+    return BC_OK;
+}
+
+int bch_render_hip(bch_scene* s, const double* camera10, const bch_settings* st, const volatile int32_t* cancel,
+                   bch_chunk_fn on_chunk, void* user, double* linear, bch_guide* guides, int64_t* rays_out,
+                   int32_t* scene_action, char* device_name, int name_cap, char* err, int err_cap) {
+    if (!s || !camera10 || !st || !linear) {
+        put(err, err_cap, "invalid arguments");
+        return BC_ERR_ARG;
+    }
+    try {
+        const bcx::RenderSettings rs = settings_from(*st);
+        bcx::ChunkFn fn;
+        int w = 0, h = 0;
+        bcx::internalSize(rs, w, h);
+        std::vector<double> flat;
+        if (on_chunk)
+            fn = [&](int y0, int rows, const std::vector<Vec3>& data, double progress) {
+                flat.resize(data.size() * 3);
+                for (size_t i = 0; i < data.size(); i++) {
+                    flat[3 * i] = data[i].x; flat[3 * i + 1] = data[i].y; flat[3 * i + 2] = data[i].z;
+                }
+                on_chunk(user, y0, rows, flat.data(), progress);
+            };
+        std::vector<Vec3> lin;
+        std::vector<bcx::Guide> gd;
+        int64_t rays = 0;
+        int action = 0;
+        std::string name, msg;
+        const int rc = bcx::renderHIP(s->s, camera_from(camera10), rs, cancel, fn, lin, gd, rays, name, msg, &action);
+        put(device_name, name_cap, name);
+        if (scene_action) *scene_action = action;
+        if (rc != BC_OK) {
+            put(err, err_cap, msg);
+            return rc;
+        }
+        for (size_t i = 0; i < lin.size(); i++) {
+            linear[3 * i] = lin[i].x; linear[3 * i + 1] = lin[i].y; linear[3 * i + 2] = lin[i].z;
+        }
+        if (guides) {
+            for (size_t i = 0; i < (size_t)w * (size_t)h; i++) {
+                bch_guide& o = guides[i];
+                std::memset(&o, 0, sizeof o);
+                if (i >= gd.size() || !gd[i].Valid) continue;
+                const bcx::Guide& g = gd[i];
+                const Vec3 v3[3] = {g.P, g.N, g.Albedo};
+                double* d3[3] = {o.p, o.n, o.albedo};
+                for (int k = 0; k < 3; k++) { d3[k][0] = v3[k].x; d3[k][1] = v3[k].y; d3[k][2] = v3[k].z; }
+                o.depth = g.Depth;
+                o.valid = 1;
+            }
+        }
+        if (rays_out) *rays_out = rays;
+        return BC_OK;
+    } catch (const std::bad_alloc&) {
+        put(err, err_cap, "out of host memory");
+        return BC_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        put(err, err_cap, e.what());
+        return BC_ERR_DEVICE;
+    }
+}
+
+void bch_render_hip_release(void) { bcx::renderHIPForget(nullptr); }
+
+}  // extern "C"

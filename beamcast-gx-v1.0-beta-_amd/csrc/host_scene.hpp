@@ -119,11 +119,20 @@ struct Guide {  // renderer.go:726-731
     bool Valid = false;
 };
 
-// renderOpenCL's contract (gpu_opencl_windows.go:565-566) over the HIP backend.
+// renderOpenCL's contract (gpu_opencl_windows.go:565-566) over the HIP backend.  cancel (may be
+// null) is the atomic.Bool the Go caller sets, polled before every chunk.  sceneAction (may be
+// null) reports what ensureSceneLocked did: 0 the resident scene was reused, 1 refit on the
+// device (only the geometry revision changed), 2 full upload.
 using ChunkFn = std::function<void(int y0, int rows, const std::vector<Vec3>& data, double progress)>;
-int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, std::atomic<bool>* cancel,
+int renderHIP(Scene& scene, const Camera& cam, const RenderSettings& settings, const volatile int32_t* cancel,
               const ChunkFn& onChunk, std::vector<Vec3>& linear, std::vector<Guide>& guides,
-              int64_t& rays, std::string& deviceName, std::string& err);
+              int64_t& rays, std::string& deviceName, std::string& err, int* sceneAction = nullptr);
+// The bc_render_desc renderHIP renders a frame with: internalSize, effectiveSPP, the sampler and
+// light settings, the camera basis and the reference kernel's seed.
+bc_render_desc renderDescFor(const Camera& cam, const RenderSettings& settings);
+// The persistent runtime forgets `scene` (it is being freed), so a later scene at the same
+// address is uploaded rather than taken for the resident one; null releases the runtime.
+void renderHIPForget(const Scene* scene);
 
 }  // namespace bcx
 

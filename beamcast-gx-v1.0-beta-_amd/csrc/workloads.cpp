@@ -122,7 +122,10 @@ double vnoise(uint64_t seed, double x, double y) {
 extern "C" {
 
 bch_scene* bch_scene_new(void) { return new bch_scene(); }
-void bch_scene_free(bch_scene* s) { delete s; }
+void bch_scene_free(bch_scene* s) {
+    if (s) bcx::renderHIPForget(&s->s);  // the persistent runtime must not mistake a later scene for it
+    delete s;
+}
 int bch_add_material(bch_scene* s, int kind, const double* d) {
     bcx::Material m;
     m.Kind = kind;
@@ -429,16 +432,12 @@ bch_scene* bch_workload_scene(int kind, uint64_t seed, int64_t param, int bins, 
             const int m = pal[(size_t)(sm64(seed ^ it) % pal.size())];
             add_box(s, V(x, 0, z), w, h, d, 360 * u01(seed, it, 9), m);
         }
-        // analytic spheres
-        const int64_t nsph = std::max<int64_t>(4, (int64_t)(tris / 200));
         for (int64_t b = 0; b < nsph; b++) {
             const uint64_t it = i * 0x100000000ull + 0x80000000ull + (uint64_t)b;
             const double r = .1 + .4 * u01(seed, it, 10);
             const double x = -lot + 2 * lot * u01(seed, it, 11), z = -lot + 2 * lot * u01(seed, it, 12);
             s.AddSphere(V(x, r + 2 * u01(seed, it, 13), z), r, pal[(size_t)(sm64(seed ^ it ^ 0x5555) % pal.size())]);
         }
-        // 2-16 emissive quads overhead
-        const int nl = 2 + (int)(u01(seed, i, 14) * 15);
         for (int l = 0; l < nl; l++) {
             const uint64_t it = i * 64 + (uint64_t)l;
             bcx::Material lm;

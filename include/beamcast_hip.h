@@ -160,8 +160,10 @@ typedef struct bc_render_desc {
  *   rgba   rows*W*4 floats: mean radiance rgb, w = path segments (gpu_opencl_windows.go:820)
  *   guides rows*W*3 u32 (may be NULL): depth bits, oct-normal|material|valid, jitter
  *          (gpu_opencl_windows.go:787-797, decoded at :688-717)
- * rays_out receives the path-segment count (Meta.Rays).  cancel (may be NULL) is polled
- * between sample passes; a set flag returns BC_ERR_CANCELLED. */
+ * rays_out receives the path-segment count (Meta.Rays).  cancel (may be NULL) is read once, when
+ * the call starts: a set flag returns BC_ERR_CANCELLED before anything is rendered.  A frame is
+ * cancelled between chunks, as renderOpenCL polls its flag before each chunk
+ * (gpu_opencl_windows.go:644-646); bch_render_hip (beamcast_host.h) does that. */
 int bc_render(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* rgba,
               uint32_t* guides, uint64_t* rays_out, const volatile int32_t* cancel);
 /* Same, with rgba/guides in device memory (no host copy). shadow_rays_out may be NULL.

@@ -101,6 +101,45 @@ int bch_quality(const char* name, const char* backend, int32_t* out20, double* f
 /* effectiveSPP (renderer.go:2395-2417) of Quality(name) with its SPP, MSAA and TSAA replaced */
 int bch_effective_spp(const char* name, int spp, int msaa, int tsaa);
 
+/* The RenderSettings fields the GPU render path reads (renderer.go:1027-1062). */
+typedef struct bch_settings {
+    int32_t width, height, spp, bounces, rr_depth;
+    int32_t sampler;                 /* 0 Random, 1 Halton, 2 R2 */
+    int32_t mis, power_lights;
+    int32_t msaa, tsaa, taa, txaa, hbao, hbao_plus, debug_view, denoise;  /* effectiveSPP, guides */
+    int32_t progressive, publish_hz; /* ProgressiveUpdates, PublishHz: the chunk height */
+    int32_t upscale;                 /* internalSize: 0 Off, 1 Ultra Quality, 2 Balanced, 3 Performance */
+    double firefly_clamp;
+} bch_settings;
+/* Quality(name) (renderer.go:1064-1117) as bch_settings; 0, or BC_ERR_ARG without an output */
+int bch_settings_quality(const char* name, bch_settings* out);
+/* The render description renderOpenCL's host loop renders a frame with (internalSize, effectiveSPP,
+ * sampler and light settings, Camera.basis, the kernel's seed 0x9e3779b9); camera10 = pos.xyz,
+ * target.xyz, up.xyz, fov. */
+int bch_render_desc_for(const double* camera10, const bch_settings* settings, bc_render_desc* out);
+/* Guide (renderer.go:726-731) as renderOpenCL decodes it from the compact guides
+ * (gpu_opencl_windows.go:686-716). */
+typedef struct bch_guide {
+    double p[3], n[3], albedo[3], depth;
+    int32_t valid, pad;
+} bch_guide;
+/* onChunk (renderer.go:2657-2677): rows [y0, y0+rows) of the linear image, rows*W*3 doubles */
+typedef void (*bch_chunk_fn)(void* user, int32_t y0, int32_t rows, const double* rgb, double progress);
+/* renderOpenCL (gpu_opencl_windows.go:565-723) over the HIP backend: one process-wide runtime
+ * (context, lock and scene cache, as persistentCL), ensureSceneLocked's choice keyed on the scene
+ * and its revisions (*scene_action: 0 resident scene reused, 1 refit on the device because only
+ * GPUGeometryRevision changed, 2 full upload), the frame rendered in row chunks of 64 rows (32 at
+ * PublishHz >= 60, 128 at <= 15, all rows without progressive updates) with on_chunk called after
+ * each, the cancel flag polled before every chunk (BC_ERR_CANCELLED, "render cancelled"), and
+ * the outputs decoded as there: linear (W*H*3 doubles), guides (W*H, may be NULL; filled when the
+ * settings need guides: denoise, HBAO, TAA/TSAA/TXAA or a debug view), the path-segment count and
+ * the device name.  Errors return a status with the message in err. */
+int bch_render_hip(bch_scene* s, const double* camera10, const bch_settings* settings, const volatile int32_t* cancel,
+                   bch_chunk_fn on_chunk, void* user, double* linear, bch_guide* guides, int64_t* rays_out,
+                   int32_t* scene_action, char* device_name, int name_cap, char* err, int err_cap);
+/* Destroys the process-wide runtime of bch_render_hip (the next call creates it again). */
+void bch_render_hip_release(void);
+
 /* Camera.basis (renderer.go:947-960): out12 = origin, lower, horiz, vert */
 void bch_camera_basis(const double* pos, const double* target, const double* up, double fov,
                       double aspect, double* out12);
