@@ -102,6 +102,8 @@ def lib():
             "bc_render": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), vp]),
             "bc_render_device": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_render_noalloc": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
+            "bc_trace_counts": (c_int, [vp, vp, ctypes.c_int64, c_int, vp, vp]),
+            "bc_valu_probe": (c_int, [vp, c_int, c_int, c_int, POINTER(ctypes.c_float)]),
             "bc_render_tuning": (c_int, [vp, POINTER(c_int32)]),
             "bc_batch_upload": (c_int, [vp, c_int, vp]),
             "bc_postprocess_device": (c_int, [vp, POINTER(PostDesc), c_int, c_int, vp, vp, vp, vp]),
@@ -661,6 +663,18 @@ class Device:
         self._check(lib().bc_render_device(self.ctx, ctypes.byref(desc), y0, rows, d_rgba, d_guides,
                                            ctypes.byref(rays), ctypes.byref(shadow)))
         return int(rays.value), int(shadow.value)
+
+    def trace_counts(self, d_rays, n, any_hit, d_out, d_counts):
+        """bc_trace_counts: the query of trace_device plus per-ray (node visits, record tests,
+        id reads, 0) as n x 4 uint32 at d_counts (asynchronous)."""
+        self._check(lib().bc_trace_counts(self.ctx, d_rays, n, int(any_hit), d_out, d_counts))
+
+    def valu_probe(self, blocks, iters, mix=0):
+        """bc_valu_probe: milliseconds of `blocks` x 256 threads running `iters` slab-test visits
+        (mix 0) or rounds of eight independent FMAs (mix 1)."""
+        ms = ctypes.c_float()
+        self._check(lib().bc_valu_probe(self.ctx, mix, blocks, iters, ctypes.byref(ms)))
+        return float(ms.value)
 
     def render_noalloc(self, desc: RenderDesc, y0, rows, d_rgba, d_guides=None):
         """bc_render_noalloc: the plain schedule with its state in the library's fixed scratch array."""

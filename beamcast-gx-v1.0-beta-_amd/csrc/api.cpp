@@ -541,6 +541,27 @@ int bc_trace_device(bc_ctx* c, const void* d_rays, int64_t n, int any_hit, void*
     return ok(c, e, "trace launch") ? BC_OK : BC_ERR_DEVICE;
 }
 
+int bc_trace_counts(bc_ctx* c, const void* d_rays, int64_t n, int any_hit, void* d_out, uint32_t* d_counts) {
+    if (!c || n < 0 || (n > 0 && (!d_rays || !d_out || !d_counts))) return BC_ERR_ARG;
+    if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
+    hipSetDevice(c->device);
+    hipError_t e = bcd::launch_trace_count(c->stack, any_hit != 0, c->d_nodes, c->d_prims, c->d_ids, (const bc_ray*)d_rays,
+                                           n, d_out, c->d_err, (uint4*)d_counts, c->stream);
+    return ok(c, e, "counting trace launch") ? BC_OK : BC_ERR_DEVICE;
+}
+
+int bc_valu_probe(bc_ctx* c, int mix, int blocks, int iters, float* ms) {
+    if (!c || mix < 0 || mix > 1 || blocks <= 0 || iters <= 0 || !ms) return BC_ERR_ARG;
+    hipSetDevice(c->device);
+    if (!ok(c, hipEventRecord(c->ev0, c->stream), "probe") ||
+        !ok(c, bcd::launch_valu_probe(mix, blocks, (uint32_t)iters, 0x5EEDu, (uint32_t*)c->d_err + 8, c->stream),
+            "probe launch") ||
+        !ok(c, hipEventRecord(c->ev1, c->stream), "probe") || !ok(c, hipEventSynchronize(c->ev1), "probe") ||
+        !ok(c, hipEventElapsedTime(ms, c->ev0, c->ev1), "probe"))
+        return BC_ERR_DEVICE;
+    return BC_OK;
+}
+
 int bc_trace(bc_ctx* c, const float* rays, int64_t n, int any_hit, void* out) {
     if (!c || n < 0 || (n > 0 && (!rays || !out))) return BC_ERR_ARG;
     if (n == 0) return BC_OK;

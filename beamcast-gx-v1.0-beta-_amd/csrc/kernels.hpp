@@ -45,6 +45,15 @@ hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc
                         const bc_ray* rays, int64_t n, void* out, int refill, int* err, hipStream_t st,
                         const bc_ray* sorted = nullptr, const uint32_t* perm = nullptr, const int* oflag = nullptr);
 
+// The traversal of launch_trace, one ray per thread, that also writes each ray's node visits,
+// primitive-record tests and id-table reads to counts[i] (x, y, z; w = 0).
+hipError_t launch_trace_count(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids,
+                              const bc_ray* rays, int64_t n, void* out, int* err, uint4* counts, hipStream_t st);
+// VALU issue probes: `blocks` workgroups of 256 threads, each thread `iters` register-only
+// eight-child slab tests (mix 0) or rounds of eight independent FMAs (mix 1).  sink receives
+// nothing in practice (it keeps the loops from being dead).
+hipError_t launch_valu_probe(int mix, int blocks, uint32_t iters, uint32_t seed, uint32_t* sink, hipStream_t st);
+
 // Refit: records (and materials) of the moved primitives in leaf order from the view's arrays.
 hipError_t launch_refit_records(const double* geom, const int32_t* kind, const int32_t* mat, const uint32_t* ids, uint32_t n,
                                 bc_prim* prims, int32_t* pmat, int* err, hipStream_t st);

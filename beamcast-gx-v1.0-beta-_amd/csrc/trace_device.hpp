@@ -193,6 +193,8 @@ struct TravState {
     uint32_t ng_base, ng_bits, pg_base, pg_mask, node;
     int sp;
     bool have_node;
+    // the counting instantiation (COUNT = true) only: node visits, record tests and id-table reads
+    uint32_t n_node, n_rec, n_id;
 };
 
 // root: index of the scene's root node (0 for a single scene; scenes packed in one arena
@@ -210,12 +212,14 @@ __device__ __forceinline__ void trav_init(TravState& st, const Ray& r, float tma
     st.node = root;
     st.sp = 0;
     st.have_node = true;
+    st.n_node = st.n_rec = st.n_id = 0;
 }
 
 // One step; returns true when the traversal of this lane's ray is complete (st.best final).
 // ANY=true stops at the first primitive hit in [tmin, tmax].  stk points at this lane's column
-// of the LDS stack (entry i at stk[i * stride], i < cap).
-template <bool ANY>
+// of the LDS stack (entry i at stk[i * stride], i < cap).  COUNT=true also tallies the memory
+// work of the step in st.n_node / n_rec / n_id (the traversal itself is unchanged).
+template <bool ANY, bool COUNT = false>
 __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restrict__ nodes, const bc_prim* __restrict__ prims,
                                           const uint32_t* __restrict__ ids, uint2* stk, int stride, int cap,
                                           int* __restrict__ err) {
@@ -223,6 +227,7 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
     if (st.have_node) {
         visit_node(nodes, st.node, r, st.ix, st.iy, st.iz, st.oct, st.best.t, st.ng_base, st.ng_bits, st.pg_base, st.pg_mask);
         st.have_node = false;
+        if (COUNT) st.n_node++;
     }
     const bool sph_node = (st.pg_base & BC_NODE_HAS_SPHERES) != 0;
     const uint32_t pbase = st.pg_base & ~BC_NODE_HAS_SPHERES;
@@ -247,6 +252,10 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
         bool sp[kPrimBatch];
 #pragma unroll
         for (int u = 0; u < kPrimBatch; u++) sp[u] = sph_node && (ids[rr[u]] & BC_SPHERE_BIT);
+        if (COUNT) {
+            st.n_rec += (uint32_t)cnt;
+            st.n_id += sph_node ? (uint32_t)cnt : 0u;
+        }
 #pragma unroll
         for (int u = 0; u < kPrimBatch; u++) {
             if (u >= cnt) break;
@@ -254,6 +263,7 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
             float t;
             const bool h = sp[u] ? sphere_test(g[u], r, r.tmin, st.best.t, t) : tri_test(g[u], r, r.tmin, st.best.t, t);
             if (h) {
+                if (COUNT && (ANY || !(st.best.rec == 0xffffffffu || t < st.best.t))) st.n_id += ANY ? 1u : 2u;
                 if (ANY) {
                     st.best.t = t;
                     st.best.id = (int32_t)(ids[ri] & ~BC_SPHERE_BIT);
@@ -306,7 +316,7 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
     }
 }
 
-// The closest hit resolves its primitive index once, at the end.
+// The closest hit resolves its primitive index once, at the end (one more id-table read).
 __device__ __forceinline__ HitRec trav_result(const TravState& st, const uint32_t* __restrict__ ids, bool any) {
     HitRec h = st.best;
     if (!any && h.rec != 0xffffffffu) h.id = (int32_t)(ids[h.rec] & ~BC_SPHERE_BIT);

@@ -161,6 +161,112 @@ hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc
                    : launch_trace_t<false>(stack, nodes, prims, ids, rays, n, out, refill, err, st, sorted, perm, oflag);
 }
 
+// Counting instantiation of k_trace: the same traversal of each ray (same results), plus the
+// work it did: counts[i] = (node visits, primitive-record tests, id-table reads, traversal steps).
+// The algorithmic bytes of a launch follow from the first three (80 B per node, 36 B per record,
+// 4 B per id); steps minus node visits are the steps in which the ray only popped its stack.
+template <bool ANY>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_count(const bc_node8* __restrict__ nodes,
+                                                             const bc_prim* __restrict__ prims,
+                                                             const uint32_t* __restrict__ ids,
+                                                             const bc_ray* __restrict__ rays, int64_t n,
+                                                             void* __restrict__ out, int cap, int* __restrict__ err,
+                                                             uint4* __restrict__ counts) {
+    extern __shared__ uint2 stk[];
+    const int64_t i = (int64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    Ray r;
+    float tmax;
+    load_ray(rays, i, r, tmax);
+    TravState st;
+    trav_init(st, r, tmax, 0u);
+    uint32_t steps = 1;
+    while (!trav_step<ANY, true>(st, nodes, prims, ids, stk + threadIdx.x, kTraceBlock, cap, err)) steps++;
+    const HitRec h = trav_result(st, ids, ANY);
+    const uint32_t final_id = (!ANY && h.rec != 0xffffffffu) ? 1u : 0u;
+    store_hit(out, i, h, ANY);
+    counts[i] = make_uint4(st.n_node, st.n_rec, st.n_id + final_id, steps);
+}
+
+hipError_t launch_trace_count(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids,
+                              const bc_ray* rays, int64_t n, void* out, int* err, uint4* counts, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    if (stack < 1 || stack > kMaxStack) return hipErrorInvalidValue;
+    const size_t lds = (size_t)stack * kTraceBlock * sizeof(uint2);
+    const int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
+    if (any_hit)
+        hipLaunchKernelGGL((k_trace_count<true>), dim3((unsigned)blocks), dim3(kTraceBlock), lds, st, nodes, prims, ids,
+                           rays, n, out, stack, err, counts);
+    else
+        hipLaunchKernelGGL((k_trace_count<false>), dim3((unsigned)blocks), dim3(kTraceBlock), lds, st, nodes, prims, ids,
+                           rays, n, out, stack, err, counts);
+    return hipGetLastError();
+}
+
+// VALU issue probes, no memory access in their loops, so their instruction rate is what the SIMDs
+// issue of their instruction mix.  k_valu_probe: the eight-child slab test of a node visit
+// (byte-to-float conversions, FMAs, three-operand min/max, compares and mask building) on
+// operands held in registers; each thread runs `iters` visits on its own operands and writes one
+// word only in a case that does not occur, so nothing is dead.  k_fma_probe: eight independent
+// fp32 FMA chains per thread, `iters` rounds of eight FMAs.
+__global__ __launch_bounds__(256) void k_fma_probe(uint32_t iters, uint32_t seed, uint32_t* __restrict__ sink) {
+    const float b = __uint_as_float(0x3f7ffff0u), c = __uint_as_float(0x3a800000u | (seed & 0xffu));
+    float a[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = (float)(threadIdx.x + (uint32_t)k);
+    for (uint32_t it = 0; it < iters; it++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) a[k] = __builtin_fmaf(a[k], b, c);
+    }
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) t += a[k];
+    if (t == -1.0f) sink[0] = 1u;  // never: the chains stay live
+}
+
+__global__ __launch_bounds__(256) void k_valu_probe(uint32_t iters, uint32_t seed, uint32_t* __restrict__ sink) {
+    uint32_t w0 = seed ^ (blockIdx.x * 256u + threadIdx.x) * 0x9E3779B9u;
+    uint32_t w1 = w0 * 0x85EBCA6Bu + 1u, w2 = w1 * 0xC2B2AE35u + 3u, w3 = w2 * 0x27D4EB2Fu + 5u;
+    float ax = __uint_as_float(0x3f800000u | (w0 >> 9)), ay = __uint_as_float(0x3f800000u | (w1 >> 9));
+    float az = __uint_as_float(0x3f800000u | (w2 >> 9));
+    float ox = -ax, oy = -ay, oz = -az;
+    uint32_t acc = 0;
+    for (uint32_t it = 0; it < iters; it++) {
+        float tn_min = 3.0e38f;
+        uint32_t hits = 0;
+#pragma unroll
+        for (int s = 0; s < 8; s++) {
+            const uint32_t sh = (uint32_t)(s & 3) * 8u;
+            const uint32_t a = s < 4 ? w0 : w1, b = s < 4 ? w2 : w3;
+            const float tnx = __builtin_fmaf(u8f(a, sh), ax, ox), tny = __builtin_fmaf(u8f(b, sh), ay, oy);
+            const float tnz = __builtin_fmaf(u8f(a ^ b, sh), az, oz);
+            const float tfx = __builtin_fmaf(u8f(b, sh ^ 8u), ax, ox), tfy = __builtin_fmaf(u8f(a, sh ^ 8u), ay, oy);
+            const float tfz = __builtin_fmaf(u8f(a + b, sh), az, oz);
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.0f));
+            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, 1.0e30f));
+            const bool h = tn <= tf;
+            hits |= (h ? 1u : 0u) << s;
+            tn_min = fminf(tn_min, h ? tn : 3.0e38f);
+        }
+        acc += hits + (__float_as_uint(tn_min) >> 16);
+        // new operands for the next visit: cheap integer steps, as a traversal's new node words
+        w0 = w0 * 1664525u + 1013904223u;
+        w1 ^= w0 >> 7;
+        w2 += w1 << 3;
+        w3 ^= w2 >> 5;
+        ox += 0.5f;
+    }
+    if (acc == 0x12345678u) sink[0] = acc;  // practically never: keeps the loop alive
+}
+
+hipError_t launch_valu_probe(int mix, int blocks, uint32_t iters, uint32_t seed, uint32_t* sink, hipStream_t st) {
+    if (mix == 1)
+        hipLaunchKernelGGL(k_fma_probe, dim3((unsigned)blocks), dim3(256), 0, st, iters, seed, sink);
+    else
+        hipLaunchKernelGGL(k_valu_probe, dim3((unsigned)blocks), dim3(256), 0, st, iters, seed, sink);
+    return hipGetLastError();
+}
+
 int stack_for_depth(int depth) {
     const int need = depth - 1;
     if (need > kMaxStack) return -1;

@@ -9,10 +9,14 @@ and incoherent rays (origins uniform in the cube, directions uniform on the sphe
 step = the four launches closest-hit and any-hit over both sets = 67,108,864 ray queries,
 inputs resident in HBM.  value = ray queries per second of the whole step, in Mrays/s.
 
-Also reported: the roofline of the dominant kernel (algorithmic bytes per ray counted by the
-oracle's threaded traversal on the reference's 32-byte node layout, over HIP-event kernel
-durations) and the CPU baseline: the oracle's fp64 restatement of Scene.Hit / Scene.Occluded
-(renderer.go:777-895) on all host cores over a bounded sample of the same rays.
+Also reported: the roofline of the dominant kernel -- algorithmic bytes of the BVH8 traversal
+that runs (80 B per node visit, 36 B per primitive-record test, 4 B per id-table read, the ray and
+its result), counted per ray by a counting instantiation of the same traversal over the whole
+launch, over the kernel's HIP-event duration, against the 8 TB/s HBM peak -- with the VALU issue
+rate of the same kernel against a calibrated peak as the second roofline; and the CPU baseline:
+the oracle's fp64 restatement of Scene.Hit / Scene.Occluded (renderer.go:777-895) on every host
+core this job may use (affinity and cgroup quota, both recorded with the CPU model) over a
+bounded sample of the same rays.
 """
 import argparse
 import concurrent.futures as cf
@@ -51,6 +55,29 @@ def gen_rays(bc, set_id, width, any_hit, threads):
     return out
 
 
+def fixed_schedule(args, d):
+    """--schedule pool,iters,segment,sort: the persistent schedule given explicitly (no autotune)."""
+    if args.schedule:
+        pool, iters, seg, sort = (int(x) for x in args.schedule.split(","))
+        d.reserved[0], d.reserved[1], d.reserved[2], d.reserved[4] = pool, iters, seg, sort
+    return d
+
+
+def go_facing(dev, desc, reps=1):
+    """The renderer's own path to the GPU: bc_render into host buffers in 64-row chunks with the
+    compact guides, as renderOpenCL chunks a frame (gpu_opencl_windows.go:631-641) and the cgo
+    shim calls it; seconds per frame after one untimed frame."""
+    def frame():
+        for y0 in range(0, desc.height, 64):
+            dev.render(desc, y0=y0, rows=min(64, desc.height - y0), guides=True)
+
+    frame()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        frame()
+    return (time.perf_counter() - t0) / reps
+
+
 def bench_render(bc, dev, args, threads):
     """Closed box 256x256, 16 spp, 4 segments: path-samples/s of the wavefront integrator
     (rgb + guides left in HBM) next to the oracle's fp64 integrator on the host cores."""
@@ -77,7 +104,8 @@ def bench_render(bc, dev, args, threads):
            "mrays_s_segments": round(segs / sec / 1e6, 2), "mrays_s_total": round((segs + shadow) / sec / 1e6, 2),
            "segments_per_sample": round(segs / samples, 4)}
     # batched-persistent schedule (in-place regeneration, hipGraph replay, autotuned shape)
-    dp = bc.render_desc(box, w, h, spp, bounces, rr_depth=3, sampler=1, mis=True, power=True, mode=1)
+    dp = fixed_schedule(args, bc.render_desc(box, w, h, spp, bounces, rr_depth=3, sampler=1, mis=True, power=True,
+                                             mode=1))
     d_rgba = dev.alloc(w * h * 16)
     d_g = dev.alloc(w * h * 12)
     dev.render_device(dp, 0, h, d_rgba, d_g)  # autotunes and captures once
@@ -89,6 +117,10 @@ def bench_render(bc, dev, args, threads):
     dev.free(d_g)
     out["persistent"] = {"ms_per_frame": round(psec * 1e3, 3), "path_samples_per_s": round(samples / psec, 1),
                          "over_plain": round(sec / psec, 3), "tuning": dev.tuning()}
+    gsec = go_facing(dev, d, reps=5)
+    out["go_facing"] = {"ms_per_frame": round(gsec * 1e3, 3), "path_samples_per_s": round(samples / gsec, 1),
+                        "path": "bc_render, 64-row chunks, rgba + compact guides copied to host per chunk",
+                        "over_device_resident_time": round(gsec / sec, 3)}
     if not args.no_cpu:
         import oracle as orc
 
@@ -100,7 +132,8 @@ def bench_render(bc, dev, args, threads):
         csec = time.perf_counter() - t0
         cpu_rate = w * rows * spp / csec
         out["cpu_baseline"] = {"path_samples_per_s": round(cpu_rate, 1), "cores": threads, "kind": "port",
-                               "sample": f"rows 120-{120 + rows} of the same frame, fp64 trace restatement"}
+                               "sample": f"rows 120-{120 + rows} of the same frame, fp64 trace restatement",
+                               "host": cpu_info()}
         out["gpu_over_cpu"] = round(samples / sec / cpu_rate, 1)
         # matched image error: both 16-spp bands against an independent-seed fp64 render at 4096 spp
         dref = bc.render_desc(box, w, h, 4096, bounces, rr_depth=3, sampler=1, mis=True, power=True, seed=0x51ED)
@@ -134,6 +167,14 @@ def bench_terrain(bc, dev, args, threads):
     d = bc.render_desc(s, w, h, spp, bounces, **kw)
     d_rgba = dev.alloc(w * h * 16)
     d_g = dev.alloc(w * h * 12)
+    dp = fixed_schedule(args, bc.render_desc(s, w, h, spp, bounces, mode=1, **kw))
+    if args.profile_light:  # one plain and one persistent frame, for counter runs
+        segs, shadow = dev.render_device(d, 0, h, d_rgba)
+        psegs, _ = dev.render_device(dp, 0, h, d_rgba)
+        dev.free(d_rgba)
+        dev.free(d_g)
+        return {"profile_light": True, "segments": segs, "shadow_rays": shadow, "persistent_segments": psegs,
+                "tuning": dev.tuning()}
     dev.render_device(d, 0, h, d_rgba, d_g)
     t0 = time.perf_counter()
     segs, shadow = dev.render_device(d, 0, h, d_rgba)
@@ -153,7 +194,6 @@ def bench_terrain(bc, dev, args, threads):
     t0 = time.perf_counter()
     ssegs, _ = dev.render_device(dsp, 0, h, d_rgba)
     ssec = time.perf_counter() - t0
-    dp = bc.render_desc(s, w, h, spp, bounces, mode=1, **kw)
     t0 = time.perf_counter()
     dev.render_device(dp, 0, h, d_rgba)  # autotunes and captures once
     tune = time.perf_counter() - t0
@@ -163,6 +203,9 @@ def bench_terrain(bc, dev, args, threads):
     dev.free(d_rgba)
     samples = w * h * spp
     assert psegs == segs
+    g_plain = go_facing(dev, d)
+    g_pers = go_facing(dev, dp)
+    log(f"[bench] terrain Go-facing chunks: plain {g_plain * 1e3:.1f} ms, persistent {g_pers * 1e3:.1f} ms")
     log(f"[bench] terrain: plain {sec * 1e3:.1f} ms, persistent {psec * 1e3:.1f} ms (autotune + first {tune:.1f}s)")
     out = {"workload": f"terrain {args.terrain_cells}^2 cells ({s.counts()[0]} tris, 64 materials), sun + sky 0.3, "
                        f"{w}x{h}, {spp} spp Halton, {bounces} segments, RR from bounce 3, resampled direct "
@@ -176,7 +219,12 @@ def bench_terrain(bc, dev, args, threads):
            "post": {"ms_per_frame": round(post_ms, 3), "stages": "a-trous denoise 3 passes, FXAA 2, TXAA 1, toRGBA",
                     "readback_bytes_per_pixel": {"with_device_post": 4, "without": 28}},
            "spatial_reuse": {"ms_per_frame": round(ssec * 1e3, 2), "neighbours": 4, "radius_px": 16,
-                             "over_plain_time": round(ssec / sec, 3)}}
+                             "over_plain_time": round(ssec / sec, 3)},
+           "go_facing": {"path": "bc_render, 64-row chunks, rgba + compact guides copied to host per chunk",
+                         "plain_ms_per_frame": round(g_plain * 1e3, 2),
+                         "persistent_ms_per_frame": round(g_pers * 1e3, 2),
+                         "plain_path_samples_per_s": round(samples / g_plain, 1),
+                         "persistent_path_samples_per_s": round(samples / g_pers, 1)}}
     if not args.no_cpu:
         import oracle as orc
 
@@ -195,7 +243,7 @@ def bench_terrain(bc, dev, args, threads):
         cpu_rate = w * rows * spp / csec
         out["cpu_baseline"] = {"path_samples_per_s": round(cpu_rate, 1), "cores": threads, "kind": "port",
                                "sample": f"rows {y0}-{y0 + rows} of the same frame, fp64 trace restatement, "
-                                         f"{csec:.1f} s"}
+                                         f"{csec:.1f} s", "host": cpu_info()}
         out["gpu_over_cpu"] = round(samples / sec / cpu_rate, 1)
         del o
     return out
@@ -212,10 +260,12 @@ def batch_resolution(i, seed=0xC5):
 
 
 def bench_batch(bc, dev, args, threads):
-    """Multi-scene batch: 32 seeded scenes (log-uniform 1e4..2e6 triangles of instanced boxes and
+    """Multi-scene batch: 32 seeded scenes (log-uniform 1e4..2e6 primitives of instanced boxes and
     spheres, 2-16 quad lights, resolutions from {640x360, 1280x720, 2560x1440}), Ultra transport
-    (16 segments, RR from bounce 6, no clamp) at --batch-spp samples per pixel.  Plain: the scenes
-    one after another; persistent: one schedule over all scenes' samples (autotuned)."""
+    (256 spp by default, 16 segments, RR from bounce 6, no clamp).  Plain: the scenes one after
+    another; persistent: one schedule over all scenes' samples.  The persistent shape is autotuned
+    on an 8-spp render of the same frames: the autotuner's trials hold one sample per pixel at this
+    frame size whatever the spp, so they are the trials a 256-spp render would run."""
     n = args.batch_scenes
     t0 = time.time()
     with cf.ThreadPoolExecutor(threads) as ex:
@@ -233,25 +283,42 @@ def bench_batch(bc, dev, args, threads):
     pix = sum(d.width * d.height for d in descs)
     bufs = [dev.alloc(d.width * d.height * 16) for d in descs]
     out = {"workload": f"{n} batch scenes ({tris} primitives, {pix} pixels), {args.batch_spp} spp Halton, 16 segments, "
-                       "RR from bounce 6, no clamp, MIS + power NEE over emissive quads"}
+                       "RR from bounce 6, no clamp, MIS + power NEE over emissive quads",
+           "is_c5": n == 32 and args.batch_spp == 256}
+    spp = args.batch_spp
     for mode in (0, 1):
+        tuning = None
         for d in descs:
             d.mode = mode
-        first = None
-        if mode == 1 or not args.batch_no_warmup:
-            t0 = time.perf_counter()
-            dev.render_batch_device(descs, bufs)  # mode 1: autotunes and captures once
-            first = time.perf_counter() - t0
+            d.spp = 1 if mode == 0 else min(8, spp)
+        # untimed: plain, the passes' buffers at one sample; persistent, the autotuner at 8 spp
+        t0 = time.perf_counter()
+        if mode == 0 or (spp != 1 and not args.schedule):
+            dev.render_batch_device(descs, bufs)
+        first = time.perf_counter() - t0
+        if mode == 1 and args.schedule:
+            for d in descs:
+                fixed_schedule(args, d)
+            tuning = dict(zip(("pool", "iters_per_graph", "segment", "material_sort"),
+                              (int(x) for x in args.schedule.split(","))))
+        elif mode == 1:
+            tuning = dev.tuning()
+            for d in descs:
+                d.mode = 0 if tuning["plain"] else 1
+                d.reserved[0], d.reserved[1] = tuning["pool"], tuning["iters_per_graph"]
+                d.reserved[2], d.reserved[4] = tuning["segment"], tuning["material_sort"]
+        for d in descs:
+            d.spp = spp
         t0 = time.perf_counter()
         segs, shadow = dev.render_batch_device(descs, bufs)
         sec = time.perf_counter() - t0
         key = "plain" if mode == 0 else "persistent"
-        out[key] = {"seconds": round(sec, 4), "path_samples_per_s": round(pix * args.batch_spp / sec, 1),
+        out[key] = {"seconds": round(sec, 4), "path_samples_per_s": round(pix * spp / sec, 1),
                     "mrays_s_total": round((segs + shadow) / sec / 1e6, 2),
-                    "segments_per_sample": round(segs / (pix * args.batch_spp), 4)}
+                    "segments_per_sample": round(segs / (pix * spp), 4)}
         if mode == 1:
-            out[key]["tuning"] = dev.tuning()
-            out[key]["autotune_and_first_s"] = round(first, 2)
+            out[key]["tuning"] = dict(tuning or {}, windows=dev.tuning()["windows"], given=bool(args.schedule))
+            out[key]["autotune_at_8spp_s"] = round(first, 2)
         log(f"[bench] batch {key}: {sec:.3f}s")
     out["persistent_over_plain"] = round(out["plain"]["seconds"] / out["persistent"]["seconds"], 3)
     for b in bufs:
@@ -290,7 +357,7 @@ def bench_ingest(bc, dev, args):
     import tempfile
 
     src = bc.HostScene.workload(bc.SOUP, param=args.tris, bins=32, leaf=4)
-    threads = min(16, host_cores())
+    threads = cpu_info()["threads"]
     with tempfile.TemporaryDirectory() as tmp:
         path = os.path.join(tmp, "soup.obj")
         src.write_obj(path)
@@ -381,34 +448,74 @@ def bench_adjoint(bc, dev):
 PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_soup1M_16Mrays.json")
 
 
-def roofline_entry(kernel, achieved, alg_bytes, kernel_ms=None):
-    """Roofline of the dominant kernel.  traffic: HBM-side bytes per launch of that kernel from the
-    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE passes, scaled by a 1 GiB copy),
-    with the VALU-busy fraction of the same runs as the secondary roofline."""
-    out = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "alg_bytes_per_ray": round(alg_bytes, 1)}
+def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_cfg, probe_ms):  # noqa: C901
+    """Roofline of the dominant kernel (the slowest launch of the step).
+
+    achieved: the BVH8 traversal's algorithmic bytes per launch over its HIP-event duration, against
+    the HBM peak.  traffic: HBM-side bytes per launch of that kernel from the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by what it reads for a 1 GiB
+    device copy).  valu: the same kernel's VALU wave-instructions per second (SQ_INSTS_VALU of the
+    committed summary over this run's duration) against the issue rate of the FMA probe (peak) and
+    of the slab-test probe (the node visit's instruction mix alone), whose instruction counts per
+    wave the same summary holds and whose durations this run measures."""
+    out = {"bound": "hbm", "kernel": f"k_trace ({launch})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+           "alg_bytes_per_ray": round(alg_bytes, 1), "ref_layout_alg_bytes_per_ray": round(alg_ref_bytes, 1),
+           "valu": None}
     try:
         with open(PROFILE_SUMMARY) as f:
             prof = json.load(f)
-        k = prof["launches"].get(kernel[len("k_trace ("):-1])
-        if k:
-            out["traffic"] = k["traffic_bytes"]
-            out["traffic_source"] = os.path.relpath(PROFILE_SUMMARY, ROOT)
-            if kernel_ms:  # the measured HBM-side bytes over this run's launch time
-                out["traffic_gbs"] = round(k["traffic_bytes"] / (kernel_ms * 1e-3) / 1e9, 1)
-                out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
-            out["valu_busy"] = k.get("valu_busy")
-            out["l2_hit"] = k.get("l2_hit")
-    except (OSError, ValueError, KeyError):
-        pass
+    except (OSError, ValueError):
+        return out
+    k = prof.get("launches", {}).get(launch)
+    if k:
+        out["traffic"] = k["traffic_bytes"]
+        out["traffic_source"] = os.path.relpath(PROFILE_SUMMARY, ROOT)
+        out["traffic_gbs"] = round(k["traffic_bytes"] / (kernel_ms * 1e-3) / 1e9, 1)
+        out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
+        out["l2_hit"] = k.get("l2_hit")
+        out["active_lanes_per_valu_instr"] = k.get("active_lanes")
+    pf, pm = prof.get("fma_probe"), prof.get("valu_probe")
+    ok = all(p and (p.get("blocks"), p.get("iters")) == tuple(probe_cfg) for p in (pf, pm))
+    if k and ok and k.get("valu_insts_per_wave"):
+        rate = k["valu_insts_per_wave"] * k["waves"] / (kernel_ms * 1e-3)
+        peak = pf["valu_insts_per_wave"] * pf["waves"] / (probe_ms[1] * 1e-3)
+        mix = pm["valu_insts_per_wave"] * pm["waves"] / (probe_ms[0] * 1e-3)
+        out["valu"] = {"bound": "valu", "achieved": round(rate / 1e9, 2), "peak": round(peak / 1e9, 2),
+                       "unit": "G wave-instr/s", "frac": round(rate / peak, 4),
+                       "slab_mix_rate": round(mix / 1e9, 2), "over_slab_mix": round(rate / mix, 4),
+                       "probe_ms": {"fma": round(probe_ms[1], 3), "slab_mix": round(probe_ms[0], 3)},
+                       "source": os.path.relpath(PROFILE_SUMMARY, ROOT)}
     return out
 
 
-def host_cores():
+def cpu_info():
+    """Host cores this process may run on: the affinity set, limited by the cgroup CPU quota when
+    one is set (a quota of q CPUs lets the job use q cores' worth however many it sees), plus the
+    CPU model, for the CPU-baseline records."""
     try:
-        return len(os.sched_getaffinity(0))
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        return os.cpu_count() or 1
+        aff = os.cpu_count() or 1
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = aff if quota is None else max(1, min(aff, int(quota + 0.5)))
+    return {"threads": threads, "nproc": os.cpu_count(), "affinity": aff, "cgroup_cpu_quota": quota, "model": model}
 
 
 def main():
@@ -428,8 +535,12 @@ def main():
     ap.add_argument("--only", choices=("render", "terrain", "batch", "adjoint", "refit", "build", "ingest"),
                     help="run this render leg alone (for profiling) and print its result")
     ap.add_argument("--batch-scenes", type=int, default=32)
-    ap.add_argument("--batch-spp", type=int, default=8, help="samples per pixel of the batch leg (Ultra: 256)")
-    ap.add_argument("--batch-no-warmup", action="store_true", help="time the plain batch render without a warm-up")
+    ap.add_argument("--batch-spp", type=int, default=256, help="samples per pixel of the batch leg (Ultra: 256)")
+    ap.add_argument("--profile-light", action="store_true",
+                    help="terrain leg: one plain and one persistent frame only (for counter runs)")
+    ap.add_argument("--schedule", default=None,
+                    help="pool,iters,segment,sort: run the persistent legs with this schedule instead of the "
+                         "autotuner's (for profiling runs)")
     ap.add_argument("--any-refill", type=int, default=8,
                     help="any-hit queries and integrator shadow rays: refill a wave's lanes once this many are idle "
                          "(0: one ray per thread)")
@@ -448,7 +559,9 @@ def main():
 
     import beamcast as bc
 
-    threads = min(16, host_cores())
+    cinfo = cpu_info()
+    threads = cinfo["threads"]
+    log(f"[bench] host: {cinfo}")
     if args.only:
         dev = bc.Device(0)
         dev.set_trace_refill(args.closest_refill, args.any_refill)
@@ -529,15 +642,51 @@ def main():
             hits[k] = float((h["prim"] >= 0).mean())
     dominant = max(kernel_ms, key=kernel_ms.get)
 
-    # algorithmic bytes per ray: 32 (ray) + out + 32 * node fetches + 44 * primitive tests,
-    # counted by the oracle's threaded traversal on the reference node layout over a sample
+    # algorithmic bytes of the BVH8 traversal that runs, counted for every ray of each launch by
+    # the counting instantiation of the same traversal (whose results must equal the timed
+    # launch's): 32 B ray + 8 B (closest) or 4 B (any hit) result + 80 B per node visit + 36 B per
+    # primitive-record test + 4 B per id-table read
+    d_cnt, d_chk = dev.alloc(n * 16), dev.alloc(n * 8)
+    alg, work = {}, {}
+    for k in launches:
+        out_b = 4 if k[1] else 8
+        dev.trace_counts(d_rays[k], n, k[1], d_chk, d_cnt)
+        cnt = dev.download(d_cnt, np.empty((n, 4), np.uint32))
+        c = cnt.sum(axis=0, dtype=np.int64) / n
+        # steps of the slowest ray of each 64-ray wave (k_trace runs one ray per lane to completion)
+        wave_steps = float(cnt[: n // 64 * 64, 3].reshape(-1, 64).max(axis=1).mean())
+        del cnt
+        a = dev.download(d_chk, np.empty(n * out_b, np.uint8))
+        b = dev.download(d_out[k], np.empty(n * out_b, np.uint8))
+        if not np.array_equal(a, b):
+            raise RuntimeError("the counting traversal's results differ from the timed launch's")
+        alg[k] = 32 + out_b + 80 * c[0] + 36 * c[1] + 4 * c[2]
+        work[k] = {"node_visits_per_ray": round(float(c[0]), 3), "record_tests_per_ray": round(float(c[1]), 3),
+                   "id_reads_per_ray": round(float(c[2]), 3), "steps_per_ray": round(float(c[3]), 3),
+                   "steps_per_wave": round(wave_steps, 2)}
+    dev.free(d_cnt)
+    dev.free(d_chk)
+    achieved = alg[dominant] * n / (kernel_ms[dominant] * 1e-3) / 1e9
+
+    # VALU issue probes, 8 waves per SIMD, no memory access: eight independent FMA chains per
+    # thread (the issue ceiling the traversal's VALU rate is compared with) and the node visit's
+    # slab-test instruction mix on registers (what that mix alone issues)
+    probe_cfg = (2048, 4000)
+    probe_ms = {}
+    for mix in (1, 0):
+        dev.valu_probe(*probe_cfg, mix=mix)
+        probe_ms[mix] = min(dev.valu_probe(*probe_cfg, mix=mix) for _ in range(3))
+
+    # the reference layout's algorithmic bytes on the same rays (32 + out + 32 per threaded node
+    # fetch + 44 per primitive test, counted by the oracle's threaded traversal over a sample), as
+    # what the reference's OpenCL kernel would have moved
     import oracle as orc
 
     t0 = time.time()
     oscene = orc.Scene.from_host(scene, bins=32, leaf=4)
     log(f"[bench] oracle scene build: {time.time() - t0:.2f}s")
     sample_n = 65536
-    alg = {}
+    alg_ref = {}
     for k in launches:
         idx = np.linspace(0, n - 1, sample_n).astype(np.int64)
         smp = sets[k][idx]
@@ -546,9 +695,7 @@ def main():
         else:
             _, _, nv, pt = oscene.hit32(smp, threads=threads, counts=True)
         out_b = 4 if k[1] else 8
-        alg[k] = 32 + out_b + 32 * float(nv.mean()) + 44 * float(pt.mean())
-    dom_bytes = alg[dominant] * n
-    achieved = dom_bytes / (kernel_ms[dominant] * 1e-3) / 1e9
+        alg_ref[k] = 32 + out_b + 32 * float(nv.mean()) + 44 * float(pt.mean())
 
     cpu = None
     if not args.no_cpu:
@@ -568,7 +715,7 @@ def main():
         cpu = {"value": round(cpu_rate, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
                "sample": f"{total} rays ({m} from each of the 4 launches, strided), fp64 Scene.Hit/Occluded "
                          f"restatement, {secs:.1f} s",
-               "gpu_over_cpu": round(value / cpu_rate, 1)}
+               "host": cinfo, "gpu_over_cpu": round(value / cpu_rate, 1)}
         log(f"[bench] CPU baseline: {cpu_rate:.2f} Mrays/s on {threads} threads ({secs:.1f}s)")
 
     render = None if args.no_render else bench_render(bc, dev, args, threads)
@@ -603,7 +750,8 @@ def main():
             "bvh8_depth": int(st["depth8"]),
             "scene_bytes": int(st["bytes"]),
         },
-        "roofline": roofline_entry(f"k_trace ({names[dominant]})", achieved, alg[dominant], kernel_ms[dominant]),
+        "roofline": roofline_entry(names[dominant], achieved, alg[dominant], alg_ref[dominant], kernel_ms[dominant],
+                                   probe_cfg, probe_ms),
         "cpu_baseline": cpu,
         "render": render,
         "adjoint": adjoint,
@@ -613,8 +761,10 @@ def main():
         "build": build,
         "ingest": ingest,
         "kernels": {names[k]: {"ms": round(kernel_ms[k], 4), "mrays_s": round(n / kernel_ms[k] / 1e3, 1),
-                               "alg_bytes_per_ray": round(alg[k], 1), "hit_fraction": round(hits[k], 4),
-                               "binned": binned[k]}
+                               "alg_bytes_per_ray": round(alg[k], 1),
+                               "alg_gbs": round(alg[k] * n / (kernel_ms[k] * 1e-3) / 1e9, 1),
+                               "ref_layout_alg_bytes_per_ray": round(alg_ref[k], 1), **work[k],
+                               "hit_fraction": round(hits[k], 4), "binned": binned[k]}
                     for k in launches},
     }
     for k in launches:

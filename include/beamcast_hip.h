@@ -249,6 +249,18 @@ int bc_trace(bc_ctx* ctx, const float* rays, int64_t n, int any_hit, void* out);
 /* Same with device pointers, enqueued on the context stream (asynchronous). */
 int bc_trace_device(bc_ctx* ctx, const void* d_rays, int64_t n, int any_hit, void* d_out);
 
+/* Diagnostics with no counterpart in the reference (the roofline accounting of the benchmark).
+ * bc_trace_counts: bc_trace_device's query (same results in d_out), one ray per thread, that also
+ * writes per ray d_counts[4i .. 4i+3] = node visits (80-byte bc_node8 each), primitive-record tests
+ * (36-byte records), id-table reads (4 bytes), traversal steps (a step visits at most one node;
+ * the others only pop the stack).  Asynchronous on the context stream.
+ * bc_valu_probe: times (HIP events, *ms) `blocks` workgroups of 256 threads that each run `iters`
+ * register-only eight-child slab tests, the VALU instruction mix of a node visit (mix 0), or
+ * `iters` rounds of eight independent fp32 FMAs (mix 1), to calibrate the VALU issue rates the
+ * traversal is compared with. */
+int bc_trace_counts(bc_ctx* ctx, const void* d_rays, int64_t n, int any_hit, void* d_out, uint32_t* d_counts);
+int bc_valu_probe(bc_ctx* ctx, int mix, int blocks, int iters, float* ms);
+
 /* Device memory and stream helpers for callers that keep inputs resident in HBM. */
 void* bc_alloc(bc_ctx* ctx, uint64_t bytes);
 int bc_free(bc_ctx* ctx, void* d_ptr);
