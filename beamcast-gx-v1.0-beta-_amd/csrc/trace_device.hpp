@@ -130,8 +130,10 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
     const uint32_t ny0 = ny ? hiy0 : loy0, ny1 = ny ? hiy1 : loy1, fy0 = ny ? loy0 : hiy0, fy1 = ny ? loy1 : hiy1;
     const uint32_t nz0 = nz ? hiz0 : loz0, nz1 = nz ? hiz1 : loz1, fz0 = nz ? loz0 : hiz0, fz1 = nz ? loz1 : hiz1;
     const uint32_t meta0 = w1.z, meta1 = w1.w;
-    uint32_t hits = 0, ord = 0;
-    float tnmin = 3.0e38f;  // smallest entry distance among hit internal children
+    uint32_t hits = 0;
+    // smallest entry distance among the hit children: a lower bound for the internal ones that a
+    // pop tests against the current hit (leaf and empty slots can only lower it)
+    float tnmin = 3.0e38f;
 #pragma unroll
     for (int s = 0; s < 8; s++) {
         const uint32_t sh = (uint32_t)(s & 3) * 8u;
@@ -147,13 +149,16 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
         // an empty slot's box is inverted (255 / 0); rounding can still let it pass here, so
         // emptiness is checked where it matters, for leaf hits below (internal slots never are)
         const bool h = tn <= tf;
-        const bool hin = h && ((imask >> s) & 1u);
         hits |= (h ? 1u : 0u) << s;
-        // internal children go straight to near-to-far order: bit k is slot k ^ oct
-        ord |= (hin ? 1u : 0u) << ((uint32_t)s ^ oct);
         // the minimum of canonical operands needs no NaN quieting step
-        tnmin = fminf(tnmin, hin ? tn : 3.0e38f);
+        tnmin = fminf(tnmin, h ? tn : 3.0e38f);
     }
+    // internal hits in near-to-far order, bit k = slot k ^ oct: the slot mask with its bit
+    // positions XORed by the octant, one swap stage per octant bit
+    uint32_t ord = hits & imask;
+    ord = (oct & 1u) ? (((ord & 0x55u) << 1) | ((ord >> 1) & 0x55u)) : ord;
+    ord = (oct & 2u) ? (((ord & 0x33u) << 2) | ((ord >> 2) & 0x33u)) : ord;
+    ord = (oct & 4u) ? (((ord & 0x0fu) << 4) | ((ord >> 4) & 0x0fu)) : ord;
     ng_base = w1.x;
     // bits 16-31: upper half of the smallest entry distance.  Truncating a non-negative
     // float rounds towards zero, so it is a lower bound usable to cull the group after a
@@ -281,39 +286,41 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
         }
     }
     // next node: the nearest remaining child of the current group, else of the nearest pending
-    // group on the stack.  Any hit pops and descends in the same step, so that every step of
-    // every lane is one node visit and a wave's lanes fetch their nodes in the same iterations.
-    // Closest hit returns after a pop (its next step descends or pops again): descending in
-    // the same step measured slower for it (more registers live across the step).
-    for (;;) {
-        if (st.ng_bits & 0xffu) {
-            const uint32_t k = (uint32_t)__builtin_ctz(st.ng_bits & 0xffu);
-            st.ng_bits &= ~(1u << k);
-            const uint32_t slot = k ^ st.oct;
-            const uint32_t imask = (st.ng_bits >> 8) & 0xffu;
-            st.node = st.ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
-            if (st.ng_bits & 0xffu) {
-                if (st.sp >= cap) {  // the host sizes cap to the tree depth; report instead of overflowing
-                    atomicOr(err, 1);
-                    return true;
-                }
-                stk[st.sp * stride] = make_uint2(st.ng_base, st.ng_bits);
-                st.sp++;
-            }
-            st.have_node = true;
-            return false;
-        }
-        if (st.sp == 0) return true;
-        st.sp--;
-        const uint2 e = stk[st.sp * stride];
-        st.ng_base = e.x;
-        st.ng_bits = e.y;
-        if (!ANY) {
-            // closest hit: a group whose nearest child starts beyond the current hit is done
-            if (__uint_as_float(e.y & 0xffff0000u) > st.best.t) st.ng_bits = 0;
-            return false;
+    // group on the stack, popped and descended into in the same step, so that every step of a
+    // lane that still has work visits a node.  Closest hit pops at most one group per step (one
+    // straight-line pop keeps the step's live registers within the kernel's budget) and drops a
+    // group whose nearest child starts beyond the current hit, which then ends the step.
+    if (!(st.ng_bits & 0xffu)) {
+        if (ANY) {
+            if (st.sp == 0) return true;
+            st.sp--;
+            const uint2 e = stk[st.sp * stride];
+            st.ng_base = e.x;
+            st.ng_bits = e.y;
+        } else {
+            if (st.sp == 0) return true;
+            st.sp--;
+            const uint2 e = stk[st.sp * stride];
+            st.ng_base = e.x;
+            st.ng_bits = __uint_as_float(e.y & 0xffff0000u) > st.best.t ? 0u : e.y;
+            if (!(st.ng_bits & 0xffu)) return false;
         }
     }
+    const uint32_t k = (uint32_t)__builtin_ctz(st.ng_bits & 0xffu);
+    st.ng_bits &= ~(1u << k);
+    const uint32_t slot = k ^ st.oct;
+    const uint32_t imask = (st.ng_bits >> 8) & 0xffu;
+    st.node = st.ng_base + (uint32_t)__builtin_popcount(imask & ((1u << slot) - 1u));
+    if (st.ng_bits & 0xffu) {
+        if (st.sp >= cap) {  // the host sizes cap to the tree depth; report instead of overflowing
+            atomicOr(err, 1);
+            return true;
+        }
+        stk[st.sp * stride] = make_uint2(st.ng_base, st.ng_bits);
+        st.sp++;
+    }
+    st.have_node = true;
+    return false;
 }
 
 // The closest hit resolves its primitive index once, at the end (one more id-table read).

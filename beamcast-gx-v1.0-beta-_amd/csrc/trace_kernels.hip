@@ -8,12 +8,13 @@
 namespace bcd {
 
 // Resident waves per SIMD the register allocation is held to (64 VGPRs for 8 waves, 72 for 7).
-// The any-hit refill loop fits 64 without spilling; the other loops fit 72 and would spill at 64.
+// Every loop fits 64 VGPRs; closest hit keeps the ray's index in scratch across its traversal
+// loop (one store before, one load after), which costs less than the eighth wave gains.
 #define BC_TRACE_WAVES(N) __attribute__((amdgpu_waves_per_eu(N, N)))
 // Consecutive workgroups of k_trace that one XCD takes per turn (see xcd_chunk): 4096 rays.
 constexpr uint32_t kTraceXcdRun = 64;
 #ifndef BC_CLOSEST_WAVES
-#define BC_CLOSEST_WAVES 7
+#define BC_CLOSEST_WAVES 8
 #endif
 
 // The LDS traversal stack is dynamic: cap entries per lane, cap * (threads per block) uint2 per
