@@ -446,6 +446,9 @@ def bench_adjoint(bc, dev):
 
 
 PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_soup1M_16Mrays.json")
+# MI355X's published fp32 vector rate, 157.3 TFLOP/s, in wave64 FMA instructions per second
+# (64 lanes x 2 flops each): one per SIMD every 2 clocks, 1024 SIMDs at 2.4 GHz
+VALU_PEAK = 157.3e12 / 128
 
 
 def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_cfg, probe_ms):  # noqa: C901
@@ -455,9 +458,10 @@ def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_
     the HBM peak.  traffic: HBM-side bytes per launch of that kernel from the committed rocprofv3
     counter summary (FETCH_SIZE and WRITE_SIZE passes, each scaled by what it reads for a 1 GiB
     device copy).  valu: the same kernel's VALU wave-instructions per second (SQ_INSTS_VALU of the
-    committed summary over this run's duration) against the issue rate of the FMA probe (peak) and
-    of the slab-test probe (the node visit's instruction mix alone), whose instruction counts per
-    wave the same summary holds and whose durations this run measures."""
+    committed summary over this run's duration) against the published fp32 vector issue rate, next
+    to the rates this run measures for the FMA probe and the slab-test probe (the node visit's
+    instruction mix alone, on registers), and the SIMD clocks per VALU instruction the summary's
+    counter run gives for each (GRBM_GUI_ACTIVE per XCD over VALU instructions per SIMD)."""
     out = {"bound": "hbm", "kernel": f"k_trace ({launch})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
            "alg_bytes_per_ray": round(alg_bytes, 1), "ref_layout_alg_bytes_per_ray": round(alg_ref_bytes, 1),
@@ -479,11 +483,17 @@ def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_
     ok = all(p and (p.get("blocks"), p.get("iters")) == tuple(probe_cfg) for p in (pf, pm))
     if k and ok and k.get("valu_insts_per_wave"):
         rate = k["valu_insts_per_wave"] * k["waves"] / (kernel_ms * 1e-3)
-        peak = pf["valu_insts_per_wave"] * pf["waves"] / (probe_ms[1] * 1e-3)
+        fma = pf["valu_insts_per_wave"] * pf["waves"] / (probe_ms[1] * 1e-3)
         mix = pm["valu_insts_per_wave"] * pm["waves"] / (probe_ms[0] * 1e-3)
-        out["valu"] = {"bound": "valu", "achieved": round(rate / 1e9, 2), "peak": round(peak / 1e9, 2),
-                       "unit": "G wave-instr/s", "frac": round(rate / peak, 4),
+        cyc = {name: (x.get("valu_class_per_wave") or {}).get("simd_cycles_per_instr")
+               for name, x in (("launch", k), ("fma_probe", pf), ("slab_mix_probe", pm))}
+        out["valu"] = {"bound": "valu", "achieved": round(rate / 1e9, 2), "peak": round(VALU_PEAK / 1e9, 2),
+                       "unit": "G wave-instr/s", "frac": round(rate / VALU_PEAK, 4),
+                       "peak_basis": "fp32 vector 157.3 TFLOP/s: one wave64 FMA per SIMD per 2 clocks, 1024 SIMDs, 2.4 GHz",
+                       "fma_probe_rate": round(fma / 1e9, 2), "over_fma_probe": round(rate / fma, 4),
                        "slab_mix_rate": round(mix / 1e9, 2), "over_slab_mix": round(rate / mix, 4),
+                       "simd_cycles_per_instr": cyc,
+                       "f32_arith_share": (k.get("valu_class_per_wave") or {}).get("f32_arith_share"),
                        "probe_ms": {"fma": round(probe_ms[1], 3), "slab_mix": round(probe_ms[0], 3)},
                        "source": os.path.relpath(PROFILE_SUMMARY, ROOT)}
     return out
