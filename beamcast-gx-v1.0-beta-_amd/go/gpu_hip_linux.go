@@ -42,6 +42,12 @@ type hipRuntime struct {
 
 var persistentHIP hipRuntime
 
+// gpuFullTransport tells RenderState.Start's GPU gate (renderer.go:2649, widened by
+// renderer_gate.patch) that this backend renders the CPU integrator's whole path transport:
+// MIS, power-weighted light selection, Halton and R2 sampling and the firefly clamp.  The
+// OpenCL kernel and the stub do not (gpu_transport_opencl.go).
+const gpuFullTransport = true
+
 func openCLGPUAvailable() (bool, string) {
 	buf := (*C.char)(C.malloc(256))
 	defer C.free(unsafe.Pointer(buf))

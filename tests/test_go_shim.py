@@ -98,3 +98,111 @@ def test_renderer_identifiers_exist_in_reference():
     node = re.search(r"^type BVHNode struct \{(.*?)^\}", ref, flags=re.M | re.S).group(1)
     for field in ["Box", "Left", "Right", "Start", "Count"]:
         assert re.search(r"\b" + field + r"\b", node), field
+
+
+# --- the GPU gate (renderer.go:2649) and its widening patch ---------------------------------------
+
+PATCH = os.path.join(GO, "renderer_gate.patch")
+GATE_VARS = ["Integrator", "Sampler", "MIS", "PowerLightSampling", "FireflyClamp", "AdaptiveSampling"]
+
+
+def patch_hunks():
+    """(old_start, old_len, new_start, new_len, lines) of each hunk of the patch."""
+    text = open(PATCH).read()
+    assert "--- a/windows_portable/renderer.go\n+++ b/windows_portable/renderer.go\n" in text
+    body = text[text.index("--- a/"):].split("\n")[2:]
+    hunks, cur = [], None
+    for line in body:
+        m = re.match(r"@@ -(\d+),(\d+) \+(\d+),(\d+) @@", line)
+        if m:
+            cur = tuple(int(g) for g in m.groups()) + ([],)
+            hunks.append(cur)
+        elif cur is not None and line[:1] in (" ", "-", "+"):
+            cur[4].append(line)
+    return hunks
+
+
+def go_gate(expr, full, s):
+    """Evaluates a Go gpuCompatible expression for one settings assignment (a translation of its
+    &&, ||, ! and comparisons to Python)."""
+    py = expr.replace("&&", " and ").replace("||", " or ")
+    py = re.sub(r"!(?!=)", " not ", py)
+    for c in ("IntegratorPath", "IntegratorRecursive", "SamplerRandom", "SamplerHalton", "SamplerR2"):
+        py = re.sub(r"(?<![.\w])" + c + r"\b", repr(c), py)
+    for v in GATE_VARS:
+        py = py.replace("settings." + v, repr(s[v]))
+    py = py.replace("gpuFullTransport", repr(full))
+    return eval(py, {"__builtins__": {}})  # noqa: S307 - a fixed expression of this patch
+
+
+def gate_exprs():
+    (hunk,) = patch_hunks()
+    old = [ln[1:] for ln in hunk[4] if ln[:1] == "-"]
+    new = [ln[1:] for ln in hunk[4] if ln[:1] == "+"]
+    assert len(old) == 1
+    strip = lambda lines: " ".join(ln.strip() for ln in lines).split(":=", 1)[1].strip()  # noqa: E731
+    assert old[0].strip().startswith("gpuCompatible :=") and new[0].strip().startswith("gpuCompatible :=")
+    return strip(old), strip(new)
+
+
+def test_gate_patch_hunk_is_consistent():
+    (hunk,) = patch_hunks()
+    o0, ol, n0, nl, lines = hunk
+    assert o0 == n0 == 2646
+    assert ol == sum(1 for ln in lines if ln[:1] in (" ", "-"))
+    assert nl == sum(1 for ln in lines if ln[:1] in (" ", "+"))
+
+
+def test_gate_truth_table():
+    """With gpuFullTransport false the widened gate is the reference gate for every combination of
+    the settings it reads; with it true only the integrator and adaptive sampling keep a render on
+    the CPU."""
+    old, new = gate_exprs()
+    n = 0
+    import itertools
+    for integ, samp, mis, power, clamp, adapt in itertools.product(
+            ("IntegratorPath", "IntegratorRecursive"), ("SamplerRandom", "SamplerHalton", "SamplerR2"),
+            (False, True), (False, True), (0.0, -1.0, 4.0), (False, True)):
+        s = dict(Integrator=integ, Sampler=samp, MIS=mis, PowerLightSampling=power, FireflyClamp=clamp,
+                 AdaptiveSampling=adapt)
+        ref = go_gate(old, None, s)
+        assert go_gate(new, False, s) == ref, s
+        assert go_gate(new, True, s) == (integ == "IntegratorPath" and not adapt), s
+        n += ref
+    assert 0 < n < 2 * 3 * 2 * 2 * 3 * 2
+
+
+def test_full_transport_flag_per_backend():
+    """gpuFullTransport is defined once per build: true by the HIP shim, false under exactly the
+    complementary build constraint (the OpenCL kernel and the stub), and the shim maps every setting
+    the widened gate lets through onto the render description."""
+    shim = open(os.path.join(GO, "gpu_hip_linux.go")).read()
+    other = open(os.path.join(GO, "gpu_transport_opencl.go")).read()
+    assert other.startswith("//go:build !(linux && cgo && beamcast_hip)\n")
+    assert re.search(r"^const gpuFullTransport = true$", shim, flags=re.M)
+    assert re.search(r"^const gpuFullTransport = false$", other, flags=re.M)
+    src = go_src("gpu_hip_linux.go")
+    fn = src[src.index("func hipRenderDesc"):]
+    fn = fn[: fn.index("\n}\n")]
+    for needle in ("case SamplerHalton", "case SamplerR2", "settings.MIS", "settings.PowerLightSampling",
+                   "settings.FireflyClamp"):
+        assert needle in fn, needle
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_GO), reason="reference tree not mounted")
+def test_gate_patch_applies_to_reference(tmp_path):
+    """git apply --check against a copy of the reference renderer, then the applied file differs from
+    the original in the gate line only."""
+    import shutil
+    import subprocess
+    dst = tmp_path / "windows_portable"
+    dst.mkdir()
+    shutil.copy(os.path.join(REF_GO, "renderer.go"), dst / "renderer.go")
+    subprocess.run(["git", "apply", "--check", PATCH], cwd=tmp_path, check=True, capture_output=True)
+    subprocess.run(["git", "apply", PATCH], cwd=tmp_path, check=True, capture_output=True)
+    before = open(os.path.join(REF_GO, "renderer.go")).read().split("\n")
+    after = (dst / "renderer.go").read_text().split("\n")
+    assert len(after) == len(before) + 1
+    assert before[:2648] == after[:2648] and before[2649:] == after[2650:]
+    old, new = gate_exprs()
+    assert before[2648].strip() == "gpuCompatible := " + old
