@@ -417,7 +417,11 @@ int bc_batch_upload(bc_ctx* c, int n, const bc_scene_view* views) {
         for (int s; (s = next.fetch_add(1)) < n;) okb[(size_t)s] = bcx::build_bvh8(views[s], bv[(size_t)s], errs[(size_t)s]);
     };
     std::vector<std::thread> pool;
-    for (unsigned t = 1; t < nth; t++) pool.emplace_back(work);
+    try {  // a thread that cannot start leaves its scenes to the others (the queue is shared)
+        pool.reserve(nth);
+        for (unsigned t = 1; t < nth; t++) pool.emplace_back(work);
+    } catch (...) {
+    }
     work();
     for (auto& t : pool) t.join();
     int depth = 0;

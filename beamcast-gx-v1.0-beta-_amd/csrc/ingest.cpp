@@ -247,9 +247,17 @@ bool load_obj(const char* path, bcx::Scene& s, double* cam10, std::string& err, 
         ch[(size_t)t].e = base + end;
         pos = end;
     }
+    // chunk t on thread t; a chunk whose thread cannot start runs on the calling thread, so no
+    // exception leaves the C entry points
     auto run = [&](auto fn) {
         std::vector<std::thread> th;
-        for (int t = 1; t < T; t++) th.emplace_back(fn, t);
+        int t = 1;
+        try {
+            th.reserve((size_t)T);
+            for (; t < T; t++) th.emplace_back(fn, t);
+        } catch (...) {
+        }
+        for (int u = t; u < T; u++) fn(u);
         fn(0);
         for (auto& x : th) x.join();
     };

@@ -19,19 +19,31 @@ namespace {
 constexpr size_t kDirectMax = size_t(4) << 20;  // up to this, one plain hipMemcpyAsync
 constexpr unsigned kMaxThreads = 8;
 
-void par_copy(char* dst, const char* src, size_t n, unsigned nth) {
+// Copies n bytes with up to nth threads (the caller's among them).  It never throws: it is called
+// inside the C entry points, so a thread that cannot be started (std::system_error) or a vector
+// that cannot grow leaves its part of the copy to the calling thread, and the threads already
+// started are joined before returning.
+void par_copy(char* dst, const char* src, size_t n, unsigned nth) noexcept {
     if (nth <= 1 || n < (size_t(1) << 20)) {
         std::memcpy(dst, src, n);
         return;
     }
     const size_t per = ((n + nth - 1) / nth + 4095) & ~size_t(4095);
-    std::vector<std::thread> pool;
-    for (size_t a = per; a < n; a += per) {
+    std::thread pool[kMaxThreads];
+    unsigned started = 0;
+    size_t a = per;  // [per, a) is covered by started threads
+    for (; a < n && started < kMaxThreads; a += per) {
         const size_t b = std::min(n, a + per);
-        pool.emplace_back([=] { std::memcpy(dst + a, src + a, b - a); });
+        try {
+            pool[started] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
+        } catch (...) {
+            break;
+        }
+        started++;
     }
     std::memcpy(dst, src, std::min(n, per));
-    for (auto& t : pool) t.join();
+    if (a < n) std::memcpy(dst + a, src + a, n - a);
+    for (unsigned i = 0; i < started; i++) pool[i].join();
 }
 
 bool ensure_pinned(bc_ctx* c) {
