@@ -461,7 +461,9 @@ def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_
     committed summary over this run's duration) against the published fp32 vector issue rate, next
     to the rates this run measures for the FMA probe and the slab-test probe (the node visit's
     instruction mix alone, on registers), and the SIMD clocks per VALU instruction the summary's
-    counter run gives for each (GRBM_GUI_ACTIVE per XCD over VALU instructions per SIMD)."""
+    counter run gives for each (GRBM_GUI_ACTIVE per XCD over VALU instructions per SIMD).  limiter:
+    the same launch's texture-addresser and texture-data busy fractions and VALU instructions per
+    SIMD clock from the summary's memory-pipe counter run, next to the probes' VALU rates."""
     out = {"bound": "hbm", "kernel": f"k_trace ({launch})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
            "alg_bytes_per_ray": round(alg_bytes, 1), "ref_layout_alg_bytes_per_ray": round(alg_ref_bytes, 1),
@@ -496,6 +498,16 @@ def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_
                        "f32_arith_share": (k.get("valu_class_per_wave") or {}).get("f32_arith_share"),
                        "probe_ms": {"fma": round(probe_ms[1], 3), "slab_mix": round(probe_ms[0], 3)},
                        "source": os.path.relpath(PROFILE_SUMMARY, ROOT)}
+    mp = (k or {}).get("memory_pipe")
+    if mp:
+        # the two limits the counter runs show saturated together: the texture-data return path
+        # (busy fraction of clocks, per CU) and VALU issue against the slab-mix probe's own rate
+        mix_clk = pm.get("valu_per_simd_clock") if pm else None
+        out["limiter"] = {"td_busy": mp["td_busy"], "ta_busy": mp["ta_busy"],
+                          "td_clocks_per_ray": mp["td_clocks_per_ray"], "tcp_accesses_per_ray": mp["tcp_accesses_per_ray"],
+                          "valu_per_simd_clock": mp["valu_per_simd_clock"], "slab_mix_probe_valu_per_simd_clock": mix_clk,
+                          "fma_probe_valu_per_simd_clock": pf.get("valu_per_simd_clock") if pf else None,
+                          "source": os.path.relpath(PROFILE_SUMMARY, ROOT)}
     return out
 
 
