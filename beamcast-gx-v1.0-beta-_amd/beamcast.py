@@ -545,15 +545,28 @@ class Device:
         self._check(lib().bc_scene_stats(self.ctx, o))
         return {"nodes8": o[0], "depth8": o[1], "prims": o[2], "bytes": o[3]}
 
-    NODE8 = np.dtype([("o", "<f4", 3), ("e", "u1", 3), ("imask", "u1"), ("child_base", "<u4"), ("prim_base", "<u4"),
-                      ("meta", "u1", 8), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
+    NODE8 = np.dtype([("link", "<u4"), ("grid", "<u4", 3), ("qlo", "u1", (3, 8)), ("qhi", "u1", (3, 8))])
+    REC_HOLE = 0xFFFFFFFF
+
+    @staticmethod
+    def node_fields(nodes):
+        """Decoded bc_node8 fields (bc_layout.h): child_base, imask, lmask (leaf slots), spheres,
+        k [n, 3] (origin grid integers), e [n, 3] (grid exponents), o [n, 3] (origin = k 2^e)."""
+        link = nodes["link"].astype(np.uint32)
+        g = nodes["grid"].astype(np.uint32)
+        k = ((g << np.uint32(11)).view(np.int32) >> 11).astype(np.int64)
+        e = ((g >> np.uint32(21)) & np.uint32(0xFF)).astype(np.int64) - 127
+        lmask = (g[:, 0] >> 29) | ((g[:, 1] >> 29) << 3) | (((g[:, 2] >> 29) & 3) << 6)
+        return {"child_base": link & np.uint32(0xFFFFFF), "imask": link >> np.uint32(24), "lmask": lmask,
+                "spheres": (g[:, 2] >> 31) != 0, "k": k, "e": e, "o": np.ldexp(k.astype(np.float64), e)}
 
     def scene_arrays(self):
-        """The resident BVH8: (nodes [NODE8], records [n, 9] float32, ids [n] uint32)."""
+        """The resident BVH8: (nodes [NODE8], records [8 n, 9] float32, ids [8 n] uint32): eight record
+        slots per node, leaf slot s of node i at 8 i + s, holes with id REC_HOLE."""
         st = self.stats()
         nodes = np.zeros(st["nodes8"], self.NODE8)
-        recs = np.zeros((st["prims"], 9), np.float32)
-        ids = np.zeros(st["prims"], np.uint32)
+        recs = np.zeros((8 * st["nodes8"], 9), np.float32)
+        ids = np.zeros(8 * st["nodes8"], np.uint32)
         self._check(lib().bc_scene_arrays(self.ctx, _ptr(nodes), _ptr(recs), _ptr(ids)))
         return nodes, recs, ids
 

@@ -194,13 +194,15 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     std::string err;
     if (!bcx::validate_view(*v, err)) { c->err = err; return BC_ERR_SCENE; }
     const bool on_device = c->flatten == 2 || (c->flatten == 0 && v->nprim >= kDeviceFlattenMin);
+    bcx::PreparedView pv;
+    const bc_scene_view& w = bcx::prepare_view(*v, pv);  // binary leaves of at most eight primitives
     bcx::Bvh8 b;
-    if (!on_device && !bcx::build_bvh8(*v, b, err)) { c->err = err; return BC_ERR_SCENE; }
+    if (!on_device && !bcx::build_bvh8(w, b, err)) { c->err = err; return BC_ERR_SCENE; }
     c->has_scene = false;
     c->batch.clear();
     bcd::FlattenResult fr;
     if (on_device) {
-        const int rc = bcd::flatten_device(c, *v, fr);
+        const int rc = bcd::flatten_device(c, w, fr);
         if (rc != BC_OK) return rc;
     } else {
         fr.nnode8 = (int64_t)b.nodes.size();
@@ -241,11 +243,12 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     }
     if (!ok(c, hipStreamSynchronize(c->stream), "scene upload")) return BC_ERR_DEVICE;
     c->nnode8 = fr.nnode8;
+    c->nrec = (int64_t)BC_REC_PER_NODE * fr.nnode8;
     c->depth8 = fr.depth;
     c->stack = stack;
     c->topo = topo;
     c->geom = geom;
-    c->scene_bytes = (int64_t)((size_t)fr.nnode8 * sizeof(bc_node8) + (size_t)v->nprim * (sizeof(bc_prim) + 8));
+    c->scene_bytes = (int64_t)((size_t)fr.nnode8 * sizeof(bc_node8) + (size_t)c->nrec * (sizeof(bc_prim) + 8));
     c->has_scene = true;
     return BC_OK;
 }
@@ -351,12 +354,13 @@ extern "C" {
 int bc_scene_refit(bc_ctx* c, const bc_scene_view* v, uint64_t geom) {
     if (!c || !v) return BC_ERR_ARG;
     if (!c->has_scene || !c->batch.empty()) { c->err = "bc_scene_refit needs a resident single scene"; return BC_ERR_SCENE; }
-    if (v->nprim != c->nprim || (int64_t)c->h_ids.size() != (int64_t)v->nprim) {
+    if (v->nprim != c->nprim || (int64_t)c->h_ids.size() != c->nrec) {
         c->err = "bc_scene_refit: the primitive count differs from the resident scene (topology changed)";
         return BC_ERR_ARG;
     }
     if (!check_view(c, *v)) return BC_ERR_SCENE;
     for (size_t r = 0; r < c->h_ids.size(); r++) {
+        if (c->h_ids[r] == BC_REC_HOLE) continue;
         const int id = (int)(c->h_ids[r] & ~BC_SPHERE_BIT);
         if ((v->prim_kind[id] == 0) != ((c->h_ids[r] & BC_SPHERE_BIT) != 0)) {
             c->err = "bc_scene_refit: a primitive changed kind (topology changed)";
@@ -376,7 +380,7 @@ int bc_scene_refit(bc_ctx* c, const bc_scene_view* v, uint64_t geom) {
     if (!bc_h2d(c, g, v->prim_geom, 80 * n, c->stream, "refit upload") ||
         !bc_h2d(c, kd, v->prim_kind, 4 * n, c->stream, "refit upload") ||
         !bc_h2d(c, mt, v->prim_mat, 4 * n, c->stream, "refit upload") ||
-        !ok(c, bcd::launch_refit_records(g, kd, mt, c->d_ids, (uint32_t)n, c->d_prims, c->d_pmat, c->d_err, c->stream),
+        !ok(c, bcd::launch_refit_records(g, kd, mt, c->d_ids, (uint32_t)c->nrec, c->d_prims, c->d_pmat, c->d_err, c->stream),
             "refit records"))
         return BC_ERR_DEVICE;
     void* nb = c->d_nbox;
@@ -444,21 +448,21 @@ int bc_batch_upload(bc_ctx* c, int n, const bc_scene_view* views) {
     for (int s = 0; s < n; s++) {
         const bcx::Bvh8& b = bv[(size_t)s];
         const bc_scene_view& v = views[s];
-        const uint32_t nbase = (uint32_t)all.nodes.size(), rbase = (uint32_t)all.prims.size();
+        const uint32_t nbase = (uint32_t)all.nodes.size();
         const int mbase = (int)mkind.size();
-        if ((uint64_t)nbase + b.nodes.size() >= 0x80000000ull || (uint64_t)rbase + b.prims.size() >= 0x80000000ull ||
-            (uint64_t)pbase + (uint64_t)v.nprim >= 0x80000000ull) {
-            c->err = "batch exceeds 2^31 nodes or primitives";
+        if ((uint64_t)nbase + b.nodes.size() > (uint64_t)BC_NODE_MAX || (uint64_t)pbase + (uint64_t)v.nprim >= 0x80000000ull) {
+            c->err = "batch exceeds 2^24 BVH8 nodes or 2^31 primitives";
             return BC_ERR_SCENE;
         }
+        // node links rebased; the record slots follow the nodes (8 per node), so they stay aligned
         for (bc_node8 nd : b.nodes) {
-            nd.child_base += nbase;
-            nd.prim_base = ((nd.prim_base & ~BC_NODE_HAS_SPHERES) + rbase) | (nd.prim_base & BC_NODE_HAS_SPHERES);
+            bc_node_set_link(nd, bc_node_child_base(nd) + nbase, bc_node_imask(nd));
             all.nodes.push_back(nd);
         }
         all.prims.insert(all.prims.end(), b.prims.begin(), b.prims.end());
-        for (uint32_t id : b.ids) all.ids.push_back(((id & ~BC_SPHERE_BIT) + pbase) | (id & BC_SPHERE_BIT));
-        for (int32_t m : b.mats) all.mats.push_back(m + mbase);
+        for (uint32_t id : b.ids)
+            all.ids.push_back(id == BC_REC_HOLE ? id : (((id & ~BC_SPHERE_BIT) + pbase) | (id & BC_SPHERE_BIT)));
+        for (int32_t m : b.mats) all.mats.push_back(m < 0 ? m : m + mbase);
         mkind.insert(mkind.end(), v.mat_kind, v.mat_kind + v.nmat);
         mdata.insert(mdata.end(), v.mat_data, v.mat_data + 8 * (size_t)v.nmat);
         const int l0 = (int)c->h_lights.size();
@@ -482,6 +486,7 @@ int bc_batch_upload(bc_ctx* c, int n, const bc_scene_view* views) {
         !ok(c, hipStreamSynchronize(c->stream), "batch upload"))
         return BC_ERR_DEVICE;
     c->nnode8 = (int64_t)all.nodes.size();
+    c->nrec = (int64_t)all.prims.size();
     c->depth8 = depth;
     c->stack = stack;
     c->topo = c->geom = 0;
@@ -502,9 +507,9 @@ int bc_scene_arrays(bc_ctx* c, void* nodes, void* prims, uint32_t* ids) {
     (void)hipSetDevice(c->device);
     if (nodes && !ok(c, hipMemcpy(nodes, c->d_nodes, (size_t)c->nnode8 * sizeof(bc_node8), hipMemcpyDeviceToHost), "read nodes"))
         return BC_ERR_DEVICE;
-    if (prims && !ok(c, hipMemcpy(prims, c->d_prims, (size_t)c->nprim * sizeof(bc_prim), hipMemcpyDeviceToHost), "read records"))
+    if (prims && !ok(c, hipMemcpy(prims, c->d_prims, (size_t)c->nrec * sizeof(bc_prim), hipMemcpyDeviceToHost), "read records"))
         return BC_ERR_DEVICE;
-    if (ids && !ok(c, hipMemcpy(ids, c->d_ids, (size_t)c->nprim * sizeof(uint32_t), hipMemcpyDeviceToHost), "read ids"))
+    if (ids && !ok(c, hipMemcpy(ids, c->d_ids, (size_t)c->nrec * sizeof(uint32_t), hipMemcpyDeviceToHost), "read ids"))
         return BC_ERR_DEVICE;
     return BC_OK;
 }

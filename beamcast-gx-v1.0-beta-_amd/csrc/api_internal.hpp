@@ -33,6 +33,7 @@ struct bc_ctx {
     size_t cap_nodes = 0, cap_prims = 0, cap_mats = 0, cap_lights = 0, cap_pdf = 0, cap_light_of = 0;
     int64_t nnode8 = 0, scene_bytes = 0;
     int depth8 = 0, nprim = 0, nmat = 0, nlight = 0, stack = 0, nmat_scene = 0;
+    int64_t nrec = 0;  // record slots resident: 8 per BVH8 node
     std::vector<bc_light> h_lights;  // light geometry (emission/CDF filled per material update)
     std::vector<double> h_light_area;
     // light table groups normalised separately ([begin, end) into h_lights): one per scene

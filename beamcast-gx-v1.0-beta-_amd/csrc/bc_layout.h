@@ -3,9 +3,10 @@
 // The reference GPU path uploads a threaded binary BVH of 32-byte nodes and 40-byte
 // triangles (gpu_opencl_windows.go:54-77, packed at :337-377).  This backend replaces
 // that with an 8-wide BVH whose child boxes are quantised to 8 bits per plane relative
-// to a per-node power-of-two grid, and with primitive records stored in leaf order so a
-// leaf's primitives are one contiguous run.
+// to a per-node power-of-two grid, one primitive per leaf slot, and primitive records at fixed
+// positions of their node (eight record slots per node), so a node needs no record base.
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIPCC__)
@@ -14,38 +15,94 @@
 #define BC_HD
 #endif
 
-// One 8-wide node: 80 bytes = five 16-byte loads.
-//   child box (slot i, axis a) = [o_a + qlo_a[i] * 2^(e_a-127), o_a + qhi_a[i] * 2^(e_a-127)]
+// One 8-wide node: 64 bytes = four 16-byte loads.
+//   link           : bits 0-23 child_base, bits 24-31 imask
+//   grid[a]        : bits 0-20 origin k_a (two's complement), bits 21-28 e_a + 127, bits 29-31
+//                    three bits of lmask and the sphere flag (bc_node_lmask, bc_node_spheres)
+//   child box (slot i, axis a) = [o_a + qlo_a[i] * 2^e_a, o_a + qhi_a[i] * 2^e_a], o_a = k_a * 2^e_a
 //   imask bit i    : slot i is an internal child, node index = child_base + popcount(imask & ((1<<i)-1))
-//   meta[i] (leaf) : bits 0-4 offset of the first primitive (relative to prim_base), bits 5-7 count-1
-//   meta[i] = 0xFF : empty slot
+//   lmask bit i    : slot i is a leaf holding one primitive, whose record is 8 * node + i
+//   neither        : empty slot
 // Slot i holds the child lying most towards (+-x,+-y,+-z) = (i&1, i&2, i&4): a ray whose
 // direction sign bits are oct visits slots in the order k ^ oct, k = 0..7 (near to far).
 struct alignas(16) bc_node8 {
-    float ox, oy, oz;
-    uint8_t ex, ey, ez, imask;
-    uint32_t child_base;
-    uint32_t prim_base;
-    uint8_t meta[8];
+    uint32_t link;
+    uint32_t grid[3];
     uint8_t qlox[8], qloy[8], qloz[8];
     uint8_t qhix[8], qhiy[8], qhiz[8];
 };
-static_assert(sizeof(bc_node8) == 80, "bc_node8 must be 80 bytes");
+static_assert(sizeof(bc_node8) == 64, "bc_node8 must be 64 bytes");
 
-#define BC_META_EMPTY 0xFFu
+#define BC_NODE_MAX (1u << 24)     // child_base field: nodes per scene (or batch arena)
+#define BC_GRID_K_BITS 21          // origin grid integer: |k| < 2^20
+#define BC_REC_PER_NODE 8u         // record slots per node: record of leaf slot s of node i = 8 i + s
+#define BC_REC_HOLE 0xFFFFFFFFu    // id of a record slot that holds no primitive
+
+BC_HD static inline uint32_t bc_node_child_base(const bc_node8& n) { return n.link & 0xFFFFFFu; }
+BC_HD static inline uint32_t bc_node_imask(const bc_node8& n) { return n.link >> 24; }
+BC_HD static inline uint32_t bc_node_lmask(const bc_node8& n) {
+    return (n.grid[0] >> 29) | ((n.grid[1] >> 29) << 3) | (((n.grid[2] >> 29) & 3u) << 6);
+}
+BC_HD static inline bool bc_node_spheres(const bc_node8& n) { return (n.grid[2] >> 31) != 0u; }
+BC_HD static inline int32_t bc_node_k(const bc_node8& n, int a) { return ((int32_t)(n.grid[a] << 11)) >> 11; }
+BC_HD static inline int bc_node_e(const bc_node8& n, int a) { return (int)((n.grid[a] >> 21) & 0xFFu) - 127; }
+BC_HD static inline void bc_node_set_link(bc_node8& n, uint32_t child_base, uint32_t imask) {
+    n.link = (child_base & 0xFFFFFFu) | (imask << 24);
+}
+BC_HD static inline void bc_node_set_grid(bc_node8& n, int a, int32_t k, int e) {
+    n.grid[a] = ((uint32_t)k & 0x1FFFFFu) | ((uint32_t)(e + 127) << 21) | (n.grid[a] & 0xE0000000u);
+}
+BC_HD static inline void bc_node_set_flags(bc_node8& n, uint32_t lmask, bool spheres) {
+    n.grid[0] = (n.grid[0] & 0x1FFFFFFFu) | ((lmask & 7u) << 29);
+    n.grid[1] = (n.grid[1] & 0x1FFFFFFFu) | (((lmask >> 3) & 7u) << 29);
+    n.grid[2] = (n.grid[2] & 0x1FFFFFFFu) | (((lmask >> 6) & 3u) << 29) | (spheres ? 0x80000000u : 0u);
+}
+
+// The quantisation grid of one axis of a node whose children's union box is [mn, mx] (fp64):
+// outward padding pad (relative 2^-20), the smallest power-of-two step 2^e (e >= -126) for which
+// the origin k 2^e = floor((mn - pad) / 2^e) 2^e has |k| < 2^20 and 255 steps from it reach
+// mx + pad.  The same arithmetic runs on the host and the device (exact fp64 operations only),
+// so both produce the same bytes.  Returns false when the step would exceed 2^127 or the origin
+// leaves the fp32 range.
+BC_HD static inline bool bc_grid_axis(double mn, double mx, double& pad, int32_t& k, int& e, double& org, double& scale) {
+    const double mag = fmax(fabs(mn), fabs(mx));
+    const double ext = mx - mn;
+    pad = fmax((mag + ext) * 0x1p-20, 1e-20);
+    const double lo = mn - pad, hi = mx + pad;
+    int k2;
+    const double m2 = frexp((hi - lo) / 255.0, &k2);  // smallest e with 2^e >= (hi - lo) / 255, exactly
+    e = m2 == 0.5 ? k2 - 1 : k2;
+    if (e < -126) e = -126;
+    for (;;) {
+        if (e > 127) return false;
+        const double step = ldexp(1.0, e);
+        const double kd = floor(lo / step);
+        if (fabs(kd) >= 1048576.0 || (kd + 255.0) * step < hi) {
+            e++;
+            continue;
+        }
+        k = (int32_t)kd;
+        org = kd * step;
+        scale = step;
+        return fabs(org) < 0x1p127 && fabs(org + 255.0 * step) < 0x1p127;
+    }
+}
+
 #define BC_SPHERE_BIT 0x80000000u
 
-// Primitive record (36 bytes, nine floats), in BVH8 leaf order: exactly the operands of the
+// Primitive record (36 bytes, nine floats), at 8 * node + slot of its leaf slot (record slots
+// of internal and empty slots are holes: zero record, id BC_REC_HOLE): exactly the operands of the
 // intersection test.
 //   triangle: A.xyz, E1.xyz, E2.xyz
 //   sphere  : C.xyz, r, r*r, 1/r, 0, 0, 0
 // Parallel arrays by record: id = Scene.Primitives index (renderer.go:233) | SPHERE_BIT for
 // spheres, and material.  Traversal reads the id only to resolve an accepted hit, and the
-// kind only in nodes flagged as holding spheres (bit 31 of prim_base).
+// kind only in nodes flagged as holding spheres (bc_node_spheres).
 struct bc_prim {
     float v[9];
 };
 static_assert(sizeof(bc_prim) == 36, "bc_prim must be 36 bytes");
+// bit 31 of a traversal's record base: the node holds spheres
 #define BC_NODE_HAS_SPHERES 0x80000000u
 
 // Material record: kinds as renderer.go:86-91.

@@ -8,10 +8,10 @@
 // 1..8 it keeps the cheapest way to represent the subtree as at most that many children of
 // one wide node -- a leaf of at most kLeafMax primitives, a wide internal node, or the budget
 // split between the two binary children -- with cost = area x (kNodeCost per node visited,
-// kPrimCost per primitive tested).  Leaf slots hold kLeafMax primitives (more only for binary
-// leaves that would otherwise need more than eight slots, bc_leaf_run), so a node's leaves fit
-// its 32-primitive run; larger binary leaves become several leaf slots, and a leaf that needs
-// more than four slots may instead become a wide node of its own.
+// kPrimCost per primitive tested).  Leaf slots hold one primitive each; a binary leaf becomes
+// several leaf slots, or a wide node of its own when it needs more than four (binary leaves of
+// more than eight primitives are split in two beforehand, prepare_view).  A node's leaf records
+// sit at its eight record slots, 8 * node + slot.
 // Every primitive appears in exactly one leaf; which leaf only changes culling, never a hit
 // result (equal distances resolve to the smaller primitive index).
 #include "bvh8_build.hpp"
@@ -49,12 +49,6 @@ struct Child {
     Box box;
 };
 
-float f_down(double v) {
-    float f = (float)v;
-    if ((double)f > v) f = std::nextafter(f, -INFINITY);
-    return f;
-}
-
 Box node_box(const bc_scene_view& v, int i) {
     Box b;
     for (int a = 0; a < 3; a++) {
@@ -80,13 +74,20 @@ Box prim_box(const bc_scene_view& v, int p) {
     return b;
 }
 
-void write_prim(const bc_scene_view& v, int id, Bvh8& out) {
-    out.prims.push_back(prim_record(v, id));
-    out.ids.push_back((uint32_t)id | (v.prim_kind[id] == 0 ? BC_SPHERE_BIT : 0u));
-    out.mats.push_back(v.prim_mat[id]);
+// Record slot r of the output (grown to hold it, holes filled as empty slots).
+void write_prim(const bc_scene_view& v, int id, size_t r, Bvh8& out) {
+    if (out.prims.size() <= r) {
+        const size_t n = (r / BC_REC_PER_NODE + 1) * BC_REC_PER_NODE;
+        out.prims.resize(n, bc_prim{});
+        out.ids.resize(n, BC_REC_HOLE);
+        out.mats.resize(n, -1);
+    }
+    out.prims[r] = prim_record(v, id);
+    out.ids[r] = (uint32_t)id | (v.prim_kind[id] == 0 ? BC_SPHERE_BIT : 0u);
+    out.mats[r] = v.prim_mat[id];
 }
 
-constexpr int kLeafMax = BC_LEAF_MAX;       // primitives per leaf slot (8 slots x kLeafMax <= 32)
+constexpr int kLeafMax = 1;                 // primitives per leaf slot
 constexpr double kNodeCost = 1.0;           // relative cost of visiting one wide node
 constexpr double kPrimCost = BC_PRIM_COST;  // relative cost of one primitive test
 
@@ -113,12 +114,11 @@ void expand(const bc_scene_view& v, const std::vector<Plan>& plan, int n, int sl
             c.box = node_box(v, n);
             out.push_back(c);
         } else {
-            const int run = bc_leaf_run(P.count);
-            for (int s = 0; s < P.count; s += run) {
+            for (int s = 0; s < P.count; s++) {
                 Child c;
-                c.leaf = true; c.node2 = -1; c.start = P.start + s; c.count = std::min(run, P.count - s);
+                c.leaf = true; c.node2 = -1; c.start = P.start + s; c.count = 1;
                 c.box.empty();
-                for (int k = 0; k < c.count; k++) c.box.grow(prim_box(v, v.order[c.start + k]));
+                c.box.grow(prim_box(v, v.order[c.start]));
                 out.push_back(c);
             }
         }
@@ -142,11 +142,10 @@ void make_plans(const bc_scene_view& v, std::vector<Plan>& plan) {
         Plan& P = plan[(size_t)n];
         const int32_t* l = v.node_link + 4 * (size_t)n;
         const double A = node_box(v, n).area();
-        if (l[3] > 0) {  // binary leaf: ceil(count / run) leaf slots, at most eight
+        if (l[3] > 0) {  // binary leaf: one slot per primitive, at most eight
             P.start = l[2];
             P.count = l[3];
-            const int run = bc_leaf_run(l[3]);
-            const int need = (l[3] + run - 1) / run;
+            const int need = l[3];
             const float c = (float)(A * kPrimCost * l[3]);
             // a leaf needing more than half a node may also be a wide node of its own (one slot)
             const float c_wide = need > 4 ? c + (float)(A * kNodeCost) : inf;
@@ -218,18 +217,58 @@ bool validate_view(const bc_scene_view& v, std::string& err) {
     return true;
 }
 
-bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
+const bc_scene_view& prepare_view(const bc_scene_view& v, PreparedView& pv) {
+    size_t big = 0;
+    for (int i = 0; i < v.nnode; i++) big += v.node_link[4 * (size_t)i + 3] > BC_BINARY_LEAF_MAX ? 1 : 0;
+    if (big == 0) return v;
+    const size_t n0 = (size_t)v.nnode, n = n0 + 2 * big;
+    pv.link.assign(v.node_link, v.node_link + 4 * n0);
+    pv.link.resize(4 * n);
+    pv.box.assign(v.node_box, v.node_box + 6 * n0);
+    pv.box.resize(6 * n);
+    size_t next = n0;
+    for (size_t i = 0; i < n0; i++) {
+        const int32_t start = pv.link[4 * i + 2], cnt = pv.link[4 * i + 3];
+        if (cnt <= BC_BINARY_LEAF_MAX) continue;
+        const int32_t first = (cnt + 1) / 2;
+        const int32_t part[2][2] = {{start, first}, {start + first, cnt - first}};
+        pv.link[4 * i + 0] = (int32_t)next;
+        pv.link[4 * i + 1] = (int32_t)next + 1;
+        pv.link[4 * i + 2] = 0;
+        pv.link[4 * i + 3] = 0;
+        for (int h = 0; h < 2; h++, next++) {
+            int32_t* c = &pv.link[4 * next];
+            c[0] = c[1] = -1;
+            c[2] = part[h][0];
+            c[3] = part[h][1];
+            Box b;
+            b.empty();
+            for (int k = 0; k < part[h][1]; k++) b.grow(prim_box(v, v.order[part[h][0] + k]));
+            for (int a = 0; a < 3; a++) {
+                pv.box[6 * next + (size_t)a] = b.mn[a];
+                pv.box[6 * next + 3 + (size_t)a] = b.mx[a];
+            }
+        }
+    }
+    pv.view = v;
+    pv.view.nnode = (int32_t)n;
+    pv.view.node_link = pv.link.data();
+    pv.view.node_box = pv.box.data();
+    return pv.view;
+}
+
+bool build_bvh8(const bc_scene_view& v0, Bvh8& out, std::string& err) {
     out.nodes.clear();
     out.prims.clear();
     out.ids.clear();
     out.mats.clear();
     out.node_depth.clear();
     out.depth = 0;
-    if (!validate_view(v, err)) return false;
-    out.prims.reserve((size_t)v.nprim);
-    out.ids.reserve((size_t)v.nprim);
-    out.mats.reserve((size_t)v.nprim);
+    if (!validate_view(v0, err)) return false;
+    PreparedView pv;
+    const bc_scene_view& v = prepare_view(v0, pv);
     out.nodes.reserve((size_t)v.nnode / 3 + 1);
+    int64_t written = 0;
     std::vector<Plan> plan;
     make_plans(v, plan);
     if (!(plan[0].cost[7] < INFINITY)) { err = "internal: no BVH8 plan for the scene"; return false; }
@@ -293,34 +332,31 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
 
         bc_node8 nd;
         std::memset(&nd, 0, sizeof nd);
-        // quantisation grid (conservative: padded box, origin rounded down, power-of-two step)
+        // quantisation grid (conservative: padded box, origin on the grid below it, power-of-two step)
         double pad[3], org[3], scale[3];
         for (int a = 0; a < 3; a++) {
-            double mag = std::max(std::fabs(nb.mn[a]), std::fabs(nb.mx[a]));
-            double ext = nb.mx[a] - nb.mn[a];
-            pad[a] = std::max((mag + ext) * 0x1p-20, 1e-20);
-            float o = f_down(nb.mn[a] - pad[a]);
-            double ext2 = (nb.mx[a] + pad[a]) - (double)o;
-            int e = ceil_log2(ext2 / 255.0);
-            while (std::ldexp(255.0, e) < ext2) e++;
-            if (e < -126) e = -126;
-            if (e > 127) { err = "scene extent too large for quantised BVH8"; return false; }
-            org[a] = o;
-            scale[a] = std::ldexp(1.0, e);
-            (&nd.ox)[a] = o;
-            (&nd.ex)[a] = (uint8_t)(e + 127);
+            int32_t k;
+            int e;
+            if (!bc_grid_axis(nb.mn[a], nb.mx[a], pad[a], k, e, org[a], scale[a])) {
+                err = "scene extent too large for quantised BVH8";
+                return false;
+            }
+            bc_node_set_grid(nd, a, k, e);
         }
         int ninternal = 0;
         for (auto& c : fin) ninternal += c.leaf ? 0 : 1;
-        uint32_t child_base = (uint32_t)out.nodes.size();
-        nd.child_base = child_base;
-        nd.prim_base = (uint32_t)out.prims.size();
+        const uint32_t child_base = (uint32_t)out.nodes.size();
+        if ((uint64_t)child_base + (uint64_t)ninternal > (uint64_t)BC_NODE_MAX) {
+            err = "scene needs more than 2^24 BVH8 nodes";
+            return false;
+        }
         out.nodes.resize(out.nodes.size() + (size_t)ninternal);
-        int rank = 0, poff = 0;
+        uint32_t imask = 0, lmask = 0;
+        bool spheres = false;
+        int rank = 0;
         for (int s = 0; s < 8; s++) {
             int i = child_at_slot[s];
             if (i < 0) {
-                nd.meta[s] = BC_META_EMPTY;
                 nd.qlox[s] = nd.qloy[s] = nd.qloz[s] = 255;
                 nd.qhix[s] = nd.qhiy[s] = nd.qhiz[s] = 0;
                 continue;
@@ -335,28 +371,32 @@ bool build_bvh8(const bc_scene_view& v, Bvh8& out, std::string& err) {
                 qhi[a][s] = (uint8_t)std::min(255.0, std::max(0.0, hi));
             }
             if (c.leaf) {
-                nd.meta[s] = (uint8_t)(((c.count - 1) << 5) | poff);
-                for (int k = 0; k < c.count; k++) {
-                    const int id = v.order[c.start + k];
-                    write_prim(v, id, out);
-                    if (v.prim_kind[id] == 0) nd.prim_base |= BC_NODE_HAS_SPHERES;
-                }
-                poff += c.count;
+                if (c.count != 1) { err = "internal: a BVH8 leaf slot holds more than one primitive"; return false; }
+                const int id = v.order[c.start];
+                write_prim(v, id, BC_REC_PER_NODE * (size_t)w.i8 + (size_t)s, out);
+                written++;
+                lmask |= 1u << s;
+                spheres = spheres || v.prim_kind[id] == 0;
             } else {
-                nd.imask |= (uint8_t)(1u << s);
-                nd.meta[s] = 0;
+                imask |= 1u << s;
                 work.push_back({c.node2, (int)child_base + rank, w.depth + 1});
                 rank++;
             }
         }
+        bc_node_set_link(nd, child_base, imask);
+        bc_node_set_flags(nd, lmask, spheres);
         out.nodes[(size_t)w.i8] = nd;
         if (out.node_depth.size() < out.nodes.size()) out.node_depth.resize(out.nodes.size(), 0);
         out.node_depth[(size_t)w.i8] = w.depth;
     }
-    if ((int64_t)out.prims.size() != v.nprim) {
+    if (written != v.nprim) {
         err = "BVH leaves do not cover every primitive exactly once";
         return false;
     }
+    const size_t nrec = BC_REC_PER_NODE * out.nodes.size();
+    out.prims.resize(nrec, bc_prim{});
+    out.ids.resize(nrec, BC_REC_HOLE);
+    out.mats.resize(nrec, -1);
     return true;
 }
 

@@ -2,21 +2,20 @@
 //
 // Same algorithm and arithmetic as the host flattening (bvh8_build.cpp): the surface-area cost
 // plans of every binary node for 1..8 slots (Ylitie, Karras and Laine, HPG 2017), their top-down
-// expansion into wide nodes (leaf runs of at most kLeafMax primitives), the octant slot
+// expansion into wide nodes (one primitive per leaf slot), the octant slot
 // assignment and the conservative 8-bit quantisation.  The host emits wide nodes from a LIFO
 // work stack; here the wide tree is expanded level by level and the stack's numbering is
 // recovered from subtree sizes: the stack processes a node, allocates its internal children
 // contiguously, then processes the children's subtrees from the highest slot down, so
 //     child_base(C) = child_base(W) + nint(W) + sum over internal siblings C' of C at higher
 //                     slots of (wide nodes in subtree(C') - 1),
-//     first record(C) = first record(W) + records(W) + sum over the same siblings of the
-//                     records in subtree(C'),
-// with the root at index 0, child_base 1, first record 0.  Nodes, records, ids and materials are
-// therefore identical to the host's and every traversal result is unchanged.
+// with the root at index 0, child_base 1.  Records sit at 8 * node + slot.  Nodes, records, ids
+// and materials are therefore identical to the host's and every traversal result is unchanged.
 //
 // Passes: binary levels (top-down), plans (bottom-up, one launch per level), wide levels
 // (top-down: children, slots, quantised node), subtree sums (bottom-up), numbering (top-down),
-// emission (nodes, records in leaf order, refit level lists).
+// emission (nodes, records at their record slots, refit level lists).  The view must be
+// prepared (bcx::prepare_view: binary leaves of at most eight primitives).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,7 +28,7 @@
 namespace bcd {
 namespace {
 
-constexpr int kLeafMax = BC_LEAF_MAX;
+constexpr int kLeafMax = 1;  // primitives per leaf slot
 constexpr double kNodeCost = 1.0;
 constexpr double kPrimCost = BC_PRIM_COST;
 constexpr int kT = 256;
@@ -126,8 +125,7 @@ __global__ void k_plans(Inputs in, const int* __restrict__ level, int n, Plan* _
     if (l[3] > 0) {
         P.start = l[2];
         P.count = l[3];
-        const int run = bc_leaf_run(l[3]);
-        const int need = (l[3] + run - 1) / run;
+        const int need = l[3];  // one slot per primitive (leaves of at most eight: prepare_view)
         const float c = (float)(A * kPrimCost * l[3]);
         const float c_wide = need > 4 ? c + (float)(A * kNodeCost) : inf;
         for (int s = 1; s <= 8; s++) {
@@ -171,13 +169,12 @@ __global__ void k_plans(Inputs in, const int* __restrict__ level, int n, Plan* _
 // wide levels
 
 struct WNode {
-    bc_node8 q;            // quantised node; child_base and prim_base filled by the numbering
+    bc_node8 q;            // quantised node; child_base filled by the numbering
     int32_t node2, depth;
-    int32_t child[8];      // internal slot: temporary id of the child wide node; leaf slot: run start
-    int32_t count[8];      // leaf slot: run length (0 for internal and empty slots)
-    int32_t nint, nrec, sph;
-    int32_t wsize, rsize;  // wide nodes and records in the subtree
-    int32_t idx, alloc, rec;
+    int32_t child[8];      // internal slot: temporary id of the child wide node; leaf slot: Order index
+    int32_t nint;
+    int32_t wsize;         // wide nodes in the subtree
+    int32_t idx, alloc;
 };
 
 struct Child {
@@ -206,13 +203,12 @@ __device__ int expand(const Inputs& in, const Plan* __restrict__ plan, int n0, i
                 c.node2 = -1; c.start = P.start; c.count = P.count;
                 c.box = node_box(in, n);
             } else {
-                const int run = bc_leaf_run(P.count);
-                for (int s = 0; s < P.count; s += run) {
+                for (int s = 0; s < P.count; s++) {
                     if (nout >= 8) { atomicOr(err, 2); return nout; }
                     Child& c = out[nout++];
-                    c.node2 = -1; c.start = P.start + s; c.count = min(run, P.count - s);
+                    c.node2 = -1; c.start = P.start + s; c.count = 1;
                     box_empty(c.box);
-                    for (int k = 0; k < c.count; k++) box_grow(c.box, prim_box(in, in.order[c.start + k]));
+                    box_grow(c.box, prim_box(in, in.order[c.start]));
                 }
             }
         } else if (h == 2) {
@@ -295,29 +291,20 @@ __global__ void k_wide(Inputs in, const Plan* __restrict__ plan, WNode* __restri
     memset(&nd, 0, sizeof nd);
     double pad[3], org[3], scale[3];
     for (int a = 0; a < 3; a++) {
-        const double mag = fmax(fabs(nb.mn[a]), fabs(nb.mx[a]));
-        const double ext = nb.mx[a] - nb.mn[a];
-        pad[a] = fmax((mag + ext) * 0x1p-20, 1e-20);
-        const float o = f_down(nb.mn[a] - pad[a]);
-        const double ext2 = (nb.mx[a] + pad[a]) - (double)o;
-        int e = ceil_log2(ext2 / 255.0);
-        while (ldexp(255.0, e) < ext2) e++;
-        if (e < -126) e = -126;
-        if (e > 127) { atomicOr(err, 4); e = 127; }
-        org[a] = o;
-        scale[a] = ldexp(1.0, e);
-        (&nd.ox)[a] = o;
-        (&nd.ex)[a] = (uint8_t)(e + 127);
+        int32_t k = 0;
+        int e = 127;
+        if (!bc_grid_axis(nb.mn[a], nb.mx[a], pad[a], k, e, org[a], scale[a])) atomicOr(err, 4);
+        bc_node_set_grid(nd, a, k, e);
     }
     uint8_t* qlo[3] = {nd.qlox, nd.qloy, nd.qloz};
     uint8_t* qhi[3] = {nd.qhix, nd.qhiy, nd.qhiz};
-    int nint = 0, poff = 0, sph = 0;
+    uint32_t imask = 0, lmask = 0;
+    int nint = 0;
+    bool sph = false;
     for (int s = 0; s < 8; s++) {
         W.child[s] = -1;
-        W.count[s] = 0;
         const int i = at[s];
         if (i < 0) {
-            nd.meta[s] = BC_META_EMPTY;
             qlo[0][s] = qlo[1][s] = qlo[2][s] = 255;
             qhi[0][s] = qhi[1][s] = qhi[2][s] = 0;
             continue;
@@ -330,14 +317,12 @@ __global__ void k_wide(Inputs in, const Plan* __restrict__ plan, WNode* __restri
             qhi[a][s] = (uint8_t)fmin(255.0, fmax(0.0, hi));
         }
         if (c.node2 < 0) {
-            nd.meta[s] = (uint8_t)(((c.count - 1) << 5) | poff);
+            if (c.count != 1) atomicOr(err, 2);
+            lmask |= 1u << s;
             W.child[s] = c.start;
-            W.count[s] = c.count;
-            for (int k = 0; k < c.count; k++) sph |= in.kind[in.order[c.start + k]] == 0;
-            poff += c.count;
+            sph = sph || in.kind[in.order[c.start]] == 0;
         } else {
-            nd.imask |= (uint8_t)(1u << s);
-            nd.meta[s] = 0;
+            imask |= 1u << s;
             const int cid = atomicAdd(counter, 1);
             if (cid >= cap) { atomicOr(err, 8); continue; }
             WNode& C = w[cid];
@@ -347,73 +332,68 @@ __global__ void k_wide(Inputs in, const Plan* __restrict__ plan, WNode* __restri
             nint++;
         }
     }
-    if (poff > 32) atomicOr(err, 2);
+    bc_node_set_link(nd, 0u, imask);
+    bc_node_set_flags(nd, lmask, sph);
     W.q = nd;
     W.nint = nint;
-    W.nrec = poff;
-    W.sph = sph;
 }
 
 __global__ void k_wsums(WNode* __restrict__ w, int first, int n) {
     const int t = blockIdx.x * kT + threadIdx.x;
     if (t >= n) return;
     WNode& W = w[first + t];
-    int ws = 1, rs = W.nrec;
+    const uint32_t imask = bc_node_imask(W.q);
+    int ws = 1;
     for (int s = 0; s < 8; s++)
-        if ((W.q.imask >> s) & 1) { ws += w[W.child[s]].wsize; rs += w[W.child[s]].rsize; }
+        if ((imask >> s) & 1) ws += w[W.child[s]].wsize;
     W.wsize = ws;
-    W.rsize = rs;
 }
 
 __global__ void k_wnumber(WNode* __restrict__ w, int first, int n) {
     const int t = blockIdx.x * kT + threadIdx.x;
     if (t >= n) return;
     const WNode& W = w[first + t];
+    const uint32_t imask = bc_node_imask(W.q);
     int rank = 0;
     for (int s = 0; s < 8; s++)
-        if ((W.q.imask >> s) & 1) w[W.child[s]].idx = W.alloc + rank++;
-    int alloc = W.alloc + W.nint, rec = W.rec + W.nrec;
+        if ((imask >> s) & 1) w[W.child[s]].idx = W.alloc + rank++;
+    int alloc = W.alloc + W.nint;
     for (int s = 7; s >= 0; s--) {  // the LIFO stack visits the highest slot first
-        if (!((W.q.imask >> s) & 1)) continue;
+        if (!((imask >> s) & 1)) continue;
         WNode& C = w[W.child[s]];
         C.alloc = alloc;
-        C.rec = rec;
         alloc += C.wsize - 1;
-        rec += C.rsize;
     }
 }
 
+// Nodes, and the records of their leaf slots at 8 * node + slot (the holes were filled before).
 __global__ void k_wemit(Inputs in, const WNode* __restrict__ w, int n, bc_node8* __restrict__ nodes, bc_prim* __restrict__ prims,
-                        uint32_t* __restrict__ ids, int32_t* __restrict__ mats, int nnode_out, int nprim) {
+                        uint32_t* __restrict__ ids, int32_t* __restrict__ mats, int nnode_out) {
     const int t = blockIdx.x * kT + threadIdx.x;
     if (t >= n) return;
     const WNode& W = w[t];
     if (W.idx < 0 || W.idx >= nnode_out) return;
     bc_node8 nd = W.q;
-    nd.child_base = (uint32_t)W.alloc;
-    nd.prim_base = (uint32_t)W.rec | (W.sph ? BC_NODE_HAS_SPHERES : 0u);
+    bc_node_set_link(nd, (uint32_t)W.alloc, bc_node_imask(nd));
     nodes[W.idx] = nd;
+    const uint32_t lmask = bc_node_lmask(nd);
     for (int s = 0; s < 8; s++) {
-        if (nd.meta[s] == BC_META_EMPTY || ((nd.imask >> s) & 1)) continue;
-        const int base = W.rec + (nd.meta[s] & 31);
-        for (int k = 0; k < W.count[s]; k++) {
-            const int r = base + k;
-            if (r >= nprim) return;
-            const int id = in.order[W.child[s] + k];
-            const double* g = in.geom + 10 * (size_t)id;
-            bc_prim p;
-            for (int q = 0; q < 9; q++) p.v[q] = 0.f;
-            if (in.kind[id] == 0) {
-                const double rad = g[3];
-                p.v[0] = (float)g[0]; p.v[1] = (float)g[1]; p.v[2] = (float)g[2];
-                p.v[3] = (float)rad; p.v[4] = (float)(rad * rad); p.v[5] = (float)(1.0 / rad);
-            } else {
-                for (int q = 0; q < 9; q++) p.v[q] = (float)g[q];
-            }
-            prims[r] = p;
-            ids[r] = (uint32_t)id | (in.kind[id] == 0 ? BC_SPHERE_BIT : 0u);
-            mats[r] = in.mat[id];
+        if (!((lmask >> s) & 1u)) continue;
+        const size_t r = (size_t)W.idx * BC_REC_PER_NODE + (size_t)s;
+        const int id = in.order[W.child[s]];
+        const double* g = in.geom + 10 * (size_t)id;
+        bc_prim p;
+        for (int q = 0; q < 9; q++) p.v[q] = 0.f;
+        if (in.kind[id] == 0) {
+            const double rad = g[3];
+            p.v[0] = (float)g[0]; p.v[1] = (float)g[1]; p.v[2] = (float)g[2];
+            p.v[3] = (float)rad; p.v[4] = (float)(rad * rad); p.v[5] = (float)(1.0 / rad);
+        } else {
+            for (int q = 0; q < 9; q++) p.v[q] = (float)g[q];
         }
+        prims[r] = p;
+        ids[r] = (uint32_t)id | (in.kind[id] == 0 ? BC_SPHERE_BIT : 0u);
+        mats[r] = in.mat[id];
     }
 }
 
@@ -429,7 +409,6 @@ __global__ void k_wroot(WNode* w) {
     r.depth = 1;
     r.idx = 0;
     r.alloc = 1;
-    r.rec = 0;
 }
 
 inline unsigned grid(size_t n) { return (unsigned)((n + kT - 1) / kT); }
@@ -529,25 +508,34 @@ int flatten_device(bc_ctx* c, const bc_scene_view& v, FlattenResult& res) {
     for (int l = 0; l < nlevels; l++)
         hipLaunchKernelGGL(k_wnumber, dim3(grid((size_t)(woff[l + 1] - woff[l]))), dim3(kT), 0, st, wn.as<WNode>(), woff[l],
                            woff[l + 1] - woff[l]);
-    // outputs
+    if ((size_t)total > (size_t)BC_NODE_MAX) {
+        c->err = "scene needs more than 2^24 BVH8 nodes";
+        return BC_ERR_SCENE;
+    }
+    // outputs: nodes, and eight record slots per node (holes: zero record, id BC_REC_HOLE, material -1)
+    const size_t nrec = BC_REC_PER_NODE * (size_t)total;
     void* p;
     p = c->d_nodes;
     if (!bc_ensure(c, p, c->cap_nodes, sizeof(bc_node8) * (size_t)total, "upload nodes")) return BC_ERR_NOMEM;
     c->d_nodes = (bc_node8*)p;
     p = c->d_prims;
-    if (!bc_ensure(c, p, c->cap_prims, sizeof(bc_prim) * np, "upload prims")) return BC_ERR_NOMEM;
+    if (!bc_ensure(c, p, c->cap_prims, sizeof(bc_prim) * nrec, "upload prims")) return BC_ERR_NOMEM;
     c->d_prims = (bc_prim*)p;
     p = c->d_ids;
-    if (!bc_ensure(c, p, c->cap_ids, 4 * np, "upload prim ids")) return BC_ERR_NOMEM;
+    if (!bc_ensure(c, p, c->cap_ids, 4 * nrec, "upload prim ids")) return BC_ERR_NOMEM;
     c->d_ids = (uint32_t*)p;
     p = c->d_pmat;
-    if (!bc_ensure(c, p, c->cap_pmat, 4 * np, "upload prim materials")) return BC_ERR_NOMEM;
+    if (!bc_ensure(c, p, c->cap_pmat, 4 * nrec, "upload prim materials")) return BC_ERR_NOMEM;
     c->d_pmat = (int32_t*)p;
     p = c->d_levels;
     if (!bc_ensure(c, p, c->cap_levels, 4 * (size_t)total, "upload node levels")) return BC_ERR_NOMEM;
     c->d_levels = (uint32_t*)p;
+    if (!bc_check(c, hipMemsetAsync(c->d_prims, 0, sizeof(bc_prim) * nrec, st), "record holes") ||
+        !bc_check(c, hipMemsetAsync(c->d_ids, 0xFF, 4 * nrec, st), "record holes") ||
+        !bc_check(c, hipMemsetAsync(c->d_pmat, 0xFF, 4 * nrec, st), "record holes"))
+        return BC_ERR_DEVICE;
     hipLaunchKernelGGL(k_wemit, dim3(grid((size_t)total)), dim3(kT), 0, st, in, wn.as<WNode>(), total, c->d_nodes, c->d_prims,
-                       c->d_ids, c->d_pmat, total, (int)np);
+                       c->d_ids, c->d_pmat, total);
     // refit lists: deepest level first
     res.level_off.assign(1, 0u);
     uint32_t off = 0;
@@ -558,18 +546,29 @@ int flatten_device(bc_ctx* c, const bc_scene_view& v, FlattenResult& res) {
         res.level_off.push_back(off);
     }
     if (!bc_check(c, hipGetLastError(), "flatten launch")) return BC_ERR_DEVICE;
-    // the root's subtree must hold every record exactly once
+    // the root's subtree must be every wide node, and the leaf slots every primitive exactly once
     WNode root;
     if (!bc_check(c, hipMemcpyAsync(&root, wn.p, sizeof root, hipMemcpyDeviceToHost, st), "flatten") ||
         !bc_check(c, hipStreamSynchronize(st), "flatten"))
         return BC_ERR_DEVICE;
-    if (root.wsize != total || root.rsize != (int)np) {
+    res.ids.resize(nrec);
+    if (root.wsize != total || !bc_d2h(c, res.ids.data(), c->d_ids, 4 * nrec, st, "read ids")) {
+        if (root.wsize != total) c->err = "flatten: the wide tree does not cover its nodes";
+        return root.wsize != total ? BC_ERR_SCENE : BC_ERR_DEVICE;
+    }
+    std::vector<char> seen(np, 0);
+    size_t nseen = 0;
+    for (uint32_t id : res.ids) {
+        if (id == BC_REC_HOLE) continue;
+        const uint32_t q = id & ~BC_SPHERE_BIT;
+        if (q >= np || seen[q]) { nseen = np + 1; break; }
+        seen[q] = 1;
+        nseen++;
+    }
+    if (nseen != np) {
         c->err = "BVH leaves do not cover every primitive exactly once";
         return BC_ERR_SCENE;
     }
-    res.ids.resize(np);
-    if (!bc_d2h(c, res.ids.data(), c->d_ids, 4 * np, st, "read ids"))
-        return BC_ERR_DEVICE;
     res.nnode8 = total;
     res.depth = nlevels;
     return BC_OK;

@@ -33,21 +33,21 @@ struct Grid {
     float ox, oy, oz, sx, sy, sz, cmax;
 };
 
-// The root node's quantisation grid spans the scene: origin o, extent 255 * 2^(e - 127) per
-// axis.  Cells per unit length = 2^bits / extent.
+// The root node's quantisation grid spans the scene: origin k * 2^e, extent 255 * 2^e per axis.
+// Cells per unit length = 2^bits / extent.
 __device__ __forceinline__ Grid root_grid(const bc_node8* __restrict__ nodes, int bits) {
-    const uint4 w0 = *reinterpret_cast<const uint4*>(nodes);
+    const bc_node8 root = nodes[0];
     Grid g;
-    g.ox = __uint_as_float(w0.x);
-    g.oy = __uint_as_float(w0.y);
-    g.oz = __uint_as_float(w0.z);
+    float o[3], s[3];
     const float cells = (float)(1u << bits);
-    const float sx = cells / (255.0f * __uint_as_float((w0.w & 0xffu) << 23));
-    const float sy = cells / (255.0f * __uint_as_float(((w0.w >> 8) & 0xffu) << 23));
-    const float sz = cells / (255.0f * __uint_as_float(((w0.w >> 16) & 0xffu) << 23));
-    g.sx = isfinite(sx) ? sx : 0.0f;
-    g.sy = isfinite(sy) ? sy : 0.0f;
-    g.sz = isfinite(sz) ? sz : 0.0f;
+    for (int a = 0; a < 3; a++) {
+        const float step = __uint_as_float((uint32_t)(bc_node_e(root, a) + 127) << 23);
+        o[a] = (float)bc_node_k(root, a) * step;
+        const float c = cells / (255.0f * step);
+        s[a] = isfinite(c) ? c : 0.0f;
+    }
+    g.ox = o[0]; g.oy = o[1]; g.oz = o[2];
+    g.sx = s[0]; g.sy = s[1]; g.sz = s[2];
     g.cmax = cells - 1.0f;
     return g;
 }

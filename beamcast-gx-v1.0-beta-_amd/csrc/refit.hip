@@ -7,7 +7,7 @@
 // one thread per node: the union of the node's child boxes (leaf runs: the boxes of their fp32
 // primitive records; internal children: the union box the child's own thread stored), then the
 // node's quantisation grid and child planes with the conservative rule of the host flattening
-// (outward padding, origin rounded down, power-of-two step).  Topology, slot layout and leaf
+// (bc_grid_axis: outward padding, origin on the grid below the box, power-of-two step).  Topology, slot layout and leaf
 // runs are unchanged, so traversal results stay exact; only culling tightness can change.
 #include <hip/hip_runtime.h>
 
@@ -29,33 +29,30 @@ __device__ __forceinline__ void dbox_grow(DBox& b, double x, double y, double z)
     for (int a = 0; a < 3; a++) { b.mn[a] = fmin(b.mn[a], p[a]); b.mx[a] = fmax(b.mx[a], p[a]); }
 }
 
-// Box of one child slot of node nd (slot s is not empty).
-__device__ __forceinline__ DBox slot_box(const bc_node8& nd, int s, const bc_prim* __restrict__ prims,
+// Box of one child slot s of node idx (slot s is not empty).
+__device__ __forceinline__ DBox slot_box(const bc_node8& nd, uint32_t idx, int s, const bc_prim* __restrict__ prims,
                                          const uint32_t* __restrict__ ids, const float4* __restrict__ nbox) {
     DBox b;
     dbox_empty(b);
-    if ((nd.imask >> s) & 1u) {
-        const uint32_t child = nd.child_base + (uint32_t)__popc((uint32_t)nd.imask & ((1u << s) - 1u));
+    const uint32_t imask = bc_node_imask(nd);
+    if ((imask >> s) & 1u) {
+        const uint32_t child = bc_node_child_base(nd) + (uint32_t)__popc(imask & ((1u << s) - 1u));
         const float4 lo = nbox[2 * (size_t)child], hi = nbox[2 * (size_t)child + 1];
         b.mn[0] = lo.x; b.mn[1] = lo.y; b.mn[2] = lo.z;
         b.mx[0] = hi.x; b.mx[1] = hi.y; b.mx[2] = hi.z;
         return b;
     }
-    const uint32_t meta = nd.meta[s], off = meta & 31u, cnt = (meta >> 5) + 1u;
-    const uint32_t base = nd.prim_base & ~BC_NODE_HAS_SPHERES;
-    for (uint32_t k = 0; k < cnt; k++) {
-        const uint32_t r = base + off + k;
-        const float* v = prims[r].v;
-        if (ids[r] & BC_SPHERE_BIT) {
-            const double rad = v[3];
-            dbox_grow(b, (double)v[0] - rad, (double)v[1] - rad, (double)v[2] - rad);
-            dbox_grow(b, (double)v[0] + rad, (double)v[1] + rad, (double)v[2] + rad);
-        } else {  // the fp32 triangle A, A+E1, A+E2 (the sums are exact in fp64)
-            const double ax = v[0], ay = v[1], az = v[2];
-            dbox_grow(b, ax, ay, az);
-            dbox_grow(b, ax + (double)v[3], ay + (double)v[4], az + (double)v[5]);
-            dbox_grow(b, ax + (double)v[6], ay + (double)v[7], az + (double)v[8]);
-        }
+    const size_t r = (size_t)idx * BC_REC_PER_NODE + (size_t)s;
+    const float* v = prims[r].v;
+    if (ids[r] & BC_SPHERE_BIT) {
+        const double rad = v[3];
+        dbox_grow(b, (double)v[0] - rad, (double)v[1] - rad, (double)v[2] - rad);
+        dbox_grow(b, (double)v[0] + rad, (double)v[1] + rad, (double)v[2] + rad);
+    } else {  // the fp32 triangle A, A+E1, A+E2 (the sums are exact in fp64)
+        const double ax = v[0], ay = v[1], az = v[2];
+        dbox_grow(b, ax, ay, az);
+        dbox_grow(b, ax + (double)v[3], ay + (double)v[4], az + (double)v[5]);
+        dbox_grow(b, ax + (double)v[6], ay + (double)v[7], az + (double)v[8]);
     }
     return b;
 }
@@ -78,39 +75,27 @@ __global__ __launch_bounds__(256) void k_refit_level(bc_node8* __restrict__ node
     if (i >= n) return;
     const uint32_t idx = list[i];
     bc_node8 nd = nodes[idx];
+    const uint32_t used = bc_node_imask(nd) | bc_node_lmask(nd);
     DBox nb;
     dbox_empty(nb);
     for (int s = 0; s < 8; s++) {
-        if (nd.meta[s] == BC_META_EMPTY && !((nd.imask >> s) & 1u)) continue;
-        const DBox c = slot_box(nd, s, prims, ids, nbox);
+        if (!((used >> s) & 1u)) continue;
+        const DBox c = slot_box(nd, idx, s, prims, ids, nbox);
         dbox_grow(nb, c.mn[0], c.mn[1], c.mn[2]);
         dbox_grow(nb, c.mx[0], c.mx[1], c.mx[2]);
     }
     double pad[3], org[3], scale[3];
-    uint8_t* ex = &nd.ex;
-    float* o3 = &nd.ox;
     for (int a = 0; a < 3; a++) {
-        const double mag = fmax(fabs(nb.mn[a]), fabs(nb.mx[a]));
-        const double ext = nb.mx[a] - nb.mn[a];
-        pad[a] = fmax((mag + ext) * 0x1p-20, 1e-20);
-        const float o = f_down(nb.mn[a] - pad[a]);
-        const double ext2 = (nb.mx[a] + pad[a]) - (double)o;
-        int k2;
-        const double m2 = frexp(ext2 / 255.0, &k2);  // smallest e with 2^e >= ext2 / 255, exactly
-        int e = m2 == 0.5 ? k2 - 1 : k2;
-        while (ldexp(255.0, e) < ext2) e++;
-        if (e < -126) e = -126;
-        if (e > 127) { atomicOr(err, 4); e = 127; }
-        org[a] = o;
-        scale[a] = ldexp(1.0, e);
-        o3[a] = o;
-        ex[a] = (uint8_t)(e + 127);
+        int32_t k;
+        int e;
+        if (!bc_grid_axis(nb.mn[a], nb.mx[a], pad[a], k, e, org[a], scale[a])) atomicOr(err, 4);
+        bc_node_set_grid(nd, a, k, e);
     }
     uint8_t* qlo[3] = {nd.qlox, nd.qloy, nd.qloz};
     uint8_t* qhi[3] = {nd.qhix, nd.qhiy, nd.qhiz};
     for (int s = 0; s < 8; s++) {
-        if (nd.meta[s] == BC_META_EMPTY && !((nd.imask >> s) & 1u)) continue;
-        const DBox c = slot_box(nd, s, prims, ids, nbox);
+        if (!((used >> s) & 1u)) continue;
+        const DBox c = slot_box(nd, idx, s, prims, ids, nbox);
         for (int a = 0; a < 3; a++) {
             const double lo = floor(((c.mn[a] - pad[a]) - org[a]) / scale[a]);
             const double hi = ceil(((c.mx[a] + pad[a]) - org[a]) / scale[a]);
@@ -123,14 +108,15 @@ __global__ __launch_bounds__(256) void k_refit_level(bc_node8* __restrict__ node
     nbox[2 * (size_t)idx + 1] = make_float4(f_up(nb.mx[0]), f_up(nb.mx[1]), f_up(nb.mx[2]), 0.0f);
 }
 
-// Records of the moved primitives in leaf order (prim_record of the host flattening) and their
-// materials; a primitive whose kind differs from the resident record's sets error bit 16.
+// Records of the moved primitives at their record slots (prim_record of the host flattening) and
+// their materials, over all n record slots (holes skipped); a primitive whose kind differs from
+// the resident record's sets error bit 16.
 __global__ __launch_bounds__(256) void k_refit_records(const double* __restrict__ geom, const int32_t* __restrict__ kind,
                                                        const int32_t* __restrict__ mat, const uint32_t* __restrict__ ids,
                                                        uint32_t n, bc_prim* __restrict__ prims, int32_t* __restrict__ pmat,
                                                        int* __restrict__ err) {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    if (r >= n) return;
+    if (r >= n || ids[r] == BC_REC_HOLE) return;
     const uint32_t id = ids[r] & ~BC_SPHERE_BIT;
     const bool sphere = (ids[r] & BC_SPHERE_BIT) != 0;
     if ((kind[id] == 0) != sphere) { atomicOr(err, 16); return; }

@@ -3,10 +3,10 @@
 // Replaces closest()/boxhit()/phit() of the reference OpenCL kernel
 // (gpu_opencl_windows.go:736-777) and Scene.Hit / Scene.Occluded (renderer.go:777-895).
 //
-// One ray per lane.  A node visit loads one 80-byte bc_node8 (five 16-byte loads), tests
+// One ray per lane.  A node visit loads one 64-byte bc_node8 (four 16-byte loads), tests
 // its eight quantised child boxes with FMA slab tests and produces
 //   * a node group  : (child_base, imask, hit bits in near-to-far slot order)
-//   * a prim group  : (prim_base, 32-bit mask of primitives in hit leaves)
+//   * a prim group  : (record base 8 * node, mask of hit leaf slots, one primitive each)
 // Groups that still hold children are pushed to a per-lane stack in LDS laid out
 // [depth][lane] so that a wave's pushes and pops are conflict-free 8-byte accesses.
 // A group consumes one stack entry per tree level above the deepest, so a BVH8 of depth D
@@ -108,28 +108,32 @@ struct NodeAxis {
     float adj, off;
 };
 
-// Tests the 8 children of node `nd`.  Outputs: ordered internal-hit bits (bit k <=> slot k^oct),
-// the node's imask and child_base, and the primitive group.
+// Tests the 8 children of node `idx`.  Outputs: ordered internal-hit bits (bit k <=> slot k^oct),
+// the node's imask and child_base, and the primitive group: its record base (8 * idx, bit 31 set
+// when the node holds spheres) and the mask of hit leaf slots (record = base + slot).
 __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, uint32_t idx, const Ray& r,
                                            float ix, float iy, float iz, uint32_t oct, float tmax,
                                            uint32_t& ng_base, uint32_t& ng_bits, uint32_t& pg_base,
                                            uint32_t& pg_mask) {
     const uint4* p = reinterpret_cast<const uint4*>(nodes + idx);
-    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3], w4 = p[4];
-    const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
-    const uint32_t ex = w0.w & 0xffu, ey = (w0.w >> 8) & 0xffu, ez = (w0.w >> 16) & 0xffu;
-    const uint32_t imask = w0.w >> 24;
-    const float sxf = __uint_as_float(ex << 23), syf = __uint_as_float(ey << 23), szf = __uint_as_float(ez << 23);
+    const uint4 w0 = p[0], w1 = p[1], w2 = p[2], w3 = p[3];
+    const uint32_t imask = w0.x >> 24;
+    // per axis: grid step 2^e (the exponent field shifted into place) and origin k * 2^e, exact
+    const float sxf = __uint_as_float(((w0.y >> 21) & 0xffu) << 23);
+    const float syf = __uint_as_float(((w0.z >> 21) & 0xffu) << 23);
+    const float szf = __uint_as_float(((w0.w >> 21) & 0xffu) << 23);
+    const float ox = (float)(((int32_t)(w0.y << 11)) >> 11) * sxf;
+    const float oy = (float)(((int32_t)(w0.z << 11)) >> 11) * syf;
+    const float oz = (float)(((int32_t)(w0.w << 11)) >> 11) * szf;
     const float adjx = sxf * ix, adjy = syf * iy, adjz = szf * iz;
     const float offx = (ox - r.ox) * ix, offy = (oy - r.oy) * iy, offz = (oz - r.oz) * iz;
     // near/far planes per axis by direction sign: words (lo0-3, lo4-7) and (hi0-3, hi4-7)
-    const uint32_t lox0 = w2.x, lox1 = w2.y, loy0 = w2.z, loy1 = w2.w, loz0 = w3.x, loz1 = w3.y;
-    const uint32_t hix0 = w3.z, hix1 = w3.w, hiy0 = w4.x, hiy1 = w4.y, hiz0 = w4.z, hiz1 = w4.w;
+    const uint32_t lox0 = w1.x, lox1 = w1.y, loy0 = w1.z, loy1 = w1.w, loz0 = w2.x, loz1 = w2.y;
+    const uint32_t hix0 = w2.z, hix1 = w2.w, hiy0 = w3.x, hiy1 = w3.y, hiz0 = w3.z, hiz1 = w3.w;
     const bool nx = ix < 0.0f, ny = iy < 0.0f, nz = iz < 0.0f;
     const uint32_t nx0 = nx ? hix0 : lox0, nx1 = nx ? hix1 : lox1, fx0 = nx ? lox0 : hix0, fx1 = nx ? lox1 : hix1;
     const uint32_t ny0 = ny ? hiy0 : loy0, ny1 = ny ? hiy1 : loy1, fy0 = ny ? loy0 : hiy0, fy1 = ny ? loy1 : hiy1;
     const uint32_t nz0 = nz ? hiz0 : loz0, nz1 = nz ? hiz1 : loz1, fz0 = nz ? loz0 : hiz0, fz1 = nz ? loz1 : hiz1;
-    const uint32_t meta0 = w1.z, meta1 = w1.w;
     uint32_t hits = 0;
     // smallest entry distance among the hit children: a lower bound for the internal ones that a
     // pop tests against the current hit (leaf and empty slots can only lower it)
@@ -146,8 +150,8 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
         const float tfz = __builtin_fmaf(u8f(hiw ? fz1 : fz0, sh), adjz, offz);
         const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tmin));
         const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        // an empty slot's box is inverted (255 / 0); rounding can still let it pass here, so
-        // emptiness is checked where it matters, for leaf hits below (internal slots never are)
+        // an empty slot's box is inverted (255 / 0); rounding can still let it pass here, which
+        // is harmless: it is in neither imask nor the leaf mask
         const bool h = tn <= tf;
         hits |= (h ? 1u : 0u) << s;
         // the minimum of canonical operands needs no NaN quieting step
@@ -159,24 +163,16 @@ __device__ __forceinline__ void visit_node(const bc_node8* __restrict__ nodes, u
     ord = (oct & 1u) ? (((ord & 0x55u) << 1) | ((ord >> 1) & 0x55u)) : ord;
     ord = (oct & 2u) ? (((ord & 0x33u) << 2) | ((ord >> 2) & 0x33u)) : ord;
     ord = (oct & 4u) ? (((ord & 0x0fu) << 4) | ((ord >> 4) & 0x0fu)) : ord;
-    ng_base = w1.x;
+    ng_base = w0.x & 0xffffffu;
     // bits 16-31: upper half of the smallest entry distance.  Truncating a non-negative
     // float rounds towards zero, so it is a lower bound usable to cull the group after a
     // pop; a negative distance (only with tmin < 0) is stored as -inf, which never culls.
     const uint32_t tnb = tnmin >= 0.0f ? (__float_as_uint(tnmin) & 0xffff0000u) : 0xff800000u;
     ng_bits = ord | (imask << 8) | tnb;
-    // leaf children: union of their primitive runs
-    uint32_t lh = hits & ~imask & 0xffu, pm = 0;
-    while (lh) {
-        const uint32_t s = (uint32_t)__builtin_ctz(lh);
-        lh &= lh - 1u;
-        const uint32_t meta = ((s >= 4 ? meta1 : meta0) >> ((s & 3u) * 8u)) & 0xffu;
-        if (meta == BC_META_EMPTY) continue;
-        const uint32_t cnt = (meta >> 5) + 1u, offs = meta & 31u;
-        pm |= (uint32_t)(((1ull << cnt) - 1ull) << offs);
-    }
-    pg_base = w1.y;
-    pg_mask = pm;
+    // leaf slots (one primitive each, record 8 * idx + slot): bits 29-31 of the three grid words
+    const uint32_t lmask = (w0.y >> 29) | ((w0.z >> 29) << 3) | (((w0.w >> 29) & 3u) << 6);
+    pg_base = (idx * BC_REC_PER_NODE) | (w0.w & BC_NODE_HAS_SPHERES);
+    pg_mask = hits & lmask;
 }
 
 // Primitive records fetched together in a leaf (see the leaf loop of trav_step).

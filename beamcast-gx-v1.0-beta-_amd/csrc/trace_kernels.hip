@@ -164,7 +164,7 @@ hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc
 
 // Counting instantiation of k_trace: the same traversal of each ray (same results), plus the
 // work it did: counts[i] = (node visits, primitive-record tests, id-table reads, traversal steps).
-// The algorithmic bytes of a launch follow from the first three (80 B per node, 36 B per record,
+// The algorithmic bytes of a launch follow from the first three (64 B per node, 36 B per record,
 // 4 B per id); steps minus node visits are the steps in which the ray only popped its stack.
 template <bool ANY>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_count(const bc_node8* __restrict__ nodes,

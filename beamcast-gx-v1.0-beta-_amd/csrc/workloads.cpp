@@ -152,17 +152,20 @@ int bch_flatten_check(bch_scene* s, int64_t* out4) {
     std::vector<int> seen((size_t)v.nprim, 0);
     int64_t slots = 0;
     bool ok = true;
-    for (const bc_node8& n : b.nodes)
+    for (size_t i = 0; i < b.nodes.size(); i++) {
+        const uint32_t lmask = bc_node_lmask(b.nodes[i]);
+        if (lmask & bc_node_imask(b.nodes[i])) ok = false;  // a slot is either internal or a leaf
         for (int k = 0; k < 8; k++) {
-            if (((n.imask >> k) & 1u) || n.meta[k] == BC_META_EMPTY) continue;
-            const uint32_t cnt = (n.meta[k] >> 5) + 1u, off = n.meta[k] & 31u;
-            if (off + cnt > 32u) { ok = false; continue; }
-            for (uint32_t j = 0; j < cnt; j++) {
-                const uint32_t id = b.ids[(n.prim_base & ~BC_NODE_HAS_SPHERES) + off + j] & ~BC_SPHERE_BIT;
-                if (id < (uint32_t)v.nprim) seen[id]++; else ok = false;
+            const uint32_t id = b.ids[BC_REC_PER_NODE * i + (size_t)k];
+            if (!((lmask >> k) & 1u)) {
+                if (id != BC_REC_HOLE) ok = false;
+                continue;
             }
+            const uint32_t q = id & ~BC_SPHERE_BIT;
+            if (id != BC_REC_HOLE && q < (uint32_t)v.nprim) seen[q]++; else ok = false;
             slots++;
         }
+    }
     for (int x : seen) ok = ok && x == 1;
     out4[0] = (int64_t)b.nodes.size();
     out4[1] = b.depth;

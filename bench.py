@@ -10,7 +10,7 @@ step = the four launches closest-hit and any-hit over both sets = 67,108,864 ray
 inputs resident in HBM.  value = ray queries per second of the whole step, in Mrays/s.
 
 Also reported: the roofline of the dominant kernel -- algorithmic bytes of the BVH8 traversal
-that runs (80 B per node visit, 36 B per primitive-record test, 4 B per id-table read, the ray and
+that runs (64 B per node visit, 36 B per primitive-record test, 4 B per id-table read, the ray and
 its result), counted per ray by a counting instantiation of the same traversal over the whole
 launch, over the kernel's HIP-event duration, against the 8 TB/s HBM peak -- with the VALU issue
 rate of the same kernel against a calibrated peak as the second roofline; and the CPU baseline:
@@ -666,7 +666,7 @@ def main():
 
     # algorithmic bytes of the BVH8 traversal that runs, counted for every ray of each launch by
     # the counting instantiation of the same traversal (whose results must equal the timed
-    # launch's): 32 B ray + 8 B (closest) or 4 B (any hit) result + 80 B per node visit + 36 B per
+    # launch's): 32 B ray + 8 B (closest) or 4 B (any hit) result + 64 B per node visit + 36 B per
     # primitive-record test + 4 B per id-table read
     d_cnt, d_chk = dev.alloc(n * 16), dev.alloc(n * 8)
     alg, work = {}, {}
@@ -682,7 +682,7 @@ def main():
         b = dev.download(d_out[k], np.empty(n * out_b, np.uint8))
         if not np.array_equal(a, b):
             raise RuntimeError("the counting traversal's results differ from the timed launch's")
-        alg[k] = 32 + out_b + 80 * c[0] + 36 * c[1] + 4 * c[2]
+        alg[k] = 32 + out_b + 64 * c[0] + 36 * c[1] + 4 * c[2]
         work[k] = {"node_visits_per_ray": round(float(c[0]), 3), "record_tests_per_ray": round(float(c[1]), 3),
                    "id_reads_per_ray": round(float(c[2]), 3), "steps_per_ray": round(float(c[3]), 3),
                    "steps_per_wave": round(wave_steps, 2)}

@@ -59,7 +59,8 @@ typedef struct bc_scene_view {
  * (gpu_opencl_windows.go:379-495) including validateGPUSceneEncoding (:470-482).
  * Every call flattens and uploads the given scene; the caller decides when to call, keyed by
  * (scene, topo_rev, geom_rev) as ensureSceneLocked does.  The revisions are recorded with the
- * scene (bc_backward checks that the scene did not change since bc_forward). */
+ * scene (bc_backward checks that the scene did not change since bc_forward).  A scene may take
+ * at most 2^24 BVH8 nodes (about 90 million primitives); a larger one is BC_ERR_SCENE. */
 int bc_scene_upload(bc_ctx* ctx, const bc_scene_view* scene, uint64_t topo_rev, uint64_t geom_rev);
 /* Replaces the material part of uploadFullSceneLocked (gpu_opencl_windows.go:384-396) and the
  * light distribution of rebuildLightDistribution (renderer.go:363-407) without touching the
@@ -111,9 +112,11 @@ int bc_set_flatten(bc_ctx* ctx, int mode);
 /* Statistics of the uploaded scene: nodes8, depth8, prims, bytes resident. */
 int bc_scene_stats(const bc_ctx* ctx, int64_t* out4);
 /* Copies the resident BVH8 to host memory for inspection (no counterpart in the reference, which
- * never reads its packed buffers back): nodes (nodes8 x 80 bytes, the bc_node8 layout of
- * csrc/bc_layout.h), records (prims x 36 bytes, nine floats) and record ids (prims x uint32,
- * primitive index | 0x80000000 for spheres), counts from bc_scene_stats; NULL skips an array. */
+ * never reads its packed buffers back): nodes (nodes8 x 64 bytes, the bc_node8 layout of
+ * csrc/bc_layout.h), records (8 x nodes8 record slots of 36 bytes, nine floats: the record of
+ * leaf slot s of node i at 8 i + s, zero elsewhere) and record ids (8 x nodes8 uint32, primitive
+ * index | 0x80000000 for spheres, 0xFFFFFFFF in slots without a primitive), nodes8 from
+ * bc_scene_stats; NULL skips an array. */
 int bc_scene_arrays(bc_ctx* ctx, void* nodes, void* records, uint32_t* ids);
 
 /* Render description: the RenderSettings fields the path reads (renderer.go:1027-1062), the
@@ -251,7 +254,7 @@ int bc_trace_device(bc_ctx* ctx, const void* d_rays, int64_t n, int any_hit, voi
 
 /* Diagnostics with no counterpart in the reference (the roofline accounting of the benchmark).
  * bc_trace_counts: bc_trace_device's query (same results in d_out), one ray per thread, that also
- * writes per ray d_counts[4i .. 4i+3] = node visits (80-byte bc_node8 each), primitive-record tests
+ * writes per ray d_counts[4i .. 4i+3] = node visits (64-byte bc_node8 each), primitive-record tests
  * (36-byte records), id-table reads (4 bytes), traversal steps (a step visits at most one node;
  * the others only pop the stack).  Asynchronous on the context stream.
  * bc_valu_probe: times (HIP events, *ms) `blocks` workgroups of 256 threads that each run `iters`
