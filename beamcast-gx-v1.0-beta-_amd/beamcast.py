@@ -519,7 +519,8 @@ class Device:
         self._check(lib().bc_scene_refit(self.ctx, scene.view(nee_triangles), geom))
 
     def set_trace_refill(self, closest, any_hit):
-        """Ray-query schedule: refill a wave's finished lanes once that many are idle (0: off)."""
+        """Ray-query schedule: refill a wave's finished lanes once that many are idle (0: off; closest
+        -1: chosen per batch by the library's coherence probe, the default)."""
         self._check(lib().bc_set_trace_refill(self.ctx, closest, any_hit))
 
     def set_trace_order(self, mode, cell_bits=4):

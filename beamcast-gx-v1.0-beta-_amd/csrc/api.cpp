@@ -22,6 +22,7 @@
 constexpr int kDeviceFlattenMin = 1 << 16;
 // Ray batches of at least this many rays may be binned before a query (bc_set_trace_order).
 constexpr int64_t kOrderMinRays = 1 << 16;
+constexpr int kAutoRefillLanes = 8;  // automatic closest-hit refill threshold (measured best: 8-12)
 
 namespace {
 thread_local std::string g_create_error;
@@ -172,7 +173,8 @@ void bc_destroy(bc_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox, c->d_stage, c->d_order, c->d_sorted};
+                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox, c->d_stage, c->d_order, c->d_sorted,
+                    c->d_sel};
     for (void* q : ptrs)
         if (q) hipFree(q);
     bc_render_release(c);
@@ -254,7 +256,7 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
 }
 
 int bc_set_trace_refill(bc_ctx* c, int closest_lanes, int any_lanes) {
-    if (!c || closest_lanes < 0 || closest_lanes > 64 || any_lanes < 0 || any_lanes > 64) return BC_ERR_ARG;
+    if (!c || closest_lanes < -1 || closest_lanes > 64 || any_lanes < 0 || any_lanes > 64) return BC_ERR_ARG;
     c->trace_refill[0] = closest_lanes;
     c->trace_refill[1] = any_lanes;
     return BC_OK;
@@ -545,8 +547,24 @@ int bc_trace_device(bc_ctx* c, const void* d_rays, int64_t n, int any_hit, void*
         oflag = s.flag;
         c->last_order_flag = s.flag;
     }
-    hipError_t e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, c->d_ids, rays, n, d_out,
-                                     c->trace_refill[any_hit != 0 ? 1 : 0], c->d_err, c->stream, sorted, perm, oflag);
+    int refill = c->trace_refill[any_hit != 0 ? 1 : 0];
+    const int* sel = nullptr;
+    if (refill < 0) {
+        // automatic (closest hit): lane refill for incoherent batches, one ray per thread for
+        // coherent ones (where refilled lanes lose the wave's shared nodes), chosen on the device
+        refill = kAutoRefillLanes;
+        if (oflag == nullptr && n >= kOrderMinRays) {
+            if (!bc_ensure(c, c->d_sel, c->cap_sel, 2 * sizeof(int), "hipMalloc(coherence flag)")) return BC_ERR_NOMEM;
+            if (!ok(c, bcd::launch_coherence_probe(c->order_bits, rays, n, c->d_nodes, (int*)c->d_sel, c->stream),
+                    "coherence probe"))
+                return BC_ERR_DEVICE;
+            sel = (const int*)c->d_sel;
+        } else {
+            refill = 0;
+        }
+    }
+    hipError_t e = bcd::launch_trace(c->stack, any_hit != 0, c->d_nodes, c->d_prims, c->d_ids, rays, n, d_out, refill,
+                                     c->d_err, c->stream, sorted, perm, oflag, sel);
     return ok(c, e, "trace launch") ? BC_OK : BC_ERR_DEVICE;
 }
 

@@ -57,7 +57,9 @@ struct bc_ctx {
     int flatten = 0;       // BVH8 flattening: 0 auto (device for large scenes), 1 host, 2 device
     // ray queries (closest, any hit): refill a wave's lanes once this many are idle (0: one ray
     // per thread).  Any-hit rays end at very different depths, closest-hit rays much less so.
-    int trace_refill[2] = {0, 8};
+    int trace_refill[2] = {-1, 8};  // closest (-1: automatic), any hit
+    void* d_sel = nullptr;          // coherence probe flag of automatic closest-hit refill
+    size_t cap_sel = 0;
     // ray binning before queries (order.hip): mode 0 off, 1 always, 2 when a sample of the
     // batch is incoherent; bits = log2 of the origin cells per axis.  Off by default: on the 1M
     // soup it lifts the incoherent L2 hit rate from 0.47 to 0.95 but saves only about what its

@@ -36,6 +36,11 @@ size_t order_bins(int bits);
 // G workgroups count and scatter one contiguous chunk each.
 hipError_t launch_order(int bits, int mode, const bc_ray* rays, int64_t n, const bc_node8* nodes, int G,
                         const OrderScratch& s, hipStream_t st);
+// One block: flag[0] = 1 when at least half of 4,096 sampled adjacent ray pairs fall into different
+// (origin cell on a 2^bits grid over the scene box, direction octant) bins, else 0; flag[1] = the
+// pairs that differ.  n >= 2.
+hipError_t launch_coherence_probe(int bits, const bc_ray* rays, int64_t n, const bc_node8* nodes, int* flag,
+                                  hipStream_t st);
 
 // Ray queries: any_hit writes int32 occlusion flags, else bc_hit records.  stack: LDS stack
 // entries per lane (stack_for_depth).  *err is set non-zero by the kernel if a traversal stack
@@ -43,7 +48,11 @@ hipError_t launch_order(int bits, int mode, const bc_ray* rays, int64_t n, const
 // the rays from `sorted` and write the result of sorted ray j to out[perm[j]].
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids,
                         const bc_ray* rays, int64_t n, void* out, int refill, int* err, hipStream_t st,
-                        const bc_ray* sorted = nullptr, const uint32_t* perm = nullptr, const int* oflag = nullptr);
+                        const bc_ray* sorted = nullptr, const uint32_t* perm = nullptr, const int* oflag = nullptr,
+                        const int* sel = nullptr);
+// sel (optional): a device flag written by launch_coherence_probe on the same stream; launch_trace
+// then launches both the one-ray-per-thread kernel and the lane-refill kernel (threshold refill),
+// and each returns at once unless the flag chose it (nonzero: refill).
 
 // The traversal of launch_trace, one ray per thread, that also writes each ray's node visits,
 // primitive-record tests and id-table reads to counts[i] (x, y, z; w = 0).

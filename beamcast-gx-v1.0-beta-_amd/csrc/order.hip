@@ -225,6 +225,13 @@ hipError_t launch_order_t(const bc_ray* rays, int64_t n, const bc_node8* nodes, 
 
 }  // namespace
 
+hipError_t launch_coherence_probe(int bits, const bc_ray* rays, int64_t n, const bc_node8* nodes, int* flag,
+                                  hipStream_t st) {
+    if (bits < kOrderMinBits || bits > kOrderMaxBits || n < 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_order_probe, dim3(1), dim3(kProbeBlock), 0, st, rays, n, nodes, bits, 2, flag);
+    return hipGetLastError();
+}
+
 size_t order_bins(int bits) { return (size_t)1 << (3 * bits + 3); }
 
 hipError_t launch_order(int bits, int mode, const bc_ray* rays, int64_t n, const bc_node8* nodes, int G,

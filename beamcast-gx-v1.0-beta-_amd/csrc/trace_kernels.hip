@@ -29,8 +29,10 @@ __global__ __launch_bounds__(kTraceBlock) BC_TRACE_WAVES(BC_CLOSEST_WAVES) void 
                                                                            int* __restrict__ err,
                                                                            const bc_ray* __restrict__ sorted,
                                                                            const uint32_t* __restrict__ perm,
-                                                                           const int* __restrict__ oflag) {
+                                                                           const int* __restrict__ oflag,
+                                                                           const int* __restrict__ sel) {
     extern __shared__ uint2 stk[];
+    if (sel != nullptr && sel[0] != 0) return;  // the probe found the batch incoherent: k_trace_refill runs it
     const int64_t i = (int64_t)xcd_chunk(blockIdx.x, gridDim.x, kTraceXcdRun) * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const bool ord = oflag != nullptr && oflag[0] != 0;  // uniform: a scalar load
@@ -86,8 +88,10 @@ __global__ __launch_bounds__(kRefillBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_W
                                                                                   int* __restrict__ err,
                                                                                   const bc_ray* __restrict__ sorted,
                                                                                   const uint32_t* __restrict__ perm,
-                                                                                  const int* __restrict__ oflag) {
+                                                                                  const int* __restrict__ oflag,
+                                                                                  const int* __restrict__ sel) {
     extern __shared__ uint2 stk[];
+    if (sel != nullptr && sel[0] == 0) return;  // the probe found the batch coherent: k_trace runs it
     const bool ord = oflag != nullptr && oflag[0] != 0;  // uniform: a scalar load
     const int lane = (int)(threadIdx.x & 63u);
     // the run's bounds are wave-uniform (scalar registers); lanes hold 32-bit offsets into it
@@ -133,33 +137,42 @@ __global__ __launch_bounds__(kRefillBlock) BC_TRACE_WAVES(ANY ? 8 : BC_CLOSEST_W
 }
 
 template <bool ANY>
+static hipError_t launch_refill_t(int cap, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
+                                  int64_t n, void* out, int refill, int* err, hipStream_t st, const bc_ray* sorted,
+                                  const uint32_t* perm, const int* oflag, const int* sel) {
+    const size_t lds = (size_t)cap * kRefillBlock * sizeof(uint2);
+    const int64_t per_block = (int64_t)kRefillRays * (kRefillBlock / 64);
+    const int64_t blocks = (n + per_block - 1) / per_block;
+    hipLaunchKernelGGL((k_trace_refill<ANY>), dim3((unsigned)blocks), dim3(kRefillBlock), lds, st, nodes, prims, ids, rays,
+                       n, out, refill, cap, err, sorted, perm, oflag, sel);
+    return hipGetLastError();
+}
+
+template <bool ANY>
 static hipError_t launch_trace_t(int cap, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
                                  int64_t n, void* out, int refill, int* err, hipStream_t st, const bc_ray* sorted,
-                                 const uint32_t* perm, const int* oflag) {
-    if (refill > 0) {
-        const size_t lds = (size_t)cap * kRefillBlock * sizeof(uint2);
-        const int64_t per_block = (int64_t)kRefillRays * (kRefillBlock / 64);
-        const int64_t blocks = (n + per_block - 1) / per_block;
-        hipLaunchKernelGGL((k_trace_refill<ANY>), dim3((unsigned)blocks), dim3(kRefillBlock), lds, st, nodes, prims, ids, rays,
-                           n, out, refill, cap, err, sorted, perm, oflag);
-        return hipGetLastError();
-    }
+                                 const uint32_t* perm, const int* oflag, const int* sel) {
+    if (refill > 0 && sel == nullptr)
+        return launch_refill_t<ANY>(cap, nodes, prims, ids, rays, n, out, refill, err, st, sorted, perm, oflag, nullptr);
     const size_t lds = (size_t)cap * kTraceBlock * sizeof(uint2);
     const int64_t blocks = (n + kTraceBlock - 1) / kTraceBlock;
     hipLaunchKernelGGL((k_trace<ANY>), dim3((unsigned)blocks), dim3(kTraceBlock), lds, st, nodes, prims, ids, rays, n, out,
-                       cap, err, sorted, perm, oflag);
-    return hipGetLastError();
+                       cap, err, sorted, perm, oflag, sel);
+    if (sel == nullptr) return hipGetLastError();
+    // the probe's choice is read by both kernels; the one not chosen returns at once
+    return launch_refill_t<ANY>(cap, nodes, prims, ids, rays, n, out, refill, err, st, sorted, perm, oflag, sel);
 }
 
 hipError_t launch_trace(int stack, bool any_hit, const bc_node8* nodes, const bc_prim* prims, const uint32_t* ids, const bc_ray* rays,
                         int64_t n, void* out, int refill, int* err, hipStream_t st, const bc_ray* sorted,
-                        const uint32_t* perm, const int* oflag) {
+                        const uint32_t* perm, const int* oflag, const int* sel) {
     if (n <= 0) return hipSuccess;
     if (oflag && (!sorted || !perm)) return hipErrorInvalidValue;
     if (stack < 1 || stack > kMaxStack) return hipErrorInvalidValue;
+    if (sel && refill <= 0) return hipErrorInvalidValue;
     if (refill > 64) refill = 64;
-    return any_hit ? launch_trace_t<true>(stack, nodes, prims, ids, rays, n, out, refill, err, st, sorted, perm, oflag)
-                   : launch_trace_t<false>(stack, nodes, prims, ids, rays, n, out, refill, err, st, sorted, perm, oflag);
+    return any_hit ? launch_trace_t<true>(stack, nodes, prims, ids, rays, n, out, refill, err, st, sorted, perm, oflag, sel)
+                   : launch_trace_t<false>(stack, nodes, prims, ids, rays, n, out, refill, err, st, sorted, perm, oflag, sel);
 }
 
 // Counting instantiation of k_trace: the same traversal of each ray (same results), plus the

@@ -566,8 +566,9 @@ def main():
     ap.add_argument("--any-refill", type=int, default=8,
                     help="any-hit queries and integrator shadow rays: refill a wave's lanes once this many are idle "
                          "(0: one ray per thread)")
-    ap.add_argument("--closest-refill", type=int, default=0,
-                    help="closest-hit queries: refill a wave's lanes once this many are idle (0: one ray per thread)")
+    ap.add_argument("--closest-refill", type=int, default=-1,
+                    help="closest-hit queries: refill a wave's lanes once this many are idle (0: one ray per thread, "
+                         "-1: chosen per batch by the library's coherence probe)")
     ap.add_argument("--trace-order", type=int, default=0, choices=(0, 1, 2),
                     help="ray binning before queries: 0 off, 1 always, 2 when the batch is incoherent")
     ap.add_argument("--order-bits", type=int, default=4, choices=(2, 3, 4), help="log2 of origin cells per axis")

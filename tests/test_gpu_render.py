@@ -239,7 +239,7 @@ def test_shadow_refill_equals_one_ray_per_thread(dev, mode):
         dev.set_trace_refill(0, lanes)
         img, _, rays = dev.render(bc.render_desc(s, 72, 40, 5, 6, **kw))
         imgs.append((img, rays))
-    dev.set_trace_refill(0, 8)
+    dev.set_trace_refill(-1, 8)
     for img, rays in imgs[1:]:
         assert rays == imgs[0][1] and np.array_equal(img, imgs[0][0])
 

@@ -171,12 +171,27 @@ def test_refill_schedules(dev, n):
     o = orc.Scene.from_host(s)
     prim, t = o.hit32(rays)
     occ = o.any32(ra)
-    for closest, anyh in ((0, 0), (1, 1), (8, 8), (33, 64), (64, 4)):
+    for closest, anyh in ((0, 0), (1, 1), (8, 8), (33, 64), (64, 4), (-1, 8)):
         dev.set_trace_refill(closest, anyh)
         got = dev.trace(rays)
         assert np.array_equal(got["prim"], prim) and np.array_equal(got["t"], t), (closest, anyh)
         assert np.array_equal(dev.trace(ra, any_hit=True), occ), (closest, anyh)
-    dev.set_trace_refill(0, 8)
+    dev.set_trace_refill(-1, 8)
+
+
+def test_automatic_refill_coherent_and_incoherent(dev):
+    """The automatic closest-hit schedule (the default) on batches above the probe's 65,536 rays,
+    one coherent (consecutive pinhole primaries) and one incoherent: the oracle's results either way."""
+    print("case: automatic closest-hit refill, 70,001 coherent and incoherent rays", flush=True)
+    s = bc.HostScene.workload(bc.SOUP, param=100_000)
+    dev.upload(s)
+    dev.set_trace_refill(-1, 8)
+    o = orc.Scene.from_host(s)
+    for ray_set in (0, 1):
+        rays = bc.workload_rays(ray_set, 512, count=512 * 512)[:70_001]
+        prim, t = o.hit32(rays)
+        got = dev.trace(rays)
+        assert np.array_equal(got["prim"], prim) and np.array_equal(got["t"], t), ray_set
 
 
 @pytest.mark.parametrize("bits", [2, 3, 4])
@@ -203,7 +218,7 @@ def test_ray_order(dev, bits):
     base = None
     for mode in (0, 1, 2):
         dev.set_trace_order(mode, bits)
-        for closest, anyh in ((0, 8), (0, 0), (8, 8)):
+        for closest, anyh in ((0, 8), (0, 0), (8, 8), (-1, 8)):
             dev.set_trace_refill(closest, anyh)
             got = dev.trace(rays)
             binned = dev.trace_order_last()
@@ -216,7 +231,7 @@ def test_ray_order(dev, bits):
             assert np.array_equal(got["prim"], base[0]["prim"]) and np.array_equal(got["t"].view(np.int32),
                                                                                     base[0]["t"].view(np.int32))
             assert np.array_equal(g_any, base[1])
-    dev.set_trace_refill(0, 8)
+    dev.set_trace_refill(-1, 8)
     dev.set_trace_order(0, 4)
 
 
