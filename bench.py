@@ -464,7 +464,7 @@ def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_
     counter run gives for each (GRBM_GUI_ACTIVE per XCD over VALU instructions per SIMD).  limiter:
     the same launch's texture-addresser and texture-data busy fractions and VALU instructions per
     SIMD clock from the summary's memory-pipe counter run, next to the probes' VALU rates."""
-    out = {"bound": "hbm", "kernel": f"k_trace ({launch})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    out = {"bound": "hbm", "kernel": f"BVH8 traversal ({launch})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
            "alg_bytes_per_ray": round(alg_bytes, 1), "ref_layout_alg_bytes_per_ray": round(alg_ref_bytes, 1),
            "valu": None}
