@@ -166,8 +166,9 @@ def test_median_build_invariants():
                                         (bc.TERRAIN, 40), (bc.BOX_RETESS, 3000)])
 @pytest.mark.parametrize("leaf", [1, 4, 8, 16])
 def test_bvh8_flatten_covers_every_primitive(kind, param, leaf):
-    """The BVH8 collapse puts every primitive in exactly one leaf slot for Go leaves of up to 16
-    primitives (BVHLeafSize 16), with a node's leaf runs inside its 32-primitive window."""
+    """The BVH8 collapse puts every primitive in exactly one leaf slot, one primitive per slot, its
+    record at 8 * node + slot and holes elsewhere, for Go leaves of up to 16 primitives
+    (BVHLeafSize 16: leaves of more than eight are split in two before the collapse)."""
     s = bc.HostScene.workload(kind, param=param, bins=16, leaf=leaf)
     nodes, depth, slots, ok = s.flatten_check()
-    assert ok and nodes >= 1 and depth >= 1 and slots <= s.counts()[0]
+    assert ok and nodes >= 1 and depth >= 1 and slots == s.counts()[0]
