@@ -167,6 +167,54 @@ int bch_flatten_check(bch_scene* s, int64_t* out4) {
         }
     }
     for (int x : seen) ok = ok && x == 1;
+    // conservative boxes: every record's exact box (fp64 of the fp32 operands) lies inside the
+    // quantised slot box of its leaf and of every internal slot on its path from the root, and
+    // every grid origin is k 2^e with |k| < 2^20
+    const size_t nn = b.nodes.size();
+    std::vector<uint32_t> parent(nn, 0xFFFFFFFFu), pslot(nn, 0);
+    for (size_t i = 0; i < nn && ok; i++) {
+        const bc_node8& n = b.nodes[i];
+        const uint32_t im = bc_node_imask(n);
+        for (int a = 0; a < 3; a++) ok = ok && std::abs(bc_node_k(n, a)) < (1 << 20);
+        for (int k = 0, r = 0; k < 8; k++) {
+            if (!((im >> k) & 1u)) continue;
+            const size_t c = (size_t)bc_node_child_base(n) + (size_t)r++;
+            if (c >= nn || parent[c] != 0xFFFFFFFFu) { ok = false; break; }
+            parent[c] = (uint32_t)i;
+            pslot[c] = (uint32_t)k;
+        }
+    }
+    auto inside = [&](size_t node, int slot, const double* lo, const double* hi) {
+        const bc_node8& n = b.nodes[node];
+        const uint8_t* ql[3] = {n.qlox, n.qloy, n.qloz};
+        const uint8_t* qh[3] = {n.qhix, n.qhiy, n.qhiz};
+        for (int a = 0; a < 3; a++) {
+            const double step = std::ldexp(1.0, bc_node_e(n, a)), o = (double)bc_node_k(n, a) * step;
+            if (o + ql[a][slot] * step > lo[a] || o + qh[a][slot] * step < hi[a]) return false;
+        }
+        return true;
+    };
+    for (size_t i = 0; i < nn && ok; i++) {
+        const uint32_t lm = bc_node_lmask(b.nodes[i]);
+        for (int k = 0; k < 8 && ok; k++) {
+            if (!((lm >> k) & 1u)) continue;
+            const size_t r = BC_REC_PER_NODE * i + (size_t)k;
+            const float* g = b.prims[r].v;
+            double lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                if (b.ids[r] & BC_SPHERE_BIT) {
+                    lo[a] = (double)g[a] - (double)g[3];
+                    hi[a] = (double)g[a] + (double)g[3];
+                } else {
+                    const double A = g[a], B = A + (double)g[3 + a], C = A + (double)g[6 + a];
+                    lo[a] = std::min(A, std::min(B, C));
+                    hi[a] = std::max(A, std::max(B, C));
+                }
+            }
+            ok = inside(i, k, lo, hi);
+            for (size_t c = i; ok && parent[c] != 0xFFFFFFFFu; c = parent[c]) ok = inside(parent[c], (int)pslot[c], lo, hi);
+        }
+    }
     out4[0] = (int64_t)b.nodes.size();
     out4[1] = b.depth;
     out4[2] = slots;

@@ -36,9 +36,11 @@ void bch_build_median(bch_scene* s, int leaf);
  * bch_build; returns the bc_build_bvh status (message via bc_last_error(ctx)) */
 int bch_build_device(bch_scene* s, bc_ctx* ctx, int bins, int leaf);
 /* The host BVH8 flattening of bc_scene_upload, run without a device: out4 = wide nodes, depth,
- * leaf slots, and 1 if every primitive lies in exactly one leaf slot (else 0).  Returns 0, or -1 if
- * the view cannot be encoded.  No counterpart in the reference (its GPU path uploads the binary
- * tree). */
+ * leaf slots, and 1 if every primitive lies in exactly one leaf slot with its record at
+ * 8 * node + slot (holes elsewhere), every node is reached once, every grid origin is k 2^e with
+ * |k| < 2^20, and every record's exact box lies inside the quantised boxes of its leaf slot and of
+ * all internal slots above it (else 0).  Returns 0, or -1 if the view cannot be encoded.  No
+ * counterpart in the reference (its GPU path uploads the binary tree). */
 int bch_flatten_check(bch_scene* s, int64_t* out4);
 /* Primitive.Box (min xyz, max xyz) and Primitive.Centroid of every primitive: nprim x 6, x 3 */
 void bch_export_bounds(const bch_scene* s, double* box6, double* centroid3);
