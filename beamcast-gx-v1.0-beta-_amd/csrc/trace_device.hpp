@@ -245,14 +245,22 @@ __device__ __forceinline__ bool trav_step(TravState& st, const bc_node8* __restr
             cnt += any ? 1 : 0;
             st.pg_mask &= st.pg_mask - 1u;
         }
+        // a lane with fewer records than the batch loads only its own (no repeated fetch of its
+        // first record: every lane-load costs texture-data time)
         float g[kPrimBatch][9];
-#pragma unroll
-        for (int u = 0; u < kPrimBatch; u++)
-#pragma unroll
-            for (int q = 0; q < 9; q++) g[u][q] = prims[rr[u]].v[q];
         bool sp[kPrimBatch];
 #pragma unroll
-        for (int u = 0; u < kPrimBatch; u++) sp[u] = sph_node && (ids[rr[u]] & BC_SPHERE_BIT);
+        for (int u = 0; u < kPrimBatch; u++) {
+            if (u == 0 || u < cnt) {
+#pragma unroll
+                for (int q = 0; q < 9; q++) g[u][q] = prims[rr[u]].v[q];
+                sp[u] = sph_node && (ids[rr[u]] & BC_SPHERE_BIT);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; q++) g[u][q] = 0.0f;
+                sp[u] = false;
+            }
+        }
         if (COUNT) {
             st.n_rec += (uint32_t)cnt;
             st.n_id += sph_node ? (uint32_t)cnt : 0u;
