@@ -285,3 +285,51 @@ def test_sah_and_median_agree(dev):
     prim, t = orc.Scene.from_host(bc.HostScene.workload(bc.STRESS, bins=16, leaf=4), bins=16).hit32(rays)
     assert np.array_equal(med["prim"], prim) and np.array_equal(med["t"], t)
     assert rays_sah == rays_med and np.array_equal(img_sah, img_med)
+
+
+def test_trace_counts(dev):
+    """The counting traversal (bc_trace_counts, the algorithmic bytes of the benchmark's roofline):
+    the same results as the timed query, and per-ray counts that obey what the traversal must do --
+    the root is visited by every ray and no node twice; a ray that hits tests at least its own
+    record and reads the id table for its index; a ray that misses reads no id; steps are at least
+    node visits.  A one-triangle scene pins the counts exactly (one node, one record test
+    on a hit, none on a miss)."""
+    print("case: counting traversal", flush=True)
+    one = bc.HostScene()
+    m = one.add_material(0, (0.5, 0.5, 0.5))
+    one.add_triangle((0.0, 0.0, 0.0), (1.0, 0.0, 0.0), (0.0, 1.0, 0.0), m)
+    one.build(16, 4)
+    soup = bc.HostScene.workload(bc.SOUP, param=50_000)
+    hit_ray = [0.2, 0.2, 1.0, 1e-4, 0.0, 0.0, -1.0, 1e30]
+    miss_ray = [5.0, 5.0, 1.0, 1e-4, 0.0, 0.0, -1.0, 1e30]
+    for scene, rays in ((one, np.array([hit_ray, miss_ray], np.float32)),
+                        (soup, np.concatenate([bc.workload_rays(0, 256, count=256 * 256)[::5],
+                                               bc.workload_rays(1, 256, count=20_000)]))):
+        dev.upload(scene)
+        st = dev.stats()
+        n = len(rays)
+        d_rays = dev.to_device(np.ascontiguousarray(rays, np.float32))
+        for any_hit in (False, True):
+            out_b = 4 if any_hit else 8
+            d_out, d_ref, d_cnt = dev.alloc(n * out_b), dev.alloc(n * out_b), dev.alloc(n * 16)
+            dev.trace_counts(d_rays, n, any_hit, d_out, d_cnt)
+            dev.trace_device(d_rays, n, any_hit, d_ref)
+            got = dev.download(d_out, np.empty(n * out_b, np.uint8))
+            ref = dev.download(d_ref, np.empty(n * out_b, np.uint8))
+            cnt = dev.download(d_cnt, np.empty((n, 4), np.uint32)).astype(np.int64)
+            for p in (d_out, d_ref, d_cnt):
+                dev.free(p)
+            assert np.array_equal(got, ref), (scene is one, any_hit)
+            nodes, recs, ids, steps = cnt.T
+            assert (nodes >= 1).all() and (steps >= nodes).all()
+            if any_hit:
+                hit = got.view(np.int32) != 0
+            else:
+                hit = got.view([("t", np.float32), ("prim", np.int32)])["prim"] >= 0
+            assert (recs[hit] >= 1).all() and (nodes <= st["nodes8"]).all()
+            # the id table is read for a hit's index (closest hit: once at the end, and on exact ties;
+            # any hit: once) and only then (no spheres here)
+            assert (ids[~hit] == 0).all() and (ids[hit] >= 1).all()
+            if scene is one:
+                assert nodes.tolist() == [1, 1] and recs.tolist() == [1, 0] and hit.tolist() == [True, False]
+                assert ids.tolist() == [1, 0]
