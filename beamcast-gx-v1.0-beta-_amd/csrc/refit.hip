@@ -4,11 +4,12 @@
 // primitive on the host and re-uploads them (updateGeometryLocked, gpu_opencl_windows.go:447-468,
 // after Scene.Refit, renderer.go:451-473).  Here only the moved primitive records are uploaded;
 // node boxes are recomputed bottom-up on the device, one launch per tree level (deepest first),
-// one thread per node: the union of the node's child boxes (leaf runs: the boxes of their fp32
-// primitive records; internal children: the union box the child's own thread stored), then the
+// one thread per node: the union of the node's child boxes (leaf slots: the box of their fp32
+// primitive record; internal children: the union box the child's own thread stored), then the
 // node's quantisation grid and child planes with the conservative rule of the host flattening
-// (bc_grid_axis: outward padding, origin on the grid below the box, power-of-two step).  Topology, slot layout and leaf
-// runs are unchanged, so traversal results stay exact; only culling tightness can change.
+// (bc_grid_axis: outward padding, origin on the grid below the box, power-of-two step).
+// Topology, slot layout and record slots are unchanged, so traversal results stay exact; only
+// culling tightness can change.
 #include <hip/hip_runtime.h>
 
 #include <cmath>

@@ -103,11 +103,6 @@ __device__ __forceinline__ float safe_rcp(float d) {
     return __builtin_amdgcn_rcpf(a);
 }
 
-// Per-axis quantised slab terms of one node for this ray.
-struct NodeAxis {
-    float adj, off;
-};
-
 // Tests the 8 children of node `idx`.  Outputs: ordered internal-hit bits (bit k <=> slot k^oct),
 // the node's imask and child_base, and the primitive group: its record base (8 * idx, bit 31 set
 // when the node holds spheres) and the mask of hit leaf slots (record = base + slot).
