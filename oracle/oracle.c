@@ -65,8 +65,9 @@ typedef struct {
 typedef struct { int kind; v3 color; double rough, ior; v3 emission; } mat_t;
 typedef struct { box3 box; int left, right, start, count; } node_t;
 
-/* fp32 records: what the reference packer uploads (gpu_opencl_windows.go:337-377) with
- * outward-rounded node boxes so that culling stays conservative. */
+/* fp32 records: what the reference packer uploads (gpu_opencl_windows.go:337-377), with node
+ * boxes that cover the rounded records and are rounded outward, so that culling stays
+ * conservative for the fp32 primitive tests. */
 typedef struct { float mn[3], mx[3]; int escape; uint32_t meta; } node32_t;
 typedef struct {
     int kind, mat, id;
@@ -341,19 +342,6 @@ orc_scene* orc_scene_new(int nmat, const int32_t* mat_kind, const double* md, in
     if (s->nnode > 0) assign_escape(s, 0, -1);
 
     /* fp32 records (gpu_opencl_windows.go:337-377) */
-    s->n32 = (node32_t*)calloc((size_t)s->nnode + 1, sizeof(node32_t));
-    for (int i = 0; i < s->nnode; i++) {
-        const node_t* n = &s->nodes[i];
-        node32_t o;
-        o.mn[0] = f_down(n->box.mn.x); o.mn[1] = f_down(n->box.mn.y); o.mn[2] = f_down(n->box.mn.z);
-        o.mx[0] = f_up(n->box.mx.x); o.mx[1] = f_up(n->box.mx.y); o.mx[2] = f_up(n->box.mx.z);
-        o.escape = s->escape[i];
-        if (n->count > 0)
-            o.meta = 0x80000000u | ((uint32_t)(n->count - 1) & 0xf) << 27 | ((uint32_t)n->start & 0x07ffffffu);
-        else
-            o.meta = (uint32_t)n->left & 0x07ffffffu;
-        s->n32[i] = o;
-    }
     s->p32 = (prim32_t*)calloc((size_t)s->nprim + 1, sizeof(prim32_t));
     for (int i = 0; i < s->nprim; i++) {
         const prim_t* p = &s->prims[i];
@@ -372,6 +360,54 @@ orc_scene* orc_scene_new(int nmat, const int32_t* mat_kind, const double* md, in
         }
         s->p32[i] = q;
     }
+    /* Node boxes for the fp32 traversal.  The fp32 primitive test sees the rounded record
+     * (a triangle a + u e1 + v e2 of fp32 a, e1, e2; a sphere of fp32 centre and r^2), which
+     * can lie outside the fp64 box by half an ulp of the coordinates -- far from the origin a
+     * large fraction of a small triangle.  Each box therefore covers the fp64 box and the
+     * extent of the rounded records (a + e1, a + e2 are exact in double), is widened by
+     * 2^-21 (|coordinate| + extent) for the rounding inside the box and primitive tests, and
+     * is rounded outward to fp32: culling never drops a ray that the fp32 test would hit, so
+     * the result equals a test of every primitive.  Children come after their parent in
+     * s->nodes, so one reverse pass builds the boxes bottom-up. */
+    s->n32 = (node32_t*)calloc((size_t)s->nnode + 1, sizeof(node32_t));
+    box3* eb = (box3*)malloc(sizeof(box3) * ((size_t)s->nnode + 1));
+    for (int i = s->nnode - 1; i >= 0; i--) {
+        const node_t* n = &s->nodes[i];
+        box3 b = n->box;
+        if (n->count > 0) {
+            for (int k = n->start; k < n->start + n->count; k++) {
+                const prim32_t* q = &s->p32[s->order[k]];
+                if (q->kind == ORC_TRIANGLE) {
+                    v3 a = V(q->a[0], q->a[1], q->a[2]);
+                    b = expand_pt(b, a);
+                    b = expand_pt(b, add(a, V(q->e1[0], q->e1[1], q->e1[2])));
+                    b = expand_pt(b, add(a, V(q->e2[0], q->e2[1], q->e2[2])));
+                } else {
+                    double r = sqrt((double)q->r2);
+                    b = expand_pt(b, V(q->c[0] - r, q->c[1] - r, q->c[2] - r));
+                    b = expand_pt(b, V(q->c[0] + r, q->c[1] + r, q->c[2] + r));
+                }
+            }
+        } else {
+            b = expand_box(b, eb[n->left]);
+            b = expand_box(b, eb[n->right]);
+        }
+        eb[i] = b;
+        double lo[3] = {b.mn.x, b.mn.y, b.mn.z}, hi[3] = {b.mx.x, b.mx.y, b.mx.z};
+        node32_t o;
+        for (int k = 0; k < 3; k++) {
+            const double pad = ldexp(fmax(fabs(lo[k]), fabs(hi[k])) + (hi[k] - lo[k]), -21);
+            o.mn[k] = f_down(lo[k] - pad);
+            o.mx[k] = f_up(hi[k] + pad);
+        }
+        o.escape = s->escape[i];
+        if (n->count > 0)
+            o.meta = 0x80000000u | ((uint32_t)(n->count - 1) & 0xf) << 27 | ((uint32_t)n->start & 0x07ffffffu);
+        else
+            o.meta = (uint32_t)n->left & 0x07ffffffu;
+        s->n32[i] = o;
+    }
+    free(eb);
     s->m32 = (mat32_t*)calloc((size_t)s->nmat + 1, sizeof(mat32_t));
     for (int i = 0; i < s->nmat; i++) {
         const mat_t* m = &s->mats[i];

@@ -333,3 +333,34 @@ def test_trace_counts(dev):
             if scene is one:
                 assert nodes.tolist() == [1, 1] and recs.tolist() == [1, 0] and hit.tolist() == [True, False]
                 assert ids.tolist() == [1, 0]
+
+
+def test_soup_far_from_origin(dev):
+    """Node grid origins are 21-bit integers on the node's own power-of-two grid; far from the
+    scene origin that bound coarsens the grid of small nodes (step at least 2^-20 of the distance).
+    A 20k-triangle soup moved to (1.5e5, -2.5e5, 4e4) still traverses exactly (closest and any hit
+    against the fp32 oracle), for the host and the device flattening."""
+    print("case: soup 20k far from the origin", flush=True)
+    s = bc.HostScene.workload(bc.SOUP, param=20_000)
+    off = np.array([1.5e5, -2.5e5, 4.0e4])
+    s.translate(0, s.counts()[0], tuple(off))
+    s.build(32, 4)
+    rng = np.random.default_rng(11)
+    n = 20_000
+    o = off + rng.random((n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.column_stack([o, np.full(n, 1e-4), d, np.full(n, 1e30)]).astype(np.float32)
+    ra = rays.copy()
+    ra[:, 7] = 0.05
+    orc_s = orc.Scene.from_host(s)
+    prim, t = orc_s.hit32(rays)
+    occ = orc_s.any32(ra)
+    assert (prim >= 0).mean() > 0.2 and 0.02 < occ.mean() < 0.98
+    for mode in (1, 2):
+        dev.set_flatten(mode)
+        dev.upload(s)
+        got = dev.trace(rays)
+        assert np.array_equal(got["prim"], prim) and np.array_equal(got["t"], t), mode
+        assert np.array_equal(dev.trace(ra, any_hit=True), occ), mode
+    dev.set_flatten(0)

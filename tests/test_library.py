@@ -172,3 +172,13 @@ def test_bvh8_flatten_covers_every_primitive(kind, param, leaf):
     s = bc.HostScene.workload(kind, param=param, bins=16, leaf=leaf)
     nodes, depth, slots, ok = s.flatten_check()
     assert ok and nodes >= 1 and depth >= 1 and slots == s.counts()[0]
+
+
+def test_bvh8_flatten_far_from_origin():
+    """The grid rule far from the scene origin (origin integers |k| < 2^20): every record still
+    inside all its enclosing quantised boxes."""
+    s = bc.HostScene.workload(bc.SOUP, param=20_000)
+    s.translate(0, s.counts()[0], (1.5e5, -2.5e5, 4.0e4))
+    s.build(32, 4)
+    nodes, depth, slots, ok = s.flatten_check()
+    assert ok and slots == s.counts()[0]

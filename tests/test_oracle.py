@@ -209,6 +209,31 @@ def test_traversal_matches_brute_force():
     assert diff.mean() <= 1e-4 or np.all(np.abs(t[diff] - u[diff]) / np.maximum(u[diff], 1e-9) < 1e-5)
 
 
+def test_traversal_matches_brute_force_far_from_origin():
+    """1e5 away from the origin a coordinate's fp32 rounding (up to 0.008) is a sizeable part of a
+    small triangle, so the rounded records stick out of boxes built from the fp64 geometry; the
+    fp32 traversal must still find exactly what testing every primitive finds."""
+    s = bc.HostScene.workload(bc.SOUP, param=20_000)
+    off = np.array([1.5e5, -2.5e5, 4.0e4])
+    s.translate(0, s.counts()[0], tuple(off))
+    s.build(32, 4)
+    o = orc.Scene.from_host(s)
+    rng = np.random.default_rng(11)
+    n = 4000
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.column_stack([off + rng.random((n, 3)), np.full(n, 1e-4), d, np.full(n, 1e30)]).astype(np.float32)
+    p, t = o.hit32(rays)
+    bp, bt = o.brute32(rays)
+    assert (p >= 0).mean() > 0.2
+    assert np.array_equal(p, bp) and np.array_equal(t, bt)
+    ra = rays.copy()
+    ra[:, 7] = 0.05
+    bq, _ = o.brute32(ra)
+    occ = o.any32(ra)
+    assert 0.02 < occ.mean() < 0.98 and np.array_equal(occ, (bq >= 0).astype(np.int32))
+
+
 def test_light_distribution():
     """TestPowerWeightedLightPDF / TestCachedLightDistribution (renderer_test.go:310-329, 563-582)."""
     s = bc.HostScene()
