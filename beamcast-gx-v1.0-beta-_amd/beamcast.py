@@ -115,6 +115,7 @@ def lib():
             "bc_scene_refit": (c_int, [vp, vp, c_uint64]),
             "bc_scene_arrays": (c_int, [vp, vp, vp, vp]),
             "bc_set_flatten": (c_int, [vp, c_int]),
+            "bc_set_render_ahead": (c_int, [vp, c_int]),
             "bc_set_trace_refill": (c_int, [vp, c_int, c_int]),
             "bc_set_trace_order": (c_int, [vp, c_int, c_int]),
             "bc_trace_order_last": (c_int, [vp, vp]),
@@ -533,6 +534,11 @@ class Device:
         out = np.zeros(2, np.int32)
         self._check(lib().bc_trace_order_last(self.ctx, _ptr(out)))
         return int(out[0]), int(out[1])
+
+    def set_render_ahead(self, on):
+        """bc_set_render_ahead: 1 (default) lets a persistent-schedule bc_render of a row chunk
+        render the following chunks with it (one pool drain per block); 0 renders each alone."""
+        self._check(lib().bc_set_render_ahead(self.ctx, 1 if on else 0))
 
     def set_flatten(self, mode):
         """BVH8 flattening of later uploads: 0 automatic, 1 host, 2 device (same bytes)."""

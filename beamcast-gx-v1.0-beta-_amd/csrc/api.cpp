@@ -173,7 +173,7 @@ void bc_destroy(bc_ctx* c) {
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf, c->d_light_of, c->d_err,
-                    c->tmp_in, c->tmp_out, c->out_buf, c->d_levels, c->d_nbox, c->d_stage, c->d_order, c->d_sorted,
+                    c->tmp_in, c->tmp_out, c->out_buf, c->ahead_buf, c->d_levels, c->d_nbox, c->d_stage, c->d_order, c->d_sorted,
                     c->d_sel};
     for (void* q : ptrs)
         if (q) hipFree(q);
@@ -200,6 +200,7 @@ int bc_scene_upload(bc_ctx* c, const bc_scene_view* v, uint64_t topo, uint64_t g
     const bc_scene_view& w = bcx::prepare_view(*v, pv);  // binary leaves of at most eight primitives
     bcx::Bvh8 b;
     if (!on_device && !bcx::build_bvh8(w, b, err)) { c->err = err; return BC_ERR_SCENE; }
+    c->scene_epoch++;
     c->has_scene = false;
     c->batch.clear();
     bcd::FlattenResult fr;
@@ -280,6 +281,13 @@ int bc_trace_order_last(bc_ctx* c, int* out2) {
     return BC_OK;
 }
 
+int bc_set_render_ahead(bc_ctx* c, int on) {
+    if (!c || on < 0 || on > 1) return BC_ERR_ARG;
+    c->ahead_on = on != 0;
+    c->ahead_valid = false;
+    return BC_OK;
+}
+
 int bc_set_flatten(bc_ctx* c, int mode) {
     if (!c || mode < 0 || mode > 2) return BC_ERR_ARG;
     c->flatten = mode;
@@ -298,6 +306,7 @@ int bc_update_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const 
 }  // extern "C"
 
 int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data) {
+    c->scene_epoch++;
     std::vector<bc_mat> mats((size_t)nmat);
     for (int i = 0; i < nmat; i++) {
         const double* d = mat_data + 8 * (size_t)i;
@@ -356,6 +365,7 @@ extern "C" {
 int bc_scene_refit(bc_ctx* c, const bc_scene_view* v, uint64_t geom) {
     if (!c || !v) return BC_ERR_ARG;
     if (!c->has_scene || !c->batch.empty()) { c->err = "bc_scene_refit needs a resident single scene"; return BC_ERR_SCENE; }
+    c->scene_epoch++;
     if (v->nprim != c->nprim || (int64_t)c->h_ids.size() != c->nrec) {
         c->err = "bc_scene_refit: the primitive count differs from the resident scene (topology changed)";
         return BC_ERR_ARG;
@@ -412,6 +422,7 @@ int bc_batch_upload(bc_ctx* c, int n, const bc_scene_view* views) {
     if (!c || n <= 0 || !views) return BC_ERR_ARG;
     for (int s = 0; s < n; s++)
         if (!check_view(c, views[s])) { c->err = "batch scene " + std::to_string(s) + ": " + c->err; return BC_ERR_SCENE; }
+    c->scene_epoch++;
     hipSetDevice(c->device);
     // the scenes' BVH8 builds are independent: spread them over host threads
     std::vector<bcx::Bvh8> bv((size_t)n);

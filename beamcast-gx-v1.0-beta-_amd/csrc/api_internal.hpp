@@ -84,6 +84,17 @@ struct bc_ctx {
     // staging for host-pointer renders
     void* out_buf = nullptr;
     size_t out_cap = 0;
+    // bumped by every change of the resident scene or its materials (upload, refit, material
+    // update, batch upload): rows that bc_render rendered ahead belong to one epoch
+    uint64_t scene_epoch = 0;
+    // rows rendered ahead by bc_render on the persistent schedule: rgba, then guides, of rows
+    // [ahead_y0, ahead_y0 + ahead_rows) of ahead_desc (integrator.hip)
+    bool ahead_on = true, ahead_valid = false, ahead_guides = false;
+    int ahead_y0 = 0, ahead_rows = 0;
+    uint64_t ahead_epoch = 0;
+    bc_render_desc ahead_desc{};
+    void* ahead_buf = nullptr;
+    size_t ahead_cap = 0;
     // integrator
     RenderState* rs = nullptr;
     // post stack

@@ -170,9 +170,20 @@ typedef struct bc_render_desc {
  * rays_out receives the path-segment count (Meta.Rays).  cancel (may be NULL) is read once, when
  * the call starts: a set flag returns BC_ERR_CANCELLED before anything is rendered.  A frame is
  * cancelled between chunks, as renderOpenCL polls its flag before each chunk
- * (gpu_opencl_windows.go:644-646); bch_render_hip (beamcast_host.h) does that. */
+ * (gpu_opencl_windows.go:644-646); bch_render_hip (beamcast_host.h) does that.
+ * Render-ahead (mode 1, on unless bc_set_render_ahead turns it off): a call whose rows are not
+ * already rendered renders them together with the following rows, up to a multiple of `rows`
+ * that one window of per-sample results holds (W x rows x spp <= 2^25 samples, at most the rest
+ * of the frame), into a buffer of the context; later calls with the same description for rows
+ * inside that block, with no scene or material change in between, copy them out.  Every pixel
+ * depends only on the description, the scene and its index, so the output is what rendering
+ * each chunk alone gives; the pool drains once per block instead of once per chunk.  rays_out
+ * is then the sum of the rows' per-pixel segment counts (rgba w), as renderOpenCL sums them. */
 int bc_render(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* rgba,
               uint32_t* guides, uint64_t* rays_out, const volatile int32_t* cancel);
+/* No counterpart in the reference: render-ahead of bc_render on the persistent schedule, 1 on
+ * (the default) or 0 off; either call drops rows rendered ahead. */
+int bc_set_render_ahead(bc_ctx* ctx, int on);
 /* Same, with rgba/guides in device memory (no host copy). shadow_rays_out may be NULL.
  * mode 1 (persistent): a pool of path slots refilled in place as paths end, the bounce loop
  * replayed as a captured hipGraph; reserved[0] = pool slots, reserved[1] = iterations per

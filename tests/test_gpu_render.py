@@ -221,6 +221,67 @@ def test_persistent_sample_windows(dev, window, pool, iters):
         dev.render(bad)
 
 
+def test_render_ahead_chunks(dev):
+    """bc_render on the persistent schedule renders the rows after a chunk with it and serves the
+    following chunks from them: each chunk equals the chunk rendered alone and its slice of the
+    whole frame, the per-chunk segment counts add up to the frame's, and a material update, a
+    refit, another description or a later request for guides is never answered with stale rows."""
+    print("case: render-ahead of persistent row chunks, showcase 64x40", flush=True)
+    s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
+    dev.upload(s)
+    dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=4096, iters=4)
+    full, gfull, rfull = dev.render(dp, guides=True)
+
+    def chunks(rows=8, guides=True):
+        return [dev.render(dp, y0=y, rows=min(rows, 40 - y), guides=guides) for y in range(0, 40, rows)]
+
+    ahead = chunks()
+    assert np.array_equal(np.concatenate([c[0] for c in ahead]), full)
+    assert np.array_equal(np.concatenate([c[1] for c in ahead]), gfull)
+    assert sum(c[2] for c in ahead) == rfull
+    dev.set_render_ahead(False)
+    alone = chunks()
+    dev.set_render_ahead(True)
+    for a, b in zip(ahead, alone):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+    def fresh(desc, y0, rows, guides=False):
+        dev.set_render_ahead(False)
+        out = dev.render(desc, y0=y0, rows=rows, guides=guides)
+        dev.set_render_ahead(True)
+        return out
+
+    # material update between two chunks of one block
+    kind, mat, geom, mk, md = s.export()
+    dev.render(dp, y0=0, rows=8)
+    md2 = md.copy()
+    md2[:, 0:3] *= 0.5
+    dev.update_materials(mk, md2)
+    got = dev.render(dp, y0=8, rows=8)
+    want = fresh(dp, 8, 8)
+    assert np.array_equal(got[0], want[0]) and got[2] == want[2]
+    assert not np.array_equal(got[0], ahead[1][0])
+    dev.update_materials(mk, md)
+    # refit between two chunks
+    dev.render(dp, y0=0, rows=8)
+    s.translate(0, s.counts()[0] // 2, (0.05, 0.0, 0.0))
+    dev.refit(s)
+    got = dev.render(dp, y0=16, rows=8)
+    want = fresh(dp, 16, 8)
+    assert np.array_equal(got[0], want[0]) and got[2] == want[2]
+    # another description (seed) with the block of the first still held
+    dev.render(dp, y0=0, rows=8)
+    d2 = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=4096, iters=4)
+    d2.seed = dp.seed + 1
+    got = dev.render(d2, y0=8, rows=8)
+    assert np.array_equal(got[0], fresh(d2, 8, 8)[0])
+    # guides asked for after a block rendered without them
+    dev.render(dp, y0=0, rows=8)
+    got = dev.render(dp, y0=8, rows=8, guides=True)
+    want = fresh(dp, 8, 8, guides=True)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
 SUN = dict(sun_dir=(0.3, 0.8, 0.2), sun=(3.0, 3.0, 3.0))
 
 
