@@ -153,8 +153,9 @@ struct bc_hit {
 #define BC_NTRY_COS 8
 #define BC_NTRY_METAL 9
 /* resampled direct lighting at a vertex of bounce b: the vertex key is bc_rng_u32(path key,
- * BC_DIM_RIS0 + b); candidate k draws dims k * BC_RIS_DIMS + o of the vertex key, o = 0 component,
- * 1 light pick, 2-3 triangle point, 4-19 sphere point / sky direction tries, 20 reservoir update */
+ * BC_DIM_RIS0 + b); candidate k draws dims k * BC_RIS_DIMS + o of the vertex key, o = 0 component
+ * (fewer candidates than components only), 1 light pick, 2-3 triangle point (2: the sky sample's
+ * squared disk radius), 4-19 sphere point / sky angle tries, 20 reservoir update */
 #define BC_DIM_RIS0 0x100000u
 #define BC_RIS_DIMS 32u
 #define BC_RIS_MAX 64

@@ -275,13 +275,18 @@ __device__ __forceinline__ float3 hemi_dir(float3 N, float x, float y, float z) 
 }
 
 // Resampled direct lighting at a diffuse vertex (an extension; the Go integrator has
-// one-sample NEE): RIS over P.restir candidates drawn uniformly from the components {sun
-// (delta), emitters (chooseLight + point sampling of directLight), sky (cosine hemisphere)},
-// target = luminance of the unshadowed contribution, one streaming reservoir.  Each candidate
-// weighs target / (component probability x its own pdf); the survivor y carries
-// W = sum(w) / (M target(y)).  Returns false when nothing survives; otherwise the unshadowed
-// contribution F(y) W (before throughput) and the visibility ray: towards the point y when
-// `point`, else along the direction y.
+// one-sample NEE): RIS over P.restir candidates from the components {sun (delta), emitters
+// (chooseLight + point sampling of directLight), sky (cosine hemisphere)}, target = luminance of
+// the unshadowed contribution, one streaming reservoir.  With at least one candidate per
+// component (M >= nc) the candidates are split into contiguous strata, candidate k taking
+// component k * nc / M, so the lanes of a wave evaluate the same component at each k; component
+// c's M_c candidates each weigh target x (M / M_c) / pdf, since only those M_c candidates can
+// produce a sample in c's domain (the domains are disjoint: a delta direction, emitter points,
+// sky directions).  With fewer candidates than components each draws its component with
+// probability 1 / nc and weighs target x nc / pdf.  Either way the survivor y carries
+// W = sum(w) / (M target(y)), unbiased.  ris_direct returns false when nothing survives;
+// otherwise the unshadowed contribution F(y) W (before throughput) and the visibility ray:
+// towards the point y when `point`, else along the direction y.
 // The reservoir of resampled direct lighting after its candidates: the survivor, its target and
 // weight sum (ris_sample), from which ris_direct forms F(y) W and spatial reuse starts.
 struct Reservoir {
@@ -293,6 +298,27 @@ struct Reservoir {
     int M;
 };
 
+// (x, y) in the tangent plane of a cosine-weighted hemisphere sample whose squared disk radius is
+// u1 (Malley's method: cos(theta) = sqrt(1 - u1)): the radius sqrt(u1) along a uniform angle,
+// the angle of a point drawn uniformly in the unit disk by rejection from dims 4-19 of the
+// candidate (after BC_NTRY_COS misses, probability 5e-6, the angle is 0).
+__device__ __forceinline__ void sky_disk(uint32_t rk, uint32_t base, float u1, float& x, float& y) {
+    const float r = sqrtf(u1);
+    x = r;
+    y = 0.0f;
+    for (int t = 0; t < BC_NTRY_COS; t++) {
+        const float a = bc_rng(rk, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+        const float b = bc_rng(rk, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
+        const float ss = a * a + b * b;
+        if (ss < 1.0f && ss > (float)1e-18) {
+            const float s = r / sqrtf(ss);
+            x = a * s;
+            y = b * s;
+            break;
+        }
+    }
+}
+
 __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParams& P, float3 p, float3 N, float3 color,
                                            uint32_t key, int bounce, Reservoir& rs) {
     const int M = P.restir < BC_RIS_MAX ? P.restir : BC_RIS_MAX;
@@ -301,19 +327,38 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
     const bool sun_on = (E.x > 0.0f || E.y > 0.0f || E.z > 0.0f) && (sd.x != 0.0f || sd.y != 0.0f || sd.z != 0.0f);
     const bool area_on = P.nlight > 0;
     const int nc = 1 + (sun_on ? 1 : 0) + (area_on ? 1 : 0);
-    const float pc = 1.0f / (float)nc;
+    const bool strat = M >= nc;
     const float3 f = mul(color, INVPI_F);
+    // the sun's candidates are all the same sample (a delta direction)
+    float3 F_sun = f3(0.0f, 0.0f, 0.0f);
+    float ph_sun = 0.0f;
+    if (sun_on) {
+        const float cs = dot(N, sd);
+        if (cs > 0.0f) {
+            F_sun = mul(had(f, E), cs);
+            ph_sun = lum(F_sun);
+        }
+    }
     float wsum = 0.0f, ph_s = 0.0f, cosL_s = 0.0f, dist2_s = 0.0f;
     float3 F_s = f3(0.0f, 0.0f, 0.0f), y_s = F_s, ln_s = F_s;
     bool have = false;
-    int comp_s = -1, li_s = -1;
+    int comp_s = -1, li_s = -1, k_s = 0;
     // The candidates draw from a key of their own per vertex, so the dimension half of every hash
     // depends only on (k, offset): the same for all lanes of a wave, whatever their bounce.
     const uint32_t rk = bc_rng_u32(key, BC_DIM_RIS0 + (uint32_t)bounce);
     for (int k = 0; k < M; k++) {
         const uint32_t base = (uint32_t)k * BC_RIS_DIMS;
-        int ci = (int)(bc_rng(rk, base) * (float)nc);
-        if (ci >= nc) ci = nc - 1;
+        int ci;
+        float mult;  // M / M_c (strata) or nc (random components)
+        if (strat) {
+            ci = k * nc / M;
+            const int c0 = (ci * M + nc - 1) / nc, c1 = ((ci + 1) * M + nc - 1) / nc;
+            mult = (float)M / (float)(c1 - c0);
+        } else {
+            ci = (int)(bc_rng(rk, base) * (float)nc);
+            if (ci >= nc) ci = nc - 1;
+            mult = (float)nc;
+        }
         int comp;  // 0 sun, 1 emitters, 2 sky
         if (sun_on) comp = ci == 0 ? 0 : ((area_on && ci == 1) ? 1 : 2);
         else comp = (area_on && ci == 0) ? 1 : 2;
@@ -321,11 +366,10 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
         float3 F = f3(0.0f, 0.0f, 0.0f), y = F, ln_c = F;
         int li_c = -1;
         if (comp == 0) {
-            const float cs = dot(N, sd);
-            if (cs > 0.0f) {
-                F = mul(had(f, E), cs);
-                ph = lum(F);
-                w = ph / pc;
+            if (ph_sun > 0.0f) {
+                F = F_sun;
+                ph = ph_sun;
+                w = ph_sun * mult;
                 y = sd;
             }
         } else if (comp == 1) {
@@ -382,7 +426,7 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
                     if (pin > 0.0f) {
                         F = mul(had(f, f3(Lr.le[0], Lr.le[1], Lr.le[2])), nd);
                         ph = lum(F);
-                        w = ph / (pc * pin);
+                        w = (ph * mult) / pin;
                         y = qpt;
                         ln_c = lnrm;
                         li_c = li;
@@ -392,27 +436,23 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
                 }
             }
         } else {
-            float x = 0.0f, yy = 0.0f, ss = 0.0f;
-            bool ok = false;
-            for (int t = 0; t < BC_NTRY_COS && !ok; t++) {
-                x = bc_rng(rk, base + 4u + 2u * (uint32_t)t) * 2.0f - 1.0f;
-                yy = bc_rng(rk, base + 5u + 2u * (uint32_t)t) * 2.0f - 1.0f;
-                ss = x * x + yy * yy;
-                ok = ss < 1.0f;
-            }
-            if (ok) {
-                // cos(theta) of the sample is its z about N; the world direction is needed only
-                // for a sky that varies with direction, and for the survivor (built after the loop)
-                const float z = sqrtf(rmax(0.0f, 1.0f - ss));
-                const float cv = z;
-                if (cv > 0.0f) {
-                    const float pin = cv / PI_F;
-                    const float3 Ls = P.sky_mode == 1 ? f3(P.sky[0], P.sky[1], P.sky[2]) : sky_radiance(P, hemi_dir(N, x, yy, z));
-                    F = mul(had(f, Ls), cv);
-                    ph = lum(F);
-                    w = ph / (pc * pin);
-                    y = f3(x, yy, z);
+            // cos(theta) of a cosine-weighted sample from its squared disk radius u1; the angle
+            // about N is drawn only where the direction is needed: a sky that varies with
+            // direction, and the survivor (after the loop)
+            const float u1 = bc_rng(rk, base + 2u);
+            const float z = sqrtf(rmax(0.0f, 1.0f - u1));
+            if (z > 0.0f) {
+                const float pin = z / PI_F;
+                float3 Ls = f3(P.sky[0], P.sky[1], P.sky[2]);
+                if (P.sky_mode != 1) {
+                    float x, yy;
+                    sky_disk(rk, base, u1, x, yy);
+                    Ls = sky_radiance(P, hemi_dir(N, x, yy, z));
                 }
+                F = mul(had(f, Ls), z);
+                ph = lum(F);
+                w = (ph * mult) / pin;
+                y = f3(u1, 0.0f, z);
             }
         }
         if (w > 0.0f) {
@@ -427,12 +467,19 @@ __device__ __forceinline__ void ris_sample(const SceneBufs& S, const RenderParam
                 ln_s = ln_c;
                 cosL_s = cosL_c;
                 dist2_s = dist2_c;
+                k_s = k;
             }
         }
     }
     rs.comp = have ? comp_s : -1;
     rs.li = li_s;
-    rs.y = comp_s == 2 ? hemi_dir(N, y_s.x, y_s.y, y_s.z) : y_s;  // sky: local sample -> direction
+    if (comp_s == 2) {  // sky: the survivor's local sample -> direction
+        float x, yy;
+        sky_disk(rk, (uint32_t)k_s * BC_RIS_DIMS, y_s.x, x, yy);
+        rs.y = hemi_dir(N, x, yy, y_s.z);
+    } else {
+        rs.y = y_s;
+    }
     rs.lnrm = ln_s;
     rs.F = F_s;
     rs.ph = ph_s;

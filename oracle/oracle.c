@@ -1044,8 +1044,9 @@ void orc_camera_basis(const double* pos, const double* target, const double* up,
 #define NTRY_COS 8
 #define NTRY_METAL 9
 /* resampled direct lighting at a vertex of bounce b: the vertex key is orc_rng_u32(path key,
- * DIM_RIS0 + b); candidate k draws dims k * RIS_DIMS + o of the vertex key, o = 0 component,
- * 1 light pick, 2-3 triangle point, 4-19 sphere point / sky direction tries, 20 reservoir update */
+ * DIM_RIS0 + b); candidate k draws dims k * RIS_DIMS + o of the vertex key, o = 0 component
+ * (fewer candidates than components only), 1 light pick, 2-3 triangle point (2: the sky sample's
+ * squared disk radius), 4-19 sphere point / sky angle tries, 20 reservoir update */
 #define DIM_RIS0 0x100000u
 #define RIS_DIMS 32u
 #define RIS_MAX 64

@@ -368,13 +368,25 @@ def test_ris_plane_closed_form(bounces):
     got = img[..., :3].mean()
     assert abs(got / want - 1) < 0.01
     assert shadow == 24 * 24 * 16  # one visibility ray per diffuse vertex
-    # the sun alone (black sky): sky candidates weigh zero but still count in M, so each sample
-    # is (suns drawn / M) x 2 x the sun term, unbiased, and every rgb channel is equal
+    # the sun alone (black sky): sky candidates weigh zero but still count in M; the 32 candidates
+    # split into 16 sun and 16 sky strata, each sun candidate weighs its target x 32 / 16, so
+    # every sample is (16 x 2 / 32) x the sun term: exactly the sun term, in every pixel (to the
+    # fp32 rounding of the image and of the stored sun direction)
     d.sky_color[0] = d.sky_color[1] = d.sky_color[2] = 0.0
     img2, _, _, _ = o.render(d, precision=64)
     sun = np.float32(a) * e * sd[1] / np.pi
-    assert abs(img2[..., :3].mean() / sun - 1) < 0.01
-    assert np.all(img2[..., 0] == img2[..., 2]) and img2[..., 0].min() > 0
+    assert np.allclose(img2[..., :3], sun, rtol=1e-6, atol=0)
+    # three candidates over sun and sky: strata of 2 and 1 (weights x 3/2 and x 3); with a black
+    # sky each sample is (2 x 3/2 / 3) x the sun term, again exactly
+    d.restir_candidates = 3
+    img3, _, _, _ = o.render(d, precision=64)
+    assert np.allclose(img3[..., :3], sun, rtol=1e-6, atol=0)
+    # one candidate (fewer than the two components): a random component with weight x 2, so a
+    # pixel is 2 x the sun term times the fraction of its samples that drew the sun
+    d.restir_candidates = 1
+    img1, _, _, _ = o.render(d, precision=64)
+    frac = img1[..., 0] / (2 * sun)
+    assert np.allclose(frac * 16, np.round(frac * 16), atol=1e-4) and abs(frac.mean() - 0.5) < 0.05
 
 
 def test_ris_matches_one_sample_nee():
@@ -406,8 +418,11 @@ def test_spatial_reuse_without_neighbours_is_ris():
 
 def test_spatial_reuse_on_sunlit_plane():
     """On a diffuse plane under a constant sky and a sun every pixel has the same expected
-    radiance albedo (Ls + E (n.s) / pi): reusing neighbours' reservoirs keeps the mean and cuts
-    the per-pixel spread."""
+    radiance albedo (Ls + E (n.s) / pi).  With one candidate (a random component) reusing
+    neighbours' reservoirs keeps the mean and cuts the per-pixel spread.  With two candidates
+    (one sun and one sky stratum) the estimate has no variance at all: nothing occludes the plane,
+    the grey contribution is a fixed multiple of its target, and the weight sum is the same for
+    every sky direction (cos theta cancels against the cosine pdf)."""
     a, ls, e = 0.5, float(np.float32(0.3)), 3.0
     sd = np.array([0.3, 0.8, 0.2]) / np.linalg.norm([0.3, 0.8, 0.2])
     s = sunlit_plane_scene(a)
@@ -415,12 +430,16 @@ def test_spatial_reuse_on_sunlit_plane():
     want = np.float32(a) * (ls + e * sd[1] / np.pi)
     spread = {}
     for sp in (0, 16):
-        d = orc.desc_from(bc.render_desc(s, 32, 32, 1, 1, sky_mode=1, sky_color=(ls, ls, ls), restir=2, sun_dir=sd,
+        d = orc.desc_from(bc.render_desc(s, 32, 32, 1, 1, sky_mode=1, sky_color=(ls, ls, ls), restir=1, sun_dir=sd,
                                          sun=(e, e, e), spatial=sp, radius=8, seed=11))
         img = o.render(d, precision=64)[0][..., :3]
         assert abs(img.mean() / want - 1) < 0.02
         spread[sp] = img.std() / img.mean()
     assert spread[16] < 0.6 * spread[0]
+    d = orc.desc_from(bc.render_desc(s, 32, 32, 1, 1, sky_mode=1, sky_color=(ls, ls, ls), restir=2, sun_dir=sd,
+                                     sun=(e, e, e), seed=11))
+    img = o.render(d, precision=64)[0][..., :3]
+    assert np.allclose(img, want, rtol=1e-5, atol=0)
 
 
 def test_spatial_reuse_mean_closed_box():
