@@ -94,7 +94,9 @@ int bc_scene_refit(bc_ctx* ctx, const bc_scene_view* scene, uint64_t geom_rev);
  * one-block probe of 4,096 adjacent ray pairs (origin cell and direction octant, as ray binning
  * bins them) picks refill at 8 lanes when at least half of the pairs differ (incoherent rays) and
  * one ray per thread otherwise (coherent rays, whose waves share upper nodes); smaller batches run
- * one ray per thread.  Defaults -1 (closest) and 8 (any hit).  Results do not depend on it. */
+ * one ray per thread.  Defaults -1 (closest) and 8 (any hit).  The renders use the same settings:
+ * shadow rays the any-hit one; the second segment's extension rays of the plain schedule the
+ * closest-hit one (-1: 8 lanes).  Results do not depend on it. */
 int bc_set_trace_refill(bc_ctx* ctx, int closest_lanes, int any_lanes);
 /* Ray binning of bc_trace / bc_trace_device (no counterpart in the reference): batches of 65,536
  * rays or more can be counting-sorted on the device by (Morton cell of the origin on a grid of
