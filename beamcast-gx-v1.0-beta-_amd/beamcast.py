@@ -536,8 +536,8 @@ class Device:
         return int(out[0]), int(out[1])
 
     def set_render_ahead(self, on):
-        """bc_set_render_ahead: 1 (default) lets a persistent-schedule bc_render of a row chunk
-        render the following chunks with it (one pool drain per block); 0 renders each alone."""
+        """bc_set_render_ahead: 1 (default) lets bc_render of a row chunk render the following
+        chunks with it (one render per block); 0 renders each alone."""
         self._check(lib().bc_set_render_ahead(self.ctx, 1 if on else 0))
 
     def set_flatten(self, mode):

@@ -87,10 +87,10 @@ struct bc_ctx {
     // bumped by every change of the resident scene or its materials (upload, refit, material
     // update, batch upload): rows that bc_render rendered ahead belong to one epoch
     uint64_t scene_epoch = 0;
-    // rows rendered ahead by bc_render on the persistent schedule: rgba, then guides, of rows
-    // [ahead_y0, ahead_y0 + ahead_rows) of ahead_desc (integrator.hip)
+    // rows rendered ahead by bc_render: rgba, then guides, of rows [ahead_y0, ahead_y0 +
+    // ahead_rows) of ahead_desc (integrator.hip); rows from ahead_next on are not yet served
     bool ahead_on = true, ahead_valid = false, ahead_guides = false;
-    int ahead_y0 = 0, ahead_rows = 0;
+    int ahead_y0 = 0, ahead_rows = 0, ahead_next = 0;
     uint64_t ahead_epoch = 0;
     bc_render_desc ahead_desc{};
     void* ahead_buf = nullptr;

@@ -594,12 +594,13 @@ __global__ __launch_bounds__(kBlock) BC_WAVES(BC_P_EXTEND_WAVES) void k_p_extend
                     if constexpr (MULTI) {
                         const uint32_t sc = scene_of(gpre, nscene, gn);
                         const RenderParams& Q = Pd[sc];
-                        const uint32_t gl = gn - gpre[sc];
-                        const uint32_t sl = gl / (uint32_t)Q.npix, pl = gl - sl * (uint32_t)Q.npix;
+                        uint32_t sl, pl;
+                        window_sample(Q, gn - gpre[sc], sl, pl);
                         init_camera_path(Q, B, j, (int)pl, Q.s0 + (int)sl);
                         B.sid[j] = sc;
                     } else {
-                        const uint32_t sl = gn / (uint32_t)P.npix, pl = gn - sl * (uint32_t)P.npix;
+                        uint32_t sl, pl;
+                        window_sample(P, gn, sl, pl);
                         init_camera_path(P, B, j, (int)pl, P.s0 + (int)sl);
                     }
                     B.gid[j] = gn;
@@ -741,7 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_p_reduce(const RenderParams* __restr
     for (int pl = blockIdx.x * kBlock + threadIdx.x; pl < P.npix; pl += gridDim.x * kBlock) {
         float4 a = acc[pl];
         for (int sl = 0; sl < P.ns; sl++) {
-            const float4 r = B.res[(size_t)sl * (size_t)P.npix + (size_t)pl];
+            const float4 r = B.res[window_index(P, (uint32_t)sl, (uint32_t)pl)];
             const float4 c = clamp_sample(P, r);
             a.x = a.x + c.x; a.y = a.y + c.y; a.z = a.z + c.z;
             a.w = a.w + r.w;
@@ -905,7 +906,7 @@ struct RenderState {
 
 namespace {
 
-constexpr size_t kResultCap = (size_t)1 << 25;   // per-sample results held per window (512 MB)
+constexpr size_t kResultCap = (size_t)1 << 27;   // per-sample results held per window (2 GB)
 constexpr size_t kBatchCap = (size_t)1 << 27;    // the same for a batch of scenes (2 GB)
 constexpr int kMaxJobs = 256;                    // scenes of one batch render
 
@@ -1032,6 +1033,7 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
     for (int k = 0; k < 3; k++) P.sun[k] = d->sun_irradiance[k];
     P.spatial = d->restir_candidates > 0 ? d->restir_spatial : 0;
     P.radius = d->restir_radius > 0 ? d->restir_radius : 16;
+    P.band = P.npix;
     return P;
 }
 
@@ -1123,38 +1125,61 @@ int finish(bc_ctx* c, const bcd::RenderParams& P, float* d_rgba, int grid_pix, u
 }
 
 // Plain wavefront schedule: passes of pixels x samples, three kernels per bounce.  scene: batch
-// scene index (-1: the single resident scene).
+// scene index (-1: the single resident scene).  A pass holds up to 2^22 paths.  When the rows'
+// samples need several passes, they run band by band: bands of whole rows whose every sample
+// fits one pass (one row per band, in passes of its samples, when a row's samples do not), so
+// that the rays in flight come from one region of the image and share the scene's nodes in
+// cache, instead of passes of a few samples over the whole image.  Each pixel still adds its
+// samples in sample order, so the result does not depend on the split.  Spatial reuse (which
+// reads neighbouring pixels among the rendered rows) and the forward pass of the adjoint (which
+// keeps every sample by index) run whole-image passes.
 int render_plain(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* d_rgba, uint32_t* d_guides,
                  uint64_t* rays_out, uint64_t* shadow_out, float4* keep = nullptr, int scene = -1) {
-    const size_t npix = (size_t)rows * (size_t)d->width;
+    const size_t W = (size_t)d->width;
+    const size_t npix = (size_t)rows * W;
     const size_t max_paths = std::max<size_t>(npix, (size_t)1 << 22);
-    int spp_pass = d->samples_per_pass > 0 ? d->samples_per_pass : (int)std::max<size_t>(1, max_paths / npix);
-    spp_pass = std::min(spp_pass, d->spp);
-    const size_t paths = npix * (size_t)spp_pass;
-    const bool reservoirs = d->restir_candidates > 0 && d->restir_spatial > 0;
-    if (!ensure_state(c, paths, npix, d_guides != nullptr, 1, plain_segment(c, paths), 1, reservoirs)) return BC_ERR_NOMEM;
+    const bool spatial = d->restir_candidates > 0 && d->restir_spatial > 0;
+    int band = rows;  // rows per band
+    int spp_pass;
+    if (d->samples_per_pass > 0) {
+        spp_pass = std::min(d->samples_per_pass, d->spp);
+    } else if (npix * (size_t)d->spp <= max_paths || spatial || keep) {
+        spp_pass = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, max_paths / npix));
+    } else {
+        const size_t per_row = W * (size_t)d->spp;
+        band = (int)std::max<size_t>(1, std::min<size_t>((size_t)rows, ((size_t)1 << 22) / per_row));
+        spp_pass = (int)std::min<size_t>((size_t)d->spp, std::max<size_t>(1, ((size_t)1 << 22) / ((size_t)band * W)));
+    }
+    const size_t paths = (size_t)band * W * (size_t)spp_pass;
+    if (!ensure_state(c, paths, npix, d_guides != nullptr, 1, plain_segment(c, paths), 1, spatial)) return BC_ERR_NOMEM;
     RenderState& R = *c->rs;
     R.B.xg = xcd_group(d);
     const bcd::SceneBufs S = scene_bufs(c);
-    bcd::RenderParams P = make_params(c, d, y0, rows, d_guides != nullptr, scene);
-    const uint32_t nseg = R.nseg;
-    const int grid = seg_grid(c, nseg);
-    const int grid_pix = pix_grid(c, npix);
+    const bcd::RenderParams Pall = make_params(c, d, y0, rows, d_guides != nullptr, scene);
     hipStream_t st = c->stream;
     if (!clear_tally(c)) return BC_ERR_DEVICE;
     uint32_t* guides = d_guides ? d_guides : R.guide_tmp;
-    for (int s0 = 0; s0 < d->spp; s0 += spp_pass) {
-        P.s0 = s0;
-        P.ns = std::min(spp_pass, d->spp - s0);
-        hipLaunchKernelGGL(bcd::k_raygen, dim3(grid), dim3(bcd::kBlock), 0, st, P, R.B, (float*)R.acc, guides, nseg);
-        for (int b = 0; b < d->bounces; b++) {
-            const int cur = b & 1;
-            launch_bounce(S, R.B, P, cur, b, guides, grid, nseg, st);
+    for (int yb = 0; yb < rows; yb += band) {
+        const int hb = std::min(band, rows - yb);
+        const size_t off = (size_t)yb * W, bpix = (size_t)hb * W;
+        bcd::RenderParams P = make_params(c, d, y0 + yb, hb, d_guides != nullptr, scene);
+        const uint32_t nseg = (uint32_t)segments_for(bpix * (size_t)spp_pass, R.B.seg);
+        const int grid = seg_grid(c, nseg);
+        const int grid_pix = pix_grid(c, bpix);
+        for (int s0 = 0; s0 < d->spp; s0 += spp_pass) {
+            P.s0 = s0;
+            P.ns = std::min(spp_pass, d->spp - s0);
+            hipLaunchKernelGGL(bcd::k_raygen, dim3(grid), dim3(bcd::kBlock), 0, st, P, R.B, (float*)(R.acc + off),
+                               guides + 3 * off, nseg);
+            for (int b = 0; b < d->bounces; b++) {
+                const int cur = b & 1;
+                launch_bounce(S, R.B, P, cur, b, guides + 3 * off, grid, nseg, st);
+            }
+            hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, R.B, R.acc + off, keep);
+            if (!bc_check(c, hipGetLastError(), "render launch")) return BC_ERR_DEVICE;
         }
-        hipLaunchKernelGGL(bcd::k_accumulate, dim3(grid_pix), dim3(bcd::kBlock), 0, st, P, R.B, R.acc, keep);
-        if (!bc_check(c, hipGetLastError(), "render launch")) return BC_ERR_DEVICE;
     }
-    return finish(c, P, d_rgba, grid_pix, rays_out, shadow_out);
+    return finish(c, Pall, d_rgba, pix_grid(c, npix), rays_out, shadow_out);
 }
 
 // One frame (or row chunk) of the persistent schedule: a single scene, or one scene of a batch.
@@ -1239,6 +1264,10 @@ int render_persistent_jobs(bc_ctx* c, const std::vector<Job>& jobs, uint64_t* ra
             P[k].s0 = s0;
             P[k].ns = std::max(0, std::min(window_ns, jobs[k].d->spp - s0));
             gpre[k + 1] = gpre[k] + (uint32_t)P[k].npix * (uint32_t)P[k].ns;
+            // bands of whole rows holding about one pool of samples each (window_sample)
+            const size_t row = (size_t)P[k].width * (size_t)std::max(1, P[k].ns);
+            const size_t rows_b = std::max<size_t>(1, slots / row);
+            P[k].band = (int32_t)std::min<size_t>((size_t)P[k].npix, rows_b * (size_t)P[k].width);
         }
         const size_t window_total = gpre[njobs];
         std::memcpy(R.h_params, P.data(), njobs * sizeof(bcd::RenderParams));
@@ -1697,23 +1726,30 @@ static_assert(offsetof(bc_render_desc, origin) == 14 * 4 && offsetof(bc_render_d
               "bc_render_desc is compared bytewise: no padding between its fields");
 constexpr int kNotAhead = 1 << 30;
 
-// Render-ahead for bc_render on the persistent schedule.  A frame rendered in row chunks pays one
-// drain of the path pool per call (terrain 1080p, 64 spp: 147 ms in 64-row chunks against 111 ms
-// in one call); so the first call of a run of chunks renders as many further rows (a multiple of
-// its chunk height) as one window of per-sample results holds, into a buffer of the context, and
-// the calls for those rows copy them out.  Every pixel depends only on the description, the
-// scene and its own index, so the rows are what rendering them alone gives.  Returns kNotAhead
-// when the call should render its rows alone.
+// Render-ahead for bc_render.  A frame rendered in row chunks pays one drain of the path pool
+// (persistent schedule) or one short run of passes (plain schedule) per call, plus a host round
+// trip (terrain 1080p, 64 spp, persistent: 147 ms in 64-row chunks against 111 ms in one call);
+// so the first call of a run of chunks renders as many further rows (a multiple of its chunk
+// height) as one window of per-sample results holds (persistent) or as kAheadPixels allows
+// (plain), into a buffer of the context, and the calls for those rows copy them out.  Every pixel
+// depends only on the description, the scene and its own index (spatial reuse, which reads
+// neighbouring pixels' reservoirs among the rendered rows, never renders ahead), so the rows are
+// what rendering them alone gives.  Rows are served once, in increasing order: a request that
+// starts before the end of the last rows served (a new frame, a repeated chunk) renders anew, so
+// the block is a prefetch of the frame being read, never a cache of earlier frames.  Returns
+// kNotAhead when the call should render its rows alone.
+constexpr size_t kAheadPixels = size_t(1) << 23;
 int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rgba, uint32_t* guides,
                  uint64_t* rays_out) {
-    const bool have = c->ahead_valid && c->ahead_epoch == c->scene_epoch && y0 >= c->ahead_y0 &&
+    const bool have = c->ahead_valid && c->ahead_epoch == c->scene_epoch && y0 >= c->ahead_next &&
                       y0 + rows <= c->ahead_y0 + c->ahead_rows && (!guides || c->ahead_guides) &&
                       std::memcmp(&c->ahead_desc, d, kDescBytes) == 0;
     if (!have) {
         if (!c->has_scene) { c->err = "no scene uploaded"; return BC_ERR_SCENE; }
         if (!valid_desc(c, d, y0, rows)) return BC_ERR_ARG;
-        const size_t per_row = (size_t)d->width * (size_t)d->spp;
-        const int fit = (int)std::min<size_t>((size_t)d->height, kResultCap / per_row);
+        const size_t per_row = d->mode == 1 ? (size_t)d->width * (size_t)d->spp : (size_t)d->width;
+        const size_t cap = d->mode == 1 ? kResultCap : kAheadPixels;
+        const int fit = (int)std::min<size_t>((size_t)d->height, cap / per_row);
         const int block = std::min(d->height - y0, std::max(rows, fit / rows * rows));
         if (block <= rows) return kNotAhead;
         const size_t bpix = (size_t)block * (size_t)d->width;
@@ -1725,6 +1761,7 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
         c->ahead_valid = true;
         c->ahead_desc = *d;
         c->ahead_y0 = y0;
+        c->ahead_next = y0;
         c->ahead_rows = block;
         c->ahead_guides = guides != nullptr;
         c->ahead_epoch = c->scene_epoch;
@@ -1735,6 +1772,7 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
     if (!bc_d2h(c, rgba, (const float*)c->ahead_buf + off * 4, npix * 16, c->stream, "copy image") ||
         (guides && !bc_d2h(c, guides, dg + off * 3, npix * 12, c->stream, "copy guides")))
         return BC_ERR_DEVICE;
+    c->ahead_next = y0 + rows;
     // the rows' path segments, from the per-pixel counts as renderOpenCL sums them
     // (gpu_opencl_windows.go:681-685)
     if (rays_out) {
@@ -1752,7 +1790,7 @@ extern "C" int bc_render(bc_ctx* c, const bc_render_desc* d, int y0, int rows, f
     if (!c || !d || !rgba) return BC_ERR_ARG;
     if (cancel && *cancel) { c->err = "render cancelled"; return BC_ERR_CANCELLED; }
     if (d->width <= 0 || rows <= 0) { c->err = "invalid render description or row range"; return BC_ERR_ARG; }
-    if (c->ahead_on && d->mode == 1 && d->bounces > 0) {
+    if (c->ahead_on && d->bounces > 0 && !(d->restir_candidates > 0 && d->restir_spatial > 0)) {
         const int rc = render_ahead(c, d, y0, rows, rgba, guides, rays_out);
         if (rc != kNotAhead) return rc;
     }

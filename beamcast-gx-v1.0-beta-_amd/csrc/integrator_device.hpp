@@ -27,7 +27,28 @@ struct RenderParams {
     int32_t restir;            // candidates of the resampled direct lighting (0 = one-sample NEE)
     float sun_dir[3], sun[3];  // directional light (restir only)
     int32_t spatial, radius;   // spatial reuse at primary vertices: neighbours, pixel radius (0 = off)
+    int32_t band;              // persistent window: pixels per band of its sample order (window_sample)
 };
+
+// Order of a persistent window's samples (g < npix * ns of one job): bands of P.band consecutive
+// pixels (the last one shorter), each holding all of the window's samples of its pixels, sample
+// after sample.  The path pool restarts samples in g order, so the paths in flight come from one
+// or two bands of the image and share the scene's nodes in cache.
+__device__ __forceinline__ void window_sample(const RenderParams& P, uint32_t g, uint32_t& sl, uint32_t& pl) {
+    const uint32_t band = (uint32_t)P.band, per = band * (uint32_t)P.ns;
+    const uint32_t b = g / per, b0 = b * band;
+    const uint32_t bs = min(band, (uint32_t)P.npix - b0);
+    const uint32_t r = g - b * per;
+    sl = r / bs;
+    pl = b0 + (r - sl * bs);
+}
+// The inverse: the window index g of sample slot sl of pixel pl.
+__device__ __forceinline__ uint32_t window_index(const RenderParams& P, uint32_t sl, uint32_t pl) {
+    const uint32_t band = (uint32_t)P.band;
+    const uint32_t b = pl / band, b0 = b * band;
+    const uint32_t bs = min(band, (uint32_t)P.npix - b0);
+    return b * band * (uint32_t)P.ns + sl * bs + (pl - b0);
+}
 
 // Path slots are grouped in segments of B.seg consecutive slots (a multiple of the workgroup
 // size).  One workgroup owns a segment in every kernel: it walks the segment's queues, and

@@ -221,15 +221,18 @@ def test_persistent_sample_windows(dev, window, pool, iters):
         dev.render(bad)
 
 
-def test_render_ahead_chunks(dev):
-    """bc_render on the persistent schedule renders the rows after a chunk with it and serves the
-    following chunks from them: each chunk equals the chunk rendered alone and its slice of the
-    whole frame, the per-chunk segment counts add up to the frame's, and a material update, a
-    refit, another description or a later request for guides is never answered with stale rows."""
-    print("case: render-ahead of persistent row chunks, showcase 64x40", flush=True)
+@pytest.mark.parametrize("mode", [1, 0])
+def test_render_ahead_chunks(dev, mode):
+    """bc_render renders the rows after a chunk with it and serves the following chunks from
+    them (persistent and plain schedules): each chunk equals the chunk rendered alone and its
+    slice of the whole frame, the per-chunk segment counts add up to the frame's, and a material
+    update, a refit, another description or a later request for guides is never answered with
+    stale rows."""
+    print(f"case: render-ahead of row chunks, schedule {mode}, showcase 64x40", flush=True)
     s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
     dev.upload(s)
-    dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=4096, iters=4)
+    sched = dict(mode=1, pool=4096, iters=4) if mode == 1 else dict(mode=0)
+    dp = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, **sched)
     full, gfull, rfull = dev.render(dp, guides=True)
 
     def chunks(rows=8, guides=True):
@@ -271,7 +274,7 @@ def test_render_ahead_chunks(dev):
     assert np.array_equal(got[0], want[0]) and got[2] == want[2]
     # another description (seed) with the block of the first still held
     dev.render(dp, y0=0, rows=8)
-    d2 = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, mode=1, pool=4096, iters=4)
+    d2 = bc.render_desc(s, 64, 40, 6, 6, rr_depth=2, firefly_clamp=20, **sched)
     d2.seed = dp.seed + 1
     got = dev.render(d2, y0=8, rows=8)
     assert np.array_equal(got[0], fresh(d2, 8, 8)[0])
@@ -280,6 +283,13 @@ def test_render_ahead_chunks(dev):
     got = dev.render(dp, y0=8, rows=8, guides=True)
     want = fresh(dp, 8, 8, guides=True)
     assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    if mode == 0:
+        # spatial reuse reads neighbouring reservoirs among the rendered rows: a chunk is never
+        # rendered with the rows after it
+        ds = bc.render_desc(s, 64, 40, 2, 3, rr_depth=2, restir=8, spatial=4, radius=6)
+        dev.render(ds, y0=0, rows=8)
+        got = dev.render(ds, y0=8, rows=8)
+        assert np.array_equal(got[0], fresh(ds, 8, 8)[0])
 
 
 SUN = dict(sun_dir=(0.3, 0.8, 0.2), sun=(3.0, 3.0, 3.0))
