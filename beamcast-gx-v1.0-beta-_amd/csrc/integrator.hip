@@ -907,7 +907,7 @@ struct RenderState {
 namespace {
 
 constexpr size_t kResultCap = (size_t)1 << 27;   // per-sample results held per window (2 GB)
-constexpr size_t kBatchCap = (size_t)1 << 27;    // the same for a batch of scenes (2 GB)
+constexpr size_t kBatchCap = (size_t)1 << 30;    // the same for a batch of scenes (16 GB)
 constexpr int kMaxJobs = 256;                    // scenes of one batch render
 
 size_t segments_for(size_t paths, uint32_t ss) { return (paths + ss - 1) / ss; }
@@ -1370,7 +1370,7 @@ bool autotune(bc_ctx* c, const std::vector<Job>& jobs, int& pool, int& iters, ui
     }
     std::vector<bc_render_desc> trial(jobs.size());
     std::vector<Job> tj = jobs;
-    const int pools[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23};
+    const int pools[] = {1 << 18, 1 << 19, 1 << 20, 1 << 21, 1 << 22, 1 << 23, 1 << 24};
     const uint32_t segs[] = {256, 1024};
     float best = 1e30f;
     pool = 1 << 20;
