@@ -221,6 +221,33 @@ def test_persistent_sample_windows(dev, window, pool, iters):
         dev.render(bad)
 
 
+@pytest.mark.parametrize("w,h,spp,spp_whole", [(512, 300, 32, 27), (4096, 2, 1030, 512)])
+def test_plain_row_bands(dev, w, h, spp, spp_whole):
+    """A plain render whose samples exceed one pass (2^22 paths) runs band by band: bands of whole
+    rows holding all their samples (512x300 at 32 spp: bands of 256 and 44 rows, one pass each),
+    or single rows in several passes when a row's samples exceed a pass (4096 wide at 1030 spp:
+    passes of 1024 and 6 samples).  The image, guides and segment count equal those of
+    whole-frame passes (samples_per_pass) bit for bit, and rows across the first band boundary
+    match the oracle."""
+    print(f"case: plain row bands {w}x{h} {spp} spp", flush=True)
+    s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
+    dev.upload(s)
+    bounces = 3 if spp < 100 else 1
+    kw = dict(rr_depth=2, firefly_clamp=20)
+    img, g, rays = dev.render(bc.render_desc(s, w, h, spp, bounces, **kw), guides=True)
+    whole = bc.render_desc(s, w, h, spp, bounces, samples_per_pass=spp_whole, **kw)
+    img2, g2, rays2 = dev.render(whole, guides=True)
+    assert np.array_equal(img, img2) and np.array_equal(g, g2) and rays == rays2
+    if h > 256:
+        o = orc.Scene.from_host(s, bins=16)
+        ref, rg, _, _ = o.render(orc.desc_from(whole), precision=32, y0=254, rows=4, guides=True)
+        got = img[254:258]
+        assert np.array_equal(got[..., 3], ref[..., 3])
+        err = np.abs(got[..., :3] - ref[..., :3])
+        assert not (err > 1e-4 * np.abs(ref[..., :3]) + 1e-6).any(), err.max()
+        assert np.array_equal(g[254:258], rg)
+
+
 @pytest.mark.parametrize("mode", [1, 0])
 def test_render_ahead_chunks(dev, mode):
     """bc_render renders the rows after a chunk with it and serves the following chunks from
