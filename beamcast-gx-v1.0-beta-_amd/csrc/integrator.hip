@@ -1729,9 +1729,10 @@ constexpr int kNotAhead = 1 << 30;
 // Render-ahead for bc_render.  A frame rendered in row chunks pays one drain of the path pool
 // (persistent schedule) or one short run of passes (plain schedule) per call, plus a host round
 // trip (terrain 1080p, 64 spp, persistent: 147 ms in 64-row chunks against 111 ms in one call);
-// so the first call of a run of chunks renders as many further rows (a multiple of its chunk
-// height) as one window of per-sample results holds (persistent) or as kAheadPixels allows
-// (plain), into a buffer of the context, and the calls for those rows copy them out.  Every pixel
+// so the first call of a run of chunks renders the rest of the frame, or as many further rows (a
+// multiple of its chunk height) as one window of per-sample results holds (persistent) or as
+// kAheadPixels allows (plain), into a buffer of the context, and the calls for those rows copy
+// them out.  Every pixel
 // depends only on the description, the scene and its own index (spatial reuse, which reads
 // neighbouring pixels' reservoirs among the rendered rows, never renders ahead), so the rows are
 // what rendering them alone gives.  Rows are served once, in increasing order: a request that
@@ -1750,7 +1751,8 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
         const size_t per_row = d->mode == 1 ? (size_t)d->width * (size_t)d->spp : (size_t)d->width;
         const size_t cap = d->mode == 1 ? kResultCap : kAheadPixels;
         const int fit = (int)std::min<size_t>((size_t)d->height, cap / per_row);
-        const int block = std::min(d->height - y0, std::max(rows, fit / rows * rows));
+        // the rest of the frame when it fits, else whole chunks
+        const int block = d->height - y0 <= fit ? d->height - y0 : std::max(rows, fit / rows * rows);
         if (block <= rows) return kNotAhead;
         const size_t bpix = (size_t)block * (size_t)d->width;
         c->ahead_valid = false;

@@ -175,15 +175,16 @@ typedef struct bc_render_desc {
  * (gpu_opencl_windows.go:644-646); bch_render_hip (beamcast_host.h) does that.
  * Render-ahead (on unless bc_set_render_ahead turns it off; not with spatial reuse, whose pixels
  * read their neighbours among the rendered rows): a call whose rows are not already rendered
- * renders them together with the following rows, up to a multiple of `rows` that one window of
- * per-sample results holds (mode 1: W x rows x spp <= 2^27 samples) or up to 2^23 pixels
- * (mode 0), at most the rest of the frame, into a buffer of the context; later calls with the
- * same description for rows inside that block, with no scene or material change in between, copy
- * them out, each row once and in increasing order (a call for rows before the end of the rows
- * last served, as a new frame's first chunk, renders anew).  Every pixel depends only on the
- * description, the scene and its index, so the output is what rendering each chunk alone gives;
- * the GPU runs one render per block instead of one per chunk, with no host round trip between.  rays_out is then the sum of the rows' per-pixel
- * segment counts (rgba w), as renderOpenCL sums them. */
+ * renders them together with the rest of the frame when that fits one window of per-sample
+ * results (mode 1: W x rows x spp <= 2^27 samples) or 2^23 pixels (mode 0), else with the
+ * following rows up to the multiple of `rows` that fits, into a buffer of the context; later
+ * calls with the same description for rows inside that block, with no scene or material change
+ * in between, copy them out, each row once and in increasing order (a call for rows before the
+ * end of the rows last served, as a new frame's first chunk, renders anew).  Every pixel depends
+ * only on the description, the scene and its index, so the output is what rendering each chunk
+ * alone gives; the GPU runs one render per block instead of one per chunk, with no host round
+ * trip between.  rays_out is then the sum of the rows' per-pixel segment counts (rgba w), as
+ * renderOpenCL sums them. */
 int bc_render(bc_ctx* ctx, const bc_render_desc* desc, int y0, int rows, float* rgba,
               uint32_t* guides, uint64_t* rays_out, const volatile int32_t* cancel);
 /* No counterpart in the reference: render-ahead of bc_render, 1 on (the default) or 0 off;
