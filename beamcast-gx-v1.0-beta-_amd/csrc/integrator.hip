@@ -133,8 +133,8 @@ __global__ __launch_bounds__(kBlock) BC_WAVES(BC_EXTEND_WAVES) void k_extend(Sce
     }
 }
 
-// Extension rays with lane refill, for the segments after the first (whose rays leave scattering
-// vertices in all directions): wave w of a workgroup takes its run (queue_run) of the path queue
+// Extension rays with lane refill, for the second segment (whose rays leave scattering vertices in
+// all directions; see launch_extend): wave w of a workgroup takes its run (queue_run) of the path queue
 // of each segment the workgroup owns, and whenever at least S.extend_refill of its lanes are idle
 // hands them the next paths of the run (ballot + prefix count), as k_shadow_refill does for
 // shadow rays.  Each path is traced by one lane; results equal k_extend's.
@@ -1038,7 +1038,8 @@ bcd::RenderParams make_params(const bc_ctx* c, const bc_render_desc* d, int y0, 
 }
 
 // The resident scene as the kernels see it.  Refill thresholds: shadow rays take the any-hit
-// setting; extension rays after the first segment the closest-hit one (automatic: 8 lanes).
+// setting; the plain schedule's second-segment extension rays the closest-hit one (automatic: 8
+// lanes).
 bcd::SceneBufs scene_bufs(const bc_ctx* c) {
     const int closest = c->trace_refill[0] < 0 ? 8 : c->trace_refill[0];
     return bcd::SceneBufs{c->d_nodes, c->d_prims, c->d_ids, c->d_pmat, c->d_mats, c->d_lights, c->d_pdf,

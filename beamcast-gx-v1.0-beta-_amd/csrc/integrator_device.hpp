@@ -106,7 +106,7 @@ struct SceneBufs {
     int* err;
     int stack;  // traversal stack entries per lane (BVH8 depth - 1); kernels take that much LDS
     int shadow_refill;  // shadow rays: refill a wave's lanes once this many are idle (0: one ray per thread)
-    int extend_refill;  // extension rays after the first segment: the same for closest hit (0: one ray per thread)
+    int extend_refill;  // the plain schedule's second-segment extension rays: the same for closest hit (0: off)
 };
 
 __device__ __forceinline__ float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
