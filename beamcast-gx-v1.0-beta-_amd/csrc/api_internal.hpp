@@ -76,6 +76,7 @@ struct bc_ctx {
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     bool pin_busy[2] = {false, false};
     int pin_next = 0;
+    struct CopyPool* copy_pool = nullptr;  // host threads of the staged copies (upload.cpp)
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
     void* tmp_in = nullptr;

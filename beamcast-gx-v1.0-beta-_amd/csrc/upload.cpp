@@ -8,42 +8,106 @@
 // being reused before the transfer that reads or fills it has finished.  Small copies go
 // straight to hipMemcpyAsync.
 #include <algorithm>
+#include <condition_variable>
 #include <cstring>
+#include <mutex>
 #include <thread>
 #include <vector>
 
 #include "api_internal.hpp"
+
+// Host threads that copy slices of a staged chunk, started once per context (with its pinned
+// buffers) and kept until bc_destroy, instead of threads started for every chunk.  A copy hands
+// out slices under the pool's mutex; the calling thread takes slices too, so a copy finishes
+// whatever the workers do.
+struct CopyPool {
+    std::mutex m;
+    std::condition_variable work, done;
+    std::vector<std::thread> th;
+    char* dst = nullptr;
+    const char* src = nullptr;
+    size_t n = 0, per = 0;
+    unsigned parts = 0, next = 0, pending = 0;
+    bool stop = false;
+
+    // Takes the next slice of the current copy, if any (the mutex held).
+    bool take(size_t& a, size_t& b) {
+        if (next >= parts) return false;
+        a = (size_t)next++ * per;
+        b = std::min(n, a + per);
+        return true;
+    }
+    void finish_one() {
+        if (--pending == 0) done.notify_all();
+    }
+    void worker() {
+        std::unique_lock<std::mutex> lk(m);
+        for (;;) {
+            work.wait(lk, [&] { return stop || next < parts; });
+            if (stop) return;
+            size_t a, b;
+            while (take(a, b)) {
+                lk.unlock();
+                if (b > a) std::memcpy(dst + a, src + a, b - a);
+                lk.lock();
+                finish_one();
+            }
+        }
+    }
+};
 
 namespace {
 
 constexpr size_t kDirectMax = size_t(4) << 20;  // up to this, one plain hipMemcpyAsync
 constexpr unsigned kMaxThreads = 8;
 
-// Copies n bytes with up to nth threads (the caller's among them).  It never throws: it is called
-// inside the C entry points, so a thread that cannot be started (std::system_error) or a vector
-// that cannot grow leaves its part of the copy to the calling thread, and the threads already
-// started are joined before returning.
-void par_copy(char* dst, const char* src, size_t n, unsigned nth) noexcept {
-    if (nth <= 1 || n < (size_t(1) << 20)) {
+// Starts the context's copy threads (up to kMaxThreads - 1 besides the caller).  It never
+// throws: a thread that cannot be started leaves the pool smaller, an empty pool copies on the
+// calling thread alone.
+CopyPool* copy_pool(bc_ctx* c) {
+    if (c->copy_pool) return c->copy_pool;
+    CopyPool* p = nullptr;
+    try {
+        p = new CopyPool();
+        const unsigned hw = std::thread::hardware_concurrency();
+        const unsigned nth = std::max(1u, std::min(kMaxThreads, hw));
+        for (unsigned i = 1; i < nth; i++) p->th.emplace_back([p] { p->worker(); });
+    } catch (...) {
+    }
+    c->copy_pool = p;
+    return p;
+}
+
+// Copies n bytes with the pool's threads and the caller's.  It never throws (it runs inside the
+// C entry points): without a pool, or if its mutex fails, the caller copies alone.
+void par_copy(CopyPool* p, char* dst, const char* src, size_t n) noexcept {
+    if (!p || p->th.empty() || n < (size_t(1) << 20)) {
         std::memcpy(dst, src, n);
         return;
     }
-    const size_t per = ((n + nth - 1) / nth + 4095) & ~size_t(4095);
-    std::thread pool[kMaxThreads];
-    unsigned started = 0;
-    size_t a = per;  // [per, a) is covered by started threads
-    for (; a < n && started < kMaxThreads; a += per) {
-        const size_t b = std::min(n, a + per);
-        try {
-            pool[started] = std::thread([=] { std::memcpy(dst + a, src + a, b - a); });
-        } catch (...) {
-            break;
+    try {
+        std::unique_lock<std::mutex> lk(p->m);
+        const unsigned parts = (unsigned)p->th.size() + 1;
+        p->dst = dst;
+        p->src = src;
+        p->n = n;
+        p->per = ((n + parts - 1) / parts + 4095) & ~size_t(4095);
+        p->parts = parts;
+        p->next = 0;
+        p->pending = parts;
+        p->work.notify_all();
+        size_t a, b;
+        while (p->take(a, b)) {
+            lk.unlock();
+            if (b > a) std::memcpy(dst + a, src + a, b - a);
+            lk.lock();
+            p->finish_one();
         }
-        started++;
+        p->done.wait(lk, [&] { return p->pending == 0; });
+        p->parts = 0;
+    } catch (...) {
+        std::memcpy(dst, src, n);  // the mutex failed before any slice was handed out
     }
-    std::memcpy(dst, src, std::min(n, per));
-    if (a < n) std::memcpy(dst + a, src + a, n - a);
-    for (unsigned i = 0; i < started; i++) pool[i].join();
 }
 
 bool ensure_pinned(bc_ctx* c) {
@@ -75,8 +139,7 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
     if (bytes == 0) return true;
     if (bytes <= kDirectMax) return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what);
     if (!ensure_pinned(c)) return false;
-    const unsigned hw = std::thread::hardware_concurrency();
-    const unsigned nth = std::max(1u, std::min(kMaxThreads, hw));
+    CopyPool* pool = copy_pool(c);
     const char* s = static_cast<const char*>(src);
     char* d = static_cast<char*>(dst);
     int b = c->pin_next;
@@ -84,7 +147,7 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
         const size_t n = std::min(bc_ctx::kPinChunk, bytes - off);
         if (c->pin_busy[b] && !bc_check(c, hipEventSynchronize(c->pin_ev[b]), what)) return false;
         c->pin_busy[b] = false;
-        par_copy(static_cast<char*>(c->h_pin[b]), s + off, n, nth);
+        par_copy(pool, static_cast<char*>(c->h_pin[b]), s + off, n);
         if (!bc_check(c, hipMemcpyAsync(d + off, c->h_pin[b], n, hipMemcpyHostToDevice, st), what) ||
             !bc_check(c, hipEventRecord(c->pin_ev[b], st), what))
             return false;
@@ -105,8 +168,7 @@ bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
         c->pin_busy[i] = false;
     }
-    const unsigned hw = std::thread::hardware_concurrency();
-    const unsigned nth = std::max(1u, std::min(kMaxThreads, hw));
+    CopyPool* pool = copy_pool(c);
     const char* s = static_cast<const char*>(src);
     char* d = static_cast<char*>(dst);
     const size_t K = bc_ctx::kPinChunk, nchunk = (bytes + K - 1) / K;
@@ -121,12 +183,25 @@ bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
     for (size_t k = 0; k < nchunk; k++) {
         if (k + 1 < nchunk && !queue(k + 1)) return false;
         if (!bc_check(c, hipEventSynchronize(c->pin_ev[k & 1]), what)) return false;
-        par_copy(d + k * K, static_cast<const char*>(c->h_pin[k & 1]), std::min(K, bytes - k * K), nth);
+        par_copy(pool, d + k * K, static_cast<const char*>(c->h_pin[k & 1]), std::min(K, bytes - k * K));
     }
     return true;
 }
 
 void bc_pinned_release(bc_ctx* c) {
+    if (CopyPool* p = c->copy_pool) {
+        try {
+            std::lock_guard<std::mutex> lk(p->m);
+            p->stop = true;
+        } catch (...) {
+            p->stop = true;
+        }
+        p->work.notify_all();
+        for (std::thread& t : p->th)
+            if (t.joinable()) t.join();
+        delete p;
+        c->copy_pool = nullptr;
+    }
     for (int i = 0; i < 2; i++) {
         if (c->pin_ev[i]) {
             if (c->pin_busy[i]) (void)hipEventSynchronize(c->pin_ev[i]);
