@@ -115,6 +115,8 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
 // Device-to-host copy on stream st into pageable memory, staged the same way; returns once dst
 // holds the data.
 bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what);
+bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1, const void* src1, size_t n1,
+                 hipStream_t st, const char* what);
 void bc_pinned_release(bc_ctx* c);
 // Material table and per-group light distributions from the material data (api.cpp).
 int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data);

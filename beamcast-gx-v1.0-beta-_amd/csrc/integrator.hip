@@ -1779,8 +1779,8 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
     const size_t off = (size_t)(y0 - c->ahead_y0) * (size_t)d->width, npix = (size_t)rows * (size_t)d->width;
     const size_t bpix = (size_t)c->ahead_rows * (size_t)d->width;
     const uint32_t* dg = (const uint32_t*)((const char*)c->ahead_buf + bpix * 16);
-    if (!bc_d2h(c, rgba, (const float*)c->ahead_buf + off * 4, npix * 16, c->stream, "copy image") ||
-        (guides && !bc_d2h(c, guides, dg + off * 3, npix * 12, c->stream, "copy guides")))
+    if (!bc_d2h_pair(c, rgba, (const float*)c->ahead_buf + off * 4, npix * 16, guides, dg + off * 3,
+                     guides ? npix * 12 : 0, c->stream, "copy image and guides"))
         return BC_ERR_DEVICE;
     c->ahead_next = y0 + rows;
     // the rows' path segments, from the per-pixel counts as renderOpenCL sums them
@@ -1811,8 +1811,8 @@ extern "C" int bc_render(bc_ctx* c, const bc_render_desc* d, int y0, int rows, f
     uint32_t* dg = guides ? (uint32_t*)((char*)c->out_buf + npix * 16) : nullptr;
     int rc = bc_render_device(c, d, y0, rows, (float*)c->out_buf, dg, rays_out, nullptr);
     if (rc == BC_OK) {
-        if (!bc_d2h(c, rgba, c->out_buf, npix * 16, c->stream, "copy image") ||
-            (guides && !bc_d2h(c, guides, dg, npix * 12, c->stream, "copy guides")))
+        if (!bc_d2h_pair(c, rgba, c->out_buf, npix * 16, guides, dg, guides ? npix * 12 : 0, c->stream,
+                         "copy image and guides"))
             rc = BC_ERR_DEVICE;
     }
     return rc;

@@ -188,6 +188,28 @@ bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
     return true;
 }
 
+// Two device-to-host copies that each fit one pinned buffer (a rendered chunk's image and guides):
+// both transfers are queued into the pinned buffers and waited for once, then copied out, instead
+// of two pageable copies with a wait each.  Larger ones take bc_d2h.
+bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1, const void* src1, size_t n1,
+                 hipStream_t st, const char* what) {
+    if (n0 > bc_ctx::kPinChunk || n1 > bc_ctx::kPinChunk || n0 + n1 <= (size_t(64) << 10))
+        return bc_d2h(c, dst0, src0, n0, st, what) && bc_d2h(c, dst1, src1, n1, st, what);
+    if (!ensure_pinned(c)) return false;
+    for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
+        if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
+        c->pin_busy[i] = false;
+    }
+    if ((n0 && !bc_check(c, hipMemcpyAsync(c->h_pin[0], src0, n0, hipMemcpyDeviceToHost, st), what)) ||
+        (n1 && !bc_check(c, hipMemcpyAsync(c->h_pin[1], src1, n1, hipMemcpyDeviceToHost, st), what)) ||
+        !bc_check(c, hipStreamSynchronize(st), what))
+        return false;
+    CopyPool* pool = copy_pool(c);
+    if (n0) par_copy(pool, static_cast<char*>(dst0), static_cast<const char*>(c->h_pin[0]), n0);
+    if (n1) par_copy(pool, static_cast<char*>(dst1), static_cast<const char*>(c->h_pin[1]), n1);
+    return true;
+}
+
 void bc_pinned_release(bc_ctx* c) {
     if (CopyPool* p = c->copy_pool) {
         try {
