@@ -260,8 +260,7 @@ extern "C" int bc_render_post(bc_ctx* c, const bc_render_desc* d, const bc_post_
     uint32_t* d_px = (uint32_t*)c->ps->buf[3];
     rc = bc_postprocess_device(c, pd, d->width, d->height, d_rgba, d_g, d_px, linear ? d_rgba : nullptr);
     if (rc != BC_OK) return rc;
-    if (!bc_d2h(c, pixels, d_px, np * 4, c->stream, "copy pixels") ||
-        (linear && !bc_d2h(c, linear, d_rgba, np * 16, c->stream, "copy linear")))
+    if (!bc_d2h_pair(c, pixels, d_px, np * 4, linear, d_rgba, linear ? np * 16 : 0, c->stream, "copy pixels"))
         return BC_ERR_DEVICE;
     return BC_OK;
 }
