@@ -5,8 +5,8 @@
 // with one host thread.  Here a large copy is cut into chunks; host threads copy a chunk into
 // (or out of) one of two pinned buffers owned by the context while the DMA engine moves the
 // other, so the host copy and the PCIe transfer overlap.  An event per buffer keeps a buffer from
-// being reused before the transfer that reads or fills it has finished.  Small copies go
-// straight to hipMemcpyAsync.
+// being reused before the transfer that reads or fills it has finished.  Small copies (up to 64
+// KB) go straight to hipMemcpyAsync.
 #include <algorithm>
 #include <condition_variable>
 #include <cstring>
@@ -58,7 +58,10 @@ struct CopyPool {
 
 namespace {
 
-constexpr size_t kDirectMax = size_t(4) << 20;  // up to this, one plain hipMemcpyAsync
+// Up to this, one plain hipMemcpyAsync.  Pageable copies of a few hundred KB cost up to ~80 us each
+// in the runtime's own staging (a 64-row chunk's image and guides: 1.56 ms per closed-box frame of
+// four chunks, against 0.88 ms through the pinned buffers), so only small ones go direct.
+constexpr size_t kDirectMax = size_t(64) << 10;
 constexpr unsigned kMaxThreads = 8;
 
 // Starts the context's copy threads (up to kMaxThreads - 1 besides the caller).  It never
