@@ -755,7 +755,7 @@ extern "C" int bc_build_bvh(bc_ctx* c, int32_t n, const double* prim_box, const 
         levels.push_back(cur);
         // chunk table of this level
         ranges.resize((size_t)nl);
-        if (!bc_check(c, hipMemcpyAsync(level.p, cur.data(), 4 * cur.size(), hipMemcpyHostToDevice, st), "upload level"))
+        if (!bc_h2d(c, level.p, cur.data(), 4 * cur.size(), st, "upload level"))
             return BC_ERR_DEVICE;
         hipLaunchKernelGGL(k_ranges, dim3(grid_of((size_t)nl, 64)), dim3(64), 0, st, nodes.as<BNode>(), level.as<int>(), nl,
                            dranges.as<int2>());
@@ -776,8 +776,8 @@ extern "C" int bc_build_bvh(bc_ctx* c, int32_t n, const double* prim_box, const 
             if (!gbins.alloc(c, need, "bins")) return BC_ERR_NOMEM;
             gbins_cap = need;
         }
-        if (!bc_check(c, hipMemcpyAsync(dch.p, ch.data(), sizeof(Chunk) * ch.size(), hipMemcpyHostToDevice, st), "upload chunks") ||
-            !bc_check(c, hipMemcpyAsync(dfirst.p, first.data(), 4 * first.size(), hipMemcpyHostToDevice, st), "upload chunks") ||
+        if (!bc_h2d(c, dch.p, ch.data(), sizeof(Chunk) * ch.size(), st, "upload chunks") ||
+            !bc_h2d(c, dfirst.p, first.data(), 4 * first.size(), st, "upload chunks") ||
             !bc_check(c, hipMemsetAsync(lst, 0, sizeof(int), st), "clear level count"))
             return BC_ERR_DEVICE;
         const unsigned nch = (unsigned)ch.size();
@@ -832,13 +832,13 @@ extern "C" int bc_build_bvh(bc_ctx* c, int32_t n, const double* prim_box, const 
     // sizes bottom-up, pre-order indices top-down (one launch per level)
     for (size_t l = levels.size(); l-- > 0;) {
         const int nl = (int)levels[l].size();
-        if (!bc_check(c, hipMemcpyAsync(level.p, levels[l].data(), 4 * (size_t)nl, hipMemcpyHostToDevice, st), "upload level"))
+        if (!bc_h2d(c, level.p, levels[l].data(), 4 * (size_t)nl, st, "upload level"))
             return BC_ERR_DEVICE;
         hipLaunchKernelGGL(k_sizes, dim3(grid_of((size_t)nl, 64)), dim3(64), 0, st, nodes.as<BNode>(), level.as<int>(), nl);
     }
     for (size_t l = 0; l < levels.size(); l++) {
         const int nl = (int)levels[l].size();
-        if (!bc_check(c, hipMemcpyAsync(level.p, levels[l].data(), 4 * (size_t)nl, hipMemcpyHostToDevice, st), "upload level"))
+        if (!bc_h2d(c, level.p, levels[l].data(), 4 * (size_t)nl, st, "upload level"))
             return BC_ERR_DEVICE;
         hipLaunchKernelGGL(k_ids, dim3(grid_of((size_t)nl, 64)), dim3(64), 0, st, nodes.as<BNode>(), level.as<int>(), nl);
     }

@@ -1891,7 +1891,7 @@ extern "C" int bc_backward(bc_ctx* c, const bc_render_desc* d, int y0, int rows,
     float* d_dl = (float*)c->out_buf;
     float* d_grad = d_dl + npix * 3;
     hipStream_t st = c->stream;
-    if (!bc_check(c, hipMemcpyAsync(d_dl, dl, npix * 12, hipMemcpyHostToDevice, st), "upload d loss") ||
+    if (!bc_h2d(c, d_dl, dl, npix * 12, st, "upload d loss") ||
         !bc_check(c, hipMemsetAsync(d_grad, 0, ngc * 4, st), "clear gradient"))
         return BC_ERR_DEVICE;
     const bcd::SceneBufs S = scene_bufs(c);
