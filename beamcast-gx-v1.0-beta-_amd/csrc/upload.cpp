@@ -113,19 +113,23 @@ void par_copy(CopyPool* p, char* dst, const char* src, size_t n) noexcept {
     }
 }
 
+// Allocates the two pinned buffers on first use.  Returns false, leaving no error behind, when the
+// runtime cannot provide them (pinned memory is a limited resource): the copies then go straight
+// to hipMemcpyAsync as small ones do.
 bool ensure_pinned(bc_ctx* c) {
     if (c->h_pin[0]) return true;
     void* p[2] = {nullptr, nullptr};
     hipEvent_t e[2] = {nullptr, nullptr};
     bool good = true;
     for (int i = 0; i < 2 && good; i++)
-        good = bc_check(c, hipHostMalloc(&p[i], bc_ctx::kPinChunk, hipHostMallocDefault), "pinned upload buffer") &&
-               bc_check(c, hipEventCreateWithFlags(&e[i], hipEventDisableTiming), "upload event");
+        good = hipHostMalloc(&p[i], bc_ctx::kPinChunk, hipHostMallocDefault) == hipSuccess &&
+               hipEventCreateWithFlags(&e[i], hipEventDisableTiming) == hipSuccess;
     if (!good) {
         for (int i = 0; i < 2; i++) {
             if (p[i]) (void)hipHostFree(p[i]);
             if (e[i]) (void)hipEventDestroy(e[i]);
         }
+        (void)hipGetLastError();  // a later launch check must not report this failure
         return false;
     }
     for (int i = 0; i < 2; i++) {
@@ -140,8 +144,8 @@ bool ensure_pinned(bc_ctx* c) {
 
 bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what) {
     if (bytes == 0) return true;
-    if (bytes <= kDirectMax) return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what);
-    if (!ensure_pinned(c)) return false;
+    if (bytes <= kDirectMax || !ensure_pinned(c))
+        return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what);
     CopyPool* pool = copy_pool(c);
     const char* s = static_cast<const char*>(src);
     char* d = static_cast<char*>(dst);
@@ -163,10 +167,9 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
 
 bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what) {
     if (bytes == 0) return true;
-    if (bytes <= kDirectMax)
+    if (bytes <= kDirectMax || !ensure_pinned(c))
         return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), what) &&
                bc_check(c, hipStreamSynchronize(st), what);
-    if (!ensure_pinned(c)) return false;
     for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
         c->pin_busy[i] = false;
@@ -196,9 +199,8 @@ bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
 // of two pageable copies with a wait each.  Larger ones take bc_d2h.
 bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1, const void* src1, size_t n1,
                  hipStream_t st, const char* what) {
-    if (n0 > bc_ctx::kPinChunk || n1 > bc_ctx::kPinChunk || n0 + n1 <= (size_t(64) << 10))
+    if (n0 > bc_ctx::kPinChunk || n1 > bc_ctx::kPinChunk || n0 + n1 <= kDirectMax || !ensure_pinned(c))
         return bc_d2h(c, dst0, src0, n0, st, what) && bc_d2h(c, dst1, src1, n1, st, what);
-    if (!ensure_pinned(c)) return false;
     for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
         c->pin_busy[i] = false;
