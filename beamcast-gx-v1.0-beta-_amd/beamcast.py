@@ -624,10 +624,22 @@ class Device:
         self._check(lib().bc_timer_stop(self.ctx, ctypes.byref(ms)))
         return float(ms.value)
 
-    def render(self, desc: RenderDesc, y0=0, rows=None, guides=False, cancel=None):
+    def render(self, desc: RenderDesc, y0=0, rows=None, guides=False, cancel=None, out=None, guides_out=None):
+        """bc_render of rows [y0, y0 + rows) into host memory -> (rgba [rows, W, 4], guides [rows, W, 3]
+        or None, path segments).  out / guides_out: C-contiguous float32 / uint32 arrays of those
+        shapes to write into (a caller's chunk buffers reused across chunks) instead of new ones."""
         rows = desc.height - y0 if rows is None else rows
-        rgba = np.zeros((rows, desc.width, 4), np.float32)
-        g = np.zeros((rows, desc.width, 3), np.uint32) if guides else None
+
+        def dest(a, dtype, ch):
+            if a is None:
+                return np.zeros((rows, desc.width, ch), dtype)
+            if a.dtype != dtype or a.shape != (rows, desc.width, ch) or not a.flags.c_contiguous:
+                raise ValueError(f"output buffer must be a C-contiguous {np.dtype(dtype).name} array of shape "
+                                 f"{(rows, desc.width, ch)}")
+            return a
+
+        rgba = dest(out, np.float32, 4)
+        g = dest(guides_out, np.uint32, 3) if guides else None
         rays = c_uint64()
         cflag = None
         if cancel is not None:

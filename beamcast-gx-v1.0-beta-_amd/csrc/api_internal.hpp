@@ -76,6 +76,9 @@ struct bc_ctx {
     hipEvent_t pin_ev[2] = {nullptr, nullptr};
     bool pin_busy[2] = {false, false};
     int pin_next = 0;
+    // the pinned buffers hold the rows rendered ahead (image in [0], guides in [1]); any other
+    // staged copy clears it before it reuses them
+    bool pin_ahead = false;
     struct CopyPool* copy_pool = nullptr;  // host threads of the staged copies (upload.cpp)
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
@@ -117,6 +120,13 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
 bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st, const char* what);
 bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1, const void* src1, size_t n1,
                  hipStream_t st, const char* what);
+// Copies src0 (n0 bytes) into pinned buffer 0 and src1 (n1 bytes) into pinned buffer 1 and keeps
+// them there (sets pin_ahead): 1 when held, 0 when they do not fit or the buffers cannot be
+// allocated (nothing copied, no error), -1 on a failed copy.
+int bc_d2h_hold(bc_ctx* c, const void* src0, size_t n0, const void* src1, size_t n1, hipStream_t st,
+                const char* what);
+// Host-to-host copy by the context's copy threads (pageable destination, pinned source).
+void bc_host_copy(bc_ctx* c, void* dst, const void* src, size_t n);
 void bc_pinned_release(bc_ctx* c);
 // Material table and per-group light distributions from the material data (api.cpp).
 int bc_set_materials(bc_ctx* c, int32_t nmat, const int32_t* mat_kind, const double* mat_data);

@@ -1764,10 +1764,15 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
         if (block <= rows) return kNotAhead;
         const size_t bpix = (size_t)block * (size_t)d->width;
         c->ahead_valid = false;
+        c->pin_ahead = false;
         if (!bc_ensure(c, c->ahead_buf, c->ahead_cap, bpix * 28 + 64, "rows rendered ahead")) return BC_ERR_NOMEM;
         uint32_t* dg = guides ? (uint32_t*)((char*)c->ahead_buf + bpix * 16) : nullptr;
         const int rc = bc_render_device(c, d, y0, block, (float*)c->ahead_buf, dg, nullptr, nullptr);
         if (rc != BC_OK) return rc;
+        // a block that fits the pinned buffers is read back once, and its chunks are then copied
+        // on the host (closed box 256x256 in 64-row chunks: one transfer instead of four pairs)
+        if (bc_d2h_hold(c, c->ahead_buf, bpix * 16, dg, guides ? bpix * 12 : 0, c->stream, "read rows rendered ahead") < 0)
+            return BC_ERR_DEVICE;
         c->ahead_valid = true;
         c->ahead_desc = *d;
         c->ahead_y0 = y0;
@@ -1779,9 +1784,13 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
     const size_t off = (size_t)(y0 - c->ahead_y0) * (size_t)d->width, npix = (size_t)rows * (size_t)d->width;
     const size_t bpix = (size_t)c->ahead_rows * (size_t)d->width;
     const uint32_t* dg = (const uint32_t*)((const char*)c->ahead_buf + bpix * 16);
-    if (!bc_d2h_pair(c, rgba, (const float*)c->ahead_buf + off * 4, npix * 16, guides, dg + off * 3,
-                     guides ? npix * 12 : 0, c->stream, "copy image and guides"))
+    if (c->pin_ahead) {  // the block is still in the pinned buffers
+        bc_host_copy(c, rgba, (const char*)c->h_pin[0] + off * 16, npix * 16);
+        if (guides) bc_host_copy(c, guides, (const char*)c->h_pin[1] + off * 12, npix * 12);
+    } else if (!bc_d2h_pair(c, rgba, (const float*)c->ahead_buf + off * 4, npix * 16, guides, dg + off * 3,
+                            guides ? npix * 12 : 0, c->stream, "copy image and guides")) {
         return BC_ERR_DEVICE;
+    }
     c->ahead_next = y0 + rows;
     // the rows' path segments, from the per-pixel counts as renderOpenCL sums them
     // (gpu_opencl_windows.go:681-685)

@@ -146,6 +146,7 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
     if (bytes == 0) return true;
     if (bytes <= kDirectMax || !ensure_pinned(c))
         return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what);
+    c->pin_ahead = false;
     CopyPool* pool = copy_pool(c);
     const char* s = static_cast<const char*>(src);
     char* d = static_cast<char*>(dst);
@@ -170,6 +171,7 @@ bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
     if (bytes <= kDirectMax || !ensure_pinned(c))
         return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), what) &&
                bc_check(c, hipStreamSynchronize(st), what);
+    c->pin_ahead = false;
     for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
         c->pin_busy[i] = false;
@@ -201,6 +203,7 @@ bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1,
                  hipStream_t st, const char* what) {
     if (n0 > bc_ctx::kPinChunk || n1 > bc_ctx::kPinChunk || n0 + n1 <= kDirectMax || !ensure_pinned(c))
         return bc_d2h(c, dst0, src0, n0, st, what) && bc_d2h(c, dst1, src1, n1, st, what);
+    c->pin_ahead = false;
     for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
         c->pin_busy[i] = false;
@@ -213,6 +216,30 @@ bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1,
     if (n0) par_copy(pool, static_cast<char*>(dst0), static_cast<const char*>(c->h_pin[0]), n0);
     if (n1) par_copy(pool, static_cast<char*>(dst1), static_cast<const char*>(c->h_pin[1]), n1);
     return true;
+}
+
+int bc_d2h_hold(bc_ctx* c, const void* src0, size_t n0, const void* src1, size_t n1, hipStream_t st,
+                const char* what) {
+    if (n0 > bc_ctx::kPinChunk || n1 > bc_ctx::kPinChunk || !ensure_pinned(c)) return 0;
+    c->pin_ahead = false;
+    for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
+        if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return -1;
+        c->pin_busy[i] = false;
+    }
+    if ((n0 && !bc_check(c, hipMemcpyAsync(c->h_pin[0], src0, n0, hipMemcpyDeviceToHost, st), what)) ||
+        (n1 && !bc_check(c, hipMemcpyAsync(c->h_pin[1], src1, n1, hipMemcpyDeviceToHost, st), what)) ||
+        !bc_check(c, hipStreamSynchronize(st), what))
+        return -1;
+    c->pin_ahead = true;
+    return 1;
+}
+
+void bc_host_copy(bc_ctx* c, void* dst, const void* src, size_t n) {
+    if (n <= kDirectMax) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    par_copy(copy_pool(c), static_cast<char*>(dst), static_cast<const char*>(src), n);
 }
 
 void bc_pinned_release(bc_ctx* c) {
@@ -239,4 +266,5 @@ void bc_pinned_release(bc_ctx* c) {
         c->pin_ev[i] = nullptr;
         c->pin_busy[i] = false;
     }
+    c->pin_ahead = false;
 }

@@ -66,10 +66,15 @@ def fixed_schedule(args, d):
 def go_facing(dev, desc, reps=1):
     """The renderer's own path to the GPU: bc_render into host buffers in 64-row chunks with the
     compact guides, as renderOpenCL chunks a frame (gpu_opencl_windows.go:631-641) and the cgo
-    shim calls it; seconds per frame after one untimed frame."""
+    shim calls it (one image and one guide chunk buffer per frame, reused by every chunk); seconds
+    per frame after one untimed frame."""
     def frame():
+        # one chunk buffer of each kind per frame, reused by every chunk, as the shim mallocs them
+        raw = np.empty((64, desc.width, 4), np.float32)
+        raw_guide = np.empty((64, desc.width, 3), np.uint32)
         for y0 in range(0, desc.height, 64):
-            dev.render(desc, y0=y0, rows=min(64, desc.height - y0), guides=True)
+            rows = min(64, desc.height - y0)
+            dev.render(desc, y0=y0, rows=rows, guides=True, out=raw[:rows], guides_out=raw_guide[:rows])
 
     frame()
     t0 = time.perf_counter()

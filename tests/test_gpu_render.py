@@ -269,11 +269,32 @@ def test_render_ahead_chunks(dev, mode):
     assert np.array_equal(np.concatenate([c[0] for c in ahead]), full)
     assert np.array_equal(np.concatenate([c[1] for c in ahead]), gfull)
     assert sum(c[2] for c in ahead) == rfull
+    # one pair of chunk buffers reused by every chunk, as the cgo shim renders a frame
+    raw, raw_guide = np.empty((8, 64, 4), np.float32), np.empty((8, 64, 3), np.uint32)
+    for y in range(0, 40, 8):
+        a, g, _ = dev.render(dp, y0=y, rows=8, guides=True, out=raw, guides_out=raw_guide)
+        assert a is raw and g is raw_guide
+        assert np.array_equal(raw, full[y:y + 8]) and np.array_equal(raw_guide, gfull[y:y + 8])
+    with pytest.raises(ValueError):
+        dev.render(dp, y0=0, rows=8, out=np.empty((8, 64, 4), np.float64))
     dev.set_render_ahead(False)
     alone = chunks()
     dev.set_render_ahead(True)
     for a, b in zip(ahead, alone):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    # the block is read back once into the pinned staging buffers; a staged upload and download
+    # between two chunks reuse both buffers, and the next chunk is then read from the device
+    dev.render(dp, y0=0, rows=8, guides=True)
+    big = np.random.default_rng(3).integers(0, 256, size=(17 << 20) + 5, dtype=np.uint8)
+    p = dev.to_device(big)
+    back = np.zeros_like(big)
+    dev.download(p, back)
+    dev.free(p)
+    assert np.array_equal(back, big)
+    got = dev.render(dp, y0=8, rows=8, guides=True)
+    assert np.array_equal(got[0], ahead[1][0]) and np.array_equal(got[1], ahead[1][1]) and got[2] == ahead[1][2]
+    got = dev.render(dp, y0=16, rows=24, guides=True)
+    assert np.array_equal(got[0], full[16:]) and np.array_equal(got[1], gfull[16:])
 
     def fresh(desc, y0, rows, guides=False):
         dev.set_render_ahead(False)
