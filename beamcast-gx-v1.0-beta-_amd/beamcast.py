@@ -104,6 +104,7 @@ def lib():
             "bc_render_noalloc": (c_int, [vp, POINTER(RenderDesc), c_int, c_int, vp, vp, POINTER(c_uint64), POINTER(c_uint64)]),
             "bc_trace_counts": (c_int, [vp, vp, ctypes.c_int64, c_int, vp, vp]),
             "bc_valu_probe": (c_int, [vp, c_int, c_int, c_int, POINTER(ctypes.c_float)]),
+            "bc_line_probe": (c_int, [vp, vp, c_uint64, c_int, c_int, POINTER(ctypes.c_float)]),
             "bc_render_tuning": (c_int, [vp, POINTER(c_int32)]),
             "bc_batch_upload": (c_int, [vp, c_int, vp]),
             "bc_postprocess_device": (c_int, [vp, POINTER(PostDesc), c_int, c_int, vp, vp, vp, vp]),
@@ -700,6 +701,13 @@ class Device:
         """bc_trace_counts: the query of trace_device plus per-ray (node visits, record tests,
         id reads, 0) as n x 4 uint32 at d_counts (asynchronous)."""
         self._check(lib().bc_trace_counts(self.ctx, d_rays, n, int(any_hit), d_out, d_counts))
+
+    def line_probe(self, d_buf, nbytes, blocks, iters):
+        """bc_line_probe: milliseconds of `blocks` x 256 threads each reading `iters` 64-byte blocks at
+        hashed positions of the device buffer d_buf (nbytes, a power of two)."""
+        ms = ctypes.c_float()
+        self._check(lib().bc_line_probe(self.ctx, d_buf, nbytes, blocks, iters, ctypes.byref(ms)))
+        return ms.value
 
     def valu_probe(self, blocks, iters, mix=0):
         """bc_valu_probe: milliseconds of `blocks` x 256 threads running `iters` slab-test visits

@@ -600,6 +600,22 @@ int bc_valu_probe(bc_ctx* c, int mix, int blocks, int iters, float* ms) {
     return BC_OK;
 }
 
+int bc_line_probe(bc_ctx* c, const void* d_buf, uint64_t bytes, int blocks, int iters, float* ms) {
+    // a power-of-two number of 64-byte blocks, indexable by 32 bits
+    if (!c || !d_buf || bytes < 64 || (bytes & (bytes - 1)) != 0 || bytes / 64 - 1 > 0xFFFFFFFFull ||
+        (reinterpret_cast<uintptr_t>(d_buf) & 15) != 0 || blocks <= 0 || iters <= 0 || !ms)
+        return BC_ERR_ARG;
+    hipSetDevice(c->device);
+    if (!ok(c, hipEventRecord(c->ev0, c->stream), "probe") ||
+        !ok(c, bcd::launch_line_probe(static_cast<const uint4*>(d_buf), (uint32_t)(bytes / 64 - 1), blocks,
+                                      (uint32_t)iters, 0x5EEDu, (uint32_t*)c->d_err + 8, c->stream),
+            "probe launch") ||
+        !ok(c, hipEventRecord(c->ev1, c->stream), "probe") || !ok(c, hipEventSynchronize(c->ev1), "probe") ||
+        !ok(c, hipEventElapsedTime(ms, c->ev0, c->ev1), "probe"))
+        return BC_ERR_DEVICE;
+    return BC_OK;
+}
+
 int bc_trace(bc_ctx* c, const float* rays, int64_t n, int any_hit, void* out) {
     if (!c || n < 0 || (n > 0 && (!rays || !out))) return BC_ERR_ARG;
     if (n == 0) return BC_OK;

@@ -62,6 +62,8 @@ hipError_t launch_trace_count(int stack, bool any_hit, const bc_node8* nodes, co
 // eight-child slab tests (mix 0) or rounds of eight independent FMAs (mix 1).  sink receives
 // nothing in practice (it keeps the loops from being dead).
 hipError_t launch_valu_probe(int mix, int blocks, uint32_t iters, uint32_t seed, uint32_t* sink, hipStream_t st);
+hipError_t launch_line_probe(const uint4* buf, uint32_t block_mask, int blocks, uint32_t iters, uint32_t seed,
+                             uint32_t* sink, hipStream_t st);
 
 // Refit: records (and materials) of the moved primitives in leaf order from the view's arrays.
 hipError_t launch_refit_records(const double* geom, const int32_t* kind, const int32_t* mat, const uint32_t* ids, uint32_t n,

@@ -281,6 +281,31 @@ hipError_t launch_valu_probe(int mix, int blocks, uint32_t iters, uint32_t seed,
     return hipGetLastError();
 }
 
+// Scattered-read probe: each thread reads `iters` 64-byte blocks (four 16-byte loads, the fetch of
+// one node) at hashed positions of a buffer of block_mask + 1 blocks, and writes one word only in
+// a case that does not occur.  Over a buffer far larger than the L2 (4 MB per XCD) nearly every
+// read fills a 128-byte line from the Infinity Cache, the access an incoherent traversal makes, so
+// its rate is the ceiling of scattered line fills that traversal is compared with.
+__global__ __launch_bounds__(256) void k_line_probe(const uint4* __restrict__ buf, uint32_t block_mask, uint32_t iters,
+                                                    uint32_t seed, uint32_t* __restrict__ sink) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    uint32_t acc = 0;
+    for (uint32_t it = 0; it < iters; it++) {
+        const uint32_t b = bc_mix32((t * 0x9E3779B9u) ^ (it * 0x85EBCA6Bu) ^ seed) & block_mask;
+        const uint4* p = buf + (size_t)b * 4u;
+        const uint4 a0 = p[0], a1 = p[1], a2 = p[2], a3 = p[3];
+        acc += (a0.x ^ a0.y ^ a0.z ^ a0.w) + (a1.x ^ a1.y ^ a1.z ^ a1.w) + (a2.x ^ a2.y ^ a2.z ^ a2.w) +
+               (a3.x ^ a3.y ^ a3.z ^ a3.w);
+    }
+    if (acc == 0x12345678u) sink[0] = acc;  // practically never: keeps every load alive
+}
+
+hipError_t launch_line_probe(const uint4* buf, uint32_t block_mask, int blocks, uint32_t iters, uint32_t seed,
+                             uint32_t* sink, hipStream_t st) {
+    hipLaunchKernelGGL(k_line_probe, dim3((unsigned)blocks), dim3(256), 0, st, buf, block_mask, iters, seed, sink);
+    return hipGetLastError();
+}
+
 int stack_for_depth(int depth) {
     const int need = depth - 1;
     if (need > kMaxStack) return -1;

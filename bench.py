@@ -455,9 +455,11 @@ PROFILE_SUMMARY = os.path.join(ROOT, "profiles", "k_trace_soup1M_16Mrays.json")
 # MI355X's published fp32 vector rate, 157.3 TFLOP/s, in wave64 FMA instructions per second
 # (64 lanes x 2 flops each): one per SIMD every 2 clocks, 1024 SIMDs at 2.4 GHz
 VALU_PEAK = 157.3e12 / 128
+LINE_PROBE_BYTES = 64 << 20
 
 
-def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_cfg, probe_ms):  # noqa: C901
+def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_cfg, probe_ms, line_cfg,  # noqa: C901
+                   line_ms):
     """Roofline of the dominant kernel (the slowest launch of the step).
 
     achieved: the BVH8 traversal's algorithmic bytes per launch over its HIP-event duration, against
@@ -469,7 +471,10 @@ def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_
     instruction mix alone, on registers), and the SIMD clocks per VALU instruction the summary's
     counter run gives for each (GRBM_GUI_ACTIVE per XCD over VALU instructions per SIMD).  limiter:
     the same launch's texture-addresser and texture-data busy fractions and VALU instructions per
-    SIMD clock from the summary's memory-pipe counter run, next to the probes' VALU rates."""
+    SIMD clock from the summary's memory-pipe counter run, next to the probes' VALU rates.
+    line_fill: the launch's L2 fill bytes (FETCH_SIZE of the summary) per second of this run against
+    those of the scattered-read probe (64-byte reads at hashed positions of a 64 MiB buffer; its
+    FETCH_SIZE from the same summary, its duration from this run)."""
     out = {"bound": "hbm", "kernel": f"BVH8 traversal ({launch})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
            "alg_bytes_per_ray": round(alg_bytes, 1), "ref_layout_alg_bytes_per_ray": round(alg_ref_bytes, 1),
@@ -487,6 +492,19 @@ def roofline_entry(launch, achieved, alg_bytes, alg_ref_bytes, kernel_ms, probe_
         out["traffic_frac"] = round(out["traffic_gbs"] / HBM_PEAK_GBS, 4)
         out["l2_hit"] = k.get("l2_hit")
         out["active_lanes_per_valu_instr"] = k.get("active_lanes")
+    lp = prof.get("line_probe")
+    if k and lp and (lp.get("blocks"), lp.get("iters"), lp.get("bytes")) == (*line_cfg, LINE_PROBE_BYTES):
+        fill = k["fetch_bytes"] / (kernel_ms * 1e-3) / 1e9
+        ceil = lp["fetch_bytes"] / (line_ms * 1e-3) / 1e9
+        reads = line_cfg[0] * 256 * line_cfg[1]
+        out["line_fill"] = {"achieved": round(fill, 1), "ceiling": round(ceil, 1), "unit": "GB/s",
+                            "frac": round(fill / ceil, 4),
+                            "basis": "L2 fill bytes (FETCH_SIZE) per second: the launch against the scattered-read "
+                                     "probe (64-byte reads at hashed positions of a 64 MiB buffer)",
+                            "probe_ms": round(line_ms, 3),
+                            "probe_delivered_gbs": round(reads * 64 / (line_ms * 1e-3) / 1e9, 1),
+                            "probe_fill_bytes_per_read": round(lp["fetch_bytes"] / reads, 1),
+                            "source": os.path.relpath(PROFILE_SUMMARY, ROOT)}
     pf, pm = prof.get("fma_probe"), prof.get("valu_probe")
     ok = all(p and (p.get("blocks"), p.get("iters")) == tuple(probe_cfg) for p in (pf, pm))
     if k and ok and k.get("valu_insts_per_wave"):
@@ -705,6 +723,13 @@ def main():
     for mix in (1, 0):
         dev.valu_probe(*probe_cfg, mix=mix)
         probe_ms[mix] = min(dev.valu_probe(*probe_cfg, mix=mix) for _ in range(3))
+    # scattered 64-byte reads (one node fetch each) over 64 MiB, the size of the soup's scene: the
+    # rate of line fills from the Infinity Cache that the incoherent launch is compared with
+    line_cfg = (8192, 64)
+    d_line = dev.alloc(LINE_PROBE_BYTES)
+    dev.line_probe(d_line, LINE_PROBE_BYTES, *line_cfg)
+    line_ms = min(dev.line_probe(d_line, LINE_PROBE_BYTES, *line_cfg) for _ in range(3))
+    dev.free(d_line)
 
     # the reference layout's algorithmic bytes on the same rays (32 + out + 32 per threaded node
     # fetch + 44 per primitive test, counted by the oracle's threaded traversal over a sample), as
@@ -780,7 +805,7 @@ def main():
             "scene_bytes": int(st["bytes"]),
         },
         "roofline": roofline_entry(names[dominant], achieved, alg[dominant], alg_ref[dominant], kernel_ms[dominant],
-                                   probe_cfg, probe_ms),
+                                   probe_cfg, probe_ms, line_cfg, line_ms),
         "cpu_baseline": cpu,
         "render": render,
         "adjoint": adjoint,

@@ -281,9 +281,15 @@ int bc_trace_device(bc_ctx* ctx, const void* d_rays, int64_t n, int any_hit, voi
  * bc_valu_probe: times (HIP events, *ms) `blocks` workgroups of 256 threads that each run `iters`
  * register-only eight-child slab tests, the VALU instruction mix of a node visit (mix 0), or
  * `iters` rounds of eight independent fp32 FMAs (mix 1), to calibrate the VALU issue rates the
- * traversal is compared with. */
+ * traversal is compared with.
+ * bc_line_probe: times `blocks` workgroups of 256 threads that each read `iters` 64-byte blocks
+ * (four 16-byte loads, one node fetch) at hashed positions of the device buffer d_buf of `bytes`
+ * bytes (a power of two, at least 64; 16-byte aligned; its contents do not matter): over a buffer
+ * far larger than the L2, the rate of scattered line fills from the Infinity Cache that an
+ * incoherent traversal is compared with.  BC_ERR_ARG for any other buffer. */
 int bc_trace_counts(bc_ctx* ctx, const void* d_rays, int64_t n, int any_hit, void* d_out, uint32_t* d_counts);
 int bc_valu_probe(bc_ctx* ctx, int mix, int blocks, int iters, float* ms);
+int bc_line_probe(bc_ctx* ctx, const void* d_buf, uint64_t bytes, int blocks, int iters, float* ms);
 
 /* Device memory and stream helpers for callers that keep inputs resident in HBM. */
 void* bc_alloc(bc_ctx* ctx, uint64_t bytes);

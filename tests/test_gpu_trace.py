@@ -364,3 +364,25 @@ def test_soup_far_from_origin(dev):
         assert np.array_equal(got["prim"], prim) and np.array_equal(got["t"], t), mode
         assert np.array_equal(dev.trace(ra, any_hit=True), occ), mode
     dev.set_flatten(0)
+
+
+def test_probes(dev):
+    """The calibration probes of the traversal's rooflines run, time themselves, and refuse
+    buffers they cannot index (bc_line_probe: a power-of-two byte count of at least 64, 16-byte
+    aligned)."""
+    print("case: VALU and scattered-read probes", flush=True)
+    assert dev.valu_probe(64, 16, mix=0) > 0.0 and dev.valu_probe(64, 16, mix=1) > 0.0
+    nbytes = 1 << 20
+    d = dev.alloc(nbytes)
+    try:
+        assert dev.line_probe(d, nbytes, 64, 8) > 0.0
+        assert dev.line_probe(d, 64, 4, 2) > 0.0
+        for bad in (0, 32, nbytes - 64, 3 << 10):
+            with pytest.raises(bc.BeamcastError):
+                dev.line_probe(d, bad, 64, 8)
+        with pytest.raises(bc.BeamcastError):
+            dev.line_probe(d + 8, 1 << 10, 4, 2)  # not 16-byte aligned
+        with pytest.raises(bc.BeamcastError):
+            dev.line_probe(d, nbytes, 0, 8)
+    finally:
+        dev.free(d)
