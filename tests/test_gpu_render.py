@@ -438,3 +438,26 @@ def test_spatial_reuse_errors(dev):
         dev.render(bc.render_desc(s, 16, 16, 1, 2, restir=8, spatial=4, mode=1))  # plain schedule only
     with pytest.raises(bc.BeamcastError):
         dev.render(bc.render_desc(s, 16, 16, 1, 2, restir=8, spatial=17))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_render_ahead_block_beyond_pinned_buffers(dev, mode):
+    """A block whose image exceeds one 16 MB pinned staging buffer (1280x1000: 20.5 MB of rgba)
+    is not held on the host: each 64-row chunk is read from the device block.  Chunks equal the
+    whole-frame render bit for bit, with guides, on both schedules."""
+    print(f"case: render-ahead block beyond the pinned buffers, schedule {mode}, 1280x1000", flush=True)
+    s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
+    dev.upload(s)
+    sched = dict(mode=1, pool=1 << 18, iters=4) if mode == 1 else dict(mode=0)
+    d = bc.render_desc(s, 1280, 1000, 1, 2, rr_depth=2, firefly_clamp=20, **sched)
+    dev.set_render_ahead(False)
+    full, gfull, rfull = dev.render(d, guides=True)
+    dev.set_render_ahead(True)
+    raw, raw_guide = np.empty((64, 1280, 4), np.float32), np.empty((64, 1280, 3), np.uint32)
+    total = 0
+    for y in range(0, 1000, 64):
+        rows = min(64, 1000 - y)
+        a, g, r = dev.render(d, y0=y, rows=rows, guides=True, out=raw[:rows], guides_out=raw_guide[:rows])
+        assert np.array_equal(a, full[y:y + rows]) and np.array_equal(g, gfull[y:y + rows]), f"rows {y}.."
+        total += r
+    assert total == rfull
