@@ -62,9 +62,9 @@ struct bc_ctx {
     size_t cap_sel = 0;
     // ray binning before queries (order.hip): mode 0 off, 1 always, 2 when a sample of the
     // batch is incoherent; bits = log2 of the origin cells per axis.  Off by default: on the 1M
-    // soup it lifts the incoherent L2 hit rate from 0.47 to 0.95 but saves only about what its
-    // passes cost (closest hit is bound by VALU issue and any hit by load latency, not by L2
-    // misses).
+    // soup it lifts the incoherent L2 hit rate from 0.47 to 0.95, yet with lane refill the
+    // traversal plus its passes is slower than the traversal alone (incoherent closest hit 6.36
+    // against 7.1 ms).
     int order_mode = 0, order_bits = 4;
     void* d_order = nullptr;  // bin counts, totals, starts, permutation, decision flag
     void* d_sorted = nullptr; // the sorted rays
@@ -79,6 +79,10 @@ struct bc_ctx {
     // the pinned buffers hold the rows rendered ahead (image in [0], guides in [1]); any other
     // staged copy clears it before it reuses them
     bool pin_ahead = false;
+    // a copy of the next chunk's rows rendered ahead, started into pinned buffer pin_pre_buf (image,
+    // then guides) when the previous chunk was served; pin_pre_y0 < 0: none.  Cleared as pin_ahead.
+    int pin_pre_y0 = -1, pin_pre_rows = 0, pin_pre_buf = 0;
+    bool pin_pre_guides = false;
     struct CopyPool* copy_pool = nullptr;  // host threads of the staged copies (upload.cpp)
     int* d_err = nullptr;  // [0] kernel error flag
     // staging for host-pointer ray queries
@@ -125,6 +129,12 @@ bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1,
 // allocated (nothing copied, no error), -1 on a failed copy.
 int bc_d2h_hold(bc_ctx* c, const void* src0, size_t n0, const void* src1, size_t n1, hipStream_t st,
                 const char* what);
+// Starts an asynchronous copy of src0 (n0 bytes), then src1 (n1 bytes), into one pinned buffer and
+// records that buffer's event: the buffer index, -1 when they do not fit or the buffers cannot be
+// allocated (nothing started, no error), -2 on a failed call.  Clears pin_ahead and any earlier
+// prefetch; the caller records what the buffer holds.
+int bc_d2h_prefetch(bc_ctx* c, const void* src0, size_t n0, const void* src1, size_t n1, hipStream_t st,
+                    const char* what);
 // Host-to-host copy by the context's copy threads (pageable destination, pinned source).
 void bc_host_copy(bc_ctx* c, void* dst, const void* src, size_t n);
 void bc_pinned_release(bc_ctx* c);

@@ -116,6 +116,12 @@ void par_copy(CopyPool* p, char* dst, const char* src, size_t n) noexcept {
 // Allocates the two pinned buffers on first use.  Returns false, leaving no error behind, when the
 // runtime cannot provide them (pinned memory is a limited resource): the copies then go straight
 // to hipMemcpyAsync as small ones do.
+// The pinned buffers are about to be reused: they no longer hold rows rendered ahead.
+void drop_ahead(bc_ctx* c) {
+    c->pin_ahead = false;
+    c->pin_pre_y0 = -1;
+}
+
 bool ensure_pinned(bc_ctx* c) {
     if (c->h_pin[0]) return true;
     void* p[2] = {nullptr, nullptr};
@@ -146,7 +152,7 @@ bool bc_h2d(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
     if (bytes == 0) return true;
     if (bytes <= kDirectMax || !ensure_pinned(c))
         return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st), what);
-    c->pin_ahead = false;
+    drop_ahead(c);
     CopyPool* pool = copy_pool(c);
     const char* s = static_cast<const char*>(src);
     char* d = static_cast<char*>(dst);
@@ -171,7 +177,7 @@ bool bc_d2h(bc_ctx* c, void* dst, const void* src, size_t bytes, hipStream_t st,
     if (bytes <= kDirectMax || !ensure_pinned(c))
         return bc_check(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st), what) &&
                bc_check(c, hipStreamSynchronize(st), what);
-    c->pin_ahead = false;
+    drop_ahead(c);
     for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
         c->pin_busy[i] = false;
@@ -203,7 +209,7 @@ bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1,
                  hipStream_t st, const char* what) {
     if (n0 > bc_ctx::kPinChunk || n1 > bc_ctx::kPinChunk || n0 + n1 <= kDirectMax || !ensure_pinned(c))
         return bc_d2h(c, dst0, src0, n0, st, what) && bc_d2h(c, dst1, src1, n1, st, what);
-    c->pin_ahead = false;
+    drop_ahead(c);
     for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return false;
         c->pin_busy[i] = false;
@@ -221,7 +227,7 @@ bool bc_d2h_pair(bc_ctx* c, void* dst0, const void* src0, size_t n0, void* dst1,
 int bc_d2h_hold(bc_ctx* c, const void* src0, size_t n0, const void* src1, size_t n1, hipStream_t st,
                 const char* what) {
     if (n0 > bc_ctx::kPinChunk || n1 > bc_ctx::kPinChunk || !ensure_pinned(c)) return 0;
-    c->pin_ahead = false;
+    drop_ahead(c);
     for (int i = 0; i < 2; i++) {  // uploads still reading the buffers finish first
         if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), what)) return -1;
         c->pin_busy[i] = false;
@@ -232,6 +238,23 @@ int bc_d2h_hold(bc_ctx* c, const void* src0, size_t n0, const void* src1, size_t
         return -1;
     c->pin_ahead = true;
     return 1;
+}
+
+int bc_d2h_prefetch(bc_ctx* c, const void* src0, size_t n0, const void* src1, size_t n1, hipStream_t st,
+                    const char* what) {
+    if (n0 + n1 > bc_ctx::kPinChunk || !ensure_pinned(c)) return -1;
+    drop_ahead(c);
+    const int b = c->pin_next;
+    if (c->pin_busy[b] && !bc_check(c, hipEventSynchronize(c->pin_ev[b]), what)) return -2;
+    c->pin_busy[b] = false;
+    char* h = static_cast<char*>(c->h_pin[b]);
+    if ((n0 && !bc_check(c, hipMemcpyAsync(h, src0, n0, hipMemcpyDeviceToHost, st), what)) ||
+        (n1 && !bc_check(c, hipMemcpyAsync(h + n0, src1, n1, hipMemcpyDeviceToHost, st), what)) ||
+        !bc_check(c, hipEventRecord(c->pin_ev[b], st), what))
+        return -2;
+    c->pin_busy[b] = true;
+    c->pin_next = b ^ 1;
+    return b;
 }
 
 void bc_host_copy(bc_ctx* c, void* dst, const void* src, size_t n) {
@@ -266,5 +289,5 @@ void bc_pinned_release(bc_ctx* c) {
         c->pin_ev[i] = nullptr;
         c->pin_busy[i] = false;
     }
-    c->pin_ahead = false;
+    drop_ahead(c);
 }

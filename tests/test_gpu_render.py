@@ -443,8 +443,10 @@ def test_spatial_reuse_errors(dev):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_render_ahead_block_beyond_pinned_buffers(dev, mode):
     """A block whose image exceeds one 16 MB pinned staging buffer (1280x1000: 20.5 MB of rgba)
-    is not held on the host: each 64-row chunk is read from the device block.  Chunks equal the
-    whole-frame render bit for bit, with guides, on both schedules."""
+    is not held on the host: each 64-row chunk is read from the device block, the next one's copy
+    started into a pinned buffer as the last one is served.  Chunks equal the whole-frame render
+    bit for bit, with guides, on both schedules, also when a staged copy reuses that buffer
+    between two chunks and when the chunk heights change."""
     print(f"case: render-ahead block beyond the pinned buffers, schedule {mode}, 1280x1000", flush=True)
     s = bc.HostScene.workload(bc.SHOWCASE, bins=16, leaf=4)
     dev.upload(s)
@@ -460,4 +462,17 @@ def test_render_ahead_block_beyond_pinned_buffers(dev, mode):
         a, g, r = dev.render(d, y0=y, rows=rows, guides=True, out=raw[:rows], guides_out=raw_guide[:rows])
         assert np.array_equal(a, full[y:y + rows]) and np.array_equal(g, gfull[y:y + rows]), f"rows {y}.."
         total += r
+        if y == 128:
+            # a staged upload and download between two chunks reuse the pinned buffer that holds
+            # the next chunk's copy
+            big = np.random.default_rng(5).integers(0, 256, size=(33 << 20) + 7, dtype=np.uint8)
+            p = dev.to_device(big)
+            back = np.zeros_like(big)
+            dev.download(p, back)
+            dev.free(p)
+            assert np.array_equal(back, big)
     assert total == rfull
+    # chunks of another height than the one copied ahead, and without guides
+    for y, rows in ((0, 64), (64, 32), (96, 64)):
+        a, g, _ = dev.render(d, y0=y, rows=rows)
+        assert g is None and np.array_equal(a, full[y:y + rows])
