@@ -103,6 +103,7 @@ struct bc_ctx {
     bc_render_desc ahead_desc{};
     void* ahead_buf = nullptr;
     size_t ahead_cap = 0;
+    std::vector<uint64_t> ahead_row_segs;  // path segments of each row of the block
     // integrator
     RenderState* rs = nullptr;
     // post stack

@@ -876,6 +876,21 @@ __global__ __launch_bounds__(kBlock) BC_WAVES(6) void k_shadow_adj(SceneBufs S, 
     adj_lds_flush(lds, ng, grad, P.nmat, P.nmat * 3);
 }
 
+// Per-row sums of the per-pixel path-segment counts (rgba.w, integers held in fp32) of a block of
+// rows rendered ahead, one workgroup of 256 threads per row, so that a chunk's segment count is a
+// sum over its rows on the host.
+__global__ __launch_bounds__(256) void k_row_segments(const float4* __restrict__ rgba, int width,
+                                                      unsigned long long* __restrict__ out) {
+    const size_t base = (size_t)blockIdx.x * (size_t)width;
+    unsigned long long sum = 0;
+    for (int x = (int)threadIdx.x; x < width; x += 256) sum += (unsigned long long)(int)(rgba[base + x].w + 0.5f);
+    for (int o = 32; o > 0; o >>= 1) sum += __shfl_down(sum, o);
+    __shared__ unsigned long long part[4];
+    if ((threadIdx.x & 63u) == 0) part[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = part[0] + part[1] + part[2] + part[3];
+}
+
 }  // namespace bcd
 
 // ------------------------------------------------------------------------------ host side
@@ -1769,10 +1784,22 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
         for (int i = 0; i < 2; i++)  // a copy still reading the old block ends before it can be freed
             if (c->pin_busy[i] && !bc_check(c, hipEventSynchronize(c->pin_ev[i]), "rows rendered ahead"))
                 return BC_ERR_DEVICE;
-        if (!bc_ensure(c, c->ahead_buf, c->ahead_cap, bpix * 28 + 64, "rows rendered ahead")) return BC_ERR_NOMEM;
+        // image, guides, then the per-row segment counts (8-byte aligned)
+        const size_t row_off = (bpix * 28 + 7) & ~size_t(7);
+        if (!bc_ensure(c, c->ahead_buf, c->ahead_cap, row_off + (size_t)block * 8 + 64, "rows rendered ahead"))
+            return BC_ERR_NOMEM;
         uint32_t* dg = guides ? (uint32_t*)((char*)c->ahead_buf + bpix * 16) : nullptr;
         const int rc = bc_render_device(c, d, y0, block, (float*)c->ahead_buf, dg, nullptr, nullptr);
         if (rc != BC_OK) return rc;
+        unsigned long long* d_rows = (unsigned long long*)((char*)c->ahead_buf + row_off);
+        c->ahead_row_segs.resize((size_t)block);
+        hipLaunchKernelGGL(bcd::k_row_segments, dim3((unsigned)block), dim3(256), 0, c->stream,
+                           (const float4*)c->ahead_buf, d->width, d_rows);
+        if (!bc_check(c, hipGetLastError(), "row segment counts") ||
+            !bc_check(c, hipMemcpyAsync(c->ahead_row_segs.data(), d_rows, (size_t)block * 8, hipMemcpyDeviceToHost,
+                                        c->stream), "row segment counts") ||
+            !bc_check(c, hipStreamSynchronize(c->stream), "row segment counts"))
+            return BC_ERR_DEVICE;
         // a block that fits the pinned buffers is read back once, and its chunks are then copied
         // on the host (closed box 256x256 in 64-row chunks: one transfer instead of four pairs)
         if (bc_d2h_hold(c, c->ahead_buf, bpix * 16, dg, guides ? bpix * 12 : 0, c->stream, "read rows rendered ahead") < 0)
@@ -1828,14 +1855,11 @@ int render_ahead(bc_ctx* c, const bc_render_desc* d, int y0, int rows, float* rg
     }
     c->ahead_next = y0 + rows;
     // the rows' path segments, from the per-pixel counts as renderOpenCL sums them
-    // (gpu_opencl_windows.go:681-685)
+    // (gpu_opencl_windows.go:681-685), summed per row on the device when the block was rendered
     if (rays_out) {
-        // each count is an integer held exactly in fp32 (below 2^24), so it converts through
-        // int32, which vectorises (through uint64 it does not: 151 against 60 us for a 64-row chunk
-        // of a 1080p frame)
-        int64_t r = 0;
-        for (size_t i = 0; i < npix; i++) r += (int32_t)(rgba[4 * i + 3] + 0.5f);
-        *rays_out = (uint64_t)r;
+        uint64_t r = 0;
+        for (int k = y0 - c->ahead_y0; k < y0 - c->ahead_y0 + rows; k++) r += c->ahead_row_segs[(size_t)k];
+        *rays_out = r;
     }
     return BC_OK;
 }
